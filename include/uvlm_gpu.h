@@ -1,0 +1,177 @@
+/*
+ * uvlm_gpu.h -- C-ABI of the MI355X (gfx950) UVLM hot path.
+ *
+ * Plain C types only: pointers to host arrays of double, sizes and flags.
+ * Every entry point returns UVLM_OK (0) or a negative error code; the text
+ * of the last error is available from uvlm_last_error().  Host arrays are
+ * borrowed for the duration of the call only.  Unless stated otherwise a
+ * call is host-synchronous: its outputs are valid when it returns.
+ *
+ * Array layouts follow the reference's MultipleSheet (sheet-major,
+ * row-major, K = j + cols*(i + rows*n)) with 3 doubles per point:
+ *   nodes  [num][rows+1][cols+1][3]   ring corners of a lattice
+ *   gamma  [num][rows][cols]          ring circulations
+ * Ring (n,i,j) has corners p0=(n,i,j) p1=(n,i,j+1) p2=(n,i+1,j+1) p3=(n,i+1,j).
+ *
+ * Each entry point names the reference (xeno14/UVLM) function it replaces.
+ * One context per host thread; a context is not reentrant.
+ */
+#ifndef UVLM_GPU_H
+#define UVLM_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uvlm_ctx uvlm_ctx;
+
+/* return codes */
+#define UVLM_OK 0
+#define UVLM_ERR_ARG (-1)      /* invalid argument or shape */
+#define UVLM_ERR_HIP (-2)      /* HIP runtime error */
+#define UVLM_ERR_OOM (-3)      /* device allocation failed */
+#define UVLM_ERR_SINGULAR (-4) /* zero pivot in the dense solve */
+#define UVLM_ERR_STATE (-5)    /* call out of order (e.g. sim not initialised) */
+#define UVLM_ERR_CAPACITY (-6) /* wake row capacity exhausted */
+#define UVLM_ERR_NOCONV (-7)   /* iterative refinement did not converge */
+
+/* arithmetic of the Biot-Savart sums (flags & 0x0f) */
+#define UVLM_PREC_F64_EXACT 0x0 /* reference operation order, no FMA, IEEE div/sqrt */
+#define UVLM_PREC_F64 0x1       /* FP64 with FMA and refined hardware rsqrt/rcp */
+#define UVLM_PREC_F32C 0x2      /* FP32 tile-relative, FP64-carried accumulation,
+                                   FP64 re-evaluation of near-core segments */
+/* ring summation variants (uvlm_ring_induced_velocity) */
+#define UVLM_SCALE_PER_RING 0x10 /* sum 4 segments, then x Gamma (vortex.cpp:41-50) */
+#define UVLM_CHORDWISE_ONLY 0x20 /* segments 0 and 2 only (vortex.cpp:52-65) */
+/* one sequential sum per target (no source splitting): bitwise identical to
+ * the reference order in F64_EXACT mode */
+#define UVLM_SEQUENTIAL 0x100
+
+/* dense solve modes */
+#define UVLM_SOLVE_F64 0      /* FP64 LU, partial pivoting, FP64 MFMA updates */
+#define UVLM_SOLVE_F32_IR 1   /* FP32 LU + FP64 iterative refinement */
+#define UVLM_SOLVE_BF16_IR 2  /* BF16-input MFMA LU updates + FP64 refinement */
+
+/* wake advection schemes (AdvectFactory names, advect_factory.cpp:22-27) */
+#define UVLM_ADVECT_EULER 0
+#define UVLM_ADVECT_RK2 1
+#define UVLM_ADVECT_RK4 2 /* reference update k1+2k2+2k3+k3 (advect.cpp:172-173) */
+#define UVLM_ADVECT_AB2 3
+#define UVLM_ADVECT_RK4_CLASSIC 4 /* textbook k1+2k2+2k3+k4 (no reference counterpart) */
+
+/* --- context ------------------------------------------------------------- */
+int uvlm_create(int device, uvlm_ctx** out);
+int uvlm_destroy(uvlm_ctx* ctx);
+const char* uvlm_last_error(const uvlm_ctx* ctx);
+/* Wait for all work queued on the context's stream. */
+int uvlm_sync(uvlm_ctx* ctx);
+
+/* --- stateless kernels (host arrays) ---------------------------------------- */
+
+/* Replaces UVLM::BiotSavartLaw(result, start, end, pos) (vortex.cpp:21-39),
+ * evaluated for n independent (start, end, pos) triples [n][3] each. */
+int uvlm_biot_savart(uvlm_ctx* ctx, const double* start, const double* end,
+                     const double* pos, size_t n, unsigned flags, double* out);
+
+/* Replaces UVLM::InducedVelocity(x, pos, gamma) (velocity.h:14-24) for
+ * n_targets targets [n][3]: sum over rings in (n,i,j) order of VORING
+ * (vortex.cpp:145-160, Gamma applied per segment). */
+int uvlm_induced_velocity(uvlm_ctx* ctx, const double* targets, size_t n_targets,
+                          const double* nodes, const double* gamma, size_t num,
+                          size_t rows, size_t cols, unsigned flags, double* out);
+
+/* Replaces the legacy ring sums UVLM::InducedVelocity(result, pos, first, last)
+ * and ChordwiseInducedVelocity (shed.inl:22-33,66-77) over n_rings 4-node
+ * rings [n_rings][4][3] with segments node[(k+1)%4] -> node[k]
+ * (vortex.h:132-140).  Use UVLM_SCALE_PER_RING for VortexRing::BiotSavartLaw
+ * semantics and UVLM_CHORDWISE_ONLY for the chordwise variant. */
+int uvlm_ring_induced_velocity(uvlm_ctx* ctx, const double* targets,
+                               size_t n_targets, const double* rings,
+                               const double* gamma, size_t n_rings,
+                               unsigned flags, double* out);
+
+/* Replaces SimpleSimulator::CalcMatrix (simple_simulator.cpp:76-101):
+ * A[K][L] = VORING_L(cpos_K; Gamma=1) . normals_K, written row-major [N][N]. */
+int uvlm_influence_matrix(uvlm_ctx* ctx, const double* nodes, size_t num,
+                          size_t rows, size_t cols, const double* cpos,
+                          const double* normals, unsigned flags, double* A);
+
+/* Replaces Eigen::FullPivLU<MatrixXd>(A).solve(rhs) (simple_simulator.cpp
+ * :263-268, linear.inl:125-129): A row-major [n][n].  iters_out (nullable)
+ * receives the refinement sweeps of the mixed-precision modes. */
+int uvlm_solve(uvlm_ctx* ctx, size_t n, const double* A, const double* rhs,
+               double* x, int mode, double tol, int max_iter, int* iters_out);
+
+/* --- device-resident SimpleSimulator state ------------------------------
+ * The wake (nodes and circulations) lives in device memory as a row ring
+ * buffer, so MultipleSheet::prepend_row is O(cols).  The host keeps the
+ * kinematics (Morphing) and hands over nodes and surface velocities. */
+
+/* Allocate the state of num wings of rows x cols rings for at most
+ * max_steps shedding steps.  flags: UVLM_PREC_* of the Biot-Savart sums. */
+int uvlm_sim_init(uvlm_ctx* ctx, size_t num, size_t rows, size_t cols,
+                  size_t max_steps, unsigned flags);
+
+/* Morphed wing nodes [num][rows+1][cols+1][3] of the current step
+ * (result of Morphing::Perfome, simple_simulator.cpp:247-253).  Also saves
+ * the previous circulations (wing_gamma_prev_, :243).  Asynchronous. */
+int uvlm_sim_set_wing(uvlm_ctx* ctx, const double* nodes);
+
+/* Replaces SimpleSimulator::Shed(step) (simple_simulator.cpp:171-188):
+ * prepend the trailing-edge node row and, when step > 0, the trailing-edge
+ * circulation row of the previous solve.  Asynchronous. */
+int uvlm_sim_shed(uvlm_ctx* ctx, size_t step);
+
+/* Replaces CalcMatrix + CalcRhs + FullPivLU (simple_simulator.cpp:76-124,
+ * 263-268): vmorph_cp [N][3] is Morphing::Velocity at the reference
+ * collocation points (cols major as wing_gamma), ff the forward-flight
+ * velocity.  gamma_out [N] (nullable) receives the circulations (sync). */
+int uvlm_sim_solve(uvlm_ctx* ctx, const double* vmorph_cp, const double ff[3],
+                   int solve_mode, double* gamma_out);
+
+/* Replaces SimpleSimulator::CalcLoad (simple_simulator.cpp:195-233,
+ * joukowski.cpp:12-92).  vmorph_edges [num][rows*(2*cols+1)][3] is
+ * Morphing::Velocity at the reference midpoints of the wing's unique
+ * non-trailing-edge edges: first the spanwise edges (i,j)-(i,j+1) for
+ * i<rows, j<cols, then the chordwise edges (i,j)-(i+1,j) for i<rows,
+ * j<=cols.  ref_area [num] = chord*span.  The coefficients (CD, CL) per wing
+ * are appended to the device history (read with uvlm_sim_get_loads).
+ * Asynchronous. */
+int uvlm_sim_loads(uvlm_ctx* ctx, const double* vmorph_edges, const double ff[3],
+                   double dt, const double* ref_area);
+
+/* Replaces advect::Advection::Advect(&wake_pos_, ...) with the scheme of
+ * AdvectFactory (advect.cpp:11-216): Euler, RK2, RK4 (reference formula),
+ * AB2, or classic RK4; node rows i <= ignore_steps use Euler in RK2/RK4.
+ * Asynchronous. */
+int uvlm_sim_advect(uvlm_ctx* ctx, int scheme, size_t ignore_steps,
+                    const double ff[3], double dt);
+
+/* Replaces SimpleSimulator::EraseOldestWake (simple_simulator.cpp:235-238). */
+int uvlm_sim_erase_oldest_wake(uvlm_ctx* ctx);
+
+/* Wake shape: node rows (s+1 at step s) and circulation rows (s). */
+int uvlm_sim_wake_shape(uvlm_ctx* ctx, size_t* node_rows, size_t* gamma_rows);
+/* Copy the wake out: nodes [num][node_rows][cols+1][3], gamma
+ * [num][gamma_rows][cols] (either nullable).  Synchronous. */
+int uvlm_sim_get_wake(uvlm_ctx* ctx, double* nodes, double* gamma);
+/* Restore a wake and the current wing circulations (e.g. a snapshot). */
+int uvlm_sim_set_wake(uvlm_ctx* ctx, size_t node_rows, size_t gamma_rows,
+                      const double* nodes, const double* gamma,
+                      const double* wing_gamma);
+/* Current wing circulations [N].  Synchronous. */
+int uvlm_sim_get_gamma(uvlm_ctx* ctx, double* gamma);
+/* Load history: count_inout in = capacity in steps, out = steps recorded;
+ * out [count][num][2] (CD, CL).  Synchronous. */
+int uvlm_sim_get_loads(uvlm_ctx* ctx, double* out, size_t* count_inout);
+/* Biot-Savart pairs evaluated by the simulation so far (one pair = one
+ * segment evaluation) and the pairs the reference's own loop structure
+ * would have evaluated for the same steps. */
+int uvlm_sim_pair_counts(uvlm_ctx* ctx, double* evaluated, double* reference);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+/*
+ * uvlm_host.h -- C entry points of the host library libuvlm.so.
+ *
+ * The host library is the reference-shaped C++ API (namespace UVLM:
+ * Vector3d, MultipleSheet, VortexRing, Morphing, wing generators,
+ * advect::Advection + AdvectFactory, simulator::SimpleSimulator) built on
+ * include/uvlm_gpu.h.  These C entry points let non-C++ callers (the Python
+ * package, drivers, tests) set up a SimpleSimulator from plain data and step
+ * it; they have no reference counterpart of their own: each one wraps the
+ * SimpleSimulator member named in its comment.
+ *
+ * Motion laws are the parametric forms below (the reference accepts any
+ * std::function):
+ *   plug(t)   = plug_amp * cos(plug_omega * t + plug_phase)
+ *   flap(t)   = flap_amp * cos(flap_omega * t + flap_phase)
+ *   twist(x,t)= sum_{m<4,q<2} TA[m][q] sin((m+1) pi/2 eta) cos((q+1) w t + TP[m][q])
+ *   bend(x,t) = sum_{m<4,q<2} BA[m][q] sin(pi xi) sin((m+1) pi/2 eta)
+ *                                      cos((q+1) w t + BP[m][q])
+ * with eta = x.y / (span/2), xi = x.x / chord, w = fourier_omega.
+ */
+#ifndef UVLM_HOST_H
+#define UVLM_HOST_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UVLM_HOST_MAX_WINGS 16
+#define UVLM_GEN_RECT 0
+#define UVLM_GEN_NACA4 1
+#define UVLM_GEN_NACA00 2
+
+typedef struct {
+  int generator;
+  int digit;
+  double chord, span;
+  double origin[3];
+  double alpha;
+  double plug_amp, plug_omega, plug_phase;
+  double flap_amp, flap_omega, flap_phase;
+  double fourier_omega;
+  double twist_amp[4][2], twist_phase[4][2];
+  double bend_amp[4][2], bend_phase[4][2];
+} uvlm_wing_spec;
+
+typedef struct {
+  int num_wings;
+  uvlm_wing_spec wings[UVLM_HOST_MAX_WINGS];
+  size_t rows, cols;     /* rings per wing (cols even) */
+  double ff[3];          /* forward-flight velocity */
+  double dt;
+  int scheme;            /* UVLM_ADVECT_* */
+  size_t ignore_steps;   /* Euler rows of RK2/RK4 (reference default 5) */
+  size_t erase_wake_after;
+  int loads;             /* compute CD/CL every step */
+  unsigned precision;    /* UVLM_PREC_* */
+  int solve_mode;        /* UVLM_SOLVE_* */
+} uvlm_case_spec;
+
+typedef struct uvlm_host_sim uvlm_host_sim;
+
+/* AddWing for every wing + set_forward_flight + set_advection + Prepare. */
+int uvlm_host_sim_create(const uvlm_case_spec* spec, size_t max_steps,
+                         uvlm_host_sim** out);
+void uvlm_host_sim_destroy(uvlm_host_sim* s);
+const char* uvlm_host_last_error(void);
+/* MainLoop(step) + optional EraseOldestWake (asynchronous). */
+int uvlm_host_sim_step(uvlm_host_sim* s, size_t step);
+int uvlm_host_sim_sync(uvlm_host_sim* s);
+int uvlm_host_sim_wing_nodes(uvlm_host_sim* s, double* out);
+int uvlm_host_sim_gamma(uvlm_host_sim* s, double* out);
+int uvlm_host_sim_wake_shape(uvlm_host_sim* s, size_t* node_rows, size_t* gamma_rows);
+int uvlm_host_sim_wake(uvlm_host_sim* s, double* nodes, double* gamma);
+int uvlm_host_sim_set_wake(uvlm_host_sim* s, size_t node_rows, size_t gamma_rows,
+                           const double* nodes, const double* gamma,
+                           const double* wing_gamma);
+/* count_inout: capacity in steps in, steps recorded out; out [count][num][2]. */
+int uvlm_host_sim_loads(uvlm_host_sim* s, double* out, size_t* count_inout);
+int uvlm_host_sim_pairs(uvlm_host_sim* s, double* evaluated, double* reference);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,381 @@
+"""ctypes binding of the FP64 CPU oracle (``oracle/uvlm_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY.  This is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+import it.  The product package ``uvlm_amd`` never imports, loads or calls
+anything under ``oracle/``.
+
+Every wrapped C function restates the reference (xeno14/UVLM) and cites the
+reference file:line it follows; see ``uvlm_oracle.h``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libuvlm_oracle.so")
+
+MAX_WINGS = 16
+GEN_RECT, GEN_NACA4, GEN_NACA00 = 0, 1, 2
+EULER, RK2, RK4, AB2, RK4_CLASSIC = 0, 1, 2, 3, 4
+LU_FULLPIV, LU_PARTIALPIV = 0, 1
+SCHEMES = {"euler": EULER, "RK2": RK2, "RK4": RK4, "AB2": AB2,
+           "RK4classic": RK4_CLASSIC}
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc -O2 -fopenmp)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+class WingSpec(C.Structure):
+    _fields_ = [
+        ("generator", C.c_int), ("digit", C.c_int),
+        ("chord", C.c_double), ("span", C.c_double),
+        ("origin", C.c_double * 3), ("alpha", C.c_double),
+        ("plug_amp", C.c_double), ("plug_omega", C.c_double),
+        ("plug_phase", C.c_double),
+        ("flap_amp", C.c_double), ("flap_omega", C.c_double),
+        ("flap_phase", C.c_double),
+        ("fourier_omega", C.c_double),
+        ("twist_amp", (C.c_double * 2) * 4), ("twist_phase", (C.c_double * 2) * 4),
+        ("bend_amp", (C.c_double * 2) * 4), ("bend_phase", (C.c_double * 2) * 4),
+    ]
+
+
+class SimConfig(C.Structure):
+    _fields_ = [
+        ("num_wings", C.c_int), ("wings", WingSpec * MAX_WINGS),
+        ("rows", C.c_size_t), ("cols", C.c_size_t),
+        ("ff", C.c_double * 3), ("dt", C.c_double),
+        ("scheme", C.c_int), ("ignore_steps", C.c_size_t),
+        ("erase_wake_after", C.c_size_t), ("loads", C.c_int), ("lu", C.c_int),
+    ]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        sz = C.c_size_t
+        L.orc_biot_savart.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_voring.argtypes = [_dp, _dp, _dp, sz, sz, sz, sz, sz, sz, _dp]
+        L.orc_induced_velocity.argtypes = [_dp, sz, _dp, _dp, sz, sz, sz, _dp]
+        L.orc_ring_induced_velocity.argtypes = [_dp, sz, _dp, _dp, sz, C.c_int, _dp]
+        for f in ("orc_ring_normal", "orc_ring_tangent", "orc_ring_centroid"):
+            getattr(L, f).argtypes = [_dp, _dp]
+        L.orc_ring_area.argtypes = [_dp]
+        L.orc_ring_area.restype = C.c_double
+        L.orc_ring_impulse.argtypes = [_dp, C.c_double, _dp]
+        for f in ("orc_collocation_points", "orc_normals", "orc_panel_area"):
+            getattr(L, f).argtypes = [_dp, sz, sz, sz, _dp]
+        L.orc_influence_matrix.argtypes = [_dp, sz, sz, sz, _dp, _dp, _dp]
+        L.orc_fullpiv_solve.argtypes = [sz, _dp, _dp, _dp]
+        L.orc_fullpiv_solve.restype = sz
+        L.orc_partialpiv_solve.argtypes = [sz, _dp, _dp, _dp]
+        L.orc_naca4digit.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_naca4digit.restype = C.c_double
+        L.orc_naca00xx.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_naca00xx.restype = C.c_double
+        L.orc_linspace.argtypes = [C.c_double, C.c_double, sz, _dp]
+        L.orc_build_wing.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                     sz, sz, _dp, _dp]
+        L.orc_morph_perform.argtypes = [C.POINTER(WingSpec), _dp, C.c_double, _dp]
+        L.orc_morph_velocity.argtypes = [C.POINTER(WingSpec), _dp, C.c_double, _dp]
+        L.orc_sim_create.argtypes = [C.POINTER(SimConfig)]
+        L.orc_sim_create.restype = C.c_void_p
+        L.orc_sim_destroy.argtypes = [C.c_void_p]
+        L.orc_sim_step.argtypes = [C.c_void_p, sz]
+        for f in ("orc_sim_num_bound", "orc_sim_wake_node_rows",
+                  "orc_sim_wake_gamma_rows"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = sz
+        L.orc_sim_get_wing_nodes.argtypes = [C.c_void_p, _dp]
+        L.orc_sim_get_wing_gamma.argtypes = [C.c_void_p, _dp]
+        L.orc_sim_get_wake.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_sim_set_state.argtypes = [C.c_void_p, sz, sz, _dp, _dp, _dp]
+        L.orc_sim_get_loads.argtypes = [C.c_void_p, _dp]
+        L.orc_sim_pairs.argtypes = [C.c_void_p]
+        L.orc_sim_pairs.restype = C.c_double
+        L.orc_sim_wake_velocity.argtypes = [C.c_void_p, _dp]
+        L.orc_loads.argtypes = [_dp, _dp, _dp, _dp, sz, sz, sz, _dp, _dp, sz,
+                                C.POINTER(WingSpec), _dp, C.c_double, C.c_double, _dp]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# --- kernels ---------------------------------------------------------------
+
+def biot_savart(start, end, pos) -> np.ndarray:
+    s, e, x = _f64(start), _f64(end), _f64(pos)
+    out = np.zeros(3)
+    lib().orc_biot_savart(_p(s), _p(e), _p(x), _p(out))
+    return out
+
+
+def induced_velocity(targets, nodes, gamma) -> np.ndarray:
+    """velocity.h:14-24 over every target; nodes [num,R+1,C+1,3], gamma [num,R,C]."""
+    t = _f64(targets).reshape(-1, 3)
+    nd, g = _f64(nodes), _f64(gamma)
+    num, R, Cc = g.shape
+    assert nd.shape == (num, R + 1, Cc + 1, 3)
+    out = np.zeros_like(t)
+    lib().orc_induced_velocity(_p(t), t.shape[0], _p(nd), _p(g), num, R, Cc, _p(out))
+    return out
+
+
+def ring_induced_velocity(targets, rings, gamma, chordwise=False) -> np.ndarray:
+    t = _f64(targets).reshape(-1, 3)
+    r, g = _f64(rings).reshape(-1, 4, 3), _f64(gamma).reshape(-1)
+    out = np.zeros_like(t)
+    lib().orc_ring_induced_velocity(_p(t), t.shape[0], _p(r), _p(g), r.shape[0],
+                                    int(bool(chordwise)), _p(out))
+    return out
+
+
+def ring_geometry(ring):
+    r = _f64(ring).reshape(4, 3)
+    n, tg, c = np.zeros(3), np.zeros(3), np.zeros(3)
+    L = lib()
+    L.orc_ring_normal(_p(r), _p(n))
+    L.orc_ring_tangent(_p(r), _p(tg))
+    L.orc_ring_centroid(_p(r), _p(c))
+    return {"normal": n, "tangent": tg, "centroid": c, "area": L.orc_ring_area(_p(r))}
+
+
+def ring_impulse(ring, gamma):
+    r = _f64(ring).reshape(4, 3)
+    out = np.zeros(3)
+    lib().orc_ring_impulse(_p(r), float(gamma), _p(out))
+    return out
+
+
+def lattice_geometry(nodes):
+    nd = _f64(nodes)
+    num, R1, C1, _ = nd.shape
+    N = num * (R1 - 1) * (C1 - 1)
+    cp, nrm, area = np.zeros((N, 3)), np.zeros((N, 3)), np.zeros(N)
+    L = lib()
+    L.orc_collocation_points(_p(nd), num, R1 - 1, C1 - 1, _p(cp))
+    L.orc_normals(_p(nd), num, R1 - 1, C1 - 1, _p(nrm))
+    L.orc_panel_area(_p(nd), num, R1 - 1, C1 - 1, _p(area))
+    return cp, nrm, area
+
+
+def influence_matrix(nodes, cpos=None, normals=None) -> np.ndarray:
+    nd = _f64(nodes)
+    num, R1, C1, _ = nd.shape
+    if cpos is None or normals is None:
+        cpos, normals, _ = lattice_geometry(nd)
+    cp, nr = _f64(cpos), _f64(normals)
+    N = num * (R1 - 1) * (C1 - 1)
+    A = np.zeros((N, N))
+    lib().orc_influence_matrix(_p(nd), num, R1 - 1, C1 - 1, _p(cp), _p(nr), _p(A))
+    return A
+
+
+def fullpiv_solve(A, b):
+    A, b = _f64(A), _f64(b)
+    x = np.zeros(A.shape[0])
+    rank = lib().orc_fullpiv_solve(A.shape[0], _p(A), _p(b), _p(x))
+    return x, rank
+
+
+def partialpiv_solve(A, b):
+    A, b = _f64(A), _f64(b)
+    x = np.zeros(A.shape[0])
+    rc = lib().orc_partialpiv_solve(A.shape[0], _p(A), _p(b), _p(x))
+    if rc != 0:
+        raise np.linalg.LinAlgError("singular matrix")
+    return x
+
+
+def naca4digit(x, c, digit):
+    return lib().orc_naca4digit(float(x), float(c), int(digit))
+
+
+def naca00xx(x, c, digit):
+    return lib().orc_naca00xx(float(x), float(c), int(digit))
+
+
+def linspace(a, b, n):
+    out = np.zeros(n)
+    lib().orc_linspace(float(a), float(b), n, _p(out))
+    return out
+
+
+def build_wing(generator, digit, chord, span, rows, cols, origin=(0, 0, 0)):
+    out = np.zeros((rows + 1, cols + 1, 3))
+    o = _f64(origin)
+    rc = lib().orc_build_wing(generator, digit, chord, span, rows, cols, _p(o), _p(out))
+    if rc != 0:
+        raise ValueError("columns must be even.")
+    return out
+
+
+# --- wing specs / configs ---------------------------------------------------
+
+def wing_spec(d: dict) -> WingSpec:
+    """Build a WingSpec from a workload wing dictionary (uvlm_amd.workloads)."""
+    w = WingSpec()
+    w.generator = {"rect": GEN_RECT, "naca4": GEN_NACA4, "naca00": GEN_NACA00}[
+        d.get("generator", "rect")]
+    w.digit = int(d.get("digit", 0))
+    w.chord, w.span = float(d["chord"]), float(d["span"])
+    for k in range(3):
+        w.origin[k] = float(d.get("origin", (0, 0, 0))[k])
+    w.alpha = float(d.get("alpha", 0.0))
+    for name in ("plug", "flap"):
+        amp, omega, phase = d.get(name, (0.0, 0.0, 0.0))
+        setattr(w, name + "_amp", float(amp))
+        setattr(w, name + "_omega", float(omega))
+        setattr(w, name + "_phase", float(phase))
+    w.fourier_omega = float(d.get("fourier_omega", 0.0))
+    for name in ("twist", "bend"):
+        amp = np.asarray(d.get(name + "_amp", np.zeros((4, 2))), dtype=np.float64)
+        ph = np.asarray(d.get(name + "_phase", np.zeros((4, 2))), dtype=np.float64)
+        A = getattr(w, name + "_amp")
+        P = getattr(w, name + "_phase")
+        for m in range(4):
+            for q in range(2):
+                A[m][q] = float(amp[m, q])
+                P[m][q] = float(ph[m, q])
+    return w
+
+
+def morph_perform(spec: dict, x0, t):
+    w = wing_spec(spec)
+    x = _f64(x0)
+    out = np.zeros(3)
+    lib().orc_morph_perform(C.byref(w), _p(x), float(t), _p(out))
+    return out
+
+
+def morph_velocity(spec: dict, x0, t):
+    w = wing_spec(spec)
+    x = _f64(x0)
+    out = np.zeros(3)
+    lib().orc_morph_velocity(C.byref(w), _p(x), float(t), _p(out))
+    return out
+
+
+def sim_config(case: dict, lu: int = LU_FULLPIV) -> SimConfig:
+    cfg = SimConfig()
+    wings = case["wings"]
+    cfg.num_wings = len(wings)
+    for n, d in enumerate(wings):
+        cfg.wings[n] = wing_spec(d)
+    cfg.rows, cfg.cols = int(case["rows"]), int(case["cols"])
+    for k in range(3):
+        cfg.ff[k] = float(case["forward_flight"][k])
+    cfg.dt = float(case["dt"])
+    cfg.scheme = SCHEMES[case.get("advect", "euler")]
+    cfg.ignore_steps = int(case.get("ignore_steps", 5))
+    cfg.erase_wake_after = int(case.get("erase_wake_after", 0))
+    cfg.loads = int(bool(case.get("loads", True)))
+    cfg.lu = int(lu)
+    return cfg
+
+
+class Sim:
+    """SimpleSimulator restated (simple_simulator.cpp:240-313)."""
+
+    def __init__(self, case: dict, lu: int = LU_FULLPIV):
+        self.case = case
+        self._cfg = sim_config(case, lu)
+        self._h = lib().orc_sim_create(C.byref(self._cfg))
+        if not self._h:
+            raise ValueError("invalid oracle configuration")
+        self.num = len(case["wings"])
+        self.R, self.C = int(case["rows"]), int(case["cols"])
+        self.step_index = 0
+
+    def close(self):
+        if self._h:
+            lib().orc_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, step=None):
+        s = self.step_index if step is None else step
+        lib().orc_sim_step(self._h, s)
+        self.step_index = s + 1
+
+    def run(self, steps):
+        """Steps 0..steps inclusive, as SimpleSimulator::Run (:304)."""
+        hist = []
+        for s in range(steps + 1):
+            self.step(s)
+            hist.append(self.loads())
+        return np.array(hist)
+
+    @property
+    def pairs(self):
+        return lib().orc_sim_pairs(self._h)
+
+    def wing_nodes(self):
+        out = np.zeros((self.num, self.R + 1, self.C + 1, 3))
+        lib().orc_sim_get_wing_nodes(self._h, _p(out))
+        return out
+
+    def gamma(self):
+        out = np.zeros((self.num, self.R, self.C))
+        lib().orc_sim_get_wing_gamma(self._h, _p(out))
+        return out
+
+    def wake(self):
+        L = lib()
+        nr = L.orc_sim_wake_node_rows(self._h)
+        gr = L.orc_sim_wake_gamma_rows(self._h)
+        nodes = np.zeros((self.num, nr, self.C + 1, 3))
+        gam = np.zeros((self.num, gr, self.C))
+        L.orc_sim_get_wake(self._h, _p(nodes), _p(gam))
+        return nodes, gam
+
+    def set_state(self, wake_nodes, wake_gamma, wing_gamma=None):
+        wn, wg = _f64(wake_nodes), _f64(wake_gamma)
+        g = None if wing_gamma is None else _f64(wing_gamma)
+        rc = lib().orc_sim_set_state(self._h, wn.shape[1], wg.shape[1], _p(wn), _p(wg),
+                                     _p(g) if g is not None else None)
+        if rc != 0:
+            raise ValueError("inconsistent wake shape")
+
+    def loads(self):
+        out = np.zeros((self.num, 2))
+        lib().orc_sim_get_loads(self._h, _p(out))
+        return out
+
+    def wake_velocity(self):
+        nr = lib().orc_sim_wake_node_rows(self._h)
+        out = np.zeros((self.num, nr, self.C + 1, 3))
+        lib().orc_sim_wake_velocity(self._h, _p(out))
+        return out

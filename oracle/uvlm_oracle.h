@@ -1,0 +1,192 @@
+/*
+ * uvlm_oracle.h -- FP64 CPU restatement of the UVLM per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path
+ * is compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  The product (uvlm_amd/) never links,
+ * loads or calls it.
+ *
+ * Every function restates the algorithm of the reference (xeno14/UVLM) and
+ * cites the file:line it follows.  Arithmetic conventions (fixed here, and
+ * reproduced by the GPU's F64_EXACT mode):
+ *   dot(a,b)    = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   sqnorm(a)   = (a.x*a.x + a.y*a.y) + a.z*a.z
+ *   norm(a)     = sqrt(sqnorm(a))
+ *   normalize   = divide each component by norm when sqnorm > 0
+ *   3-term sums of vectors are evaluated left to right.
+ * The library is compiled with -O2 -ffp-contract=off (no FMA contraction),
+ * matching the reference's plain -O2 x86-64 build.
+ */
+#ifndef UVLM_ORACLE_H
+#define UVLM_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_WINGS 16
+
+/* --- Segment / ring kernels ------------------------------------------------ */
+
+/* vortex.cpp:21-39  BiotSavartLaw(result, start, end, pos) */
+void orc_biot_savart(const double start[3], const double end[3],
+                     const double pos[3], double out[3]);
+
+/* vortex.cpp:145-160  VORING(x, pos, gamma, n, i, j): per-segment Gamma.
+ * nodes: [num][rows+1][cols+1][3], gamma: [num][rows][cols]. */
+void orc_voring(const double x[3], const double* nodes, const double* gamma,
+                size_t num, size_t rows, size_t cols, size_t n, size_t i,
+                size_t j, double out[3]);
+
+/* velocity.h:14-24  InducedVelocity(x, pos, gamma) for n_targets targets. */
+void orc_induced_velocity(const double* targets, size_t n_targets,
+                          const double* nodes, const double* gamma, size_t num,
+                          size_t rows, size_t cols, double* out);
+
+/* vortex.cpp:41-65 + shed.inl:22-33,66-77  legacy ring sums: each ring is
+ * 4 nodes, segments node[(k+1)%4] -> node[k], sum then scale by Gamma.
+ * rings: [n_rings][4][3]; chordwise != 0 keeps segments k in {0,2}. */
+void orc_ring_induced_velocity(const double* targets, size_t n_targets,
+                               const double* rings, const double* gamma,
+                               size_t n_rings, int chordwise, double* out);
+
+/* vortex.cpp:82-143  ring geometry of a 4-node ring [4][3]. */
+void orc_ring_normal(const double* ring, double out[3]);
+void orc_ring_tangent(const double* ring, double out[3]);
+void orc_ring_centroid(const double* ring, double out[3]);
+double orc_ring_area(const double* ring);
+void orc_ring_impulse(const double* ring, double gamma, double out[3]);
+
+/* --- Lattice geometry (simple_simulator.cpp:19-49, joukowski.cpp:55-67) --- */
+void orc_collocation_points(const double* nodes, size_t num, size_t rows,
+                            size_t cols, double* out /*[N][3]*/);
+void orc_normals(const double* nodes, size_t num, size_t rows, size_t cols,
+                 double* out /*[N][3]*/);
+void orc_panel_area(const double* nodes, size_t num, size_t rows, size_t cols,
+                    double* out /*[N]*/);
+
+/* --- Linear problem ------------------------------------------------------- */
+
+/* simple_simulator.cpp:76-101  A(K,L) = VORING_L(cp_K; Gamma=1) . n_K,
+ * written row-major [N][N]. */
+void orc_influence_matrix(const double* nodes, size_t num, size_t rows,
+                          size_t cols, const double* cpos, const double* normals,
+                          double* A);
+
+/* Eigen::FullPivLU<MatrixXd>(A).solve(b) (simple_simulator.cpp:265-268).
+ * A row-major [n][n].  Returns the rank found (n when invertible). */
+size_t orc_fullpiv_solve(size_t n, const double* A, const double* b, double* x);
+
+/* Partial-pivoting LU solve (row max |a_ik|), same conventions. */
+int orc_partialpiv_solve(size_t n, const double* A, const double* b, double* x);
+
+/* --- Wing generation (wing/rect.cpp:18-44, naca4digit.cpp:12-37,
+ *     naca00XX.cpp:15-32, wing/wing.cpp:12-50, simple_simulator.cpp:142-169) */
+#define ORC_GEN_RECT 0
+#define ORC_GEN_NACA4 1
+#define ORC_GEN_NACA00 2
+
+double orc_naca4digit(double x, double c, int digit);
+double orc_naca00xx(double x, double c, int digit);
+void orc_linspace(double a, double b, size_t n, double* out);
+/* whole-wing node lattice [rows+1][cols+1][3] (cols even), mirrored from the
+ * half wing generated with (chord, span/2, rows, cols/2), plus origin. */
+int orc_build_wing(int generator, int digit, double chord, double span,
+                   size_t rows, size_t cols, const double origin[3],
+                   double* nodes);
+
+/* --- Kinematics (morphing.cpp:23-84) --------------------------------------
+ * The reference takes std::function motion laws; the oracle fixes the forms
+ * used by the measured workloads:
+ *   plug(t)  = plug_amp * cos(plug_omega * t + plug_phase)
+ *   flap(t)  = flap_amp * cos(flap_omega * t + flap_phase)
+ *   twist(x,t) = sum_{m<4,q<2} TA[m][q] * sin((m+1)*pi/2*eta)
+ *                                 * cos((q+1)*fourier_omega*t + TP[m][q])
+ *   bend(x,t)  = sum_{m<4,q<2} BA[m][q] * sin(pi*xi) * sin((m+1)*pi/2*eta)
+ *                                 * cos((q+1)*fourier_omega*t + BP[m][q])
+ * with eta = x_ref.y / (span/2), xi = x_ref.x / chord. */
+typedef struct {
+  int generator;
+  int digit;
+  double chord, span;
+  double origin[3];
+  double alpha;
+  double plug_amp, plug_omega, plug_phase;
+  double flap_amp, flap_omega, flap_phase;
+  double fourier_omega;
+  double twist_amp[4][2], twist_phase[4][2];
+  double bend_amp[4][2], bend_phase[4][2];
+} orc_wing_spec;
+
+void orc_morph_perform(const orc_wing_spec* w, const double x0[3], double t,
+                       double out[3]);
+void orc_morph_velocity(const orc_wing_spec* w, const double x0[3], double t,
+                        double out[3]);
+
+/* --- SimpleSimulator time loop (simple_simulator.cpp:171-313, advect.h,
+ *     advect.cpp, joukowski.cpp) ------------------------------------------ */
+#define ORC_EULER 0
+#define ORC_RK2 1
+#define ORC_RK4 2 /* reference formula: k1 + 2k2 + 2k3 + k3 (advect.cpp:172-173) */
+#define ORC_AB2 3
+#define ORC_RK4_CLASSIC 4
+
+#define ORC_LU_FULLPIV 0
+#define ORC_LU_PARTIALPIV 1
+
+typedef struct {
+  int num_wings;
+  orc_wing_spec wings[ORC_MAX_WINGS];
+  size_t rows, cols;
+  double ff[3];
+  double dt;
+  int scheme;
+  size_t ignore_steps;
+  size_t erase_wake_after;
+  int loads;
+  int lu;
+} orc_sim_config;
+
+typedef struct orc_sim orc_sim;
+
+orc_sim* orc_sim_create(const orc_sim_config* cfg);
+void orc_sim_destroy(orc_sim* s);
+/* One MainLoop(step) (simple_simulator.cpp:240-286) followed by the optional
+ * EraseOldestWake (:307-311). */
+int orc_sim_step(orc_sim* s, size_t step);
+size_t orc_sim_num_bound(const orc_sim* s);
+size_t orc_sim_wake_node_rows(const orc_sim* s);
+size_t orc_sim_wake_gamma_rows(const orc_sim* s);
+void orc_sim_get_wing_nodes(const orc_sim* s, double* out);
+void orc_sim_get_wing_gamma(const orc_sim* s, double* out);
+void orc_sim_get_wake(const orc_sim* s, double* nodes, double* gamma);
+/* Overwrite wake + previous-step circulations (for restoring a state). */
+int orc_sim_set_state(orc_sim* s, size_t node_rows, size_t gamma_rows,
+                      const double* wake_nodes, const double* wake_gamma,
+                      const double* wing_gamma);
+/* loads of the last step: [num][2] = (CD, CL) (simple_simulator.cpp:229-230) */
+void orc_sim_get_loads(const orc_sim* s, double* out);
+/* Biot-Savart pairs evaluated so far (one pair = one BiotSavartLaw call). */
+double orc_sim_pairs(const orc_sim* s);
+
+/* advect.h:16-46  wake-node velocities for the current state, TE rows -ff. */
+void orc_sim_wake_velocity(const orc_sim* s, double* out);
+
+/* Loads from an explicit state (joukowski.cpp:12-92, simple_simulator.cpp
+ * :195-233): per wing (CD, CL). */
+void orc_loads(const double* wing_nodes, const double* wing_nodes_init,
+               const double* wing_gamma, const double* wing_gamma_prev,
+               size_t num, size_t rows, size_t cols, const double* wake_nodes,
+               const double* wake_gamma, size_t wake_rows,
+               const orc_wing_spec* wings, const double ff[3], double t,
+               double dt, double* out);
+
+/* Number of OpenMP threads used by the oracle (1 when built without it). */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

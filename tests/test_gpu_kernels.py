@@ -1,0 +1,160 @@
+"""GPU kernels vs the oracle, through the C-ABI (include/uvlm_gpu.h).
+
+F64_EXACT with UVLM_SEQUENTIAL reproduces the reference's operation order
+and must agree bit for bit; source-split sums and the FMA/rsqrt mode agree
+to the stated relative tolerances.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from uvlm_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+EXACT_SEQ = N.PREC_F64_EXACT | N.SEQUENTIAL
+
+
+def rand_lattice(rng, num=2, R=3, C=4, scale=1.0):
+    base = np.zeros((num, R + 1, C + 1, 3))
+    for n in range(num):
+        for i in range(R + 1):
+            for j in range(C + 1):
+                base[n, i, j] = [0.3 * i + 2.0 * n, 0.25 * j - 0.5, 0.0]
+    nodes = base + rng.normal(scale=0.05 * scale, size=base.shape)
+    gamma = rng.normal(size=(num, R, C))
+    return nodes, gamma
+
+
+def test_segment_bitwise(ctx, orc):
+    rng = np.random.default_rng(10)
+    n = 3000
+    a = rng.normal(size=(n, 3))
+    b = rng.normal(size=(n, 3))
+    x = rng.normal(size=(n, 3)) * 2
+    # special cases: endpoints, midpoints, collinear, near the cut-off
+    x[0] = a[0]
+    x[1] = b[1]
+    x[2] = (a[2] + b[2]) / 2
+    x[3] = a[3] + 3.0 * (b[3] - a[3])
+    d = b[4] - a[4]
+    perp = np.cross(d, [0.3, 0.1, 0.9])
+    perp /= np.linalg.norm(perp)
+    for k, h in enumerate(np.geomspace(1e-8, 1e-3, 200)):
+        x[5 + k] = a[5 + k] * 0 + a[4] + 0.5 * d + h * perp
+        a[5 + k], b[5 + k] = a[4], b[4]
+    got = ctx.biot_savart(a, b, x, N.PREC_F64_EXACT)
+    ref = np.array([orc.biot_savart(a[k], b[k], x[k]) for k in range(n)])
+    np.testing.assert_array_equal(got, ref)
+    fast = ctx.biot_savart(a, b, x, N.PREC_F64)
+    scale = np.maximum(np.linalg.norm(ref, axis=1, keepdims=True), 1e-300)
+    ok = np.linalg.norm(ref, axis=1) > 0
+    np.testing.assert_allclose((fast - ref)[ok] / scale[ok], 0, atol=1e-12)
+
+
+def test_reference_known_answers_on_gpu(ctx):
+    """vortex_test.cpp:34-79 and :104-130 through the GPU."""
+    u = ctx.biot_savart([[0, 0, -1e10]] * 3, [[0, 0, 1e10]] * 3, [[1, 0, 0], [2, 0, 0], [8, 0, 0]])
+    np.testing.assert_allclose(u[:, 1], [1 / 2 / math.pi, 1 / 4 / math.pi, 1 / 16 / math.pi],
+                               atol=1e-10)
+    u = ctx.biot_savart([0, 0, -1], [0, 0, 1], [1, 0, 0])[0]
+    assert abs(u[1] - math.sqrt(2) / 4 / math.pi) < 1e-10
+    ring = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], float)
+    u = ctx.ring_induced_velocity([[0, 0, 0]], ring[None], [4 * math.pi],
+                                  N.SCALE_PER_RING | N.SEQUENTIAL)[0]
+    assert u[0] == 0 and u[1] == 0 and abs(u[2] + 4 * math.sqrt(2)) < 4e-15
+    u = ctx.ring_induced_velocity([[0, 0, 0]], np.repeat(ring[None], 20, 0),
+                                  np.full(20, 4 * math.pi),
+                                  N.SCALE_PER_RING | N.CHORDWISE_ONLY | N.SEQUENTIAL)[0]
+    assert abs(u[2] + 2 * math.sqrt(2) * 20) < 1e-13
+
+
+@pytest.mark.parametrize("num,R,C,nt", [(1, 1, 1, 5), (1, 3, 4, 300), (2, 5, 8, 1000),
+                                        (3, 7, 10, 257)])
+def test_induced_velocity_bitwise(ctx, orc, num, R, C, nt):
+    rng = np.random.default_rng(num * 100 + R)
+    nodes, gamma = rand_lattice(rng, num, R, C)
+    x = rng.normal(size=(nt, 3)) * 1.5
+    x[: min(nt, (R + 1) * (C + 1))] = nodes[0].reshape(-1, 3)[: min(nt, (R + 1) * (C + 1))]
+    ref = orc.induced_velocity(x, nodes, gamma)
+    got = ctx.induced_velocity(x, nodes, gamma, EXACT_SEQ)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_induced_velocity_split_and_fast(ctx, orc):
+    rng = np.random.default_rng(7)
+    nodes, gamma = rand_lattice(rng, 2, 30, 40)
+    x = rng.normal(size=(50, 3)) * 3  # few targets -> many source chunks
+    ref = orc.induced_velocity(x, nodes, gamma)
+    scale = np.abs(ref).max()
+    got = ctx.induced_velocity(x, nodes, gamma, N.PREC_F64_EXACT)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * scale)
+    fast = ctx.induced_velocity(x, nodes, gamma, N.PREC_F64)
+    np.testing.assert_allclose(fast, ref, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("flags", [N.SCALE_PER_RING, N.SCALE_PER_RING | N.CHORDWISE_ONLY, 0])
+def test_legacy_ring_sums_bitwise(ctx, orc, flags):
+    rng = np.random.default_rng(11)
+    rings = rng.normal(size=(37, 4, 3))
+    g = rng.normal(size=37)
+    x = rng.normal(size=(90, 3))
+    got = ctx.ring_induced_velocity(x, rings, g, flags | N.SEQUENTIAL)
+    if flags & N.SCALE_PER_RING:
+        ref = orc.ring_induced_velocity(x, rings, g, chordwise=bool(flags & N.CHORDWISE_ONLY))
+        np.testing.assert_array_equal(got, ref)
+    else:  # Gamma per segment: same sum up to rounding
+        ref = orc.ring_induced_velocity(x, rings, g)
+        np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
+
+
+def test_influence_matrix_bitwise(ctx, orc):
+    rng = np.random.default_rng(5)
+    nodes, _ = rand_lattice(rng, 2, 6, 10)
+    cp, nrm, _ = orc.lattice_geometry(nodes)
+    ref = orc.influence_matrix(nodes, cp, nrm)
+    got = ctx.influence_matrix(nodes, cp, nrm)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 257, 600])
+def test_solve_f64(ctx, orc, n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) + np.eye(n) * n ** 0.5 * 2
+    b = rng.normal(size=n)
+    x, _ = ctx.solve(A, b)
+    xr, rank = orc.fullpiv_solve(A, b)
+    assert rank == n
+    np.testing.assert_allclose(x, xr, rtol=1e-10, atol=1e-12)
+    assert np.linalg.norm(A @ x - b) <= 1e-12 * (np.linalg.norm(A) * np.linalg.norm(x) + 1)
+
+
+def test_solve_influence_matrix(ctx, orc):
+    """The real operator: a plate's influence matrix (diagonally dominant)."""
+    nodes = orc.build_wing(orc.GEN_NACA4, 44, 1.0, 6.0, 10, 40)[None]
+    cp, nrm, _ = orc.lattice_geometry(nodes)
+    A = orc.influence_matrix(nodes, cp, nrm)
+    b = -nrm @ np.array([1.0, 0.0, 0.0]) + 0.1
+    x, _ = ctx.solve(A, b)
+    xr, _ = orc.fullpiv_solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=1e-11, atol=1e-13 * np.abs(xr).max())
+
+
+@pytest.mark.parametrize("mode", [N.SOLVE_F32_IR, N.SOLVE_BF16_IR])
+def test_solve_mixed_precision(ctx, orc, mode):
+    nodes = orc.build_wing(orc.GEN_RECT, 0, 1.0, 4.0, 12, 32)[None]
+    cp, nrm, _ = orc.lattice_geometry(nodes)
+    A = orc.influence_matrix(nodes, cp, nrm)
+    b = -nrm @ np.array([-1.0, 0.0, 0.1])
+    x, iters = ctx.solve(A, b, mode=mode, tol=1e-14, max_iter=60)
+    xr, _ = orc.fullpiv_solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=1e-10, atol=1e-12 * np.abs(xr).max())
+    assert 1 <= iters <= 60
+
+
+def test_singular_matrix_reported(ctx):
+    from uvlm_amd import UVLMError
+    with pytest.raises(UVLMError) as e:
+        ctx.solve(np.zeros((4, 4)), np.ones(4))
+    assert e.value.code == N.ERR_SINGULAR

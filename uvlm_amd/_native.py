@@ -1,0 +1,141 @@
+"""ctypes bindings of ``include/uvlm_gpu.h`` and ``include/uvlm_host.h``.
+
+The libraries are built in-tree (``uvlm_amd/lib``) by ``uvlm_amd.build``.
+Loading fails loudly when they are missing: there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+GPU_LIB = os.path.join(LIBDIR, "libuvlm_gpu.so")
+HOST_LIB = os.path.join(LIBDIR, "libuvlm.so")
+
+# include/uvlm_gpu.h
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_SINGULAR, ERR_STATE, ERR_CAPACITY, ERR_NOCONV = (
+    0, -1, -2, -3, -4, -5, -6, -7)
+PREC_F64_EXACT, PREC_F64, PREC_F32C = 0x0, 0x1, 0x2
+SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
+SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR = 0, 1, 2
+ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
+SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
+           "AB2": ADVECT_AB2, "RK4classic": ADVECT_RK4_CLASSIC}
+GEN = {"rect": 0, "naca4": 1, "naca00": 2}
+MAX_WINGS = 16
+
+# Every exported symbol of the two headers (checked by the CPU test suite).
+GPU_SYMBOLS = [
+    "uvlm_create", "uvlm_destroy", "uvlm_last_error", "uvlm_sync",
+    "uvlm_biot_savart", "uvlm_induced_velocity", "uvlm_ring_induced_velocity",
+    "uvlm_influence_matrix", "uvlm_solve", "uvlm_sim_init", "uvlm_sim_set_wing",
+    "uvlm_sim_shed", "uvlm_sim_solve", "uvlm_sim_loads", "uvlm_sim_advect",
+    "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
+    "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
+    "uvlm_sim_pair_counts",
+]
+HOST_SYMBOLS = [
+    "uvlm_host_sim_create", "uvlm_host_sim_destroy", "uvlm_host_last_error",
+    "uvlm_host_sim_step", "uvlm_host_sim_sync", "uvlm_host_sim_wing_nodes",
+    "uvlm_host_sim_gamma", "uvlm_host_sim_wake_shape", "uvlm_host_sim_wake",
+    "uvlm_host_sim_set_wake", "uvlm_host_sim_loads", "uvlm_host_sim_pairs",
+]
+
+
+class WingSpec(C.Structure):
+    _fields_ = [
+        ("generator", C.c_int), ("digit", C.c_int),
+        ("chord", C.c_double), ("span", C.c_double),
+        ("origin", C.c_double * 3), ("alpha", C.c_double),
+        ("plug_amp", C.c_double), ("plug_omega", C.c_double), ("plug_phase", C.c_double),
+        ("flap_amp", C.c_double), ("flap_omega", C.c_double), ("flap_phase", C.c_double),
+        ("fourier_omega", C.c_double),
+        ("twist_amp", (C.c_double * 2) * 4), ("twist_phase", (C.c_double * 2) * 4),
+        ("bend_amp", (C.c_double * 2) * 4), ("bend_phase", (C.c_double * 2) * 4),
+    ]
+
+
+class CaseSpec(C.Structure):
+    _fields_ = [
+        ("num_wings", C.c_int), ("wings", WingSpec * MAX_WINGS),
+        ("rows", C.c_size_t), ("cols", C.c_size_t),
+        ("ff", C.c_double * 3), ("dt", C.c_double), ("scheme", C.c_int),
+        ("ignore_steps", C.c_size_t), ("erase_wake_after", C.c_size_t),
+        ("loads", C.c_int), ("precision", C.c_uint), ("solve_mode", C.c_int),
+    ]
+
+
+_gpu = None
+_host = None
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+_sz = C.c_size_t
+
+
+class MissingLibrary(RuntimeError):
+    pass
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise MissingLibrary(
+            f"{path} is not built; run `python -m uvlm_amd.build` (or "
+            "__graft_entry__.build()) first -- the UVLM hot path has no CPU fallback")
+    return C.CDLL(path)
+
+
+def gpu_lib():
+    """The gfx950 library (include/uvlm_gpu.h)."""
+    global _gpu
+    if _gpu is None:
+        L = _load(GPU_LIB)
+        L.uvlm_create.argtypes = [C.c_int, C.POINTER(_vp)]
+        L.uvlm_destroy.argtypes = [_vp]
+        L.uvlm_last_error.argtypes = [_vp]
+        L.uvlm_last_error.restype = C.c_char_p
+        L.uvlm_sync.argtypes = [_vp]
+        L.uvlm_biot_savart.argtypes = [_vp, _dp, _dp, _dp, _sz, C.c_uint, _dp]
+        L.uvlm_induced_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz,
+                                            C.c_uint, _dp]
+        L.uvlm_ring_induced_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, C.c_uint, _dp]
+        L.uvlm_influence_matrix.argtypes = [_vp, _dp, _sz, _sz, _sz, _dp, _dp, C.c_uint, _dp]
+        L.uvlm_solve.argtypes = [_vp, _sz, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                 C.POINTER(C.c_int)]
+        L.uvlm_sim_init.argtypes = [_vp, _sz, _sz, _sz, _sz, C.c_uint]
+        L.uvlm_sim_set_wing.argtypes = [_vp, _dp]
+        L.uvlm_sim_shed.argtypes = [_vp, _sz]
+        L.uvlm_sim_solve.argtypes = [_vp, _dp, _dp, C.c_int, _dp]
+        L.uvlm_sim_loads.argtypes = [_vp, _dp, _dp, C.c_double, _dp]
+        L.uvlm_sim_advect.argtypes = [_vp, C.c_int, _sz, _dp, C.c_double]
+        L.uvlm_sim_erase_oldest_wake.argtypes = [_vp]
+        L.uvlm_sim_wake_shape.argtypes = [_vp, C.POINTER(_sz), C.POINTER(_sz)]
+        L.uvlm_sim_get_wake.argtypes = [_vp, _dp, _dp]
+        L.uvlm_sim_set_wake.argtypes = [_vp, _sz, _sz, _dp, _dp, _dp]
+        L.uvlm_sim_get_gamma.argtypes = [_vp, _dp]
+        L.uvlm_sim_get_loads.argtypes = [_vp, _dp, C.POINTER(_sz)]
+        L.uvlm_sim_pair_counts.argtypes = [_vp, _dp, _dp]
+        _gpu = L
+    return _gpu
+
+
+def host_lib():
+    """The host library (include/uvlm_host.h), which links the GPU library."""
+    global _host
+    if _host is None:
+        gpu_lib()
+        L = _load(HOST_LIB)
+        L.uvlm_host_sim_create.argtypes = [C.POINTER(CaseSpec), _sz, C.POINTER(_vp)]
+        L.uvlm_host_sim_destroy.argtypes = [_vp]
+        L.uvlm_host_last_error.restype = C.c_char_p
+        L.uvlm_host_sim_step.argtypes = [_vp, _sz]
+        L.uvlm_host_sim_sync.argtypes = [_vp]
+        L.uvlm_host_sim_wing_nodes.argtypes = [_vp, _dp]
+        L.uvlm_host_sim_gamma.argtypes = [_vp, _dp]
+        L.uvlm_host_sim_wake_shape.argtypes = [_vp, C.POINTER(_sz), C.POINTER(_sz)]
+        L.uvlm_host_sim_wake.argtypes = [_vp, _dp, _dp]
+        L.uvlm_host_sim_set_wake.argtypes = [_vp, _sz, _sz, _dp, _dp, _dp]
+        L.uvlm_host_sim_loads.argtypes = [_vp, _dp, C.POINTER(_sz)]
+        L.uvlm_host_sim_pairs.argtypes = [_vp, _dp, _dp]
+        _host = L
+    return _host

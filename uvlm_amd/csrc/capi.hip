@@ -1,0 +1,846 @@
+// capi.hip -- C-ABI of the gfx950 UVLM hot path (include/uvlm_gpu.h).
+//
+// A context owns one HIP stream (default priority), the device buffers of the
+// stateless calls and the device-resident SimpleSimulator state.  Buffers
+// grow on demand and are reused; host inputs of the simulation calls go
+// through pinned staging buffers so the step stays asynchronous.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/uvlm_gpu.h"
+#include "internal.h"
+#include "step.h"
+
+using namespace uvlm;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct Staging {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+
+}  // namespace
+
+struct uvlm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  double td = 0.0;
+  // stateless scratch
+  Buf t_a, t_b, t_c, t_out, t_recs, t_partial, t_A, t_x, t_ipiv, t_info, t_work;
+  // simulation
+  bool sim = false;
+  int num = 0, R = 0, C = 0, N = 0, cap = 0, capg = 0;
+  unsigned flags = 0;
+  int wstart = 0, wrows = 0, gstart = 0, wgrows = 0;
+  int max_steps = 0;
+  Buf wing, gamma, gamma_prev, wake, wgam, cp, nrm, area, recs, partial, A,
+      rhs, ipiv, info, uw, vm, mid, uflow, vedge, pos, pst, k1, k2, k3, k4,
+      vprev, loads, ref_area, work, tmp;
+  bool has_vprev = false;
+  int vprev_rows = 0;
+  int loads_count = 0;
+  double pairs_eval = 0.0, pairs_ref = 0.0;
+  Staging st_wing, st_vm, st_ve, st_area;
+};
+
+namespace {
+
+int fail(uvlm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                    \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(ctx, UVLM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(uvlm_ctx* c, Buf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (bytes <= b.cap) return UVLM_OK;
+  if (b.p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  const size_t want = bytes + bytes / 4;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, UVLM_ERR_OOM, "device allocation of %zu bytes failed", want);
+  }
+  b.cap = want;
+  return UVLM_OK;
+}
+
+void release(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+template <typename T>
+T* P(Buf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+// host -> device through a pinned staging buffer, stream-ordered
+int stage_upload(uvlm_ctx* c, Staging& st, void* dst, const void* src,
+                 size_t bytes) {
+  if (bytes == 0) return UVLM_OK;
+  if (st.pending) {
+    HIPCHK(c, hipEventSynchronize(st.ev));
+    st.pending = false;
+  }
+  if (bytes > st.cap) {
+    if (st.host) HIPCHK(c, hipHostFree(st.host));
+    st.host = nullptr;
+    st.cap = 0;
+    HIPCHK(c, hipHostMalloc(&st.host, bytes, hipHostMallocDefault));
+    st.cap = bytes;
+  }
+  if (!st.ev) HIPCHK(c, hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+  std::memcpy(st.host, src, bytes);
+  HIPCHK(c, hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(st.ev, c->stream));
+  st.pending = true;
+  return UVLM_OK;
+}
+
+void free_staging(Staging& st) {
+  if (st.pending && st.ev) (void)hipEventSynchronize(st.ev);
+  if (st.host) (void)hipHostFree(st.host);
+  if (st.ev) (void)hipEventDestroy(st.ev);
+  st = Staging{};
+}
+
+int check_launch(uvlm_ctx* c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(c, UVLM_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
+  return UVLM_OK;
+}
+
+int prec_of(unsigned flags) { return (int)(flags & 0x0f); }
+
+#define TRY(expr)              \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != UVLM_OK) return rc_; \
+  } while (0)
+
+int set_dev(uvlm_ctx* c) {
+  HIPCHK(c, hipSetDevice(c->device));
+  return UVLM_OK;
+}
+
+int check_prec(uvlm_ctx* c, unsigned flags) {
+  const int p = prec_of(flags);
+  if (p != kF64Exact && p != kF64Fast)
+    return fail(c, UVLM_ERR_ARG, "unsupported precision flag %u", flags & 0x0fu);
+  return UVLM_OK;
+}
+
+// Launch the N-body sum of targets against recs [0, nb+nw) and return the
+// plan (partial sums in c->partial or b).
+int nbody(uvlm_ctx* c, Buf& partial, const double* targets, int nt,
+          const RingRec* recs, int nb, int nw, unsigned flags, int seg,
+          ChunkPlan* plan_out) {
+  ChunkPlan plan = plan_chunks(nt, nb, nw, (flags & UVLM_SEQUENTIAL) != 0);
+  TRY(ensure(c, partial, sizeof(double) * 3 * (size_t)std::max(1, plan.total()) *
+                             (size_t)std::max(1, nt)));
+  launch_nbody(c->stream, prec_of(flags), seg, targets, nt, recs, plan,
+               P<double>(partial), nt, c->td);
+  *plan_out = plan;
+  return check_launch(c);
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int uvlm_create(int device, uvlm_ctx** out) {
+  if (!out) return UVLM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return UVLM_ERR_HIP;
+  }
+  if (device < 0 || device >= n) return UVLM_ERR_ARG;
+  uvlm_ctx* c = new (std::nothrow) uvlm_ctx();
+  if (!c) return UVLM_ERR_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return UVLM_ERR_HIP;
+  }
+  c->td = cutoff_dsq_threshold();
+  *out = c;
+  return UVLM_OK;
+}
+
+int uvlm_destroy(uvlm_ctx* c) {
+  if (!c) return UVLM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
+                 &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
+                 &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
+                 &c->area, &c->recs, &c->partial, &c->A, &c->rhs, &c->ipiv,
+                 &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
+                 &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
+                 &c->ref_area, &c->work, &c->tmp};
+  for (Buf* b : bufs) release(*b);
+  free_staging(c->st_wing);
+  free_staging(c->st_vm);
+  free_staging(c->st_ve);
+  free_staging(c->st_area);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return UVLM_OK;
+}
+
+const char* uvlm_last_error(const uvlm_ctx* c) {
+  return c ? c->err.c_str() : "no context";
+}
+
+int uvlm_sync(uvlm_ctx* c) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int uvlm_biot_savart(uvlm_ctx* c, const double* start, const double* end,
+                     const double* pos, size_t n, unsigned flags, double* out) {
+  if (!c || (n && (!start || !end || !pos || !out))) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  TRY(check_prec(c, flags));
+  if (n == 0) return UVLM_OK;
+  const size_t bytes = sizeof(double) * 3 * n;
+  TRY(ensure(c, c->t_a, bytes));
+  TRY(ensure(c, c->t_b, bytes));
+  TRY(ensure(c, c->t_c, bytes));
+  TRY(ensure(c, c->t_out, bytes));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, start, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, end, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_c.p, pos, bytes, hipMemcpyHostToDevice, c->stream));
+  launch_segments(c->stream, prec_of(flags) == kF64Exact ? kF64Exact : kF64Fast,
+                  P<double>(c->t_a), P<double>(c->t_b), P<double>(c->t_c), (int)n,
+                  c->td, P<double>(c->t_out));
+  TRY(check_launch(c));
+  HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
+                          const double* nodes, const double* gamma, size_t num,
+                          size_t rows, size_t cols, unsigned flags, double* out) {
+  if (!c) return UVLM_ERR_ARG;
+  if (nt && (!targets || !out)) return fail(c, UVLM_ERR_ARG, "null target/out");
+  TRY(set_dev(c));
+  TRY(check_prec(c, flags));
+  if (nt > (size_t)INT32_MAX / 4) return fail(c, UVLM_ERR_ARG, "too many targets");
+  const size_t nr = num * rows * cols;
+  if (nt == 0) return UVLM_OK;
+  if (nr == 0) {
+    std::memset(out, 0, sizeof(double) * 3 * nt);
+    return UVLM_OK;
+  }
+  if (!nodes || !gamma) return fail(c, UVLM_ERR_ARG, "null nodes/gamma");
+  const size_t nn = num * (rows + 1) * (cols + 1);
+  TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
+  TRY(ensure(c, c->t_b, sizeof(double) * 3 * nn));
+  TRY(ensure(c, c->t_c, sizeof(double) * nr));
+  TRY(ensure(c, c->t_recs, sizeof(RingRec) * nr));
+  TRY(ensure(c, c->t_out, sizeof(double) * 3 * nt));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, nodes, sizeof(double) * 3 * nn,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * nr,
+                           hipMemcpyHostToDevice, c->stream));
+  launch_pack_lattice(c->stream, P<double>(c->t_b), (int)rows + 1, 0,
+                      P<double>(c->t_c), (int)rows, 0, (int)num, (int)rows,
+                      (int)cols, P<RingRec>(c->t_recs));
+  ChunkPlan plan;
+  TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
+            (int)nr, flags, kSegVoring, &plan));
+  launch_sum_chunks(c->stream, P<double>(c->t_partial), (int)nt, (int)nt,
+                    plan.total(), P<double>(c->t_out));
+  TRY(check_launch(c));
+  HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 3 * nt,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_ring_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
+                               const double* rings, const double* gamma,
+                               size_t n_rings, unsigned flags, double* out) {
+  if (!c) return UVLM_ERR_ARG;
+  if (nt && (!targets || !out)) return fail(c, UVLM_ERR_ARG, "null target/out");
+  TRY(set_dev(c));
+  TRY(check_prec(c, flags));
+  if (nt == 0) return UVLM_OK;
+  if (n_rings == 0) {
+    std::memset(out, 0, sizeof(double) * 3 * nt);
+    return UVLM_OK;
+  }
+  if (!rings || !gamma) return fail(c, UVLM_ERR_ARG, "null rings/gamma");
+  int seg = kSegRingPerSeg;
+  if (flags & UVLM_CHORDWISE_ONLY)
+    seg = kSegRingChord;
+  else if (flags & UVLM_SCALE_PER_RING)
+    seg = kSegRingPerRing;
+  TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
+  TRY(ensure(c, c->t_b, sizeof(double) * 12 * n_rings));
+  TRY(ensure(c, c->t_c, sizeof(double) * n_rings));
+  TRY(ensure(c, c->t_recs, sizeof(RingRec) * n_rings));
+  TRY(ensure(c, c->t_out, sizeof(double) * 3 * nt));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, rings, sizeof(double) * 12 * n_rings,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * n_rings,
+                           hipMemcpyHostToDevice, c->stream));
+  launch_pack_rings(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)n_rings,
+                    P<RingRec>(c->t_recs));
+  ChunkPlan plan;
+  TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
+            (int)n_rings, flags, seg, &plan));
+  launch_sum_chunks(c->stream, P<double>(c->t_partial), (int)nt, (int)nt,
+                    plan.total(), P<double>(c->t_out));
+  TRY(check_launch(c));
+  HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 3 * nt,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_influence_matrix(uvlm_ctx* c, const double* nodes, size_t num,
+                          size_t rows, size_t cols, const double* cpos,
+                          const double* normals, unsigned flags, double* A) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  (void)flags;
+  const size_t N = num * rows * cols;
+  if (N == 0) return UVLM_OK;
+  if (!nodes || !cpos || !normals || !A) return fail(c, UVLM_ERR_ARG, "null argument");
+  const size_t nn = num * (rows + 1) * (cols + 1);
+  TRY(ensure(c, c->t_a, sizeof(double) * 3 * nn));
+  TRY(ensure(c, c->t_b, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->t_c, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->t_recs, sizeof(RingRec) * N));
+  TRY(ensure(c, c->t_A, sizeof(double) * N * N));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, nodes, sizeof(double) * 3 * nn,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, cpos, sizeof(double) * 3 * N,
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_c.p, normals, sizeof(double) * 3 * N,
+                           hipMemcpyHostToDevice, c->stream));
+  launch_pack_lattice(c->stream, P<double>(c->t_a), (int)rows + 1, 0, nullptr, 0,
+                      0, (int)num, (int)rows, (int)cols, P<RingRec>(c->t_recs));
+  launch_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b),
+                P<double>(c->t_c), (int)N, c->td, P<double>(c->t_A));
+  TRY(check_launch(c));
+  // device matrix is column-major: A^T row-major == A column-major
+  double* tmp = new (std::nothrow) double[N * N];
+  if (!tmp) return fail(c, UVLM_ERR_OOM, "host buffer");
+  hipError_t e = hipMemcpyAsync(tmp, c->t_A.p, sizeof(double) * N * N,
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    delete[] tmp;
+    return fail(c, UVLM_ERR_HIP, "copy back: %s", hipGetErrorString(e));
+  }
+  for (size_t L = 0; L < N; ++L)
+    for (size_t K = 0; K < N; ++K) A[K * N + L] = tmp[L * N + K];
+  delete[] tmp;
+  return UVLM_OK;
+}
+
+int uvlm_solve(uvlm_ctx* c, size_t n, const double* A, const double* rhs,
+               double* x, int mode, double tol, int max_iter, int* iters_out) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  if (iters_out) *iters_out = 0;
+  if (n == 0) return UVLM_OK;
+  if (!A || !rhs || !x) return fail(c, UVLM_ERR_ARG, "null argument");
+  if (n > 46000) return fail(c, UVLM_ERR_ARG, "matrix too large");
+  TRY(ensure(c, c->t_A, sizeof(double) * n * n));
+  TRY(ensure(c, c->t_x, sizeof(double) * 2 * n));
+  TRY(ensure(c, c->t_ipiv, sizeof(int) * n));
+  TRY(ensure(c, c->t_info, sizeof(int) * 4));
+  // row-major host A -> column-major device A: transpose on the host
+  double* tmp = new (std::nothrow) double[n * n];
+  if (!tmp) return fail(c, UVLM_ERR_OOM, "host buffer");
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j) tmp[j * n + i] = A[i * n + j];
+  hipError_t e = hipMemcpy(c->t_A.p, tmp, sizeof(double) * n * n, hipMemcpyHostToDevice);
+  delete[] tmp;
+  if (e != hipSuccess) return fail(c, UVLM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+  double* db = P<double>(c->t_x);
+  double* dx = db + n;
+  HIPCHK(c, hipMemcpyAsync(db, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (mode == UVLM_SOLVE_F64) {
+    dense_lu_factor(c->stream, P<double>(c->t_A), (int)n, P<int>(c->t_ipiv),
+                    P<int>(c->t_info));
+    dense_lu_solve(c->stream, P<double>(c->t_A), (int)n, P<int>(c->t_ipiv), db);
+    TRY(check_launch(c));
+    int info = 0;
+    HIPCHK(c, hipMemcpyAsync(&info, c->t_info.p, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(x, db, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (info != 0) return fail(c, UVLM_ERR_SINGULAR, "Matrix is not invertible (zero pivot at column %d)", info - 1);
+    return UVLM_OK;
+  }
+  if (mode != UVLM_SOLVE_F32_IR && mode != UVLM_SOLVE_BF16_IR)
+    return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", mode);
+  TRY(ensure(c, c->t_work, dense_mixed_work_bytes((int)n)));
+  int it = 0;
+  const int rc = dense_mixed_solve(c->stream, P<double>(c->t_A), (int)n, db, dx, mode,
+                                   tol > 0 ? tol : 1e-14, max_iter > 0 ? max_iter : 30,
+                                   c->t_work.p, c->t_work.cap, &it);
+  TRY(check_launch(c));
+  if (iters_out) *iters_out = it;
+  HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (rc < 0) return fail(c, UVLM_ERR_ARG, "workspace too small");
+  if (rc > 0) return fail(c, UVLM_ERR_NOCONV, "refinement did not converge in %d sweeps", it);
+  return UVLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int uvlm_sim_init(uvlm_ctx* c, size_t num, size_t rows, size_t cols,
+                  size_t max_steps, unsigned flags) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  TRY(check_prec(c, flags));
+  if (num == 0 || rows == 0 || cols == 0)
+    return fail(c, UVLM_ERR_ARG, "empty lattice");
+  if (cols & 1) return fail(c, UVLM_ERR_ARG, "columns must be even.");
+  const size_t N = num * rows * cols;
+  const size_t cap = max_steps + 2;
+  const size_t nw_max = num * cap * cols;
+  if (N > 46000 || nw_max + N > (size_t)INT32_MAX / 8)
+    return fail(c, UVLM_ERR_ARG, "lattice too large");
+  c->num = (int)num;
+  c->R = (int)rows;
+  c->C = (int)cols;
+  c->N = (int)N;
+  c->cap = (int)cap;
+  c->capg = (int)cap;
+  c->max_steps = (int)max_steps;
+  c->flags = flags;
+  c->wstart = c->cap;
+  c->wrows = 0;
+  c->gstart = c->capg;
+  c->wgrows = 0;
+  c->has_vprev = false;
+  c->vprev_rows = 0;
+  c->loads_count = 0;
+  c->pairs_eval = c->pairs_ref = 0.0;
+  const size_t nn = num * (rows + 1) * (cols + 1);
+  const size_t E = num * rows * (2 * cols + 1);
+  const size_t V = num * cap * (cols + 1);
+  TRY(ensure(c, c->wing, sizeof(double) * 3 * nn));
+  TRY(ensure(c, c->gamma, sizeof(double) * N));
+  TRY(ensure(c, c->gamma_prev, sizeof(double) * N));
+  TRY(ensure(c, c->wake, sizeof(double) * 3 * V));
+  TRY(ensure(c, c->wgam, sizeof(double) * num * cap * cols));
+  TRY(ensure(c, c->cp, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->nrm, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->area, sizeof(double) * N));
+  TRY(ensure(c, c->recs, sizeof(RingRec) * (N + nw_max)));
+  TRY(ensure(c, c->A, sizeof(double) * N * N));
+  TRY(ensure(c, c->rhs, sizeof(double) * N));
+  TRY(ensure(c, c->ipiv, sizeof(int) * N));
+  TRY(ensure(c, c->info, sizeof(int) * 4));
+  TRY(ensure(c, c->uw, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->vm, sizeof(double) * 3 * N));
+  TRY(ensure(c, c->mid, sizeof(double) * 3 * E));
+  TRY(ensure(c, c->uflow, sizeof(double) * 3 * E));
+  TRY(ensure(c, c->vedge, sizeof(double) * 3 * E));
+  TRY(ensure(c, c->ref_area, sizeof(double) * num));
+  TRY(ensure(c, c->loads, sizeof(double) * 2 * num * (max_steps + 2)));
+  HIPCHK(c, hipMemsetAsync(c->gamma.p, 0, sizeof(double) * N, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->gamma_prev.p, 0, sizeof(double) * N, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sim = true;
+  return UVLM_OK;
+}
+
+int uvlm_sim_set_wing(uvlm_ctx* c, const double* nodes) {
+  if (!c || !nodes) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  const size_t nn = (size_t)c->num * (c->R + 1) * (c->C + 1);
+  HIPCHK(c, hipMemcpyAsync(c->gamma_prev.p, c->gamma.p, sizeof(double) * c->N,
+                           hipMemcpyDeviceToDevice, c->stream));
+  TRY(stage_upload(c, c->st_wing, c->wing.p, nodes, sizeof(double) * 3 * nn));
+  launch_pack_lattice(c->stream, P<double>(c->wing), c->R + 1, 0, P<double>(c->gamma),
+                      c->R, 0, c->num, c->R, c->C, P<RingRec>(c->recs));
+  return check_launch(c);
+}
+
+namespace {
+// Move the live rows of a ring buffer to its end when no headroom is left.
+int compact_rows(uvlm_ctx* c, Buf& buf, int width, int cap, int& start, int rows) {
+  if (start > 0 || rows == 0) {
+    if (rows == 0) start = cap;
+    return UVLM_OK;
+  }
+  const size_t row_bytes = sizeof(double) * (size_t)width;
+  const size_t bytes = row_bytes * (size_t)c->num * rows;
+  TRY(ensure(c, c->tmp, bytes));
+  for (int n = 0; n < c->num; ++n)
+    HIPCHK(c, hipMemcpyAsync((char*)c->tmp.p + row_bytes * (size_t)n * rows,
+                             (char*)buf.p + row_bytes * ((size_t)n * cap + start),
+                             row_bytes * rows, hipMemcpyDeviceToDevice, c->stream));
+  const int ns = cap - rows;
+  for (int n = 0; n < c->num; ++n)
+    HIPCHK(c, hipMemcpyAsync((char*)buf.p + row_bytes * ((size_t)n * cap + ns),
+                             (char*)c->tmp.p + row_bytes * (size_t)n * rows,
+                             row_bytes * rows, hipMemcpyDeviceToDevice, c->stream));
+  start = ns;
+  return UVLM_OK;
+}
+
+void pack_wake(uvlm_ctx* c, const double* nodes, int sheet_rows, int row_off) {
+  launch_pack_lattice(c->stream, nodes, sheet_rows, row_off, P<double>(c->wgam),
+                      c->capg, c->gstart, c->num, c->wgrows, c->C,
+                      P<RingRec>(c->recs) + c->N);
+}
+}  // namespace
+
+int uvlm_sim_shed(uvlm_ctx* c, size_t step) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  if (c->wrows + 1 > c->cap)
+    return fail(c, UVLM_ERR_CAPACITY, "wake capacity of %d rows exhausted", c->cap);
+  TRY(compact_rows(c, c->wake, 3 * (c->C + 1), c->cap, c->wstart, c->wrows));
+  c->wstart -= 1;
+  c->wrows += 1;
+  int grow = -1;
+  if (step > 0) {
+    if (c->wgrows + 1 > c->capg)
+      return fail(c, UVLM_ERR_CAPACITY, "wake capacity of %d rows exhausted", c->capg);
+    TRY(compact_rows(c, c->wgam, c->C, c->capg, c->gstart, c->wgrows));
+    c->gstart -= 1;
+    c->wgrows += 1;
+    grow = c->gstart;
+  }
+  if (c->wgrows + 1 != c->wrows)
+    return fail(c, UVLM_ERR_STATE, "wake shape out of step: %d node rows, %d gamma rows",
+                c->wrows, c->wgrows);
+  launch_shed(c->stream, P<double>(c->wing), P<double>(c->gamma), c->num, c->R, c->C,
+              P<double>(c->wake), c->cap, c->wstart, P<double>(c->wgam), c->capg, grow);
+  pack_wake(c, P<double>(c->wake), c->cap, c->wstart);
+  return check_launch(c);
+}
+
+int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
+                   int solve_mode, double* gamma_out) {
+  if (!c || !vmorph_cp || !ff) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  const int N = c->N;
+  const int Nw = c->num * c->wgrows * c->C;
+  launch_geometry(c->stream, P<double>(c->wing), c->num, c->R, c->C, P<double>(c->cp),
+                  P<double>(c->nrm), P<double>(c->area));
+  TRY(stage_upload(c, c->st_vm, c->vm.p, vmorph_cp, sizeof(double) * 3 * N));
+  // wake velocity at the collocation points
+  ChunkPlan plan;
+  TRY(nbody(c, c->partial, P<double>(c->cp), N, P<RingRec>(c->recs) + N, 0, Nw,
+            c->flags, kSegVoring, &plan));
+  launch_sum_chunks(c->stream, P<double>(c->partial), N, N, plan.total(),
+                    P<double>(c->uw));
+  launch_rhs(c->stream, P<double>(c->uw), P<double>(c->vm), P<double>(c->nrm), N, ff,
+             P<double>(c->rhs));
+  launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
+                c->td, P<double>(c->A));
+  TRY(check_launch(c));
+  c->pairs_eval += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
+  c->pairs_ref += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
+  if (solve_mode == UVLM_SOLVE_F64) {
+    dense_lu_factor(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<int>(c->info));
+    dense_lu_solve(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<double>(c->rhs));
+    HIPCHK(c, hipMemcpyAsync(c->gamma.p, c->rhs.p, sizeof(double) * N,
+                             hipMemcpyDeviceToDevice, c->stream));
+  } else if (solve_mode == UVLM_SOLVE_F32_IR || solve_mode == UVLM_SOLVE_BF16_IR) {
+    TRY(ensure(c, c->work, dense_mixed_work_bytes(N)));
+    int it = 0;
+    const int rc = dense_mixed_solve(c->stream, P<double>(c->A), N, P<double>(c->rhs),
+                                     P<double>(c->gamma), solve_mode, 1e-14, 30,
+                                     c->work.p, c->work.cap, &it);
+    if (rc < 0) return fail(c, UVLM_ERR_ARG, "workspace too small");
+    if (rc > 0) return fail(c, UVLM_ERR_NOCONV, "refinement did not converge");
+  } else {
+    return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", solve_mode);
+  }
+  launch_set_gamma(c->stream, P<double>(c->gamma), N, P<RingRec>(c->recs));
+  TRY(check_launch(c));
+  if (gamma_out) {
+    int info = 0;
+    if (solve_mode == UVLM_SOLVE_F64)
+      HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHK(c, hipMemcpyAsync(gamma_out, c->gamma.p, sizeof(double) * N,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (info != 0)
+      return fail(c, UVLM_ERR_SINGULAR, "Matrix is not invertible (zero pivot at column %d)",
+                  info - 1);
+  }
+  return UVLM_OK;
+}
+
+int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
+                   double dt, const double* ref_area) {
+  if (!c || !vmorph_edges || !ff || !ref_area) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  if (c->loads_count >= c->max_steps + 2)
+    return fail(c, UVLM_ERR_CAPACITY, "load history full");
+  const int N = c->N;
+  const int Nw = c->num * c->wgrows * c->C;
+  const int E = c->num * c->R * (2 * c->C + 1);
+  launch_edge_midpoints(c->stream, P<double>(c->wing), c->num, c->R, c->C,
+                        P<double>(c->mid));
+  TRY(stage_upload(c, c->st_ve, c->vedge.p, vmorph_edges, sizeof(double) * 3 * E));
+  TRY(stage_upload(c, c->st_area, c->ref_area.p, ref_area, sizeof(double) * c->num));
+  ChunkPlan plan;
+  TRY(nbody(c, c->partial, P<double>(c->mid), E, P<RingRec>(c->recs), N, Nw, c->flags,
+            kSegVoring, &plan));
+  launch_flow_velocity(c->stream, P<double>(c->partial), E, E, plan.cb, plan.cw, ff,
+                       P<double>(c->uflow), 0, 0);
+  const double Q = std::sqrt((ff[0] * ff[0] + ff[1] * ff[1]) + ff[2] * ff[2]);
+  launch_loads(c->stream, P<double>(c->wing), P<double>(c->gamma),
+               P<double>(c->gamma_prev), P<double>(c->nrm), P<double>(c->area),
+               P<double>(c->uflow), P<double>(c->vedge), c->num, c->R, c->C, dt, Q,
+               P<double>(c->ref_area),
+               P<double>(c->loads) + (size_t)2 * c->num * c->loads_count);
+  TRY(check_launch(c));
+  c->loads_count += 1;
+  const double L = (double)c->num * (4.0 * c->R * c->C - c->C);
+  c->pairs_eval += 4.0 * E * (double)(N + Nw);
+  c->pairs_ref += 4.0 * L * (double)(N + Nw);
+  return UVLM_OK;
+}
+
+int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double ff[3],
+                    double dt) {
+  if (!c || !ff) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (scheme < 0 || scheme > 4) return fail(c, UVLM_ERR_ARG, "unknown scheme %d", scheme);
+  TRY(set_dev(c));
+  const int N = c->N;
+  const int Nw = c->num * c->wgrows * c->C;
+  const int C1 = c->C + 1;
+  const int rows = c->wrows;
+  const int V = c->num * rows * C1;
+  if (V == 0) return UVLM_OK;
+  const size_t vb = sizeof(double) * 3 * (size_t)V;
+  TRY(ensure(c, c->pos, vb));
+  TRY(ensure(c, c->pst, vb));
+  TRY(ensure(c, c->k1, vb));
+  launch_gather_rows(c->stream, P<double>(c->wake), c->cap, c->wstart, c->num, rows, C1,
+                     P<double>(c->pos));
+  const int te_period = rows * C1;
+  auto stage = [&](const double* targets, Buf& out) -> int {
+    ChunkPlan plan;
+    TRY(nbody(c, c->partial, targets, V, P<RingRec>(c->recs), N, Nw, c->flags,
+              kSegVoring, &plan));
+    launch_flow_velocity(c->stream, P<double>(c->partial), V, V, plan.cb, plan.cw, ff,
+                         P<double>(out), te_period, C1);
+    c->pairs_eval += 4.0 * V * (double)(N + Nw);
+    return check_launch(c);
+  };
+  const size_t total = (size_t)V;
+  const int ign = (int)std::min<size_t>(ignore_steps, (size_t)INT32_MAX);
+  double* k2 = nullptr;
+  double* k3 = nullptr;
+  double* k4 = nullptr;
+  int ref_stages = 1;
+  TRY(stage(P<double>(c->pos), c->k1));  // wake recs were packed by shed
+  if (scheme == UVLM_ADVECT_RK2) {
+    TRY(ensure(c, c->k2, vb));
+    launch_stage_pos(c->stream, P<double>(c->pos), P<double>(c->k1), dt, 1.0, total,
+                     P<double>(c->pst));
+    pack_wake(c, P<double>(c->pst), rows, 0);
+    TRY(stage(P<double>(c->pst), c->k2));
+    k2 = P<double>(c->k2);
+    ref_stages = 2;
+  } else if (scheme == UVLM_ADVECT_RK4 || scheme == UVLM_ADVECT_RK4_CLASSIC) {
+    TRY(ensure(c, c->k2, vb));
+    TRY(ensure(c, c->k3, vb));
+    launch_stage_pos(c->stream, P<double>(c->pos), P<double>(c->k1), dt, 2.0, total,
+                     P<double>(c->pst));
+    pack_wake(c, P<double>(c->pst), rows, 0);
+    TRY(stage(P<double>(c->pst), c->k2));
+    launch_stage_pos(c->stream, P<double>(c->pos), P<double>(c->k2), dt, 2.0, total,
+                     P<double>(c->pst));
+    pack_wake(c, P<double>(c->pst), rows, 0);
+    TRY(stage(P<double>(c->pst), c->k3));
+    k2 = P<double>(c->k2);
+    k3 = P<double>(c->k3);
+    ref_stages = 4;  // the reference also evaluates the unused k4 stage
+    if (scheme == UVLM_ADVECT_RK4_CLASSIC) {
+      TRY(ensure(c, c->k4, vb));
+      launch_stage_pos(c->stream, P<double>(c->pos), P<double>(c->k3), dt, 1.0, total,
+                       P<double>(c->pst));
+      pack_wake(c, P<double>(c->pst), rows, 0);
+      TRY(stage(P<double>(c->pst), c->k4));
+      k4 = P<double>(c->k4);
+    }
+  }
+  const double* vprev = nullptr;
+  if (scheme == UVLM_ADVECT_AB2 && c->has_vprev) vprev = P<double>(c->vprev);
+  launch_update(c->stream, scheme, P<double>(c->pos), P<double>(c->k1), k2, k3, k4, vprev,
+                c->vprev_rows, ign, dt, c->num, rows, C1, P<double>(c->wake), c->cap,
+                c->wstart);
+  TRY(check_launch(c));
+  if (scheme == UVLM_ADVECT_AB2) {
+    TRY(ensure(c, c->vprev, vb));
+    HIPCHK(c, hipMemcpyAsync(c->vprev.p, c->k1.p, vb, hipMemcpyDeviceToDevice, c->stream));
+    c->has_vprev = true;
+    c->vprev_rows = rows;
+  }
+  c->pairs_ref += 4.0 * ref_stages * V * (double)(N + Nw);
+  return UVLM_OK;
+}
+
+int uvlm_sim_erase_oldest_wake(uvlm_ctx* c) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (c->wrows > 0) c->wrows -= 1;
+  if (c->wgrows > 0) c->wgrows -= 1;
+  return UVLM_OK;
+}
+
+int uvlm_sim_wake_shape(uvlm_ctx* c, size_t* node_rows, size_t* gamma_rows) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (node_rows) *node_rows = (size_t)c->wrows;
+  if (gamma_rows) *gamma_rows = (size_t)c->wgrows;
+  return UVLM_OK;
+}
+
+int uvlm_sim_get_wake(uvlm_ctx* c, double* nodes, double* gamma) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  const int C1 = c->C + 1;
+  for (int n = 0; n < c->num; ++n) {
+    if (nodes && c->wrows)
+      HIPCHK(c, hipMemcpyAsync(nodes + (size_t)3 * n * c->wrows * C1,
+                               P<double>(c->wake) + (size_t)3 * ((size_t)n * c->cap + c->wstart) * C1,
+                               sizeof(double) * 3 * c->wrows * C1, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (gamma && c->wgrows)
+      HIPCHK(c, hipMemcpyAsync(gamma + (size_t)n * c->wgrows * c->C,
+                               P<double>(c->wgam) + ((size_t)n * c->capg + c->gstart) * c->C,
+                               sizeof(double) * c->wgrows * c->C, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_sim_set_wake(uvlm_ctx* c, size_t node_rows, size_t gamma_rows,
+                      const double* nodes, const double* gamma,
+                      const double* wing_gamma) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  if ((int)node_rows > c->cap || (int)gamma_rows > c->capg)
+    return fail(c, UVLM_ERR_CAPACITY, "wake larger than capacity");
+  if (!(gamma_rows + 1 == node_rows || (node_rows == 0 && gamma_rows == 0)))
+    return fail(c, UVLM_ERR_ARG, "inconsistent wake shape: %zu node rows, %zu gamma rows",
+                node_rows, gamma_rows);
+  const int C1 = c->C + 1;
+  c->wrows = (int)node_rows;
+  c->wstart = c->cap - c->wrows;
+  c->wgrows = (int)gamma_rows;
+  c->gstart = c->capg - c->wgrows;
+  for (int n = 0; n < c->num; ++n) {
+    if (c->wrows)
+      HIPCHK(c, hipMemcpyAsync(P<double>(c->wake) + (size_t)3 * ((size_t)n * c->cap + c->wstart) * C1,
+                               nodes + (size_t)3 * n * c->wrows * C1,
+                               sizeof(double) * 3 * c->wrows * C1, hipMemcpyHostToDevice,
+                               c->stream));
+    if (c->wgrows)
+      HIPCHK(c, hipMemcpyAsync(P<double>(c->wgam) + ((size_t)n * c->capg + c->gstart) * c->C,
+                               gamma + (size_t)n * c->wgrows * c->C,
+                               sizeof(double) * c->wgrows * c->C, hipMemcpyHostToDevice,
+                               c->stream));
+  }
+  if (wing_gamma)
+    HIPCHK(c, hipMemcpyAsync(c->gamma.p, wing_gamma, sizeof(double) * c->N,
+                             hipMemcpyHostToDevice, c->stream));
+  c->has_vprev = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_sim_get_gamma(uvlm_ctx* c, double* gamma) {
+  if (!c || !gamma) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  HIPCHK(c, hipMemcpyAsync(gamma, c->gamma.p, sizeof(double) * c->N, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_sim_get_loads(uvlm_ctx* c, double* out, size_t* count_inout) {
+  if (!c || !count_inout) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  TRY(set_dev(c));
+  const size_t n = std::min(*count_inout, (size_t)c->loads_count);
+  if (n && out)
+    HIPCHK(c, hipMemcpyAsync(out, c->loads.p, sizeof(double) * 2 * c->num * n,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *count_inout = (size_t)c->loads_count;
+  return UVLM_OK;
+}
+
+int uvlm_sim_pair_counts(uvlm_ctx* c, double* evaluated, double* reference) {
+  if (!c) return UVLM_ERR_ARG;
+  if (evaluated) *evaluated = c->pairs_eval;
+  if (reference) *reference = c->pairs_ref;
+  return UVLM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// internal.h -- shared declarations of the gfx950 UVLM kernels.
+//
+// Data layout in HBM
+//   * lattices: AoS double3, sheet-major, row-major (the reference's
+//     MultipleSheet order K = j + cols*(i + rows*n));
+//   * wake: a row ring buffer per sheet of `cap` rows, logical row i at
+//     physical row (cap - rows + i), so prepending a row is O(cols);
+//   * source rings: packed 128-byte records (4 corners + Gamma), the unit the
+//     N-body kernels stage through LDS;
+//   * partial sums of the N-body kernels: SoA [chunk][3][ld];
+//   * dense matrices: column-major, leading dimension = n.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace uvlm {
+
+// Cut-off length of the reference's BiotSavartLaw (vortex.cpp:31).
+constexpr double kCutoff = 1e-10;
+// 1. / 4. / M_PI as the reference folds it (vortex.cpp:36).
+constexpr double kInv4Pi = 1. / 4. / 3.14159265358979323846;
+
+// One source ring: corners p0..p3 (12 doubles), Gamma, 3 pad = 128 bytes.
+struct alignas(16) RingRec {
+  double p[12];
+  double g;
+  double pad[3];
+};
+static_assert(sizeof(RingRec) == 128, "RingRec must be 128 bytes");
+
+// Segment order / scaling variants of a ring sum.
+enum SegMode : int {
+  kSegVoring = 0,      // p0->p1, p1->p2, p2->p3, p3->p0, Gamma per segment
+  kSegRingPerRing = 1, // node[(k+1)%4] -> node[k], sum then x Gamma
+  kSegRingChord = 2,   // legacy, segments k in {0,2}, sum then x Gamma
+  kSegRingPerSeg = 3,  // legacy order, Gamma per segment
+};
+
+enum Prec : int { kF64Exact = 0, kF64Fast = 1, kF32C = 2 };
+
+// Source ranges of one N-body launch: rings [0, nb) split into cb chunks
+// and rings [nb, nb + nw) split into cw chunks.
+struct ChunkPlan {
+  int nb, nw;     // ring counts of the two ranges
+  int cb, cw;     // chunks per range
+  int szb, szw;   // rings per chunk
+  int total() const { return cb + cw; }
+};
+
+ChunkPlan plan_chunks(int n_targets, int nb, int nw, bool sequential);
+
+// ---- launchers (stream-ordered, asynchronous) ----------------------------
+// Threshold T with  fl(fl(sqrt(x))^2) < kCutoff  <=>  x < T  (x >= 0).
+double cutoff_dsq_threshold();
+
+void launch_pack_lattice(hipStream_t s, const double* nodes, int sheet_rows,
+                         int row_off, const double* gamma, int g_sheet_rows,
+                         int g_row_off, int num, int rows, int cols,
+                         RingRec* out);
+void launch_pack_rings(hipStream_t s, const double* rings, const double* gamma,
+                       int n, RingRec* out);
+void launch_set_gamma(hipStream_t s, const double* gamma, int n, RingRec* out);
+
+// Partial sums partial[(c*3 + d)*ld + t] of target t over chunk c.
+void launch_nbody(hipStream_t s, int prec, int seg, const double* targets,
+                  int n_targets, const RingRec* recs, const ChunkPlan& plan,
+                  double* partial, int ld, double td);
+
+// Independent segment evaluations out[k] = BiotSavartLaw(a[k], b[k], x[k]).
+void launch_segments(hipStream_t s, int prec, const double* a, const double* b,
+                     const double* x, int n, double td, double* out);
+
+// out[t] = sum of all chunks (in chunk order), AoS.
+void launch_sum_chunks(hipStream_t s, const double* partial, int ld,
+                       int n_targets, int chunks, double* out);
+// out[t] = ((-ff + sum_bound) + sum_wake), AoS (advect.h:16-24).  When
+// te_period > 0, targets with (t % te_period) < te_width are trailing-edge
+// wake nodes and receive -ff (advect.h:41-45).  ff is a host array.
+void launch_flow_velocity(hipStream_t s, const double* partial, int ld,
+                          int n_targets, int cb, int cw, const double* ff,
+                          double* out, int te_period, int te_width);
+
+// Dense solve (column-major A, overwritten by its LU factors).
+int dense_lu_factor(hipStream_t s, double* A, int n, int* ipiv, int* info_dev);
+void dense_lu_solve(hipStream_t s, const double* LU, int n, const int* ipiv,
+                    double* b);
+// Mixed precision solve with FP64 iterative refinement; A is not modified.
+// mode: 1 = FP32 LU, 2 = BF16-input LU updates.
+int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
+                      double* x, int mode, double tol, int max_iter,
+                      void* work, size_t work_bytes, int* iters);
+size_t dense_mixed_work_bytes(int n);
+
+}  // namespace uvlm

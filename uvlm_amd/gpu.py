@@ -1,0 +1,283 @@
+"""Python view of the gfx950 C-ABI (include/uvlm_gpu.h) and of the host
+library's SimpleSimulator (include/uvlm_host.h).
+
+Arrays are numpy float64 in the reference's MultipleSheet layout:
+nodes ``[num, rows+1, cols+1, 3]``, gamma ``[num, rows, cols]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class UVLMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"UVLM error {code}: {msg}")
+        self.code = code
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One uvlm_ctx (a HIP stream plus device buffers) on ``device``."""
+
+    def __init__(self, device: int = 0):
+        self._L = N.gpu_lib()
+        h = C.c_void_p()
+        rc = self._L.uvlm_create(device, C.byref(h))
+        if rc != N.OK:
+            raise UVLMError(rc, f"uvlm_create(device={device}) failed: no usable GPU")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.uvlm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            msg = self._L.uvlm_last_error(self._h)
+            raise UVLMError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # --- stateless kernels --------------------------------------------------
+    def biot_savart(self, start, end, pos, flags=N.PREC_F64_EXACT):
+        s, e, x = _f64(start).reshape(-1, 3), _f64(end).reshape(-1, 3), _f64(pos).reshape(-1, 3)
+        out = np.zeros_like(x)
+        self._check(self._L.uvlm_biot_savart(self._h, _p(s), _p(e), _p(x), x.shape[0],
+                                             flags, _p(out)), "uvlm_biot_savart")
+        return out
+
+    def induced_velocity(self, targets, nodes, gamma, flags=N.PREC_F64_EXACT):
+        t = _f64(targets).reshape(-1, 3)
+        nd, g = _f64(nodes), _f64(gamma)
+        num, R, Cc = g.shape
+        if nd.shape != (num, R + 1, Cc + 1, 3):
+            raise ValueError("nodes must be [num, rows+1, cols+1, 3]")
+        out = np.zeros_like(t)
+        self._check(self._L.uvlm_induced_velocity(self._h, _p(t), t.shape[0], _p(nd), _p(g),
+                                                  num, R, Cc, flags, _p(out)),
+                    "uvlm_induced_velocity")
+        return out
+
+    def ring_induced_velocity(self, targets, rings, gamma, flags=N.SCALE_PER_RING):
+        t = _f64(targets).reshape(-1, 3)
+        r, g = _f64(rings).reshape(-1, 4, 3), _f64(gamma).reshape(-1)
+        out = np.zeros_like(t)
+        self._check(self._L.uvlm_ring_induced_velocity(self._h, _p(t), t.shape[0], _p(r),
+                                                       _p(g), r.shape[0], flags, _p(out)),
+                    "uvlm_ring_induced_velocity")
+        return out
+
+    def influence_matrix(self, nodes, cpos, normals, flags=N.PREC_F64_EXACT):
+        nd = _f64(nodes)
+        num, R1, C1, _ = nd.shape
+        Nn = num * (R1 - 1) * (C1 - 1)
+        A = np.zeros((Nn, Nn))
+        self._check(self._L.uvlm_influence_matrix(self._h, _p(nd), num, R1 - 1, C1 - 1,
+                                                  _p(_f64(cpos)), _p(_f64(normals)), flags,
+                                                  _p(A)), "uvlm_influence_matrix")
+        return A
+
+    def solve(self, A, b, mode=N.SOLVE_F64, tol=1e-14, max_iter=30):
+        A, b = _f64(A), _f64(b)
+        x = np.zeros(A.shape[0])
+        it = C.c_int(0)
+        self._check(self._L.uvlm_solve(self._h, A.shape[0], _p(A), _p(b), _p(x), mode, tol,
+                                       max_iter, C.byref(it)), "uvlm_solve")
+        return x, it.value
+
+    # --- raw simulation entry points ---------------------------------------
+    def sim_init(self, num, rows, cols, max_steps, flags=N.PREC_F64_EXACT):
+        self._check(self._L.uvlm_sim_init(self._h, num, rows, cols, max_steps, flags),
+                    "uvlm_sim_init")
+        self._shape = (num, rows, cols)
+
+    def sim_set_wing(self, nodes):
+        self._nodes_keep = _f64(nodes)
+        self._check(self._L.uvlm_sim_set_wing(self._h, _p(self._nodes_keep)), "uvlm_sim_set_wing")
+
+    def sim_shed(self, step):
+        self._check(self._L.uvlm_sim_shed(self._h, step), "uvlm_sim_shed")
+
+    def sim_solve(self, vmorph_cp, ff, mode=N.SOLVE_F64, want_gamma=True):
+        v = _f64(vmorph_cp)
+        f = _f64(ff)
+        num, R, Cc = self._shape
+        g = np.zeros((num, R, Cc)) if want_gamma else None
+        self._check(self._L.uvlm_sim_solve(self._h, _p(v), _p(f), mode, _p(g)),
+                    "uvlm_sim_solve")
+        return g
+
+    def sim_loads(self, vmorph_edges, ff, dt, ref_area):
+        self._check(self._L.uvlm_sim_loads(self._h, _p(_f64(vmorph_edges)), _p(_f64(ff)),
+                                           float(dt), _p(_f64(ref_area))), "uvlm_sim_loads")
+
+    def sim_advect(self, scheme, ignore_steps, ff, dt):
+        self._check(self._L.uvlm_sim_advect(self._h, scheme, ignore_steps, _p(_f64(ff)),
+                                            float(dt)), "uvlm_sim_advect")
+
+    def sim_wake(self):
+        nr, gr = C.c_size_t(), C.c_size_t()
+        self._check(self._L.uvlm_sim_wake_shape(self._h, C.byref(nr), C.byref(gr)),
+                    "uvlm_sim_wake_shape")
+        num, R, Cc = self._shape
+        nodes = np.zeros((num, nr.value, Cc + 1, 3))
+        gam = np.zeros((num, gr.value, Cc))
+        self._check(self._L.uvlm_sim_get_wake(self._h, _p(nodes), _p(gam)), "uvlm_sim_get_wake")
+        return nodes, gam
+
+    def sim_loads_history(self, max_count=1 << 20):
+        num = self._shape[0]
+        cnt = C.c_size_t(0)
+        self._check(self._L.uvlm_sim_get_loads(self._h, None, C.byref(cnt)), "loads")
+        out = np.zeros((cnt.value, num, 2))
+        c2 = C.c_size_t(cnt.value)
+        self._check(self._L.uvlm_sim_get_loads(self._h, _p(out), C.byref(c2)), "loads")
+        return out
+
+
+def case_spec(case: dict) -> N.CaseSpec:
+    """Plain-data case (uvlm_amd.workloads) -> uvlm_case_spec."""
+    cs = N.CaseSpec()
+    wings = case["wings"]
+    if not 1 <= len(wings) <= N.MAX_WINGS:
+        raise ValueError("1..16 wings")
+    cs.num_wings = len(wings)
+    for n, d in enumerate(wings):
+        w = cs.wings[n]
+        w.generator = N.GEN[d.get("generator", "rect")]
+        w.digit = int(d.get("digit", 0))
+        w.chord, w.span = float(d["chord"]), float(d["span"])
+        for k in range(3):
+            w.origin[k] = float(d.get("origin", (0, 0, 0))[k])
+        w.alpha = float(d.get("alpha", 0.0))
+        for name in ("plug", "flap"):
+            amp, om, ph = d.get(name, (0.0, 0.0, 0.0))
+            setattr(w, name + "_amp", float(amp))
+            setattr(w, name + "_omega", float(om))
+            setattr(w, name + "_phase", float(ph))
+        w.fourier_omega = float(d.get("fourier_omega", 0.0))
+        for name in ("twist", "bend"):
+            amp = np.asarray(d.get(name + "_amp", np.zeros((4, 2))), dtype=np.float64)
+            ph = np.asarray(d.get(name + "_phase", np.zeros((4, 2))), dtype=np.float64)
+            A, P = getattr(w, name + "_amp"), getattr(w, name + "_phase")
+            for m in range(4):
+                for q in range(2):
+                    A[m][q] = float(amp[m, q])
+                    P[m][q] = float(ph[m, q])
+    cs.rows, cs.cols = int(case["rows"]), int(case["cols"])
+    for k in range(3):
+        cs.ff[k] = float(case["forward_flight"][k])
+    cs.dt = float(case["dt"])
+    cs.scheme = N.SCHEMES[case.get("advect", "euler")]
+    cs.ignore_steps = int(case.get("ignore_steps", 5))
+    cs.erase_wake_after = int(case.get("erase_wake_after", 0))
+    cs.loads = int(bool(case.get("loads", True)))
+    cs.precision = int(case.get("precision", N.PREC_F64_EXACT))
+    cs.solve_mode = int(case.get("solve_mode", N.SOLVE_F64))
+    return cs
+
+
+class Simulation:
+    """UVLM::simulator::SimpleSimulator of the host library on the GPU."""
+
+    def __init__(self, case: dict, max_steps: int):
+        self._L = N.host_lib()
+        self.case = case
+        self._spec = case_spec(case)
+        h = C.c_void_p()
+        rc = self._L.uvlm_host_sim_create(C.byref(self._spec), max_steps, C.byref(h))
+        if rc != N.OK:
+            raise UVLMError(rc, self._L.uvlm_host_last_error().decode())
+        self._h = h
+        self.num = len(case["wings"])
+        self.R, self.C = int(case["rows"]), int(case["cols"])
+        self.step_index = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.uvlm_host_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            raise UVLMError(rc, what)
+
+    def step(self, step=None):
+        s = self.step_index if step is None else step
+        self._check(self._L.uvlm_host_sim_step(self._h, s), "step")
+        self.step_index = s + 1
+
+    def sync(self):
+        self._check(self._L.uvlm_host_sim_sync(self._h), "sync")
+
+    def run(self, steps):
+        """Steps 0..steps inclusive (SimpleSimulator::Run)."""
+        for s in range(steps + 1):
+            self.step(s)
+        self.sync()
+        return self.loads()
+
+    def gamma(self):
+        out = np.zeros((self.num, self.R, self.C))
+        self._check(self._L.uvlm_host_sim_gamma(self._h, _p(out)), "gamma")
+        return out
+
+    def wing_nodes(self):
+        out = np.zeros((self.num, self.R + 1, self.C + 1, 3))
+        self._check(self._L.uvlm_host_sim_wing_nodes(self._h, _p(out)), "wing_nodes")
+        return out
+
+    def wake(self):
+        nr, gr = C.c_size_t(), C.c_size_t()
+        self._check(self._L.uvlm_host_sim_wake_shape(self._h, C.byref(nr), C.byref(gr)), "shape")
+        nodes = np.zeros((self.num, nr.value, self.C + 1, 3))
+        gam = np.zeros((self.num, gr.value, self.C))
+        self._check(self._L.uvlm_host_sim_wake(self._h, _p(nodes), _p(gam)), "wake")
+        return nodes, gam
+
+    def set_wake(self, nodes, gamma, wing_gamma=None):
+        nd, g = _f64(nodes), _f64(gamma)
+        wg = None if wing_gamma is None else _f64(wing_gamma)
+        self._check(self._L.uvlm_host_sim_set_wake(self._h, nd.shape[1], g.shape[1], _p(nd),
+                                                   _p(g), _p(wg)), "set_wake")
+
+    def loads(self):
+        cnt = C.c_size_t(0)
+        self._check(self._L.uvlm_host_sim_loads(self._h, None, C.byref(cnt)), "loads")
+        out = np.zeros((cnt.value, self.num, 2))
+        c2 = C.c_size_t(cnt.value)
+        self._check(self._L.uvlm_host_sim_loads(self._h, _p(out), C.byref(c2)), "loads")
+        return out
+
+    def pairs(self):
+        e, r = C.c_double(), C.c_double()
+        self._check(self._L.uvlm_host_sim_pairs(self._h, C.byref(e), C.byref(r)), "pairs")
+        return e.value, r.value
