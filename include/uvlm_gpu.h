@@ -98,6 +98,13 @@ int uvlm_influence_matrix(uvlm_ctx* ctx, const double* nodes, size_t num,
                           size_t rows, size_t cols, const double* cpos,
                           const double* normals, unsigned flags, double* A);
 
+/* Replaces the legacy internal::CalcMatrix (linear.inl:14-32):
+ * A[i][j] = ring_j.BiotSavartLaw(targets_i, Gamma=1) . normals_i for 4-node
+ * rings [n][4][3] (segments node[(k+1)%4] -> node[k]), row-major [n][n]. */
+int uvlm_ring_influence_matrix(uvlm_ctx* ctx, const double* rings, size_t n,
+                               const double* targets, const double* normals,
+                               double* A);
+
 /* Replaces Eigen::FullPivLU<MatrixXd>(A).solve(rhs) (simple_simulator.cpp
  * :263-268, linear.inl:125-129): A row-major [n][n].  iters_out (nullable)
  * receives the refinement sweeps of the mixed-precision modes. */
@@ -170,6 +177,15 @@ int uvlm_sim_get_loads(uvlm_ctx* ctx, double* out, size_t* count_inout);
  * segment evaluation) and the pairs the reference's own loop structure
  * would have evaluated for the same steps. */
 int uvlm_sim_pair_counts(uvlm_ctx* ctx, double* evaluated, double* reference);
+
+/* --- measurement hook (no reference counterpart) ------------------------
+ * When enabled, every Biot-Savart N-body launch of the context is bracketed
+ * by HIP events on the context's stream.  uvlm_profile_read synchronises
+ * and returns the launches recorded since the last read: their count, the
+ * summed kernel time in milliseconds and the summed segment evaluations. */
+int uvlm_profile_enable(uvlm_ctx* ctx, int enable);
+int uvlm_profile_read(uvlm_ctx* ctx, size_t* launches, double* kernel_ms,
+                      double* pairs);
 
 #ifdef __cplusplus
 }

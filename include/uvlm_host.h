@@ -23,6 +23,8 @@
 
 #include <stddef.h>
 
+#include "uvlm_gpu.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -79,6 +81,8 @@ int uvlm_host_sim_set_wake(uvlm_host_sim* s, size_t node_rows, size_t gamma_rows
 /* count_inout: capacity in steps in, steps recorded out; out [count][num][2]. */
 int uvlm_host_sim_loads(uvlm_host_sim* s, double* out, size_t* count_inout);
 int uvlm_host_sim_pairs(uvlm_host_sim* s, double* evaluated, double* reference);
+/* The simulation's C-ABI context (for uvlm_profile_* and other calls). */
+uvlm_ctx* uvlm_host_sim_context(uvlm_host_sim* s);
 
 #ifdef __cplusplus
 }

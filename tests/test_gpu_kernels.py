@@ -94,6 +94,34 @@ def test_induced_velocity_split_and_fast(ctx, orc):
     np.testing.assert_allclose(fast, ref, rtol=0, atol=1e-12 * scale)
 
 
+@pytest.mark.parametrize("prec,tol", [(N.PREC_F64, 1e-12), (N.PREC_F32C, 2e-5)])
+def test_chain_modes_on_lattice_nodes(ctx, orc, prec, tol):
+    """Fast modes (segment chains) with targets on the lattice itself, as in
+    wake advection: the cut-off segments must vanish in both forms."""
+    rng = np.random.default_rng(21)
+    nodes, gamma = rand_lattice(rng, 2, 12, 16)
+    x = np.concatenate([nodes.reshape(-1, 3), rng.normal(size=(300, 3)) * 2])
+    ref = orc.induced_velocity(x, nodes, gamma)
+    got = ctx.induced_velocity(x, nodes, gamma, prec)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol * scale)
+    seq = ctx.induced_velocity(x, nodes, gamma, prec | N.SEQUENTIAL)
+    np.testing.assert_allclose(seq, ref, rtol=0, atol=tol * scale)
+
+
+def test_f32c_far_wake_coordinates(ctx, orc):
+    """F32C keeps FP32 relative accuracy for a wake far downstream (x ~ 250
+    chords): differences x - p are formed in FP64 before rounding."""
+    rng = np.random.default_rng(22)
+    nodes, gamma = rand_lattice(rng, 1, 20, 30)
+    nodes[..., 0] += 250.0
+    base = nodes.reshape(-1, 3)[::7]
+    x = base + rng.normal(scale=0.01, size=base.shape)
+    ref = orc.induced_velocity(x, nodes, gamma)
+    got = ctx.induced_velocity(x, nodes, gamma, N.PREC_F32C)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("flags", [N.SCALE_PER_RING, N.SCALE_PER_RING | N.CHORDWISE_ONLY, 0])
 def test_legacy_ring_sums_bitwise(ctx, orc, flags):
     rng = np.random.default_rng(11)

@@ -85,6 +85,18 @@ def test_fast_fp64_mode(orc):
     assert rel_err(g.loads(), ol) < 1e-9
 
 
+def test_f32c_mode_loads_within_1e4(orc):
+    """FP32 compensated Biot-Savart with FP64 near-core: CL/CD within 1e-4
+    of the FP64 oracle (relative to the largest |CL|, |CD| of the run)."""
+    case = W.naca4412_c2(steps=24, rows=6, cols=16)
+    case["precision"] = N.PREC_F32C
+    g, o, ol = run_both(orc, case, 24)
+    gl = g.loads()
+    assert rel_err(gl[..., 1], ol[..., 1]) < 1e-4
+    assert rel_err(gl[..., 0], ol[..., 0]) < 1e-4
+    assert rel_err(g.gamma(), o.gamma()) < 1e-4
+
+
 def test_plate_c1_full_run_vs_golden():
     """The CPU-runnable workload: 8x16 plate, 200 steps, Euler."""
     gold = np.load(GOLDEN)

@@ -29,17 +29,19 @@ MAX_WINGS = 16
 GPU_SYMBOLS = [
     "uvlm_create", "uvlm_destroy", "uvlm_last_error", "uvlm_sync",
     "uvlm_biot_savart", "uvlm_induced_velocity", "uvlm_ring_induced_velocity",
-    "uvlm_influence_matrix", "uvlm_solve", "uvlm_sim_init", "uvlm_sim_set_wing",
+    "uvlm_influence_matrix", "uvlm_ring_influence_matrix", "uvlm_solve",
+    "uvlm_sim_init", "uvlm_sim_set_wing",
     "uvlm_sim_shed", "uvlm_sim_solve", "uvlm_sim_loads", "uvlm_sim_advect",
     "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
-    "uvlm_sim_pair_counts",
+    "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
 ]
 HOST_SYMBOLS = [
     "uvlm_host_sim_create", "uvlm_host_sim_destroy", "uvlm_host_last_error",
     "uvlm_host_sim_step", "uvlm_host_sim_sync", "uvlm_host_sim_wing_nodes",
     "uvlm_host_sim_gamma", "uvlm_host_sim_wake_shape", "uvlm_host_sim_wake",
     "uvlm_host_sim_set_wake", "uvlm_host_sim_loads", "uvlm_host_sim_pairs",
+    "uvlm_host_sim_context",
 ]
 
 
@@ -100,6 +102,7 @@ def gpu_lib():
                                             C.c_uint, _dp]
         L.uvlm_ring_induced_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, C.c_uint, _dp]
         L.uvlm_influence_matrix.argtypes = [_vp, _dp, _sz, _sz, _sz, _dp, _dp, C.c_uint, _dp]
+        L.uvlm_ring_influence_matrix.argtypes = [_vp, _dp, _sz, _dp, _dp, _dp]
         L.uvlm_solve.argtypes = [_vp, _sz, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                  C.POINTER(C.c_int)]
         L.uvlm_sim_init.argtypes = [_vp, _sz, _sz, _sz, _sz, C.c_uint]
@@ -115,6 +118,8 @@ def gpu_lib():
         L.uvlm_sim_get_gamma.argtypes = [_vp, _dp]
         L.uvlm_sim_get_loads.argtypes = [_vp, _dp, C.POINTER(_sz)]
         L.uvlm_sim_pair_counts.argtypes = [_vp, _dp, _dp]
+        L.uvlm_profile_enable.argtypes = [_vp, C.c_int]
+        L.uvlm_profile_read.argtypes = [_vp, C.POINTER(_sz), _dp, _dp]
         _gpu = L
     return _gpu
 
@@ -137,5 +142,7 @@ def host_lib():
         L.uvlm_host_sim_set_wake.argtypes = [_vp, _sz, _sz, _dp, _dp, _dp]
         L.uvlm_host_sim_loads.argtypes = [_vp, _dp, C.POINTER(_sz)]
         L.uvlm_host_sim_pairs.argtypes = [_vp, _dp, _dp]
+        L.uvlm_host_sim_context.argtypes = [_vp]
+        L.uvlm_host_sim_context.restype = _vp
         _host = L
     return _host

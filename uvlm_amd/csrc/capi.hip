@@ -11,6 +11,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/uvlm_gpu.h"
 #include "internal.h"
@@ -47,7 +49,7 @@ struct uvlm_ctx {
   unsigned flags = 0;
   int wstart = 0, wrows = 0, gstart = 0, wgrows = 0;
   int max_steps = 0;
-  Buf wing, gamma, gamma_prev, wake, wgam, cp, nrm, area, recs, partial, A,
+  Buf wing, gamma, gamma_prev, wake, wgam, cp, nrm, area, recs, chains, partial, A,
       rhs, ipiv, info, uw, vm, mid, uflow, vedge, pos, pst, k1, k2, k3, k4,
       vprev, loads, ref_area, work, tmp;
   bool has_vprev = false;
@@ -55,6 +57,10 @@ struct uvlm_ctx {
   int loads_count = 0;
   double pairs_eval = 0.0, pairs_ref = 0.0;
   Staging st_wing, st_vm, st_ve, st_area;
+  // measurement hook
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+  std::vector<double> ev_pairs;
 };
 
 namespace {
@@ -158,7 +164,7 @@ int set_dev(uvlm_ctx* c) {
 
 int check_prec(uvlm_ctx* c, unsigned flags) {
   const int p = prec_of(flags);
-  if (p != kF64Exact && p != kF64Fast)
+  if (p != kF64Exact && p != kF64Fast && p != kF32C)
     return fail(c, UVLM_ERR_ARG, "unsupported precision flag %u", flags & 0x0fu);
   return UVLM_OK;
 }
@@ -171,8 +177,72 @@ int nbody(uvlm_ctx* c, Buf& partial, const double* targets, int nt,
   ChunkPlan plan = plan_chunks(nt, nb, nw, (flags & UVLM_SEQUENTIAL) != 0);
   TRY(ensure(c, partial, sizeof(double) * 3 * (size_t)std::max(1, plan.total()) *
                              (size_t)std::max(1, nt)));
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (c->prof && nt > 0 && nb + nw > 0) {
+    if (!c->ev_free.empty()) {
+      ev = c->ev_free.back();
+      c->ev_free.pop_back();
+    } else {
+      HIPCHK(c, hipEventCreate(&ev.first));
+      HIPCHK(c, hipEventCreate(&ev.second));
+    }
+    HIPCHK(c, hipEventRecord(ev.first, c->stream));
+  }
   launch_nbody(c->stream, prec_of(flags), seg, targets, nt, recs, plan,
                P<double>(partial), nt, c->td);
+  if (ev.first) {
+    HIPCHK(c, hipEventRecord(ev.second, c->stream));
+    c->ev_used.push_back(ev);
+    const double segs = (seg == kSegRingChord) ? 2.0 : 4.0;
+    c->ev_pairs.push_back(segs * (double)nt * (double)(nb + nw));
+  }
+  *plan_out = plan;
+  return check_launch(c);
+}
+
+// N-body of targets against the simulation's sources (bound and/or wake):
+// ring records in F64_EXACT, segment chains in the fast modes.  Counts the
+// evaluated segment pairs and returns the chunk plan for the combine.
+int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wake,
+                ChunkPlan* plan_out) {
+  const int prec = prec_of(c->flags);
+  const bool seq = (c->flags & UVLM_SEQUENTIAL) != 0;
+  if (prec == kF64Exact) {
+    const int N = c->N, Nw = c->num * c->wgrows * c->C;
+    const RingRec* base = P<RingRec>(c->recs) + (bound ? 0 : N);
+    TRY(nbody(c, c->partial, targets, nt, base, bound ? N : 0, wake ? Nw : 0, c->flags,
+              kSegVoring, plan_out));
+    c->pairs_eval += 4.0 * nt * (double)((bound ? N : 0) + (wake ? Nw : 0));
+    return UVLM_OK;
+  }
+  const int eb = chain_elems(c->num, c->R, c->C);
+  const int ew = chain_elems(c->num, c->wgrows, c->C);
+  const int nb = bound ? eb : 0, nw = wake ? ew : 0;
+  ChunkPlan plan = plan_chunks(nt, nb, nw, seq);
+  TRY(ensure(c, c->partial, sizeof(double) * 3 * (size_t)std::max(1, plan.total()) *
+                                (size_t)std::max(1, nt)));
+  const double segs = (bound ? chain_segments(c->num, c->R, c->C) : 0.0) +
+                      (wake ? chain_segments(c->num, c->wgrows, c->C) : 0.0);
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (c->prof && nt > 0 && nb + nw > 0) {
+    if (!c->ev_free.empty()) {
+      ev = c->ev_free.back();
+      c->ev_free.pop_back();
+    } else {
+      HIPCHK(c, hipEventCreate(&ev.first));
+      HIPCHK(c, hipEventCreate(&ev.second));
+    }
+    HIPCHK(c, hipEventRecord(ev.first, c->stream));
+  }
+  launch_nbody_chain(c->stream, prec, targets, nt,
+                     P<ChainRec>(c->chains) + (bound ? 0 : eb), plan,
+                     P<double>(c->partial), nt);
+  if (ev.first) {
+    HIPCHK(c, hipEventRecord(ev.second, c->stream));
+    c->ev_used.push_back(ev);
+    c->ev_pairs.push_back(segs * nt);
+  }
+  c->pairs_eval += segs * nt;
   *plan_out = plan;
   return check_launch(c);
 }
@@ -211,7 +281,7 @@ int uvlm_destroy(uvlm_ctx* c) {
   Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
                  &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
                  &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
-                 &c->area, &c->recs, &c->partial, &c->A, &c->rhs, &c->ipiv,
+                 &c->area, &c->recs, &c->chains, &c->partial, &c->A, &c->rhs, &c->ipiv,
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
                  &c->ref_area, &c->work, &c->tmp};
@@ -220,6 +290,14 @@ int uvlm_destroy(uvlm_ctx* c) {
   free_staging(c->st_vm);
   free_staging(c->st_ve);
   free_staging(c->st_area);
+  for (auto& e : c->ev_used) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : c->ev_free) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return UVLM_OK;
@@ -287,12 +365,26 @@ int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * nr,
                            hipMemcpyHostToDevice, c->stream));
-  launch_pack_lattice(c->stream, P<double>(c->t_b), (int)rows + 1, 0,
-                      P<double>(c->t_c), (int)rows, 0, (int)num, (int)rows,
-                      (int)cols, P<RingRec>(c->t_recs));
   ChunkPlan plan;
-  TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
-            (int)nr, flags, kSegVoring, &plan));
+  if (prec_of(flags) == kF64Exact) {
+    launch_pack_lattice(c->stream, P<double>(c->t_b), (int)rows + 1, 0,
+                        P<double>(c->t_c), (int)rows, 0, (int)num, (int)rows,
+                        (int)cols, P<RingRec>(c->t_recs));
+    TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
+              (int)nr, flags, kSegVoring, &plan));
+  } else {
+    const int ne = chain_elems((int)num, (int)rows, (int)cols);
+    TRY(ensure(c, c->t_recs, sizeof(ChainRec) * ne));
+    launch_pack_chains(c->stream, P<double>(c->t_b), (int)rows + 1, 0, P<double>(c->t_c),
+                       (int)rows, 0, (int)num, (int)rows, (int)cols,
+                       P<ChainRec>(c->t_recs));
+    plan = plan_chunks((int)nt, 0, ne, (flags & UVLM_SEQUENTIAL) != 0);
+    TRY(ensure(c, c->t_partial,
+               sizeof(double) * 3 * (size_t)std::max(1, plan.total()) * nt));
+    launch_nbody_chain(c->stream, prec_of(flags), P<double>(c->t_a), (int)nt,
+                       P<ChainRec>(c->t_recs), plan, P<double>(c->t_partial), (int)nt);
+    TRY(check_launch(c));
+  }
   launch_sum_chunks(c->stream, P<double>(c->t_partial), (int)nt, (int)nt,
                     plan.total(), P<double>(c->t_out));
   TRY(check_launch(c));
@@ -384,6 +476,43 @@ int uvlm_influence_matrix(uvlm_ctx* c, const double* nodes, size_t num,
   for (size_t L = 0; L < N; ++L)
     for (size_t K = 0; K < N; ++K) A[K * N + L] = tmp[L * N + K];
   delete[] tmp;
+  return UVLM_OK;
+}
+
+int uvlm_ring_influence_matrix(uvlm_ctx* c, const double* rings, size_t n,
+                               const double* targets, const double* normals, double* A) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  if (n == 0) return UVLM_OK;
+  if (!rings || !targets || !normals || !A) return fail(c, UVLM_ERR_ARG, "null argument");
+  if (n > 46000) return fail(c, UVLM_ERR_ARG, "too many rings");
+  TRY(ensure(c, c->t_a, sizeof(double) * 12 * n));
+  TRY(ensure(c, c->t_b, sizeof(double) * 3 * n));
+  TRY(ensure(c, c->t_c, sizeof(double) * 3 * n));
+  TRY(ensure(c, c->t_out, sizeof(double) * n));
+  TRY(ensure(c, c->t_recs, sizeof(RingRec) * n));
+  TRY(ensure(c, c->t_A, sizeof(double) * n * n));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, rings, sizeof(double) * 12 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, targets, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_c.p, normals, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                           c->stream));
+  // Gamma = 1 for every ring: the per-ring scale is then exact
+  std::vector<double> ones(n, 1.0);
+  HIPCHK(c, hipMemcpyAsync(c->t_out.p, ones.data(), sizeof(double) * n,
+                           hipMemcpyHostToDevice, c->stream));
+  launch_pack_rings(c->stream, P<double>(c->t_a), P<double>(c->t_out), (int)n,
+                    P<RingRec>(c->t_recs));
+  launch_ring_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b), P<double>(c->t_c),
+                     (int)n, c->td, P<double>(c->t_A));
+  TRY(check_launch(c));
+  std::vector<double> tmp(n * n);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), c->t_A.p, sizeof(double) * n * n,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t L = 0; L < n; ++L)
+    for (size_t K = 0; K < n; ++K) A[K * n + L] = tmp[L * n + K];
   return UVLM_OK;
 }
 
@@ -481,6 +610,9 @@ int uvlm_sim_init(uvlm_ctx* c, size_t num, size_t rows, size_t cols,
   TRY(ensure(c, c->nrm, sizeof(double) * 3 * N));
   TRY(ensure(c, c->area, sizeof(double) * N));
   TRY(ensure(c, c->recs, sizeof(RingRec) * (N + nw_max)));
+  if (prec_of(flags) != kF64Exact)
+    TRY(ensure(c, c->chains, sizeof(ChainRec) * ((size_t)chain_elems((int)num, (int)rows, (int)cols) +
+                                                 (size_t)chain_elems((int)num, (int)cap, (int)cols))));
   TRY(ensure(c, c->A, sizeof(double) * N * N));
   TRY(ensure(c, c->rhs, sizeof(double) * N));
   TRY(ensure(c, c->ipiv, sizeof(int) * N));
@@ -535,10 +667,17 @@ int compact_rows(uvlm_ctx* c, Buf& buf, int width, int cap, int& start, int rows
   return UVLM_OK;
 }
 
+// Wake sources from nodes (ring buffer or a compact stage copy): ring
+// records in F64_EXACT, segment chains in the fast modes.
 void pack_wake(uvlm_ctx* c, const double* nodes, int sheet_rows, int row_off) {
-  launch_pack_lattice(c->stream, nodes, sheet_rows, row_off, P<double>(c->wgam),
-                      c->capg, c->gstart, c->num, c->wgrows, c->C,
-                      P<RingRec>(c->recs) + c->N);
+  if (prec_of(c->flags) == kF64Exact)
+    launch_pack_lattice(c->stream, nodes, sheet_rows, row_off, P<double>(c->wgam),
+                        c->capg, c->gstart, c->num, c->wgrows, c->C,
+                        P<RingRec>(c->recs) + c->N);
+  else
+    launch_pack_chains(c->stream, nodes, sheet_rows, row_off, P<double>(c->wgam),
+                       c->capg, c->gstart, c->num, c->wgrows, c->C,
+                       P<ChainRec>(c->chains) + chain_elems(c->num, c->R, c->C));
 }
 }  // namespace
 
@@ -581,8 +720,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   TRY(stage_upload(c, c->st_vm, c->vm.p, vmorph_cp, sizeof(double) * 3 * N));
   // wake velocity at the collocation points
   ChunkPlan plan;
-  TRY(nbody(c, c->partial, P<double>(c->cp), N, P<RingRec>(c->recs) + N, 0, Nw,
-            c->flags, kSegVoring, &plan));
+  TRY(sum_sources(c, P<double>(c->cp), N, false, true, &plan));
   launch_sum_chunks(c->stream, P<double>(c->partial), N, N, plan.total(),
                     P<double>(c->uw));
   launch_rhs(c->stream, P<double>(c->uw), P<double>(c->vm), P<double>(c->nrm), N, ff,
@@ -590,7 +728,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
                 c->td, P<double>(c->A));
   TRY(check_launch(c));
-  c->pairs_eval += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
+  c->pairs_eval += 4.0 * N * (double)N;
   c->pairs_ref += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
   if (solve_mode == UVLM_SOLVE_F64) {
     dense_lu_factor(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<int>(c->info));
@@ -609,6 +747,9 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
     return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", solve_mode);
   }
   launch_set_gamma(c->stream, P<double>(c->gamma), N, P<RingRec>(c->recs));
+  if (prec_of(c->flags) != kF64Exact)
+    launch_pack_chains(c->stream, P<double>(c->wing), c->R + 1, 0, P<double>(c->gamma), c->R,
+                       0, c->num, c->R, c->C, P<ChainRec>(c->chains));
   TRY(check_launch(c));
   if (gamma_out) {
     int info = 0;
@@ -640,8 +781,7 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
   TRY(stage_upload(c, c->st_ve, c->vedge.p, vmorph_edges, sizeof(double) * 3 * E));
   TRY(stage_upload(c, c->st_area, c->ref_area.p, ref_area, sizeof(double) * c->num));
   ChunkPlan plan;
-  TRY(nbody(c, c->partial, P<double>(c->mid), E, P<RingRec>(c->recs), N, Nw, c->flags,
-            kSegVoring, &plan));
+  TRY(sum_sources(c, P<double>(c->mid), E, true, true, &plan));
   launch_flow_velocity(c->stream, P<double>(c->partial), E, E, plan.cb, plan.cw, ff,
                        P<double>(c->uflow), 0, 0);
   const double Q = std::sqrt((ff[0] * ff[0] + ff[1] * ff[1]) + ff[2] * ff[2]);
@@ -653,7 +793,6 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
   TRY(check_launch(c));
   c->loads_count += 1;
   const double L = (double)c->num * (4.0 * c->R * c->C - c->C);
-  c->pairs_eval += 4.0 * E * (double)(N + Nw);
   c->pairs_ref += 4.0 * L * (double)(N + Nw);
   return UVLM_OK;
 }
@@ -679,11 +818,9 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
   const int te_period = rows * C1;
   auto stage = [&](const double* targets, Buf& out) -> int {
     ChunkPlan plan;
-    TRY(nbody(c, c->partial, targets, V, P<RingRec>(c->recs), N, Nw, c->flags,
-              kSegVoring, &plan));
+    TRY(sum_sources(c, targets, V, true, true, &plan));
     launch_flow_velocity(c->stream, P<double>(c->partial), V, V, plan.cb, plan.cw, ff,
                          P<double>(out), te_period, C1);
-    c->pairs_eval += 4.0 * V * (double)(N + Nw);
     return check_launch(c);
   };
   const size_t total = (size_t)V;
@@ -833,6 +970,32 @@ int uvlm_sim_get_loads(uvlm_ctx* c, double* out, size_t* count_inout) {
                              hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *count_inout = (size_t)c->loads_count;
+  return UVLM_OK;
+}
+
+int uvlm_profile_enable(uvlm_ctx* c, int enable) {
+  if (!c) return UVLM_ERR_ARG;
+  c->prof = enable != 0;
+  return UVLM_OK;
+}
+
+int uvlm_profile_read(uvlm_ctx* c, size_t* launches, double* kernel_ms, double* pairs) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double ms = 0.0, pr = 0.0;
+  for (size_t k = 0; k < c->ev_used.size(); ++k) {
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev_used[k].first, c->ev_used[k].second));
+    ms += t;
+    pr += c->ev_pairs[k];
+    c->ev_free.push_back(c->ev_used[k]);
+  }
+  if (launches) *launches = c->ev_used.size();
+  if (kernel_ms) *kernel_ms = ms;
+  if (pairs) *pairs = pr;
+  c->ev_used.clear();
+  c->ev_pairs.clear();
   return UVLM_OK;
 }
 

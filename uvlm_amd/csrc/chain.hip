@@ -1,0 +1,326 @@
+// chain.hip -- node-shared Biot-Savart N-body over segment chains (gfx950).
+//
+// The fast modes evaluate every vortex lattice as chains of merged segments
+// (ChainRec): an interior lattice segment is shared by two rings, so it is
+// evaluated once with the difference of their circulations, and consecutive
+// segments of a chain share a node, so 1/|x - p| of each node is computed
+// once per target and carried to the next segment.  Per target and lattice
+// node this is one reciprocal square root, against two per segment and four
+// segments per ring in the reference's ring-by-ring order.
+//
+//   F64  : FP64 with FMA, refined hardware rsqrt / reciprocal.
+//   F32C : x - p in FP64 (so the FP32 difference is exact to FP32 rounding
+//          relative to |x - p|), then FP32 arithmetic with the hardware
+//          rsqrt / reciprocal; contributions are summed in FP32 over runs
+//          of 16 elements and carried in FP64 per target.  Pairs within a
+//          band around the reference's cut-off (|r| < 1e-9, |d|^2 < 1e-9 plus
+//          the FP32 rounding bound of d) are re-evaluated in FP64 exactly as
+//          the reference does.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace uvlm {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 128;  // chain elements per LDS tile (8 KB)
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+__device__ __forceinline__ double rsq64(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  double h = a * y;
+  double e = __builtin_fma(-h, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  h = a * y;
+  e = __builtin_fma(-h, y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+__device__ __forceinline__ double rcp64(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = __builtin_fma(-a, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-a, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// BiotSavartLaw of segment a->b at x, times g (vortex.cpp:21-39 formula,
+// FP64, used for the near-core pairs of F32C).
+__device__ __forceinline__ void seg_ref(double ax, double ay, double az, double bx,
+                                        double by, double bz, double x, double y,
+                                        double z, double g, double& ux, double& uy,
+                                        double& uz) {
+  const double r0x = bx - ax, r0y = by - ay, r0z = bz - az;
+  const double r1x = x - ax, r1y = y - ay, r1z = z - az;
+  const double r2x = x - bx, r2y = y - by, r2z = z - bz;
+  const double dx = r1y * r2z - r1z * r2y;
+  const double dy = r1z * r2x - r1x * r2z;
+  const double dz = r1x * r2y - r1y * r2x;
+  const double dsq = (dx * dx + dy * dy) + dz * dz;
+  const double r1n = sqrt((r1x * r1x + r1y * r1y) + r1z * r1z);
+  const double r2n = sqrt((r2x * r2x + r2y * r2y) + r2z * r2z);
+  const bool cut = (sqrt(dsq) * sqrt(dsq) < kCutoff) | (r1n < kCutoff) | (r2n < kCutoff);
+  const double K = ((r0x * (r1x / r1n - r2x / r2n) + r0y * (r1y / r1n - r2y / r2n)) +
+                    r0z * (r1z / r1n - r2z / r2n)) /
+                   dsq;
+  if (!cut) {
+    ux += dx * K * g;
+    uy += dy * K * g;
+    uz += dz * K * g;
+  }
+}
+
+__device__ __forceinline__ void chunk_range(const ChunkPlan& p, int c, int& b, int& e) {
+  if (c < p.cb) {
+    b = c * p.szb;
+    e = min(b + p.szb, p.nb);
+  } else {
+    b = p.nb + (c - p.cb) * p.szw;
+    e = min(b + p.szw, p.nb + p.nw);
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_chain_f64(
+    const double* __restrict__ targets, int nt, const ChainRec* __restrict__ recs,
+    ChunkPlan plan, double* __restrict__ partial, int ld) {
+  __shared__ double2 tile[kTile * 4];
+  int rb, re;
+  chunk_range(plan, blockIdx.y, rb, re);
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (t < nt) {
+    x = targets[3 * t + 0];
+    y = targets[3 * t + 1];
+    z = targets[3 * t + 2];
+  }
+  // carried previous node
+  double rax = 0.0, ray = 0.0, raz = 0.0, ras = 0.0, ia = 0.0;
+  if (rb > 0) {
+    const double* p = recs[rb - 1].p;
+    rax = x - p[0];
+    ray = y - p[1];
+    raz = z - p[2];
+    ras = __builtin_fma(raz, raz, __builtin_fma(ray, ray, rax * rax));
+    ia = rsq64(ras);
+  }
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  const double2* src = reinterpret_cast<const double2*>(recs);
+  for (int base = rb; base < re; base += kTile) {
+    const int cnt = min(kTile, re - base);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt * 4; q += kBlock) tile[q] = src[(size_t)base * 4 + q];
+    __syncthreads();
+    const double* tl = reinterpret_cast<const double*>(tile);
+    for (int k = 0; k < cnt; ++k) {
+      const double* e = tl + 8 * k;
+      const double rbx = x - e[0], rby = y - e[1], rbz = z - e[2];
+      const double rbs = __builtin_fma(rbz, rbz, __builtin_fma(rby, rby, rbx * rbx));
+      const double ib = rsq64(rbs);
+      const double r0x = e[4], r0y = e[5], r0z = e[6];
+      // d = r1 x r2 = r0 x (x - a)
+      const double dx = __builtin_fma(r0y, raz, -r0z * ray);
+      const double dy = __builtin_fma(r0z, rax, -r0x * raz);
+      const double dz = __builtin_fma(r0x, ray, -r0y * rax);
+      const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      const double oa = __builtin_fma(r0z, raz, __builtin_fma(r0y, ray, r0x * rax));
+      const double ob = __builtin_fma(r0z, rbz, __builtin_fma(r0y, rby, r0x * rbx));
+      const double tt = __builtin_fma(oa, ia, -ob * ib);
+      const bool cut = (dsq < kCutoff) | (ras < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
+      double K = tt * e[3] * rcp64(dsq);
+      K = cut ? 0.0 : K;
+      ux = __builtin_fma(dx, K, ux);
+      uy = __builtin_fma(dy, K, uy);
+      uz = __builtin_fma(dz, K, uz);
+      rax = rbx;
+      ray = rby;
+      raz = rbz;
+      ras = rbs;
+      ia = ib;
+    }
+  }
+  if (t < nt) {
+    const size_t c = blockIdx.y;
+    partial[(c * 3 + 0) * (size_t)ld + t] = ux;
+    partial[(c * 3 + 1) * (size_t)ld + t] = uy;
+    partial[(c * 3 + 2) * (size_t)ld + t] = uz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
+
+__global__ __launch_bounds__(kBlock) void k_chain_f32c(
+    const double* __restrict__ targets, int nt, const ChainRec* __restrict__ recs,
+    ChunkPlan plan, double* __restrict__ partial, int ld) {
+  __shared__ double2 tile[kTile * 4];
+  int rb, re;
+  chunk_range(plan, blockIdx.y, rb, re);
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (t < nt) {
+    x = targets[3 * t + 0];
+    y = targets[3 * t + 1];
+    z = targets[3 * t + 2];
+  }
+  double pax = 0.0, pay = 0.0, paz = 0.0;  // previous node (FP64, near-core path)
+  float rax = 0.f, ray = 0.f, raz = 0.f, ras = 0.f, ia = 0.f;
+  if (rb > 0) {
+    const double* p = recs[rb - 1].p;
+    pax = p[0];
+    pay = p[1];
+    paz = p[2];
+    rax = (float)(x - pax);
+    ray = (float)(y - pay);
+    raz = (float)(z - paz);
+    ras = fmaf(raz, raz, fmaf(ray, ray, rax * rax));
+    ia = __builtin_amdgcn_rsqf(ras);
+  }
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  const double2* src = reinterpret_cast<const double2*>(recs);
+  for (int base = rb; base < re; base += kTile) {
+    const int cnt = min(kTile, re - base);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt * 4; q += kBlock) tile[q] = src[(size_t)base * 4 + q];
+    __syncthreads();
+    const double* tl = reinterpret_cast<const double*>(tile);
+    for (int k0 = 0; k0 < cnt; k0 += 16) {
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      const int kend = min(cnt, k0 + 16);
+      for (int k = k0; k < kend; ++k) {
+        const double* e = tl + 8 * k;
+        const float rbx = (float)(x - e[0]), rby = (float)(y - e[1]), rbz = (float)(z - e[2]);
+        const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
+        const float ib = __builtin_amdgcn_rsqf(rbs);
+        const float r0x = (float)e[4], r0y = (float)e[5], r0z = (float)e[6];
+        const float dx = fmaf(r0y, raz, -r0z * ray);
+        const float dy = fmaf(r0z, rax, -r0x * raz);
+        const float dz = fmaf(r0x, ray, -r0y * rax);
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
+        const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
+        const float tt = fmaf(oa, ia, -ob * ib);
+        // FP32 rounding bound of |d|^2 for the near-core band
+        const float derr = 4.f * kEps32 * (float)e[7] * ras;
+        const bool near = (dsq < 1e-9f + 16.f * derr) | (ras < 1e-18f) | (rbs < 1e-18f);
+        float K = tt * (float)e[3] * __builtin_amdgcn_rcpf(dsq);
+        K = near ? 0.f : K;
+        fx = fmaf(dx, K, fx);
+        fy = fmaf(dy, K, fy);
+        fz = fmaf(dz, K, fz);
+        if (__any(near)) {
+          if (near && e[3] != 0.0)
+            seg_ref(pax, pay, paz, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
+        }
+        pax = e[0];
+        pay = e[1];
+        paz = e[2];
+        rax = rbx;
+        ray = rby;
+        raz = rbz;
+        ras = rbs;
+        ia = ib;
+      }
+      ux += (double)fx;
+      uy += (double)fy;
+      uz += (double)fz;
+    }
+  }
+  if (t < nt) {
+    const size_t c = blockIdx.y;
+    partial[(c * 3 + 0) * (size_t)ld + t] = ux;
+    partial[(c * 3 + 1) * (size_t)ld + t] = uy;
+    partial[(c * 3 + 2) * (size_t)ld + t] = uz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lattice -> chains.  Per sheet: spanwise chains (node row i = 0..rows, nodes
+// j = 0..cols; segment (i,j-1)->(i,j) carries G(i,j-1) - G(i-1,j-1)), then
+// chordwise chains (column j = 0..cols, nodes i = 0..rows; segment
+// (i-1,j)->(i,j) carries G(i-1,j-1) - G(i-1,j)), G = 0 outside the lattice.
+// These are the VORING segments p0->p1 (+G) and p2->p3 (-G of the ring
+// below) and p1->p2 (+G) and p3->p0 (-G of the ring to the right).
+__global__ void k_pack_chains(const double* __restrict__ nodes, int sheet_rows,
+                              int row_off, const double* __restrict__ gamma,
+                              int g_sheet_rows, int g_row_off, int num, int rows,
+                              int cols, ChainRec* __restrict__ out) {
+  const int per = (rows + 1) * (cols + 1);
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 2 * num * per) return;
+  const int n = q / (2 * per);
+  const int r = q % (2 * per);
+  const int C1 = cols + 1;
+  auto P = [&](int i, int j) {
+    return nodes + 3 * ((size_t)(n * sheet_rows + row_off + i) * C1 + j);
+  };
+  auto G = [&](int i, int j) -> double {
+    if (i < 0 || i >= rows || j < 0 || j >= cols) return 0.0;
+    return gamma[(size_t)(n * g_sheet_rows + g_row_off + i) * cols + j];
+  };
+  int i, j, pi = -1, pj = -1;
+  double g = 0.0;
+  if (r < per) {  // spanwise
+    i = r / C1;
+    j = r % C1;
+    if (j > 0) {
+      pi = i;
+      pj = j - 1;
+      g = G(i, j - 1) - G(i - 1, j - 1);
+    }
+  } else {  // chordwise
+    const int s = r - per;
+    j = s / (rows + 1);
+    i = s % (rows + 1);
+    if (i > 0) {
+      pi = i - 1;
+      pj = j;
+      g = G(i - 1, j - 1) - G(i - 1, j);
+    }
+  }
+  ChainRec e;
+  const double* p = P(i, j);
+  e.p[0] = p[0];
+  e.p[1] = p[1];
+  e.p[2] = p[2];
+  if (pi >= 0) {
+    const double* a = P(pi, pj);
+    e.r0[0] = p[0] - a[0];
+    e.r0[1] = p[1] - a[1];
+    e.r0[2] = p[2] - a[2];
+  } else {
+    e.r0[0] = e.r0[1] = e.r0[2] = 0.0;
+  }
+  e.r0sq = (e.r0[0] * e.r0[0] + e.r0[1] * e.r0[1]) + e.r0[2] * e.r0[2];
+  e.g = g * kInv4Pi;
+  out[q] = e;
+}
+
+}  // namespace
+
+void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows, int row_off,
+                        const double* gamma, int g_sheet_rows, int g_row_off, int num,
+                        int rows, int cols, ChainRec* out) {
+  const int total = chain_elems(num, rows, cols);
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_pack_chains, dim3(cdiv(total, 256)), dim3(256), 0, s, nodes,
+                     sheet_rows, row_off, gamma, g_sheet_rows, g_row_off, num, rows, cols,
+                     out);
+}
+
+void launch_nbody_chain(hipStream_t s, int prec, const double* targets, int nt,
+                        const ChainRec* recs, const ChunkPlan& plan, double* partial,
+                        int ld) {
+  if (nt <= 0 || plan.total() <= 0) return;
+  dim3 grid(cdiv(nt, kBlock), plan.total());
+  if (prec == kF32C)
+    hipLaunchKernelGGL(k_chain_f32c, grid, dim3(kBlock), 0, s, targets, nt, recs, plan,
+                       partial, ld);
+  else
+    hipLaunchKernelGGL(k_chain_f64, grid, dim3(kBlock), 0, s, targets, nt, recs, plan,
+                       partial, ld);
+}
+
+}  // namespace uvlm

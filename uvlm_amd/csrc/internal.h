@@ -32,6 +32,31 @@ struct alignas(16) RingRec {
 };
 static_assert(sizeof(RingRec) == 128, "RingRec must be 128 bytes");
 
+// One element of a segment chain (the FP64 fast and F32C source format):
+// node p of a polyline; the segment from the previous node to p carries the
+// net circulation g (already scaled by 1/4pi) and the vector r0 = p - prev.
+// A chain start has g = 0 and r0 = 0 (its "segment" is always cut off).
+// A vortex lattice becomes (rows+1) spanwise chains of cols+1 nodes and
+// (cols+1) chordwise chains of rows+1 nodes per sheet; an interior segment
+// shared by two rings carries the difference of their circulations.
+struct alignas(16) ChainRec {
+  double p[3];
+  double g;
+  double r0[3];
+  double r0sq;
+};
+static_assert(sizeof(ChainRec) == 64, "ChainRec must be 64 bytes");
+
+// Element count of the chains of a lattice with `rows` ring rows.
+inline int chain_elems(int num, int rows, int cols) {
+  return rows > 0 ? 2 * num * (rows + 1) * (cols + 1) : 0;
+}
+// Segments of those chains (elements minus chain starts).
+inline double chain_segments(int num, int rows, int cols) {
+  return rows > 0 ? (double)num * ((double)(rows + 1) * cols + (double)rows * (cols + 1))
+                  : 0.0;
+}
+
 // Segment order / scaling variants of a ring sum.
 enum SegMode : int {
   kSegVoring = 0,      // p0->p1, p1->p2, p2->p3, p3->p0, Gamma per segment
@@ -64,6 +89,17 @@ void launch_pack_lattice(hipStream_t s, const double* nodes, int sheet_rows,
 void launch_pack_rings(hipStream_t s, const double* rings, const double* gamma,
                        int n, RingRec* out);
 void launch_set_gamma(hipStream_t s, const double* gamma, int n, RingRec* out);
+
+// Chains of a lattice (layout as launch_pack_lattice) -> out[0, chain_elems).
+void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows,
+                        int row_off, const double* gamma, int g_sheet_rows,
+                        int g_row_off, int num, int rows, int cols,
+                        ChainRec* out);
+// Chain N-body (prec = kF64Fast or kF32C); same partial-sum layout and chunk
+// plan semantics as launch_nbody, in units of chain elements.
+void launch_nbody_chain(hipStream_t s, int prec, const double* targets,
+                        int n_targets, const ChainRec* recs,
+                        const ChunkPlan& plan, double* partial, int ld);
 
 // Partial sums partial[(c*3 + d)*ld + t] of target t over chunk c.
 void launch_nbody(hipStream_t s, int prec, int seg, const double* targets,

@@ -75,8 +75,11 @@ __device__ __forceinline__ d3 seg_exact(d3 a, d3 b, d3 x, double td) {
 }
 
 // A[L*N + K] = VORING_L(cp_K; Gamma = 1) . n_K  (simple_simulator.cpp:76-101)
+// LEGACY: ring segments node[(k+1)%4] -> node[k] summed, times Gamma = 1
+// (VortexRing::BiotSavartLaw, linear.inl:14-32).
 // Block: 256 targets x kMatRings source rings staged in LDS.
 constexpr int kMatRings = 32;
+template <bool LEGACY>
 __global__ __launch_bounds__(256) void k_matrix(const RingRec* __restrict__ recs,
                                                 const double* __restrict__ cp,
                                                 const double* __restrict__ nrm,
@@ -98,7 +101,12 @@ __global__ __launch_bounds__(256) void k_matrix(const RingRec* __restrict__ recs
     const d3 p[4] = {ld3(r), ld3(r + 3), ld3(r + 6), ld3(r + 9)};
     d3 u{0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) u = add(u, seg_exact(p[k], p[(k + 1) & 3], x, td));
+    for (int k = 0; k < 4; ++k) {
+      if (LEGACY)
+        u = add(u, seg_exact(p[(k + 1) & 3], p[k], x, td));
+      else
+        u = add(u, seg_exact(p[k], p[(k + 1) & 3], x, td));
+    }
     A[(size_t)(l0 + l) * N + K] = dot(u, nl);
   }
 }
@@ -303,7 +311,13 @@ void launch_geometry(hipStream_t s, const double* nodes, int num, int R, int C,
 void launch_matrix(hipStream_t s, const RingRec* recs, const double* cp,
                    const double* nrm, int N, double td, double* A) {
   dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
-  hipLaunchKernelGGL(k_matrix, g, dim3(256), 0, s, recs, cp, nrm, N, td, A);
+  hipLaunchKernelGGL(k_matrix<false>, g, dim3(256), 0, s, recs, cp, nrm, N, td, A);
+}
+
+void launch_ring_matrix(hipStream_t s, const RingRec* recs, const double* cp,
+                        const double* nrm, int N, double td, double* A) {
+  dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
+  hipLaunchKernelGGL(k_matrix<true>, g, dim3(256), 0, s, recs, cp, nrm, N, td, A);
 }
 
 void launch_rhs(hipStream_t s, const double* uw, const double* vm,

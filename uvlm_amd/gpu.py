@@ -277,6 +277,19 @@ class Simulation:
         self._check(self._L.uvlm_host_sim_loads(self._h, _p(out), C.byref(c2)), "loads")
         return out
 
+    def profile(self, enable: bool = True):
+        """Bracket every N-body launch with HIP events (uvlm_profile_enable)."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        self._check(N.gpu_lib().uvlm_profile_enable(ctx, int(enable)), "profile_enable")
+
+    def profile_read(self):
+        """(launches, kernel milliseconds, segment evaluations) since last read."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        n, ms, pr = C.c_size_t(), C.c_double(), C.c_double()
+        self._check(N.gpu_lib().uvlm_profile_read(ctx, C.byref(n), C.byref(ms), C.byref(pr)),
+                    "profile_read")
+        return n.value, ms.value, pr.value
+
     def pairs(self):
         e, r = C.c_double(), C.c_double()
         self._check(self._L.uvlm_host_sim_pairs(self._h, C.byref(e), C.byref(r)), "pairs")

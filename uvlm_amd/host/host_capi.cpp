@@ -165,4 +165,6 @@ int uvlm_host_sim_pairs(uvlm_host_sim* s, double* evaluated, double* reference) 
   return uvlm_sim_pair_counts(s->sim.context(), evaluated, reference);
 }
 
+uvlm_ctx* uvlm_host_sim_context(uvlm_host_sim* s) { return s ? s->sim.context() : nullptr; }
+
 }  // extern "C"
