@@ -53,6 +53,8 @@ typedef struct uvlm_ctx uvlm_ctx;
 #define UVLM_SOLVE_F64 0      /* FP64 LU, partial pivoting, FP64 MFMA updates */
 #define UVLM_SOLVE_F32_IR 1   /* FP32 LU + FP64 iterative refinement */
 #define UVLM_SOLVE_BF16_IR 2  /* BF16-input MFMA LU updates + FP64 refinement */
+#define UVLM_SOLVE_F64_FULLPIV 3 /* FP64, full pivoting in Eigen::FullPivLU's elimination
+                                   order: bitwise equal to a sequential FullPivLU */
 
 /* wake advection schemes (AdvectFactory names, advect_factory.cpp:22-27) */
 #define UVLM_ADVECT_EULER 0
@@ -107,7 +109,8 @@ int uvlm_ring_influence_matrix(uvlm_ctx* ctx, const double* rings, size_t n,
 
 /* Replaces Eigen::FullPivLU<MatrixXd>(A).solve(rhs) (simple_simulator.cpp
  * :263-268, linear.inl:125-129): A row-major [n][n].  iters_out (nullable)
- * receives the refinement sweeps of the mixed-precision modes. */
+ * receives the refinement sweeps of the mixed-precision modes, or the
+ * numerical rank (FullPivLU's default threshold) in UVLM_SOLVE_F64_FULLPIV. */
 int uvlm_solve(uvlm_ctx* ctx, size_t n, const double* A, const double* rhs,
                double* x, int mode, double tol, int max_iter, int* iters_out);
 
@@ -177,6 +180,55 @@ int uvlm_sim_get_loads(uvlm_ctx* ctx, double* out, size_t* count_inout);
  * segment evaluation) and the pairs the reference's own loop structure
  * would have evaluated for the same steps. */
 int uvlm_sim_pair_counts(uvlm_ctx* ctx, double* evaluated, double* reference);
+
+/* --- batched cases (no reference counterpart) ---------------------------
+ * The reference runs one SimpleSimulator per process (simple_simulator.cpp
+ * :288-313).  A batch advances many independent single-wing cases of
+ * different lattice sizes in lockstep: the whole timestep of every case is
+ * one fixed launch sequence (kinematics, shedding, matrix, batched LU,
+ * loads, wake advection), replayed from a hipGraph.  Kinematics follow
+ * Morphing::Perfome (morphing.cpp:23-43) with
+ *   plug(t) = plug[0] cos(plug[1] t + plug[2]),
+ *   flap(t) = flap[0] cos(flap[1] t + flap[2]),
+ * attack angle alpha, no twist or bend, origin = Morphing::origin.  Sums
+ * use UVLM_PREC_F64 segment chains; the solve is FP64 LU with partial
+ * pivoting. */
+typedef struct uvlm_batch_case {
+  size_t rows, cols; /* cols even (symmetric wing) */
+  double chord, span; /* reference area chord*span */
+  double dt, alpha;
+  double origin[3], ff[3], plug[3], flap[3];
+} uvlm_batch_case;
+
+/* Allocate a batch of n cases.  wing_nodes: the cases' reference-frame
+ * node lattices [(rows+1)][(cols+1)][3], concatenated in case order
+ * (e.g. from uvlm_host_build_wing).  At most max_steps+1 steps may run.
+ * scheme: UVLM_ADVECT_EULER, _RK2 or _RK4.  Replaces the previous batch of
+ * the context. */
+int uvlm_batch_init(uvlm_ctx* ctx, const uvlm_batch_case* cases, size_t n,
+                    const double* wing_nodes, size_t max_steps, int scheme,
+                    size_t ignore_steps);
+/* Enqueue `steps` timesteps of every case (MainLoop, simple_simulator.cpp
+ * :240-286, per case).  use_graph: replay one captured hipGraph per step.
+ * Asynchronous. */
+int uvlm_batch_run(uvlm_ctx* ctx, size_t steps, int use_graph);
+/* Time every compiled launch shape of the batch's N-body on the current
+ * state and keep the fastest.  chosen (nullable) receives its index,
+ * times_ms (nullable, max_variants entries) the measured times. */
+int uvlm_batch_autotune(uvlm_ctx* ctx, int* chosen, double* times_ms, int max_variants);
+/* Steps run so far. */
+int uvlm_batch_steps(uvlm_ctx* ctx, size_t* steps);
+/* Per-case results (synchronous): circulations [rows*cols]; wing nodes
+ * [(rows+1)(cols+1)][3]; wake nodes [steps][cols+1][3] and circulations
+ * [steps-1][cols] (MultipleSheet order, newest row first); load history
+ * [count][2] (CD, CL) with count_inout as for uvlm_sim_get_loads. */
+int uvlm_batch_get_gamma(uvlm_ctx* ctx, size_t c, double* gamma);
+int uvlm_batch_get_wing(uvlm_ctx* ctx, size_t c, double* nodes);
+int uvlm_batch_get_wake(uvlm_ctx* ctx, size_t c, double* nodes, double* gamma);
+int uvlm_batch_get_loads(uvlm_ctx* ctx, size_t c, double* out, size_t* count_inout);
+/* Segment evaluations of all steps run so far. */
+int uvlm_batch_pairs(uvlm_ctx* ctx, double* pairs);
+int uvlm_batch_free(uvlm_ctx* ctx);
 
 /* --- measurement hook (no reference counterpart) ------------------------
  * When enabled, every Biot-Savart N-body launch of the context is bracketed

@@ -81,6 +81,10 @@ int uvlm_host_sim_set_wake(uvlm_host_sim* s, size_t node_rows, size_t gamma_rows
 /* count_inout: capacity in steps in, steps recorded out; out [count][num][2]. */
 int uvlm_host_sim_loads(uvlm_host_sim* s, double* out, size_t* count_inout);
 int uvlm_host_sim_pairs(uvlm_host_sim* s, double* evaluated, double* reference);
+/* Reference-frame node lattice [(rows+1)][(cols+1)][3] of one wing, as
+ * SimpleSimulator::BuildWing lays it out (generator, WholeWing, + origin;
+ * simulator/simple_simulator.cpp:142-169): the wing_nodes input of uvlm_batch_init. */
+int uvlm_host_build_wing(const uvlm_wing_spec* w, size_t rows, size_t cols, double* nodes);
 /* The simulation's C-ABI context (for uvlm_profile_* and other calls). */
 uvlm_ctx* uvlm_host_sim_context(uvlm_host_sim* s);
 

@@ -48,9 +48,16 @@ def test_segment_bitwise(ctx, orc):
     ref = np.array([orc.biot_savart(a[k], b[k], x[k]) for k in range(n)])
     np.testing.assert_array_equal(got, ref)
     fast = ctx.biot_savart(a, b, x, N.PREC_F64)
-    scale = np.maximum(np.linalg.norm(ref, axis=1, keepdims=True), 1e-300)
-    ok = np.linalg.norm(ref, axis=1) > 0
-    np.testing.assert_allclose((fast - ref)[ok] / scale[ok], 0, atol=1e-12)
+    # FMA/rsqrt form vs the reference form: agreement limited by the
+    # conditioning of d = r1 x r2 (|r1||r2|/|d| amplifies the rounding of d)
+    r1 = np.linalg.norm(x - a, axis=1)
+    r2 = np.linalg.norm(x - b, axis=1)
+    dn = np.linalg.norm(np.cross(x - a, x - b), axis=1)
+    cond = r1 * r2 / np.maximum(dn, 1e-300)
+    scale = np.linalg.norm(ref, axis=1)
+    ok = scale > 0
+    err = np.linalg.norm(fast - ref, axis=1)
+    assert np.all(err[ok] <= 1e-14 * cond[ok] * scale[ok] + 1e-300)
 
 
 def test_reference_known_answers_on_gpu(ctx):
@@ -186,3 +193,37 @@ def test_singular_matrix_reported(ctx):
     with pytest.raises(UVLMError) as e:
         ctx.solve(np.zeros((4, 4)), np.ones(4))
     assert e.value.code == N.ERR_SINGULAR
+
+
+@pytest.mark.parametrize("n", [1, 7, 33, 128, 300])
+def test_solve_fullpiv_bitwise(ctx, orc, n):
+    """UVLM_SOLVE_F64_FULLPIV reproduces the sequential FullPivLU exactly."""
+    rng = np.random.default_rng(100 + n)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 0.5 * n ** 0.5
+    b = rng.normal(size=n)
+    x, rank = ctx.solve(A, b, mode=N.SOLVE_F64_FULLPIV)
+    xr, rr = orc.fullpiv_solve(A, b)
+    assert rank == rr == n
+    np.testing.assert_array_equal(x, xr)
+
+
+def test_solve_fullpiv_ties_and_rank(ctx, orc):
+    """Equal-magnitude pivot candidates (first in column-major order wins),
+    a zero block (early stop) and a rank-deficient matrix."""
+    A = np.array([[1.0, -3.0, 3.0, 0.0], [3.0, 1.0, -3.0, 2.0], [-3.0, 2.0, 1.0, 3.0],
+                  [2.0, 3.0, 0.0, 1.0]])
+    b = np.array([1.0, -2.0, 0.5, 4.0])
+    x, rank = ctx.solve(A, b, mode=N.SOLVE_F64_FULLPIV)
+    xr, rr = orc.fullpiv_solve(A, b)
+    assert rank == rr
+    np.testing.assert_array_equal(x, xr)
+    R = np.outer([1.0, 2.0, -1.0, 0.5], [2.0, -1.0, 1.0, 3.0])  # rank 1
+    x, rank = ctx.solve(R, b, mode=N.SOLVE_F64_FULLPIV)
+    xr, rr = orc.fullpiv_solve(R, b)
+    assert rank == rr == 1
+    np.testing.assert_array_equal(x, xr)
+    Z = np.zeros((3, 3))
+    x, rank = ctx.solve(Z, np.ones(3), mode=N.SOLVE_F64_FULLPIV)
+    xr, rr = orc.fullpiv_solve(Z, np.ones(3))
+    assert rank == rr == 0
+    np.testing.assert_array_equal(x, xr)

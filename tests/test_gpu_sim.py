@@ -97,8 +97,38 @@ def test_f32c_mode_loads_within_1e4(orc):
     assert rel_err(g.gamma(), o.gamma()) < 1e-4
 
 
-def test_plate_c1_full_run_vs_golden():
-    """The CPU-runnable workload: 8x16 plate, 200 steps, Euler."""
+def test_plate_c1_full_run_bitwise_vs_golden():
+    """The CPU-runnable workload (8x16 plate, 200 steps, Euler) in the parity
+    configuration: reference operation order in every Biot-Savart sum
+    (F64_EXACT, one sequential sum per target) and FullPivLU's elimination
+    order.  Circulations and the whole wake are bitwise those of the oracle
+    after 200 steps; the loads (GPU tree reductions) agree to 1e-13."""
+    gold = np.load(GOLDEN)
+    case = W.plate_c1(steps=200)
+    case["precision"] = N.PREC_F64_EXACT | N.SEQUENTIAL
+    case["solve_mode"] = N.SOLVE_F64_FULLPIV
+    g = Simulation(case, 200)
+    snaps = list(gold["snap_steps"])
+    for s in range(201):
+        g.step(s)
+        if s in snaps:
+            np.testing.assert_array_equal(g.gamma(), gold["gammas"][snaps.index(s)], err_msg=str(s))
+    g.sync()
+    gn, gg = g.wake()
+    np.testing.assert_array_equal(gn, gold["wake_nodes"])
+    np.testing.assert_array_equal(gg, gold["wake_gamma"])
+    assert rel_err(g.loads(), gold["loads"]) < 1e-13
+    ev, ref = g.pairs()
+    assert ref == pytest.approx(float(gold["pairs"]))
+
+
+def test_plate_c1_default_mode_vs_golden():
+    """The default configuration (source-split sums, partially pivoted
+    blocked LU) rounds differently from the oracle at the 1e-16 level.  The
+    plate's wake roll-up amplifies such differences after ~150 steps (the
+    oracle run with partial instead of full pivoting departs from itself by
+    5e-9 at step 200; test_oracle_wake_sensitivity), so the bar is 1e-10 up
+    to step 150 and 1e-5 at step 200."""
     gold = np.load(GOLDEN)
     case = W.plate_c1(steps=200)
     g = Simulation(case, 200)
@@ -106,16 +136,12 @@ def test_plate_c1_full_run_vs_golden():
     for s in range(201):
         g.step(s)
         if s in snaps:
-            k = snaps.index(s)
-            assert rel_err(g.gamma(), gold["gammas"][k]) < TOL, s
+            tol = TOL if s <= 150 else 1e-5
+            assert rel_err(g.gamma(), gold["gammas"][snaps.index(s)]) < tol, s
     g.sync()
-    gn, gg = g.wake()
-    assert rel_err(gn, gold["wake_nodes"]) < TOL
-    assert rel_err(gg, gold["wake_gamma"]) < TOL
     gl = g.loads()
-    assert rel_err(gl, gold["loads"]) < TOL
-    ev, ref = g.pairs()
-    assert ref == pytest.approx(float(gold["pairs"]))
+    assert rel_err(gl[:151], gold["loads"][:151]) < TOL
+    assert rel_err(gl, gold["loads"]) < 1e-5
 
 
 def test_restore_state_continues_identically(orc):

@@ -18,7 +18,7 @@ OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_SINGULAR, ERR_STATE, ERR_CAPACITY, ERR_NOCONV
     0, -1, -2, -3, -4, -5, -6, -7)
 PREC_F64_EXACT, PREC_F64, PREC_F32C = 0x0, 0x1, 0x2
 SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
-SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR = 0, 1, 2
+SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV = 0, 1, 2, 3
 ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
 SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
            "AB2": ADVECT_AB2, "RK4classic": ADVECT_RK4_CLASSIC}
@@ -35,13 +35,16 @@ GPU_SYMBOLS = [
     "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
+    "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
+    "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
+    "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_free",
 ]
 HOST_SYMBOLS = [
     "uvlm_host_sim_create", "uvlm_host_sim_destroy", "uvlm_host_last_error",
     "uvlm_host_sim_step", "uvlm_host_sim_sync", "uvlm_host_sim_wing_nodes",
     "uvlm_host_sim_gamma", "uvlm_host_sim_wake_shape", "uvlm_host_sim_wake",
     "uvlm_host_sim_set_wake", "uvlm_host_sim_loads", "uvlm_host_sim_pairs",
-    "uvlm_host_sim_context",
+    "uvlm_host_sim_context", "uvlm_host_build_wing",
 ]
 
 
@@ -65,6 +68,17 @@ class CaseSpec(C.Structure):
         ("ff", C.c_double * 3), ("dt", C.c_double), ("scheme", C.c_int),
         ("ignore_steps", C.c_size_t), ("erase_wake_after", C.c_size_t),
         ("loads", C.c_int), ("precision", C.c_uint), ("solve_mode", C.c_int),
+    ]
+
+
+class BatchCase(C.Structure):
+    """uvlm_batch_case (include/uvlm_gpu.h)."""
+    _fields_ = [
+        ("rows", C.c_size_t), ("cols", C.c_size_t),
+        ("chord", C.c_double), ("span", C.c_double),
+        ("dt", C.c_double), ("alpha", C.c_double),
+        ("origin", C.c_double * 3), ("ff", C.c_double * 3),
+        ("plug", C.c_double * 3), ("flap", C.c_double * 3),
     ]
 
 
@@ -120,6 +134,16 @@ def gpu_lib():
         L.uvlm_sim_pair_counts.argtypes = [_vp, _dp, _dp]
         L.uvlm_profile_enable.argtypes = [_vp, C.c_int]
         L.uvlm_profile_read.argtypes = [_vp, C.POINTER(_sz), _dp, _dp]
+        L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
+        L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
+        L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]
+        L.uvlm_batch_steps.argtypes = [_vp, C.POINTER(_sz)]
+        L.uvlm_batch_get_gamma.argtypes = [_vp, _sz, _dp]
+        L.uvlm_batch_get_wing.argtypes = [_vp, _sz, _dp]
+        L.uvlm_batch_get_wake.argtypes = [_vp, _sz, _dp, _dp]
+        L.uvlm_batch_get_loads.argtypes = [_vp, _sz, _dp, C.POINTER(_sz)]
+        L.uvlm_batch_pairs.argtypes = [_vp, _dp]
+        L.uvlm_batch_free.argtypes = [_vp]
         _gpu = L
     return _gpu
 
@@ -143,6 +167,7 @@ def host_lib():
         L.uvlm_host_sim_loads.argtypes = [_vp, _dp, C.POINTER(_sz)]
         L.uvlm_host_sim_pairs.argtypes = [_vp, _dp, _dp]
         L.uvlm_host_sim_context.argtypes = [_vp]
+        L.uvlm_host_build_wing.argtypes = [C.POINTER(WingSpec), _sz, _sz, _dp]
         L.uvlm_host_sim_context.restype = _vp
         _host = L
     return _host

@@ -1,0 +1,97 @@
+// batch.h -- the batched multi-case engine (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace uvlm {
+
+// Per-case constants in device memory.  Offsets are in points (3 doubles)
+// for node/ring/edge/wake arrays and in elements otherwise.
+struct BCase {
+  int R, C, N, E;
+  int cap;  // wake row capacity (node rows; Gamma rows use the same)
+  int pad0;
+  long node_off, ring_off, edge_off, A_off, wake_off, wgam_off, chb_off, chw_off;
+  double dt, Q, ref_area, alpha;
+  double ff[3], origin[3], plug[3], flap[3];
+};
+
+// Device-resident step counter and the work counters of the persistent
+// N-body launches.
+struct BState {
+  int step;
+  int pad;
+  unsigned ctr[8];
+};
+
+// One case as handed in by the C-ABI (see uvlm_batch_case in uvlm_gpu.h).
+struct BatchCaseSpec {
+  size_t rows, cols;
+  double chord, span, dt, alpha;
+  double origin[3], ff[3], plug[3], flap[3];
+};
+
+class BatchEngine {
+ public:
+  ~BatchEngine();
+  int init(hipStream_t s, const BatchCaseSpec* specs, const double* wing_init, size_t n,
+           size_t max_steps, int scheme, int ignore_steps, std::string* err);
+  // Enqueue `steps` timesteps of every case; with use_graph the step is
+  // captured once into a hipGraph and replayed.
+  int run(size_t steps, bool use_graph, std::string* err);
+  // Time every N-body launch shape on the current state and keep the best.
+  int autotune(std::string* err, int* chosen, double* times_ms, int max_variants);
+  int steps_done() const { return steps_done_; }
+  int n_cases() const { return n_; }
+  const BCase& case_info(int c) const { return cases_h_[c]; }
+  int get_gamma(int c, double* out, std::string* err);
+  int get_wake(int c, double* nodes, double* gamma, std::string* err);
+  int get_loads(int c, double* out, size_t* count, std::string* err);
+  int get_wing(int c, double* out, std::string* err);
+  double pairs_per_step_at(int step) const;  // segment evaluations of one step
+
+ private:
+  void release();
+  int build_items(int tile, std::string* err);
+  int step_once(std::string* err);
+  int nbody(int phase, const double* targets, double* out, int slot);
+  int capture(std::string* err);
+
+  hipStream_t stream_ = nullptr;
+  int n_ = 0, max_steps_ = 0, scheme_ = 0, ign_ = 5, steps_done_ = 0, maxN_ = 0;
+  int cus_ = 256, variant_ = 1, items_tile_ = 0;
+  long tot_nodes_ = 0, tot_rings_ = 0, tot_edges_ = 0, tot_wake_ = 0, tot_cb_ = 0, tot_cw_ = 0,
+       tot_te_ = 0;
+  std::vector<BCase> cases_h_;
+  std::vector<int> gemm_off_;
+  std::vector<void*> allocs_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  int n_rhs_ = 0, n_loads_ = 0, n_adv_ = 0, n_mat_ = 0;
+  // device arrays
+  BCase* d_cases_ = nullptr;
+  BState* d_state_ = nullptr;
+  int *d_node_case_ = nullptr, *d_ring_case_ = nullptr, *d_edge_case_ = nullptr,
+      *d_wn_case_ = nullptr, *d_te_case_ = nullptr, *d_te_j_ = nullptr;
+  long *d_chb_start_ = nullptr, *d_chw_start_ = nullptr;
+  double *d_init_ = nullptr, *d_wing_ = nullptr, *d_cp0_ = nullptr, *d_mid0_ = nullptr,
+         *d_vm_ = nullptr, *d_vme_ = nullptr, *d_gamma_ = nullptr, *d_gprev_ = nullptr,
+         *d_rhs_ = nullptr, *d_uw_ = nullptr, *d_cp_ = nullptr, *d_nrm_ = nullptr,
+         *d_area_ = nullptr, *d_mid_ = nullptr, *d_uflow_ = nullptr, *d_A_ = nullptr,
+         *d_wake_ = nullptr, *d_wgam_ = nullptr, *d_pos_ = nullptr, *d_pst_ = nullptr,
+         *d_k1_ = nullptr, *d_k2_ = nullptr, *d_k3_ = nullptr, *d_hist_ = nullptr;
+  int* d_piv_ = nullptr;
+  RingRec* d_rings_ = nullptr;
+  ChainRec *d_bch_ = nullptr, *d_wch_ = nullptr;
+  int2 *d_it_rhs_ = nullptr, *d_it_loads_ = nullptr, *d_it_adv_ = nullptr;
+  int4 *d_it_mat_ = nullptr, *d_it_gemm_ = nullptr;
+  size_t cap_rhs_ = 0, cap_loads_ = 0, cap_adv_ = 0;
+  std::vector<int> order_;
+};
+
+}  // namespace uvlm

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/uvlm_gpu.h"
+#include "batch.h"
 #include "internal.h"
 #include "step.h"
 
@@ -61,6 +62,9 @@ struct uvlm_ctx {
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
   std::vector<double> ev_pairs;
+  // batched cases
+  uvlm::BatchEngine* batch = nullptr;
+  double batch_pairs = 0.0;
 };
 
 namespace {
@@ -278,6 +282,8 @@ int uvlm_destroy(uvlm_ctx* c) {
   if (!c) return UVLM_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  delete c->batch;
+  c->batch = nullptr;
   Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
                  &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
                  &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
@@ -552,6 +558,19 @@ int uvlm_solve(uvlm_ctx* c, size_t n, const double* A, const double* rhs,
     if (info != 0) return fail(c, UVLM_ERR_SINGULAR, "Matrix is not invertible (zero pivot at column %d)", info - 1);
     return UVLM_OK;
   }
+  if (mode == UVLM_SOLVE_F64_FULLPIV) {
+    TRY(ensure(c, c->t_work, dense_fullpiv_work_bytes((int)n)));
+    dense_fullpiv_solve(c->stream, P<double>(c->t_A), (int)n, db, dx, c->t_work.p,
+                        c->t_work.cap, P<int>(c->t_info));
+    TRY(check_launch(c));
+    int rank = 0;
+    HIPCHK(c, hipMemcpyAsync(&rank, c->t_info.p, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (iters_out) *iters_out = rank;
+    return UVLM_OK;
+  }
   if (mode != UVLM_SOLVE_F32_IR && mode != UVLM_SOLVE_BF16_IR)
     return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", mode);
   TRY(ensure(c, c->t_work, dense_mixed_work_bytes((int)n)));
@@ -735,6 +754,10 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
     dense_lu_solve(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<double>(c->rhs));
     HIPCHK(c, hipMemcpyAsync(c->gamma.p, c->rhs.p, sizeof(double) * N,
                              hipMemcpyDeviceToDevice, c->stream));
+  } else if (solve_mode == UVLM_SOLVE_F64_FULLPIV) {
+    TRY(ensure(c, c->work, dense_fullpiv_work_bytes(N)));
+    dense_fullpiv_solve(c->stream, P<double>(c->A), N, P<double>(c->rhs), P<double>(c->gamma),
+                        c->work.p, c->work.cap, nullptr);
   } else if (solve_mode == UVLM_SOLVE_F32_IR || solve_mode == UVLM_SOLVE_BF16_IR) {
     TRY(ensure(c, c->work, dense_mixed_work_bytes(N)));
     int it = 0;
@@ -1003,6 +1026,117 @@ int uvlm_sim_pair_counts(uvlm_ctx* c, double* evaluated, double* reference) {
   if (!c) return UVLM_ERR_ARG;
   if (evaluated) *evaluated = c->pairs_eval;
   if (reference) *reference = c->pairs_ref;
+  return UVLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int uvlm_batch_init(uvlm_ctx* c, const uvlm_batch_case* cases, size_t n,
+                    const double* wing_nodes, size_t max_steps, int scheme,
+                    size_t ignore_steps) {
+  if (!c || !cases || !wing_nodes || n == 0) return UVLM_ERR_ARG;
+  if (n > (1u << 20) || max_steps > (1u << 20) || ignore_steps > (1u << 20))
+    return fail(c, UVLM_ERR_ARG, "batch size or step count out of range");
+  TRY(set_dev(c));
+  delete c->batch;
+  c->batch = new (std::nothrow) BatchEngine();
+  if (!c->batch) return fail(c, UVLM_ERR_OOM, "batch engine");
+  c->batch_pairs = 0.0;
+  static_assert(sizeof(BatchCaseSpec) == sizeof(uvlm_batch_case), "batch case layout");
+  std::string e;
+  const int rc = c->batch->init(c->stream, reinterpret_cast<const BatchCaseSpec*>(cases),
+                                wing_nodes, n, max_steps, scheme, (int)ignore_steps, &e);
+  if (rc != UVLM_OK) {
+    delete c->batch;
+    c->batch = nullptr;
+    return fail(c, rc, "%s", e.c_str());
+  }
+  return UVLM_OK;
+}
+
+namespace {
+int need_batch(uvlm_ctx* c) {
+  if (!c->batch) return fail(c, UVLM_ERR_STATE, "uvlm_batch_init not called");
+  return set_dev(c);
+}
+int need_case(uvlm_ctx* c, size_t k) {
+  TRY(need_batch(c));
+  if (k >= (size_t)c->batch->n_cases())
+    return fail(c, UVLM_ERR_ARG, "case %zu out of range (%d cases)", k, c->batch->n_cases());
+  return UVLM_OK;
+}
+}  // namespace
+
+int uvlm_batch_run(uvlm_ctx* c, size_t steps, int use_graph) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(need_batch(c));
+  const int s0 = c->batch->steps_done();
+  std::string e;
+  const int rc = c->batch->run(steps, use_graph != 0, &e);
+  for (int s = s0; s < c->batch->steps_done(); ++s) c->batch_pairs += c->batch->pairs_per_step_at(s);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  return UVLM_OK;
+}
+
+int uvlm_batch_autotune(uvlm_ctx* c, int* chosen, double* times_ms, int max_variants) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(need_batch(c));
+  std::string e;
+  const int rc = c->batch->autotune(&e, chosen, times_ms, max_variants);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  return UVLM_OK;
+}
+
+int uvlm_batch_steps(uvlm_ctx* c, size_t* steps) {
+  if (!c || !steps) return UVLM_ERR_ARG;
+  TRY(need_batch(c));
+  *steps = (size_t)c->batch->steps_done();
+  return UVLM_OK;
+}
+
+int uvlm_batch_get_gamma(uvlm_ctx* c, size_t k, double* gamma) {
+  if (!c || !gamma) return UVLM_ERR_ARG;
+  TRY(need_case(c, k));
+  std::string e;
+  const int rc = c->batch->get_gamma((int)k, gamma, &e);
+  return rc == UVLM_OK ? rc : fail(c, rc, "%s", e.c_str());
+}
+
+int uvlm_batch_get_wing(uvlm_ctx* c, size_t k, double* nodes) {
+  if (!c || !nodes) return UVLM_ERR_ARG;
+  TRY(need_case(c, k));
+  std::string e;
+  const int rc = c->batch->get_wing((int)k, nodes, &e);
+  return rc == UVLM_OK ? rc : fail(c, rc, "%s", e.c_str());
+}
+
+int uvlm_batch_get_wake(uvlm_ctx* c, size_t k, double* nodes, double* gamma) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(need_case(c, k));
+  std::string e;
+  const int rc = c->batch->get_wake((int)k, nodes, gamma, &e);
+  return rc == UVLM_OK ? rc : fail(c, rc, "%s", e.c_str());
+}
+
+int uvlm_batch_get_loads(uvlm_ctx* c, size_t k, double* out, size_t* count_inout) {
+  if (!c || !count_inout) return UVLM_ERR_ARG;
+  TRY(need_case(c, k));
+  std::string e;
+  const int rc = c->batch->get_loads((int)k, out, count_inout, &e);
+  return rc == UVLM_OK ? rc : fail(c, rc, "%s", e.c_str());
+}
+
+int uvlm_batch_pairs(uvlm_ctx* c, double* pairs) {
+  if (!c || !pairs) return UVLM_ERR_ARG;
+  TRY(need_batch(c));
+  *pairs = c->batch_pairs;
+  return UVLM_OK;
+}
+
+int uvlm_batch_free(uvlm_ctx* c) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  delete c->batch;
+  c->batch = nullptr;
   return UVLM_OK;
 }
 

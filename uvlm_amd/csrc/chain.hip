@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "devmath.h"
 #include "internal.h"
 
 namespace uvlm {
@@ -28,23 +29,6 @@ constexpr int kBlock = 256;
 constexpr int kTile = 128;  // chain elements per LDS tile (8 KB)
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
-
-__device__ __forceinline__ double rsq64(double a) {
-  double y = __builtin_amdgcn_rsq(a);
-  double h = a * y;
-  double e = __builtin_fma(-h, y, 1.0);
-  y = __builtin_fma(0.5 * y, e, y);
-  h = a * y;
-  e = __builtin_fma(-h, y, 1.0);
-  return __builtin_fma(0.5 * y, e, y);
-}
-__device__ __forceinline__ double rcp64(double a) {
-  double r = __builtin_amdgcn_rcp(a);
-  double e = __builtin_fma(-a, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-a, r, 1.0);
-  return __builtin_fma(r, e, r);
-}
 
 // BiotSavartLaw of segment a->b at x, times g (vortex.cpp:21-39 formula,
 // FP64, used for the near-core pairs of F32C).
@@ -97,14 +81,15 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     z = targets[3 * t + 2];
   }
   // carried previous node
-  double rax = 0.0, ray = 0.0, raz = 0.0, ras = 0.0, ia = 0.0;
+  dev::Carry64 cr;
+  dev::carry_reset(cr);
   if (rb > 0) {
     const double* p = recs[rb - 1].p;
-    rax = x - p[0];
-    ray = y - p[1];
-    raz = z - p[2];
-    ras = __builtin_fma(raz, raz, __builtin_fma(ray, ray, rax * rax));
-    ia = rsq64(ras);
+    cr.rx = x - p[0];
+    cr.ry = y - p[1];
+    cr.rz = z - p[2];
+    cr.rs = __builtin_fma(cr.rz, cr.rz, __builtin_fma(cr.ry, cr.ry, cr.rx * cr.rx));
+    cr.inv = dev::rsq64(cr.rs);
   }
   double ux = 0.0, uy = 0.0, uz = 0.0;
   const double2* src = reinterpret_cast<const double2*>(recs);
@@ -114,32 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     for (int q = threadIdx.x; q < cnt * 4; q += kBlock) tile[q] = src[(size_t)base * 4 + q];
     __syncthreads();
     const double* tl = reinterpret_cast<const double*>(tile);
-    for (int k = 0; k < cnt; ++k) {
-      const double* e = tl + 8 * k;
-      const double rbx = x - e[0], rby = y - e[1], rbz = z - e[2];
-      const double rbs = __builtin_fma(rbz, rbz, __builtin_fma(rby, rby, rbx * rbx));
-      const double ib = rsq64(rbs);
-      const double r0x = e[4], r0y = e[5], r0z = e[6];
-      // d = r1 x r2 = r0 x (x - a)
-      const double dx = __builtin_fma(r0y, raz, -r0z * ray);
-      const double dy = __builtin_fma(r0z, rax, -r0x * raz);
-      const double dz = __builtin_fma(r0x, ray, -r0y * rax);
-      const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-      const double oa = __builtin_fma(r0z, raz, __builtin_fma(r0y, ray, r0x * rax));
-      const double ob = __builtin_fma(r0z, rbz, __builtin_fma(r0y, rby, r0x * rbx));
-      const double tt = __builtin_fma(oa, ia, -ob * ib);
-      const bool cut = (dsq < kCutoff) | (ras < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
-      double K = tt * e[3] * rcp64(dsq);
-      K = cut ? 0.0 : K;
-      ux = __builtin_fma(dx, K, ux);
-      uy = __builtin_fma(dy, K, uy);
-      uz = __builtin_fma(dz, K, uz);
-      rax = rbx;
-      ray = rby;
-      raz = rbz;
-      ras = rbs;
-      ia = ib;
-    }
+    for (int k = 0; k < cnt; ++k) dev::chain_step64(tl + 8 * k, x, y, z, cr, ux, uy, uz);
   }
   if (t < nt) {
     const size_t c = blockIdx.y;
@@ -202,9 +162,10 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
         const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
         const float tt = fmaf(oa, ia, -ob * ib);
-        // FP32 rounding bound of |d|^2 for the near-core band
-        const float derr = 4.f * kEps32 * (float)e[7] * ras;
-        const bool near = (dsq < 1e-9f + 16.f * derr) | (ras < 1e-18f) | (rbs < 1e-18f);
+        // Near-core band: |d| < sqrt(10) * 1e-5 plus the FP32 rounding bound
+        // of d (~5 eps |r0| |x-a|), in squares: (a+b)^2 <= 2a^2 + 2b^2.
+        const float derr2 = 64.f * kEps32 * kEps32 * (float)e[7] * ras;
+        const bool near = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
         float K = tt * (float)e[3] * __builtin_amdgcn_rcpf(dsq);
         K = near ? 0.f : K;
         fx = fmaf(dx, K, fx);

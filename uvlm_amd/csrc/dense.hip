@@ -8,7 +8,7 @@
 // Right-looking blocked LU, panel width kNB = 32, column-major:
 //   k_getf2   panel factorisation by one 1024-thread workgroup
 //   k_laswp   row interchanges applied to the columns outside the panel
-//   k_trsm    U12 = L11^-1 A12 (one lane per column, L11 in LDS)
+//   k_trsm    U12 = L11^-1 A12 (one lane per column, L11 and X in LDS)
 //   k_gemm    A22 -= L21 U12 on the FP64 matrix cores
 //             (v_mfma_f64_16x16x4f64, 64x64 tile per workgroup)
 // followed by k_getrs (pivots, unit-lower and upper substitution) in one
@@ -16,6 +16,7 @@
 // BF16-rounded MFMA inputs) and refine the FP64 solution with FP64
 // residuals r = b - A x.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "internal.h"
@@ -124,39 +125,40 @@ __global__ void k_laswp(T* A, int lda, int n, int k0, int kb,
   }
 }
 
-// U12 = L11^-1 A12 for the columns [k0+kb, n).
+// U12 = L11^-1 A12 for the columns [k0+kb, n): one wavefront per 64
+// columns, one lane per column; L11 and the 64 partial solutions in LDS
+// (a register-resident solution vector would need a fully unrolled 32x32
+// triangle, which the compiler spills).
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm(T* A, int lda, int n, int k0,
-                                              int kb) {
+__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0,
+                                             int kb) {
   __shared__ T L[kNB][kNB + 1];
-  for (int q = threadIdx.x; q < kNB * kNB; q += 256) {
+  __shared__ T X[kNB][64 + 1];
+  for (int q = threadIdx.x; q < kNB * kNB; q += 64) {
     const int r = q % kNB, c = q / kNB;
     L[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
   }
-  __syncthreads();
-  const int col = k0 + kb + blockIdx.x * 256 + threadIdx.x;
-  if (col >= n) return;
+  const int t = threadIdx.x;
+  const int col = k0 + kb + blockIdx.x * 64 + t;
   T* a = A + (size_t)col * lda + k0;
-  T x[kNB];
-#pragma unroll
-  for (int i = 0; i < kNB; ++i) x[i] = i < kb ? a[i] : T(0);
-#pragma unroll
-  for (int i = 1; i < kNB; ++i) {
-    T s = x[i];
-#pragma unroll
-    for (int j = 0; j < i; ++j) s = s - L[i][j] * x[j];
-    x[i] = s;
+  if (col < n)
+    for (int i = 0; i < kb; ++i) X[i][t] = a[i];
+  __syncthreads();
+  if (col >= n) return;
+  for (int j = 0; j < kb; ++j) {
+    const T xj = X[j][t];
+    for (int i = j + 1; i < kb; ++i) X[i][t] = X[i][t] - L[i][j] * xj;
   }
-#pragma unroll
-  for (int i = 0; i < kNB; ++i)
-    if (i < kb) a[i] = x[i];
+  for (int i = 0; i < kb; ++i) a[i] = X[i][t];
 }
 
 // A22 -= L21 * U12 with the FP64 matrix cores.  The product is formed
 // transposed (D^T = U12^T L21^T) so that each group of 16 lanes holds 16
 // consecutive rows of one column: stores are contiguous in column-major A.
 // v_mfma_f64_16x16x4f64 operand layout: a[lane] = X[lane%16][lane/16],
-// b[lane] = Y[lane/16][lane%16], d[r] = D[4*(lane/16)+r][lane%16].
+// b[lane] = Y[lane/16][lane%16], d[r] = D[4*r+lane/16][lane%16] (the
+// result rows interleave across the four lane groups; the f32 variant
+// below keeps them blocked: d[r] = D[4*(lane/16)+r][lane%16]).
 __global__ __launch_bounds__(256) void k_gemm_f64(double* A, int lda, int n,
                                                   int k0, int kb) {
   __shared__ double sA[64][kNB + 1];  // L21 tile: rows x k
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double* A, int lda, int n,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wr * 32 + i * 16 + l16;
-        const int col = c0 + wc * 32 + j * 16 + 4 * lq + r;
+        const int col = c0 + wc * 32 + j * 16 + 4 * r + lq;
         if (row < n && col < n) {
           double* p = A + (size_t)col * lda + row;
           *p = *p - acc[i][j][r];
@@ -345,7 +347,7 @@ int lu_factor_t(hipStream_t s, T* A, int n, int* ipiv, int* info) {
                          A, n, n, k0, kb, (const int*)ipiv);
     const int rest = n - k0 - kb;
     if (rest > 0) {
-      hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 256)), dim3(256), 0, s, A,
+      hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A,
                          n, n, k0, kb);
       dim3 g(cdiv(rest, 64), cdiv(rest, 64));
       if constexpr (GEMM == 0) {
@@ -422,6 +424,189 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b
   if (i < n) b[i] = (double)a[i];
 }
 
+// --- full pivoting in the elimination order of Eigen::FullPivLU ------------
+// (simple_simulator.cpp:265-268 calls FullPivLU<MatrixXd>(A).solve(rhs)).
+// Every entry of the factors is produced by the same single IEEE operation
+// as in a sequential elimination (the rank-1 update is elementwise), so with
+// identical pivots the factors and the solution are bitwise reproducible.
+// The pivot of step k is the first maximum of |m| over the trailing block in
+// column-major scan order.  Per step: k_fp_argmax (grid partials),
+// k_fp_pivot (reduce, row and column interchange, column scale),
+// k_fp_update (rank-1 update); then k_fp_solve in one workgroup.
+struct FpState {
+  double maxpivot;
+  int nonzero, done;
+};
+constexpr int kFpParts = 512;
+
+__global__ void k_fp_init(FpState* st, int n) {
+  st->maxpivot = 0.0;
+  st->nonzero = n;
+  st->done = 0;
+}
+
+__device__ __forceinline__ void fp_better(double& v, long& q, double ov, long oq) {
+  if (ov > v || (ov == v && oq < q)) {
+    v = ov;
+    q = oq;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fp_argmax(const double* __restrict__ m, int n, int k,
+                                                   const FpState* __restrict__ st,
+                                                   double* __restrict__ pv,
+                                                   long* __restrict__ pq) {
+  if (st->done) return;
+  const long w = n - k, total = w * w;
+  double best = -1.0;
+  long bq = LONG_MAX;
+  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
+    const double a = fabs(m[(size_t)(k + q / w) * n + k + q % w]);
+    if (a > best) {
+      best = a;
+      bq = q;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    fp_better(best, bq, __shfl_down(best, off), __shfl_down(bq, off));
+  __shared__ double sv[4];
+  __shared__ long sq[4];
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = best;
+    sq[threadIdx.x >> 6] = bq;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w2 = 1; w2 < 4; ++w2) fp_better(best, bq, sv[w2], sq[w2]);
+    pv[blockIdx.x] = best;
+    pq[blockIdx.x] = bq;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_fp_pivot(double* __restrict__ m, int n, int k,
+                                                   int parts, FpState* __restrict__ st,
+                                                   const double* __restrict__ pv,
+                                                   const long* __restrict__ pq,
+                                                   int* __restrict__ rt, int* __restrict__ ct) {
+  if (st->done) return;
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  long bq = LONG_MAX;
+  for (int p = tid; p < parts; p += 1024) fp_better(best, bq, pv[p], pq[p]);
+  for (int off = 32; off > 0; off >>= 1)
+    fp_better(best, bq, __shfl_down(best, off), __shfl_down(bq, off));
+  __shared__ double sv[16];
+  __shared__ long sq[16];
+  __shared__ int s_br, s_bc, s_stop;
+  if ((tid & 63) == 0) {
+    sv[tid >> 6] = best;
+    sq[tid >> 6] = bq;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w2 = 1; w2 < 16; ++w2) fp_better(best, bq, sv[w2], sq[w2]);
+    const long w = n - k;
+    s_stop = best == 0.0;
+    if (best == 0.0) {
+      st->nonzero = k;
+      st->done = 1;
+    } else {
+      if (best > st->maxpivot) st->maxpivot = best;
+      if (bq == LONG_MAX) bq = 0;  // no comparable entry (NaN block): keep (k, k)
+      s_br = k + (int)(bq % w);
+      s_bc = k + (int)(bq / w);
+      rt[k] = s_br;
+      ct[k] = s_bc;
+    }
+  }
+  __syncthreads();
+  if (s_stop) {
+    for (int i = k + tid; i < n; i += 1024) rt[i] = ct[i] = i;
+    return;
+  }
+  const int br = s_br, bc = s_bc;
+  if (br != k)
+    for (int cc = tid; cc < n; cc += 1024) {
+      const double t = m[(size_t)cc * n + k];
+      m[(size_t)cc * n + k] = m[(size_t)cc * n + br];
+      m[(size_t)cc * n + br] = t;
+    }
+  __syncthreads();
+  if (bc != k)
+    for (int r = tid; r < n; r += 1024) {
+      const double t = m[(size_t)k * n + r];
+      m[(size_t)k * n + r] = m[(size_t)bc * n + r];
+      m[(size_t)bc * n + r] = t;
+    }
+  __syncthreads();
+  const double piv = m[(size_t)k * n + k];
+  for (int r = k + 1 + tid; r < n; r += 1024) m[(size_t)k * n + r] /= piv;
+}
+
+__global__ __launch_bounds__(256) void k_fp_update(double* __restrict__ m, int n, int k,
+                                                   const FpState* __restrict__ st) {
+  if (st->done) return;
+  const long w = n - k - 1, total = w * w;
+  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
+    const int cc = k + 1 + (int)(q / w), r = k + 1 + (int)(q % w);
+    const double u = m[(size_t)cc * n + k];
+    m[(size_t)cc * n + r] -= m[(size_t)k * n + r] * u;
+  }
+}
+
+// Rank with FullPivLU's default threshold, then P, unit-lower, upper on the
+// rank x rank block, Q -- in the order of a sequential solve.
+__global__ __launch_bounds__(1024) void k_fp_solve(const double* __restrict__ m, int n,
+                                                   const FpState* __restrict__ st,
+                                                   const int* __restrict__ rt,
+                                                   const int* __restrict__ ct,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ c, int* rank_out) {
+  const int tid = threadIdx.x;
+  const double thr = st->maxpivot * (2.220446049250313e-16 * (double)n);
+  int cnt = 0;
+  for (int k = tid; k < st->nonzero; k += 1024) cnt += fabs(m[(size_t)k * n + k]) > thr;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
+  __shared__ int s_cnt[16];
+  __shared__ int s_rank;
+  if ((tid & 63) == 0) s_cnt[tid >> 6] = cnt;
+  for (int i = tid; i < n; i += 1024) c[i] = b[i];
+  __syncthreads();
+  if (tid == 0) {
+    int r = 0;
+    for (int w2 = 0; w2 < 16; ++w2) r += s_cnt[w2];
+    s_rank = r;
+    if (rank_out) *rank_out = r;
+    for (int k = 0; k < n; ++k) {
+      const double t = c[k];
+      c[k] = c[rt[k]];
+      c[rt[k]] = t;
+    }
+  }
+  __syncthreads();
+  const int rank = s_rank;
+  for (int k = 0; k < n; ++k) {
+    const double ck = c[k];
+    for (int r = k + 1 + tid; r < n; r += 1024) c[r] -= m[(size_t)k * n + r] * ck;
+    __syncthreads();
+  }
+  for (int kk = rank - 1; kk >= 0; --kk) {
+    if (tid == 0) c[kk] /= m[(size_t)kk * n + kk];
+    __syncthreads();
+    const double ck = c[kk];
+    for (int r = tid; r < kk; r += 1024) c[r] -= m[(size_t)kk * n + r] * ck;
+    __syncthreads();
+  }
+  for (int k = rank + tid; k < n; k += 1024) c[k] = 0.0;
+  __syncthreads();
+  if (tid == 0)
+    for (int k = n - 1; k >= 0; --k) {
+      const double t = c[k];
+      c[k] = c[ct[k]];
+      c[ct[k]] = t;
+    }
+}
+
 }  // namespace
 
 int dense_lu_factor(hipStream_t s, double* A, int n, int* ipiv, int* info_dev) {
@@ -486,6 +671,43 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   }
   if (iters) *iters = it;
   if (!(h_norms[0] <= tol * h_norms[1])) return 1;
+  return 0;
+}
+
+size_t dense_fullpiv_work_bytes(int n) {
+  return 256 + kFpParts * (sizeof(double) + sizeof(long)) + 2 * (size_t)n * sizeof(int) + 64;
+}
+
+// A (column-major, destroyed) x = b with the elimination order of
+// Eigen::FullPivLU; rank_dev (nullable) receives the numerical rank.
+int dense_fullpiv_solve(hipStream_t s, double* A, int n, const double* b, double* x,
+                        void* work, size_t work_bytes, int* rank_dev) {
+  if (work_bytes < dense_fullpiv_work_bytes(n)) return -1;
+  char* w = (char*)work;
+  FpState* st = (FpState*)w;
+  w += 256;
+  double* pv = (double*)w;
+  w += kFpParts * sizeof(double);
+  long* pq = (long*)w;
+  w += kFpParts * sizeof(long);
+  int* rt = (int*)w;
+  w += (size_t)n * sizeof(int);
+  int* ct = (int*)w;
+  hipLaunchKernelGGL(k_fp_init, dim3(1), dim3(1), 0, s, st, n);
+  for (int k = 0; k < n; ++k) {
+    const long wk = n - k;
+    const int parts = (int)std::min<long>(kFpParts, cdiv(wk * wk, 256));
+    hipLaunchKernelGGL(k_fp_argmax, dim3(parts), dim3(256), 0, s, (const double*)A, n, k,
+                       (const FpState*)st, pv, pq);
+    hipLaunchKernelGGL(k_fp_pivot, dim3(1), dim3(1024), 0, s, A, n, k, parts, st,
+                       (const double*)pv, (const long*)pq, rt, ct);
+    const long wu = n - k - 1;
+    if (wu > 0)
+      hipLaunchKernelGGL(k_fp_update, dim3((int)std::min<long>(2048, cdiv(wu * wu, 256))),
+                         dim3(256), 0, s, A, n, k, (const FpState*)st);
+  }
+  hipLaunchKernelGGL(k_fp_solve, dim3(1), dim3(1024), 0, s, (const double*)A, n,
+                     (const FpState*)st, (const int*)rt, (const int*)ct, b, x, rank_dev);
   return 0;
 }
 

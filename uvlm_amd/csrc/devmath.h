@@ -1,0 +1,70 @@
+// devmath.h -- device arithmetic shared by the chain N-body kernels.
+#pragma once
+
+#include "internal.h"
+
+namespace uvlm {
+namespace dev {
+
+// Refined hardware reciprocal square root / reciprocal (two Newton steps
+// each: full FP64 accuracy from the hardware seed).
+__device__ __forceinline__ double rsq64(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  double h = a * y;
+  double e = __builtin_fma(-h, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  h = a * y;
+  e = __builtin_fma(-h, y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+__device__ __forceinline__ double rcp64(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = __builtin_fma(-a, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-a, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Carried state of a chain walk for one target: x - p of the previous node,
+// its squared norm and 1/|x - p|.
+struct Carry64 {
+  double rx, ry, rz, rs, inv;
+};
+
+__device__ __forceinline__ void carry_reset(Carry64& c) {
+  c.rx = c.ry = c.rz = c.rs = c.inv = 0.0;
+}
+
+// One chain element e (ChainRec as 8 doubles: p[3], g, r0[3], r0sq) at
+// target (x, y, z): adds g * BiotSavartLaw(prev -> p) to u and advances the
+// carry to p.  d = r1 x r2 = r0 x (x - prev); the cut-off set is the
+// reference's (|d|^2, |r1|, |r2| against 1e-10) up to rounding.
+__device__ __forceinline__ void chain_step64(const double* e, double x, double y, double z,
+                                             Carry64& c, double& ux, double& uy,
+                                             double& uz) {
+  const double rbx = x - e[0], rby = y - e[1], rbz = z - e[2];
+  const double rbs = __builtin_fma(rbz, rbz, __builtin_fma(rby, rby, rbx * rbx));
+  const double ib = rsq64(rbs);
+  const double r0x = e[4], r0y = e[5], r0z = e[6];
+  const double dx = __builtin_fma(r0y, c.rz, -r0z * c.ry);
+  const double dy = __builtin_fma(r0z, c.rx, -r0x * c.rz);
+  const double dz = __builtin_fma(r0x, c.ry, -r0y * c.rx);
+  const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  const double oa = __builtin_fma(r0z, c.rz, __builtin_fma(r0y, c.ry, r0x * c.rx));
+  const double ob = __builtin_fma(r0z, rbz, __builtin_fma(r0y, rby, r0x * rbx));
+  const double tt = __builtin_fma(oa, c.inv, -ob * ib);
+  const bool cut = (dsq < kCutoff) | (c.rs < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
+  double K = tt * e[3] * rcp64(dsq);
+  K = cut ? 0.0 : K;
+  ux = __builtin_fma(dx, K, ux);
+  uy = __builtin_fma(dy, K, uy);
+  uz = __builtin_fma(dz, K, uz);
+  c.rx = rbx;
+  c.ry = rby;
+  c.rz = rbz;
+  c.rs = rbs;
+  c.inv = ib;
+}
+
+}  // namespace dev
+}  // namespace uvlm

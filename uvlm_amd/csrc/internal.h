@@ -130,5 +130,8 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
                       double* x, int mode, double tol, int max_iter,
                       void* work, size_t work_bytes, int* iters);
 size_t dense_mixed_work_bytes(int n);
+size_t dense_fullpiv_work_bytes(int n);
+int dense_fullpiv_solve(hipStream_t s, double* A, int n, const double* b, double* x,
+                        void* work, size_t work_bytes, int* rank_dev);
 
 }  // namespace uvlm

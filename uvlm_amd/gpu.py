@@ -165,27 +165,7 @@ def case_spec(case: dict) -> N.CaseSpec:
         raise ValueError("1..16 wings")
     cs.num_wings = len(wings)
     for n, d in enumerate(wings):
-        w = cs.wings[n]
-        w.generator = N.GEN[d.get("generator", "rect")]
-        w.digit = int(d.get("digit", 0))
-        w.chord, w.span = float(d["chord"]), float(d["span"])
-        for k in range(3):
-            w.origin[k] = float(d.get("origin", (0, 0, 0))[k])
-        w.alpha = float(d.get("alpha", 0.0))
-        for name in ("plug", "flap"):
-            amp, om, ph = d.get(name, (0.0, 0.0, 0.0))
-            setattr(w, name + "_amp", float(amp))
-            setattr(w, name + "_omega", float(om))
-            setattr(w, name + "_phase", float(ph))
-        w.fourier_omega = float(d.get("fourier_omega", 0.0))
-        for name in ("twist", "bend"):
-            amp = np.asarray(d.get(name + "_amp", np.zeros((4, 2))), dtype=np.float64)
-            ph = np.asarray(d.get(name + "_phase", np.zeros((4, 2))), dtype=np.float64)
-            A, P = getattr(w, name + "_amp"), getattr(w, name + "_phase")
-            for m in range(4):
-                for q in range(2):
-                    A[m][q] = float(amp[m, q])
-                    P[m][q] = float(ph[m, q])
+        _fill_wing(cs.wings[n], d)
     cs.rows, cs.cols = int(case["rows"]), int(case["cols"])
     for k in range(3):
         cs.ff[k] = float(case["forward_flight"][k])
@@ -197,6 +177,42 @@ def case_spec(case: dict) -> N.CaseSpec:
     cs.precision = int(case.get("precision", N.PREC_F64_EXACT))
     cs.solve_mode = int(case.get("solve_mode", N.SOLVE_F64))
     return cs
+
+
+def _fill_wing(w: N.WingSpec, d: dict) -> None:
+    w.generator = N.GEN[d.get("generator", "rect")]
+    w.digit = int(d.get("digit", 0))
+    w.chord, w.span = float(d["chord"]), float(d["span"])
+    for k in range(3):
+        w.origin[k] = float(d.get("origin", (0, 0, 0))[k])
+    w.alpha = float(d.get("alpha", 0.0))
+    for name in ("plug", "flap"):
+        amp, om, ph = d.get(name, (0.0, 0.0, 0.0))
+        setattr(w, name + "_amp", float(amp))
+        setattr(w, name + "_omega", float(om))
+        setattr(w, name + "_phase", float(ph))
+    w.fourier_omega = float(d.get("fourier_omega", 0.0))
+    for name in ("twist", "bend"):
+        amp = np.asarray(d.get(name + "_amp", np.zeros((4, 2))), dtype=np.float64)
+        ph = np.asarray(d.get(name + "_phase", np.zeros((4, 2))), dtype=np.float64)
+        A, P = getattr(w, name + "_amp"), getattr(w, name + "_phase")
+        for m in range(4):
+            for q in range(2):
+                A[m][q] = float(amp[m, q])
+                P[m][q] = float(ph[m, q])
+
+
+def build_wing(wing: dict, rows: int, cols: int) -> np.ndarray:
+    """Reference-frame node lattice [rows+1, cols+1, 3] of one wing
+    (uvlm_host_build_wing: generator + WholeWing + origin)."""
+    w = N.WingSpec()
+    _fill_wing(w, wing)
+    out = np.zeros((rows + 1, cols + 1, 3))
+    L = N.host_lib()
+    rc = L.uvlm_host_build_wing(C.byref(w), rows, cols, _p(out))
+    if rc != N.OK:
+        raise UVLMError(rc, L.uvlm_host_last_error().decode())
+    return out
 
 
 class Simulation:
@@ -294,3 +310,97 @@ class Simulation:
         e, r = C.c_double(), C.c_double()
         self._check(self._L.uvlm_host_sim_pairs(self._h, C.byref(e), C.byref(r)), "pairs")
         return e.value, r.value
+
+
+class Batch:
+    """Many independent single-wing cases advanced in lockstep on one GPU
+    (uvlm_batch_*).  Each case is a workload dict with exactly one wing whose
+    motion has no twist or bend; advection euler, RK2 or RK4."""
+
+    def __init__(self, cases, max_steps: int, advect: str = "euler", ignore_steps: int = 5,
+                 ctx: Context | None = None):
+        self.ctx = ctx if ctx is not None else Context()
+        self._L = self.ctx._L
+        self.cases = list(cases)
+        n = len(self.cases)
+        specs = (N.BatchCase * n)()
+        walls = []
+        for k, case in enumerate(self.cases):
+            if len(case["wings"]) != 1:
+                raise ValueError("batched cases have exactly one wing")
+            w = case["wings"][0]
+            for name in ("twist_amp", "bend_amp"):
+                if np.any(np.asarray(w.get(name, 0.0)) != 0):
+                    raise ValueError("batched cases have no twist or bend")
+            R, Cc = int(case["rows"]), int(case["cols"])
+            sp = specs[k]
+            sp.rows, sp.cols = R, Cc
+            sp.chord, sp.span = float(w["chord"]), float(w["span"])
+            sp.dt, sp.alpha = float(case["dt"]), float(w.get("alpha", 0.0))
+            for d in range(3):
+                sp.origin[d] = float(w.get("origin", (0, 0, 0))[d])
+                sp.ff[d] = float(case["forward_flight"][d])
+                sp.plug[d] = float(w.get("plug", (0, 0, 0))[d])
+                sp.flap[d] = float(w.get("flap", (0, 0, 0))[d])
+            walls.append(build_wing(w, R, Cc).reshape(-1))
+        nodes = _f64(np.concatenate(walls))
+        self.ctx._check(self._L.uvlm_batch_init(self.ctx._h, specs, n, _p(nodes), max_steps,
+                                                N.SCHEMES[advect], ignore_steps),
+                        "uvlm_batch_init")
+        self._specs = specs
+
+    def run(self, steps: int, graph: bool = True):
+        self.ctx._check(self._L.uvlm_batch_run(self.ctx._h, steps, int(graph)), "uvlm_batch_run")
+
+    def sync(self):
+        self.ctx._check(self._L.uvlm_sync(self.ctx._h), "uvlm_sync")
+
+    def autotune(self, max_variants: int = 8):
+        chosen = C.c_int()
+        times = np.zeros(max_variants)
+        self.ctx._check(self._L.uvlm_batch_autotune(self.ctx._h, C.byref(chosen), _p(times),
+                                                    max_variants), "uvlm_batch_autotune")
+        return chosen.value, times
+
+    @property
+    def steps(self) -> int:
+        s = C.c_size_t()
+        self.ctx._check(self._L.uvlm_batch_steps(self.ctx._h, C.byref(s)), "uvlm_batch_steps")
+        return s.value
+
+    def _dims(self, k):
+        return int(self.cases[k]["rows"]), int(self.cases[k]["cols"])
+
+    def gamma(self, k):
+        R, Cc = self._dims(k)
+        out = np.zeros((R, Cc))
+        self.ctx._check(self._L.uvlm_batch_get_gamma(self.ctx._h, k, _p(out)), "gamma")
+        return out
+
+    def wing_nodes(self, k):
+        R, Cc = self._dims(k)
+        out = np.zeros((R + 1, Cc + 1, 3))
+        self.ctx._check(self._L.uvlm_batch_get_wing(self.ctx._h, k, _p(out)), "wing")
+        return out
+
+    def wake(self, k):
+        R, Cc = self._dims(k)
+        s = self.steps
+        nodes = np.zeros((s, Cc + 1, 3))
+        gam = np.zeros((max(s - 1, 0), Cc))
+        self.ctx._check(self._L.uvlm_batch_get_wake(self.ctx._h, k, _p(nodes), _p(gam)), "wake")
+        return nodes, gam
+
+    def loads(self, k):
+        cnt = C.c_size_t(0)
+        self.ctx._check(self._L.uvlm_batch_get_loads(self.ctx._h, k, None, C.byref(cnt)), "loads")
+        out = np.zeros((cnt.value, 2))
+        c2 = C.c_size_t(cnt.value)
+        self.ctx._check(self._L.uvlm_batch_get_loads(self.ctx._h, k, _p(out), C.byref(c2)),
+                        "loads")
+        return out
+
+    def pairs(self) -> float:
+        p = C.c_double()
+        self.ctx._check(self._L.uvlm_batch_pairs(self.ctx._h, C.byref(p)), "pairs")
+        return p.value

@@ -2,6 +2,7 @@
 // UVLM::simulator::SimpleSimulator.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "../../include/uvlm_host.h"
@@ -67,6 +68,32 @@ Morphing make_morphing(const uvlm_wing_spec& w) {
 extern "C" {
 
 const char* uvlm_host_last_error(void) { return g_err.c_str(); }
+
+int uvlm_host_build_wing(const uvlm_wing_spec* w, size_t rows, size_t cols, double* nodes) {
+  if (!w || !nodes) return err("null argument");
+  if (rows == 0 || cols == 0 || (cols & 1)) return err("columns must be even and non-zero");
+  std::unique_ptr<UVLM::wing::WingGenerator> gen;
+  if (w->generator == UVLM_GEN_NACA4)
+    gen.reset(new UVLM::wing::NACA4digitGenerator(w->digit));
+  else if (w->generator == UVLM_GEN_NACA00)
+    gen.reset(new UVLM::wing::NACA00XXGenerator(w->digit));
+  else
+    gen.reset(new UVLM::wing::RectGenerator());
+  // SimpleSimulator::BuildWing for one wing (simulator/simple_simulator.cpp:142-169)
+  UVLM::proto::Wing whole, half;
+  gen->Generate(&half, w->chord, w->span / 2, rows, cols / 2);
+  UVLM::wing::WholeWing(&whole, half);
+  const auto pts = UVLM::PointsToVector(whole.points());
+  if (pts.size() != (rows + 1) * (cols + 1)) return err("generator returned a wrong lattice");
+  const Vector3d o(w->origin[0], w->origin[1], w->origin[2]);
+  for (std::size_t k = 0; k < pts.size(); ++k) {
+    const Vector3d p = pts[k] + o;
+    nodes[3 * k + 0] = p.x();
+    nodes[3 * k + 1] = p.y();
+    nodes[3 * k + 2] = p.z();
+  }
+  return UVLM_OK;
+}
 
 int uvlm_host_sim_create(const uvlm_case_spec* sp, size_t max_steps, uvlm_host_sim** out) {
   if (!sp || !out) return err("null argument");
