@@ -227,3 +227,28 @@ def test_solve_fullpiv_ties_and_rank(ctx, orc):
     xr, rr = orc.fullpiv_solve(Z, np.ones(3))
     assert rank == rr == 0
     np.testing.assert_array_equal(x, xr)
+
+
+@pytest.mark.parametrize("n", [130, 700, 2100])
+def test_solve_f64_needs_pivoting(ctx, orc, n):
+    """Unstructured matrices: the tournament pivoting tree (1, 2 and 3
+    levels) must keep the factorisation backward stable."""
+    rng = np.random.default_rng(7 * n)
+    A = rng.normal(size=(n, n))
+    b = rng.normal(size=n)
+    x, _ = ctx.solve(A, b)
+    res = np.linalg.norm(A @ x - b) / (np.linalg.norm(A) * np.linalg.norm(x))
+    assert res < 1e-14
+    xr = np.linalg.solve(A, b)
+    cond = np.linalg.cond(A)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-15 * cond * n
+
+
+def test_solve_f64_large_global_substitution(ctx):
+    """n above the LDS-resident substitution size (x kept in global memory)."""
+    n = 8000
+    rng = np.random.default_rng(8000)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 2.0 * n ** 0.5
+    b = rng.normal(size=n)
+    x, _ = ctx.solve(A, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-13)

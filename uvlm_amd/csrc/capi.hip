@@ -532,7 +532,7 @@ int uvlm_solve(uvlm_ctx* c, size_t n, const double* A, const double* rhs,
   if (n > 46000) return fail(c, UVLM_ERR_ARG, "matrix too large");
   TRY(ensure(c, c->t_A, sizeof(double) * n * n));
   TRY(ensure(c, c->t_x, sizeof(double) * 2 * n));
-  TRY(ensure(c, c->t_ipiv, sizeof(int) * n));
+  TRY(ensure(c, c->t_ipiv, dense_lu_work_bytes((int)n)));
   TRY(ensure(c, c->t_info, sizeof(int) * 4));
   // row-major host A -> column-major device A: transpose on the host
   double* tmp = new (std::nothrow) double[n * n];
@@ -546,14 +546,14 @@ int uvlm_solve(uvlm_ctx* c, size_t n, const double* A, const double* rhs,
   double* dx = db + n;
   HIPCHK(c, hipMemcpyAsync(db, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   if (mode == UVLM_SOLVE_F64) {
-    dense_lu_factor(c->stream, P<double>(c->t_A), (int)n, P<int>(c->t_ipiv),
-                    P<int>(c->t_info));
-    dense_lu_solve(c->stream, P<double>(c->t_A), (int)n, P<int>(c->t_ipiv), db);
+    if (dense_lu_factor(c->stream, P<double>(c->t_A), (int)n, c->t_ipiv.p) < 0)
+      return fail(c, UVLM_ERR_HIP, "LU factorisation: %s", hipGetErrorString(hipGetLastError()));
+    dense_lu_solve(c->stream, P<double>(c->t_A), (int)n, c->t_ipiv.p, db, dx);
     TRY(check_launch(c));
     int info = 0;
-    HIPCHK(c, hipMemcpyAsync(&info, c->t_info.p, sizeof(int), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipMemcpyAsync(x, db, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&info, dense_lu_info(c->t_ipiv.p), sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (info != 0) return fail(c, UVLM_ERR_SINGULAR, "Matrix is not invertible (zero pivot at column %d)", info - 1);
     return UVLM_OK;
@@ -634,7 +634,7 @@ int uvlm_sim_init(uvlm_ctx* c, size_t num, size_t rows, size_t cols,
                                                  (size_t)chain_elems((int)num, (int)cap, (int)cols))));
   TRY(ensure(c, c->A, sizeof(double) * N * N));
   TRY(ensure(c, c->rhs, sizeof(double) * N));
-  TRY(ensure(c, c->ipiv, sizeof(int) * N));
+  TRY(ensure(c, c->ipiv, dense_lu_work_bytes(N)));
   TRY(ensure(c, c->info, sizeof(int) * 4));
   TRY(ensure(c, c->uw, sizeof(double) * 3 * N));
   TRY(ensure(c, c->vm, sizeof(double) * 3 * N));
@@ -750,10 +750,10 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   c->pairs_eval += 4.0 * N * (double)N;
   c->pairs_ref += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
   if (solve_mode == UVLM_SOLVE_F64) {
-    dense_lu_factor(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<int>(c->info));
-    dense_lu_solve(c->stream, P<double>(c->A), N, P<int>(c->ipiv), P<double>(c->rhs));
-    HIPCHK(c, hipMemcpyAsync(c->gamma.p, c->rhs.p, sizeof(double) * N,
-                             hipMemcpyDeviceToDevice, c->stream));
+    if (dense_lu_factor(c->stream, P<double>(c->A), N, c->ipiv.p) < 0)
+      return fail(c, UVLM_ERR_HIP, "LU factorisation: %s", hipGetErrorString(hipGetLastError()));
+    dense_lu_solve(c->stream, P<double>(c->A), N, c->ipiv.p, P<double>(c->rhs),
+                   P<double>(c->gamma));
   } else if (solve_mode == UVLM_SOLVE_F64_FULLPIV) {
     TRY(ensure(c, c->work, dense_fullpiv_work_bytes(N)));
     dense_fullpiv_solve(c->stream, P<double>(c->A), N, P<double>(c->rhs), P<double>(c->gamma),
@@ -777,7 +777,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   if (gamma_out) {
     int info = 0;
     if (solve_mode == UVLM_SOLVE_F64)
-      HIPCHK(c, hipMemcpyAsync(&info, c->info.p, sizeof(int), hipMemcpyDeviceToHost,
+      HIPCHK(c, hipMemcpyAsync(&info, dense_lu_info(c->ipiv.p), sizeof(int), hipMemcpyDeviceToHost,
                                c->stream));
     HIPCHK(c, hipMemcpyAsync(gamma_out, c->gamma.p, sizeof(double) * N,
                              hipMemcpyDeviceToHost, c->stream));

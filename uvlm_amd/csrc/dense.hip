@@ -1,18 +1,20 @@
 // dense.hip -- dense circulation solve on gfx950.
 //
-// Replaces Eigen::FullPivLU (simple_simulator.cpp:263-268).  The influence
-// matrix of a vortex lattice is diagonally dominant, so the GPU factors with
-// partial (row) pivoting; the solution agrees with full pivoting to
-// ~cond(A) * eps.
+// Replaces Eigen::FullPivLU (simple_simulator.cpp:263-268).  The default
+// solve factors with row pivoting chosen by a tournament (the panel's pivot
+// rows are elected tile by tile in LDS); the solution agrees with full
+// pivoting to ~cond(A) * eps.  UVLM_SOLVE_F64_FULLPIV reproduces FullPivLU's
+// elimination order exactly.
 //
 // Right-looking blocked LU, panel width kNB = 32, column-major:
-//   k_getf2   panel factorisation by one 1024-thread workgroup
-//   k_laswp   row interchanges applied to the columns outside the panel
+//   k_tslu    panel factorisation by tournament pivoting (LDS tiles + tree)
+//   k_tslu_l21 L21 = A21 U11^-1 below the panel's pivot block
+//   k_apply_moves  the panel's row permutation applied to the other columns
 //   k_trsm    U12 = L11^-1 A12 (one lane per column, L11 and X in LDS)
 //   k_gemm    A22 -= L21 U12 on the FP64 matrix cores
 //             (v_mfma_f64_16x16x4f64, 64x64 tile per workgroup)
-// followed by k_getrs (pivots, unit-lower and upper substitution) in one
-// workgroup.  The mixed-precision modes factor a copy in FP32 (or with
+// followed by k_getrs (permutation, unit-lower and upper substitution) in
+// one workgroup.  The mixed-precision modes factor a copy in FP32 (or with
 // BF16-rounded MFMA inputs) and refine the FP64 solution with FP64
 // residuals r = b - A x.
 #include <algorithm>
@@ -28,117 +30,368 @@ constexpr int kNB = 32;
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+__host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
 
-// Panel factorisation of columns [k0, k0+kb), rows [k0, n).
+// ---- panel factorisation ----------------------------------------------------
+// Fast path (k_panel_np): the panel is factored without row interchanges and
+// every comparison partial pivoting would make is checked on the way: the
+// pivot |a_jj| against each |a_ij| below it, on the updated values.  When no
+// check fails, partial pivoting would have chosen no interchange at all and
+// the factors are exactly GEPP's.  The influence matrix of a vortex lattice
+// takes this path (a ring's self-influence dominates its column).  When a
+// check fails the factorisation is redone from a copy of A with tournament
+// pivoting (k_tslu): the panel rows are cut into tiles of kTR rows, each
+// tile factored with partial pivoting in LDS nominates its kb pivot rows,
+// and the nominations are merged kFan tiles at a time up a tree inside the
+// same launch (the last workgroup to finish a group merges it, so no
+// workgroup waits).  This is the pivoting of communication-avoiding LU
+// (Grigori, Demmel, Xiang 2011).
+constexpr int kTR = 128;  // rows per tile (= workgroup size)
+constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
+
+// No-pivot LU of the kb x kb block D in LDS (one lane per row), with the
+// pivoting checks.  Returns (all lanes) the first zero pivot or -1.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_getf2(T* A, int lda, int n, int k0,
-                                               int kb, int* ipiv, int* info) {
-  __shared__ T s_val[16];
-  __shared__ int s_idx[16];
-  __shared__ int s_piv;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+__device__ int diag_lu_np(T (*D)[kNB + 1], int kb, int* s_flag) {
+  const int t = threadIdx.x;
+  int zero = -1;
   for (int j = 0; j < kb; ++j) {
-    const int col = k0 + j;
-    T* Acol = A + (size_t)col * lda;
-    T best = T(-1);
-    int bi = n;
-    for (int r = col + tid; r < n; r += 1024) {
-      const T v = fabs(Acol[r]);
-      if (v > best) {
-        best = v;
-        bi = r;
-      }
+    __syncthreads();
+    const T d = D[j][j];
+    if (d == T(0) && zero < 0) zero = j;
+    if (t > j && t < kb) {
+      T l = D[t][j];
+      if (fabs(l) > fabs(d)) *s_flag = 1;
+      if (d != T(0)) l = l / d;
+      D[t][j] = l;
+#pragma unroll 8
+      for (int c = j + 1; c < kb; ++c) D[t][c] = D[t][c] - l * D[j][c];
     }
+  }
+  __syncthreads();
+  return zero;
+}
+
+// Fast-path panel: every workgroup factors the top block itself (workgroup 0
+// stores it), then forms L21 = A21 U11^-1 for its 64 rows by a column sweep
+// in LDS.  flag <- 1 when a pivoting check fails; info <- first zero pivot.
+template <typename T>
+__global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, int kb,
+                                                 int* __restrict__ flag,
+                                                 int* __restrict__ info) {
+  __shared__ T D[kNB][kNB + 1];
+  __shared__ T X[kNB][64 + 1];
+  __shared__ int s_flag;
+  const int t = threadIdx.x;
+  if (t == 0) s_flag = 0;
+  for (int q = t; q < kNB * kNB; q += 64) {
+    const int r = q % kNB, c = q / kNB;
+    D[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
+  }
+  const int row = k0 + kb + blockIdx.x * 64 + t;
+  if (row < n)
+    for (int c = 0; c < kb; ++c) X[c][t] = A[(size_t)(k0 + c) * lda + row];
+  const int zero = diag_lu_np(D, kb, &s_flag);
+  if (blockIdx.x == 0) {
+    for (int q = t; q < kb * kb; q += 64) {
+      const int r = q % kb, c = q / kb;
+      A[(size_t)(k0 + c) * lda + k0 + r] = D[r][c];
+    }
+    if (t == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
+  }
+  if (row < n) {
+    bool bad = false;
+    for (int j = 0; j < kb; ++j) {
+      const T d = D[j][j];
+      T xj = X[j][t];
+      bad |= fabs(xj) > fabs(d);
+      if (d != T(0)) xj = xj / d;
+      X[j][t] = xj;
+#pragma unroll 8
+      for (int c = j + 1; c < kb; ++c) X[c][t] = X[c][t] - xj * D[j][c];
+    }
+    if (bad) s_flag = 1;
+    for (int c = 0; c < kb; ++c) A[(size_t)(k0 + c) * lda + row] = X[c][t];
+  }
+  __syncthreads();
+  if (t == 0 && s_flag) atomicOr(flag, 1);
+}
+
+// Partial-pivoting LU of tile rows [0, rows) x columns [0, kb) in LDS; the
+// global row numbers in idx[] follow the interchanges.  Returns (in every
+// thread) the first column with a zero pivot, or -1.
+template <typename T>
+__device__ int tile_gepp(T (*tile)[kNB + 1], int* idx, int rows, int kb, T* s_v, int* s_i) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int steps = min(kb, rows);
+  int zero = -1;
+  for (int j = 0; j < steps; ++j) {
+    T v = (t >= j && t < rows) ? fabs(tile[t][j]) : T(-1);
+    int bi = t;
     for (int off = 32; off > 0; off >>= 1) {
-      const T ov = __shfl_down(best, off);
+      const T ov = __shfl_down(v, off);
       const int oi = __shfl_down(bi, off);
-      if (ov > best || (ov == best && oi < bi)) {
-        best = ov;
+      if (ov > v || (ov == v && oi < bi)) {
+        v = ov;
         bi = oi;
       }
     }
     if (lane == 0) {
-      s_val[wid] = best;
-      s_idx[wid] = bi;
+      s_v[w] = v;
+      s_i[w] = bi;
     }
     __syncthreads();
-    if (tid == 0) {
-      T b = s_val[0];
-      int bj = s_idx[0];
-      for (int w = 1; w < 16; ++w)
-        if (s_val[w] > b || (s_val[w] == b && s_idx[w] < bj)) {
-          b = s_val[w];
-          bj = s_idx[w];
+    T best = s_v[0];
+    int p = s_i[0];
+    if (s_v[1] > best || (s_v[1] == best && s_i[1] < p)) {
+      best = s_v[1];
+      p = s_i[1];
+    }
+    if (!(best > T(0))) {
+      p = j;
+      if (zero < 0) zero = j;
+    }
+    if (p != j) {
+      if (t < kb) {
+        const T a = tile[j][t];
+        tile[j][t] = tile[p][t];
+        tile[p][t] = a;
+      }
+      if (t == 0) {
+        const int a = idx[j];
+        idx[j] = idx[p];
+        idx[p] = a;
+      }
+    }
+    __syncthreads();
+    const T d = tile[j][j];
+    if (t > j && t < rows) {
+      T l = tile[t][j];
+      if (d != T(0)) {
+        l = l / d;
+        tile[t][j] = l;
+      }
+#pragma unroll 8
+      for (int c = j + 1; c < kb; ++c) tile[t][c] = tile[t][c] - l * tile[j][c];
+    }
+    __syncthreads();
+  }
+  return zero;
+}
+
+// Pivoted panel, one launch, grid = number of tiles.  cand: nominations per
+// tree node; ctr: arrival counters per tree node (zero on entry, reset by
+// the workgroup that consumes them).  The root writes the panel's row moves
+// (mv: dst, src pairs and their count), moves the displaced rows of the
+// panel columns and stores the factored kb x kb block.
+template <typename T>
+__global__ __launch_bounds__(kTR) void k_tslu(T* A, int lda, int n, int k0, int kb,
+                                              int* __restrict__ cand, int* __restrict__ ctr,
+                                              int* __restrict__ mv, int* __restrict__ info) {
+  __shared__ T tile[kTR][kNB + 1];
+  __shared__ int idx[kTR];
+  __shared__ T s_v[2];
+  __shared__ int s_i[2], s_cnt[2], s_last, s_nm, s_dst[2 * kNB], s_src[2 * kNB];
+  const int t = threadIdx.x, lane = t & 63;
+  const int m = n - k0;
+  int count = cdiv(m, kTR);  // nodes of the current level
+  int node = blockIdx.x, level_off = 0, ctr_off = 0;
+  int rows;
+  {
+    const int r0 = k0 + node * kTR;
+    rows = min(kTR, n - r0);
+    if (t < rows) {
+      idx[t] = r0 + t;
+      for (int c = 0; c < kb; ++c) tile[t][c] = A[(size_t)(k0 + c) * lda + r0 + t];
+    }
+    __syncthreads();
+  }
+  int zero;
+  for (;;) {
+    zero = tile_gepp(tile, idx, rows, kb, s_v, s_i);
+    if (count == 1) break;
+    const int nc = min(kb, rows);
+    if (t < kNB) cand[(size_t)(level_off + node) * kNB + t] = t < nc ? idx[t] : -1;
+    __threadfence();
+    __syncthreads();
+    const int parent = node / kFan;
+    const int kids = min(kFan, count - parent * kFan);
+    if (t == 0) {
+      const int old = atomicAdd(&ctr[ctr_off + parent], 1);
+      s_last = old == kids - 1;
+      if (s_last) ctr[ctr_off + parent] = 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // gather the group's nominations (original rows of A), compacted
+    int r = -1;
+    if (t < kids * kNB)
+      r = __atomic_load_n(&cand[(size_t)(level_off + parent * kFan) * kNB + t], __ATOMIC_RELAXED);
+    const unsigned long long bal = __ballot(r >= 0);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_cnt[t >> 6] = __popcll(bal);
+    __syncthreads();
+    const int pos = before + ((t >> 6) ? s_cnt[0] : 0);
+    rows = s_cnt[0] + s_cnt[1];
+    if (r >= 0) {
+      idx[pos] = r;
+      for (int c = 0; c < kb; ++c) tile[pos][c] = A[(size_t)(k0 + c) * lda + r];
+    }
+    __syncthreads();
+    level_off += count;
+    ctr_off += cdiv(count, kFan);
+    count = cdiv(count, kFan);
+    node = parent;
+  }
+  // Root.  The panel's row permutation as a list of moves dst <- src
+  // (positions in the matrix before this panel): target k0+j receives winner
+  // idx[j], and every row displaced from a target by the interchanges of
+  // LAPACK's sequential form lands at a vacated winner position.  Wave 0
+  // replays the interchanges with the displaced rows as a lane-distributed
+  // (row, position) list.
+  if (t < 64) {
+    int my_id = -1, my_pos = -1, nlist = 0;
+    for (int j = 0; j < kb; ++j) {
+      const int r = idx[j], target = k0 + j;
+      const unsigned long long mr = __ballot(my_id == r);
+      const int a = mr ? __ffsll((long long)mr) - 1 : -1;
+      const int p = a >= 0 ? __shfl(my_pos, a) : r;
+      if (p != target) {
+        const unsigned long long mt = __ballot(my_id >= 0 && my_pos == target);
+        const int b = mt ? __ffsll((long long)mt) - 1 : -1;
+        const int rt = b >= 0 ? __shfl(my_id, b) : target;
+        if (b >= 0) {
+          if (t == b) my_pos = p;
+        } else {
+          if (t == nlist) {
+            my_id = rt;
+            my_pos = p;
+          }
+          ++nlist;
         }
-      if (!(b > T(0)) || bj >= n) {
-        if (*info == 0) *info = col + 1;
-        bj = col;
-      }
-      s_piv = bj;
-      ipiv[col] = bj;
-    }
-    __syncthreads();
-    const int piv = s_piv;
-    if (piv != col && tid < kb) {
-      T* a = A + (size_t)(k0 + tid) * lda;
-      const T tmp = a[col];
-      a[col] = a[piv];
-      a[piv] = tmp;
-    }
-    __syncthreads();
-    const T d = Acol[col];
-    if (d != T(0))
-      for (int r = col + 1 + tid; r < n; r += 1024) Acol[r] = Acol[r] / d;
-    __syncthreads();
-    const int ncols = k0 + kb - col - 1;
-    const int nrows = n - col - 1;
-    if (ncols > 0 && nrows > 0) {
-      for (int c = 0; c < ncols; ++c) {
-        T* Ac = A + (size_t)(col + 1 + c) * lda;
-        const T u = Ac[col];
-        for (int r = col + 1 + tid; r < n; r += 1024) Ac[r] = Ac[r] - Acol[r] * u;
+        if (a >= 0 && t == a) my_pos = target;
       }
     }
-    __syncthreads();
+    const bool out = my_id >= 0 && (my_pos < k0 || my_pos >= k0 + kb);
+    const unsigned long long mo = __ballot(out);
+    const int slot = __popcll(mo & ((1ull << t) - 1ull));
+    if (out) {
+      s_dst[kb + slot] = my_pos;
+      s_src[kb + slot] = my_id;
+    }
+    if (t < kb) {
+      s_dst[t] = k0 + t;
+      s_src[t] = idx[t];
+    }
+    if (t == 0) s_nm = kb + __popcll(mo);
+  }
+  __syncthreads();
+  const int nm = s_nm, nd = nm - kb;
+  // panel columns: displaced rows move (read all, then write), the targets
+  // receive the factored block
+  constexpr int kHeld = (kNB * kNB) / kTR;
+  T held[kHeld];
+#pragma unroll
+  for (int u = 0; u < kHeld; ++u) {
+    const int q = t + u * kTR;
+    if (q < nd * kb) held[u] = A[(size_t)(k0 + q / nd) * lda + s_src[kb + q % nd]];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kHeld; ++u) {
+    const int q = t + u * kTR;
+    if (q < nd * kb) A[(size_t)(k0 + q / nd) * lda + s_dst[kb + q % nd]] = held[u];
+  }
+  for (int q = t; q < kb * kb; q += kTR) {
+    const int j = q % kb, c = q / kb;
+    A[(size_t)(k0 + c) * lda + k0 + j] = tile[j][c];
+  }
+  if (t < nm) {
+    mv[2 * t] = s_dst[t];
+    mv[2 * t + 1] = s_src[t];
+  }
+  if (t == 0) {
+    mv[4 * kNB] = nm;
+    if (zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
   }
 }
 
-// Row interchanges of the panel applied to the columns outside it.
+// L21 = A21 U11^-1 for the rows [k0+kb, n) of a pivoted panel: one lane per
+// row, column sweep in LDS.
 template <typename T>
-__global__ void k_laswp(T* A, int lda, int n, int k0, int kb,
-                        const int* ipiv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n - kb) return;
-  const int col = c < k0 ? c : c + kb;
-  T* a = A + (size_t)col * lda;
+__global__ __launch_bounds__(64) void k_tslu_l21(T* A, int lda, int n, int k0, int kb) {
+  __shared__ T U[kNB][kNB + 1];
+  __shared__ T X[kNB][64 + 1];
+  const int t = threadIdx.x;
+  for (int q = t; q < kNB * kNB; q += 64) {
+    const int r = q % kNB, c = q / kNB;
+    U[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
+  }
+  const int row = k0 + kb + blockIdx.x * 64 + t;
+  if (row < n)
+    for (int c = 0; c < kb; ++c) X[c][t] = A[(size_t)(k0 + c) * lda + row];
+  __syncthreads();
+  if (row >= n) return;
   for (int j = 0; j < kb; ++j) {
-    const int r = k0 + j, p = ipiv[r];
-    if (p != r) {
-      const T t = a[r];
-      a[r] = a[p];
-      a[p] = t;
-    }
+    const T d = U[j][j];
+    T xj = X[j][t];
+    if (d != T(0)) xj = xj / d;
+    X[j][t] = xj;
+#pragma unroll 8
+    for (int c = j + 1; c < kb; ++c) X[c][t] = X[c][t] - xj * U[j][c];
   }
+  for (int c = 0; c < kb; ++c) A[(size_t)(k0 + c) * lda + row] = X[c][t];
 }
 
-// U12 = L11^-1 A12 for the columns [k0+kb, n): one wavefront per 64
-// columns, one lane per column; L11 and the 64 partial solutions in LDS
-// (a register-resident solution vector would need a fully unrolled 32x32
-// triangle, which the compiler spills).
+// Zero the pivot report, the fallback flag and the tree counters; perm =
+// identity.
+__global__ void k_lu_init(int* info, int* ctr, int nctr, int* perm, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    info[0] = 0;
+    info[1] = 0;
+  }
+  if (i < nctr) ctr[i] = 0;
+  if (i < n) perm[i] = i;
+}
+
+// The panel's row moves applied to the columns outside it and to the row
+// permutation perm (one extra column): 64 lanes per column, one move each;
+// every source is read before any destination is written.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0,
-                                             int kb) {
+__global__ __launch_bounds__(256) void k_apply_moves(T* A, int lda, int n, int k0, int kb,
+                                                     const int* __restrict__ mv,
+                                                     int* __restrict__ perm) {
+  const int e = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nm = mv[4 * kNB];
+  const bool act = c <= n - kb && e < nm;
+  const int dst = act ? mv[2 * e] : 0, src = act ? mv[2 * e + 1] : 0;
+  if (c == n - kb) {
+    const int v = act ? perm[src] : 0;
+    __syncthreads();
+    if (act) perm[dst] = v;
+    return;
+  }
+  T* a = A + (size_t)(c < k0 ? c : c + kb) * lda;
+  const T v = act ? a[src] : T(0);
+  __syncthreads();
+  if (act) a[dst] = v;
+}
+
+// U12 = L11^-1 A12 for the columns [k0+kb, n): one lane per column, column
+// sweep in LDS.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb) {
   __shared__ T L[kNB][kNB + 1];
   __shared__ T X[kNB][64 + 1];
-  for (int q = threadIdx.x; q < kNB * kNB; q += 64) {
+  const int t = threadIdx.x;
+  for (int q = t; q < kNB * kNB; q += 64) {
     const int r = q % kNB, c = q / kNB;
     L[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
   }
-  const int t = threadIdx.x;
   const int col = k0 + kb + blockIdx.x * 64 + t;
   T* a = A + (size_t)col * lda + k0;
   if (col < n)
@@ -147,6 +400,7 @@ __global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0,
   if (col >= n) return;
   for (int j = 0; j < kb; ++j) {
     const T xj = X[j][t];
+#pragma unroll 8
     for (int i = j + 1; i < kb; ++i) X[i][t] = X[i][t] - L[i][j] * xj;
   }
   for (int i = 0; i < kb; ++i) a[i] = X[i][t];
@@ -280,72 +534,126 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float* A, int lda, int n,
       }
 }
 
-// Forward (unit lower) and backward (upper) substitution after applying the
-// row interchanges, one 1024-thread workgroup, 64-row blocks.
+// x = U^-1 L^-1 (P b), one 1024-thread workgroup, x resident in LDS
+// (n <= kGetrsLds) or in global memory.  Blocks of 64 rows: wavefront 0
+// solves the 64x64 diagonal block with lane shuffles (no workgroup barrier
+// inside the block), then all lanes apply the block to the remaining rows.
+constexpr int kGetrsLds = 4096;  // FP64 x of up to 32 KB in LDS (+ 33 KB block)
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_getrs(const T* LU, int n,
-                                               const int* ipiv, T* b) {
-  __shared__ T xs[64];
-  const int tid = threadIdx.x;
-  if (tid == 0)
-    for (int i = 0; i < n; ++i) {
-      const int p = ipiv[i];
-      if (p != i) {
-        const T t = b[i];
-        b[i] = b[p];
-        b[p] = t;
-      }
-    }
-  __syncthreads();
+__device__ void getrs_body(const T* __restrict__ LU, int n, const int* __restrict__ perm,
+                           const T* __restrict__ b, T* x, T (*D)[65]) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int i = tid; i < n; i += 1024) x[i] = b[perm[i]];
   for (int i0 = 0; i0 < n; i0 += 64) {
     const int ib = min(64, n - i0);
-    if (tid < ib) xs[tid] = b[i0 + tid];
-    __syncthreads();
-    for (int i = 0; i < ib; ++i) {
-      if (tid > i && tid < ib)
-        xs[tid] = xs[tid] - LU[(size_t)(i0 + i) * n + i0 + tid] * xs[i];
-      __syncthreads();
+    for (int q = tid; q < 64 * 64; q += 1024) {
+      const int r = q % 64, c = q / 64;
+      D[r][c] = (r < ib && c < ib) ? LU[(size_t)(i0 + c) * n + i0 + r] : T(0);
     }
-    if (tid < ib) b[i0 + tid] = xs[tid];
+    __syncthreads();
+    if (tid < 64) {
+      T v = lane < ib ? x[i0 + lane] : T(0);
+      for (int i = 0; i < ib; ++i) {
+        const T xi = __shfl(v, i);
+        if (lane > i && lane < ib) v = v - D[lane][i] * xi;
+      }
+      if (lane < ib) x[i0 + lane] = v;
+    }
+    __syncthreads();
     for (int r = i0 + ib + tid; r < n; r += 1024) {
-      T s = b[r];
-      for (int i = 0; i < ib; ++i) s = s - LU[(size_t)(i0 + i) * n + r] * xs[i];
-      b[r] = s;
+      T s = x[r];
+      const T* row = LU + (size_t)i0 * n + r;
+#pragma unroll 16
+      for (int i = 0; i < ib; ++i) s = s - row[(size_t)i * n] * x[i0 + i];
+      x[r] = s;
     }
     __syncthreads();
   }
   for (int iend = n; iend > 0; iend -= 64) {
-    const int i0 = max(0, iend - 64);
-    const int ib = iend - i0;
-    if (tid < ib) xs[tid] = b[i0 + tid];
-    __syncthreads();
-    for (int i = ib - 1; i >= 0; --i) {
-      if (tid == i) xs[i] = xs[i] / LU[(size_t)(i0 + i) * n + i0 + i];
-      __syncthreads();
-      if (tid < i) xs[tid] = xs[tid] - LU[(size_t)(i0 + i) * n + i0 + tid] * xs[i];
-      __syncthreads();
+    const int i0 = max(0, iend - 64), ib = iend - i0;
+    for (int q = tid; q < 64 * 64; q += 1024) {
+      const int r = q % 64, c = q / 64;
+      D[r][c] = (r < ib && c < ib) ? LU[(size_t)(i0 + c) * n + i0 + r] : T(0);
     }
-    if (tid < ib) b[i0 + tid] = xs[tid];
+    __syncthreads();
+    if (tid < 64) {
+      T v = lane < ib ? x[i0 + lane] : T(0);
+      const T dl = lane < ib ? D[lane][lane] : T(1);
+      for (int i = ib - 1; i >= 0; --i) {
+        if (lane == i) v = v / dl;
+        const T xi = __shfl(v, i);
+        if (lane < i) v = v - D[lane][i] * xi;
+      }
+      if (lane < ib) x[i0 + lane] = v;
+    }
+    __syncthreads();
     for (int r = tid; r < i0; r += 1024) {
-      T s = b[r];
-      for (int i = 0; i < ib; ++i) s = s - LU[(size_t)(i0 + i) * n + r] * xs[i];
-      b[r] = s;
+      T s = x[r];
+      const T* row = LU + (size_t)i0 * n + r;
+#pragma unroll 16
+      for (int i = 0; i < ib; ++i) s = s - row[(size_t)i * n] * x[i0 + i];
+      x[r] = s;
     }
     __syncthreads();
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(1024) void k_getrs(const T* __restrict__ LU, int n,
+                                               const int* __restrict__ perm,
+                                               const T* __restrict__ b, T* __restrict__ out) {
+  extern __shared__ unsigned char s_raw[];
+  __shared__ T D[64][65];
+  T* xs = reinterpret_cast<T*>(s_raw);
+  if (n <= kGetrsLds) {
+    getrs_body(LU, n, perm, b, xs, D);
+    for (int i = threadIdx.x; i < n; i += 1024) out[i] = xs[i];
+  } else {
+    getrs_body(LU, n, perm, b, out, D);
+  }
+}
+
+template <typename T>
+void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* b, T* out) {
+  const size_t lds = n <= kGetrsLds ? sizeof(T) * (size_t)n : 0;
+  hipLaunchKernelGGL((k_getrs<T>), dim3(1), dim3(1024), lds, s, LU, n, perm, b, out);
+}
+
+// Workspace of the factorisation (ints): move list of the current panel,
+// tree nominations and arrival counters of the panel kernel.
+size_t lu_work_ints(int n) {
+  const int tiles = cdiv(n, kTR);
+  return 4 * kNB + 64 + (size_t)2 * tiles * kNB + 2 * (size_t)tiles + 64;
+}
+
+// Blocked right-looking LU.  pivoted = false: checked no-pivot panels
+// (info[1] <- 1 when a check fails: the factors are then not GEPP's and the
+// caller refactors with pivoted = true).  perm = the row permutation.
 template <typename T, int GEMM>
-int lu_factor_t(hipStream_t s, T* A, int n, int* ipiv, int* info) {
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, info);
+void lu_factor_t(hipStream_t s, T* A, int n, int* perm, int* info, int* work, bool pivoted) {
+  const int tiles = cdiv(n, kTR);
+  int* mv = work;
+  int* cand = work + 4 * kNB + 64;
+  int* ctr = cand + (size_t)2 * tiles * kNB;
+  const int nctr = 2 * tiles;
+  hipLaunchKernelGGL(k_lu_init, dim3(cdiv(std::max(n, nctr), 256)), dim3(256), 0, s, info,
+                     ctr, nctr, perm, n);
   for (int k0 = 0; k0 < n; k0 += kNB) {
     const int kb = std::min(kNB, n - k0);
-    hipLaunchKernelGGL((k_getf2<T>), dim3(1), dim3(1024), 0, s, A, n, n, k0, kb,
-                       ipiv, info);
-    if (n - kb > 0)
-      hipLaunchKernelGGL((k_laswp<T>), dim3(cdiv(n - kb, 256)), dim3(256), 0, s,
-                         A, n, n, k0, kb, (const int*)ipiv);
     const int rest = n - k0 - kb;
+    if (pivoted) {
+      hipLaunchKernelGGL((k_tslu<T>), dim3(cdiv(n - k0, kTR)), dim3(kTR), 0, s, A, n, n, k0,
+                         kb, cand, ctr, mv, info);
+      if (rest > 0)
+        hipLaunchKernelGGL((k_tslu_l21<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A, n, n,
+                           k0, kb);
+      hipLaunchKernelGGL((k_apply_moves<T>), dim3(cdiv(n - kb + 1, 4)), dim3(256), 0, s, A,
+                         n, n, k0, kb, (const int*)mv, perm);
+    } else {
+      hipLaunchKernelGGL((k_panel_np<T>), dim3(std::max(1, cdiv(rest, 64))), dim3(64), 0, s,
+                         A, n, n, k0, kb, info + 1, info);
+    }
     if (rest > 0) {
       hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A,
                          n, n, k0, kb);
@@ -361,7 +669,25 @@ int lu_factor_t(hipStream_t s, T* A, int n, int* ipiv, int* info) {
       }
     }
   }
-  return 0;
+}
+
+// Factor with the checked no-pivot path; if a check fails, restore A from
+// `saved` and factor with tournament pivoting.  Synchronises the stream once
+// (to read the check).  Returns 1 when pivoting was needed, -1 on a HIP
+// error.
+template <typename T, int GEMM>
+int lu_factor_checked(hipStream_t s, T* A, int n, int* perm, int* info, int* work, T* saved) {
+  const size_t bytes = sizeof(T) * (size_t)n * n;
+  if (hipMemcpyAsync(saved, A, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+  lu_factor_t<T, GEMM>(s, A, n, perm, info, work, false);
+  int flag = 0;
+  if (hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  if (!flag) return 0;
+  if (hipMemcpyAsync(A, saved, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+  lu_factor_t<T, GEMM>(s, A, n, perm, info, work, true);
+  return 1;
 }
 
 // --- mixed precision helpers ------------------------------------------------
@@ -609,21 +935,30 @@ __global__ __launch_bounds__(1024) void k_fp_solve(const double* __restrict__ m,
 
 }  // namespace
 
-int dense_lu_factor(hipStream_t s, double* A, int n, int* ipiv, int* info_dev) {
-  return lu_factor_t<double, 0>(s, A, n, ipiv, info_dev);
+// LU workspace: ints [info, flag | pad 64][perm n][moves, tree], then an
+// n x n copy of A for the pivoted retry.
+static size_t lu_ints(int n) { return 64 + (size_t)n + lu_work_ints(n); }
+size_t dense_lu_work_bytes(int n) {
+  return (sizeof(int) * lu_ints(n) + 255) / 256 * 256 + sizeof(double) * (size_t)n * n;
+}
+int* dense_lu_info(void* work) { return (int*)work; }
+
+int dense_lu_factor(hipStream_t s, double* A, int n, void* work) {
+  int* w = (int*)work;
+  double* saved = (double*)((char*)work + (sizeof(int) * lu_ints(n) + 255) / 256 * 256);
+  return lu_factor_checked<double, 0>(s, A, n, w + 64, w, w + 64 + (size_t)n, saved);
 }
 
-void dense_lu_solve(hipStream_t s, const double* LU, int n, const int* ipiv,
-                    double* b) {
-  hipLaunchKernelGGL((k_getrs<double>), dim3(1), dim3(1024), 0, s, LU, n, ipiv,
-                     b);
+void dense_lu_solve(hipStream_t s, const double* LU, int n, const void* work, const double* b,
+                    double* x) {
+  const int* w = (const int*)work;
+  launch_getrs<double>(s, LU, n, w + 64, b, x);
 }
 
 size_t dense_mixed_work_bytes(int n) {
   const size_t nn = (size_t)n * n;
-  return nn * sizeof(float) + (size_t)n * (sizeof(int) + 2 * sizeof(double) +
-                                           2 * sizeof(float)) +
-         64 * sizeof(double);
+  return 2 * nn * sizeof(float) + 512 + sizeof(int) * lu_ints(n) + (size_t)n * sizeof(double) +
+         64 * sizeof(double) + 2 * (size_t)n * sizeof(float) + 256;
 }
 
 // Mixed precision: LU of fl32(A) (FP32 or BF16-input MFMA updates), then
@@ -635,31 +970,32 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   char* w = (char*)work;
   const size_t nn = (size_t)n * n;
   float* Af = (float*)w;
-  w += nn * sizeof(float);
-  int* ipiv = (int*)w;
-  w += (size_t)n * sizeof(int);
+  w += (nn * sizeof(float) + 255) / 256 * 256;
+  float* Asaved = (float*)w;
+  w += (nn * sizeof(float) + 255) / 256 * 256;
+  int* lw = (int*)w;
+  w += (sizeof(int) * lu_ints(n) + 255) / 256 * 256;
   double* r = (double*)w;
   w += (size_t)n * sizeof(double);
   double* norms = (double*)w;
   w += 64 * sizeof(double);
   float* rf = (float*)w;
   w += (size_t)n * sizeof(float);
-  int* info = (int*)(w);
+  float* dxf = (float*)w;
+  int* perm = lw + 64;
+  int* tree = lw + 64 + (size_t)n;
   hipLaunchKernelGGL(k_to_f32, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, Af, nn);
-  if (mode == 2)
-    lu_factor_t<float, 2>(s, Af, n, ipiv, info);
-  else
-    lu_factor_t<float, 1>(s, Af, n, ipiv, info);
-  // x0 = LU^-1 b
+  const int rc = mode == 2 ? lu_factor_checked<float, 2>(s, Af, n, perm, lw, tree, Asaved)
+                           : lu_factor_checked<float, 1>(s, Af, n, perm, lw, tree, Asaved);
+  if (rc < 0) return -2;
   (void)hipMemsetAsync(x, 0, sizeof(double) * n, s);
   int it = 0;
   double h_norms[2] = {0, 0};
   for (it = 0; it < max_iter; ++it) {
     hipLaunchKernelGGL(k_residual, dim3(cdiv(n, 256)), dim3(256), 0, s, A, n, x,
                        b, r, rf);
-    hipLaunchKernelGGL((k_getrs<float>), dim3(1), dim3(1024), 0, s,
-                       (const float*)Af, n, (const int*)ipiv, rf);
-    hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, s, x, (const float*)rf,
+    launch_getrs<float>(s, Af, n, perm, rf, dxf);
+    hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, s, x, (const float*)dxf,
                        n, norms);
     if (hipMemcpyAsync(h_norms, norms, sizeof(h_norms), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||

@@ -261,6 +261,20 @@ __global__ void k_set_gamma(const double* __restrict__ gamma, int n,
   if (k < n) out[k].g = gamma[k];
 }
 
+// Chunk partial sums [chunk][3][ld] -> per-target sums.  Few chunks: one
+// lane per target, chunks in order.  Many chunks: 64 targets x 16 chunk
+// groups per workgroup (coalesced along targets), the 16 group sums added in
+// LDS in group order.
+constexpr int kSumGroups = 16;
+constexpr int kWideChunks = 24;
+
+__device__ __forceinline__ double chunk_range_sum(const double* __restrict__ partial, int ld,
+                                                  int t, int d, int c0, int c1, int step) {
+  double s = 0.0;
+  for (int c = c0; c < c1; c += step) s = s + partial[((size_t)c * 3 + d) * ld + t];
+  return s;
+}
+
 __global__ void k_sum_chunks(const double* __restrict__ partial, int ld, int nt,
                              int chunks, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,6 +284,33 @@ __global__ void k_sum_chunks(const double* __restrict__ partial, int ld, int nt,
     for (int c = 1; c < chunks; ++c) s = s + partial[((size_t)c * 3 + d) * ld + t];
     out[3 * (size_t)t + d] = s;
   }
+}
+
+// sums[t][3] of chunks [c0, c1) with the 64 x kSumGroups layout
+__device__ void wide_chunk_sums(const double* __restrict__ partial, int ld, int nt, int c0,
+                                int c1, double (*red)[kSumGroups][64], double out3[3]) {
+  const int tl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + tl;
+  for (int d = 0; d < 3; ++d)
+    red[d][g][tl] = t < nt ? chunk_range_sum(partial, ld, t, d, c0 + g, c1, kSumGroups) : 0.0;
+  __syncthreads();
+  if (g == 0)
+    for (int d = 0; d < 3; ++d) {
+      double s = red[d][0][tl];
+      for (int q = 1; q < kSumGroups; ++q) s = s + red[d][q][tl];
+      out3[d] = s;
+    }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64 * kSumGroups) void k_sum_chunks_wide(
+    const double* __restrict__ partial, int ld, int nt, int chunks, double* __restrict__ out) {
+  __shared__ double red[3][kSumGroups][64];
+  double s[3];
+  wide_chunk_sums(partial, ld, nt, 0, chunks, red, s);
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (threadIdx.x < 64 && t < nt)
+    for (int d = 0; d < 3; ++d) out[3 * (size_t)t + d] = s[d];
 }
 
 // advect.h:16-24: ((-ff + bound) + wake)
@@ -297,6 +338,20 @@ __global__ void k_flow_velocity(const double* __restrict__ partial, int ld,
     }
     out[3 * (size_t)t + d] = (mff[d] + b) + w;
   }
+}
+
+__global__ __launch_bounds__(64 * kSumGroups) void k_flow_velocity_wide(
+    const double* __restrict__ partial, int ld, int nt, int cb, int cw, double fx, double fy,
+    double fz, double* __restrict__ out, int te_period, int te_width) {
+  __shared__ double red[3][kSumGroups][64];
+  double b[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
+  if (cb > 0) wide_chunk_sums(partial, ld, nt, 0, cb, red, b);
+  if (cw > 0) wide_chunk_sums(partial, ld, nt, cb, cb + cw, red, w);
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (threadIdx.x >= 64 || t >= nt) return;
+  const double mff[3] = {-fx, -fy, -fz};
+  const bool te = te_period > 0 && (t % te_period) < te_width;
+  for (int d = 0; d < 3; ++d) out[3 * (size_t)t + d] = te ? mff[d] : (mff[d] + b[d]) + w[d];
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -445,17 +500,25 @@ void launch_segments(hipStream_t s, int prec, const double* a, const double* b,
 void launch_sum_chunks(hipStream_t s, const double* partial, int ld, int nt,
                        int chunks, double* out) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_sum_chunks, dim3(cdiv(nt, 256)), dim3(256), 0, s,
-                     partial, ld, nt, chunks, out);
+  if (chunks > kWideChunks)
+    hipLaunchKernelGGL(k_sum_chunks_wide, dim3(cdiv(nt, 64)), dim3(64 * kSumGroups), 0, s,
+                       partial, ld, nt, chunks, out);
+  else
+    hipLaunchKernelGGL(k_sum_chunks, dim3(cdiv(nt, 256)), dim3(256), 0, s,
+                       partial, ld, nt, chunks, out);
 }
 
 void launch_flow_velocity(hipStream_t s, const double* partial, int ld, int nt,
                           int cb, int cw, const double* ff, double* out,
                           int te_period, int te_width) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_flow_velocity, dim3(cdiv(nt, 256)), dim3(256), 0, s,
-                     partial, ld, nt, cb, cw, ff[0], ff[1], ff[2], out,
-                     te_period, te_width);
+  if (cb + cw > kWideChunks)
+    hipLaunchKernelGGL(k_flow_velocity_wide, dim3(cdiv(nt, 64)), dim3(64 * kSumGroups), 0, s,
+                       partial, ld, nt, cb, cw, ff[0], ff[1], ff[2], out, te_period, te_width);
+  else
+    hipLaunchKernelGGL(k_flow_velocity, dim3(cdiv(nt, 256)), dim3(256), 0, s,
+                       partial, ld, nt, cb, cw, ff[0], ff[1], ff[2], out,
+                       te_period, te_width);
 }
 
 }  // namespace uvlm
