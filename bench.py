@@ -2,19 +2,27 @@
 """UVLM hot-path benchmark on one MI355X.
 
 A step is one UVLM timestep (morph -> shed -> matrix -> RHS -> solve ->
-loads -> advect, simple_simulator.cpp:240-286) of the workload, run by the
-C++ SimpleSimulator on the gfx950 kernels.  Default workload: NACA4412
-cambered wing, 20x80 rings, root-hinged flapping +/-30 deg at k = 0.3 with
-5 deg pitch, reference "RK4" wake advection, steps 0..1000 (the wake grows to
-80,000 rings).  `--warmup W` untimed steps (from step 0), then `--steps K`
-timed steps.
+loads -> advect, simulator/simple_simulator.cpp:240-286) run by the C++
+SimpleSimulator on the gfx950 kernels.  Default workload: NACA4412-cambered
+wing, 20x80 rings, root-hinged flapping +/-30 deg at k = 0.3 with 5 deg
+pitch, reference "RK4" wake advection, 1000 steps (the wake grows to 80,000
+rings).  `--warmup W` untimed steps (from step 0), then `--steps K` timed
+steps; the defaults time steps 1..1000 of the 1001-iteration run.
 
-Prints one JSON line.  `value` is Biot-Savart pairs per second evaluated by
-the GPU over the timed steps (one pair = one segment evaluation, 56 FLOP);
-`timesteps_per_s` and the reference-loop pair model rate are reported beside
-it.  `cpu_baseline` times the oracle (a C restatement of the reference's
-loop, OpenMP over targets like the reference) on a bounded sample of the
-same sums at the final state, on the host's cores.
+Prints one JSON line.  `value` is Biot-Savart pairs per second in the
+reference's counting: the pairs the reference loop evaluates for the timed
+steps, 4 [N^2 + N Nw + L (N + Nw) + S V (N + Nw)] per step (one pair = one
+BiotSavartLaw call, vortex.cpp:21-39), divided by the wall time of those
+steps with inputs resident on the GPU.  `roofline` describes the dominant
+kernel (the FP64 segment-chain N-body): achieved = 56 FLOP x segment pairs
+it evaluated / its summed duration from HIP events on its stream.
+`cpu_baseline` times the oracle (a C restatement of the reference's loop,
+OpenMP over targets like the reference) on a bounded sample of the same
+sums at the final state, on the host's cores.
+
+Other modes: --workload c1|c3|c5 (single-case workloads), --workload c4
+(256 batched cases), --precision, --solve; --calibrate-copy runs a plain
+device copy of known size for the PMC traffic calibration.
 """
 from __future__ import annotations
 
@@ -30,15 +38,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "Biot-Savart pairs/s & timesteps/s vs plain FP64 GPU, CL/CT rel err ≤1e-4"
 FLOP_PER_PAIR = 56.0
-PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak, AMD published
-
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD published)
 PRECISIONS = {"f64exact": 0x0, "f64": 0x1, "f32c": 0x2}
-SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2}
+SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2, "fullpiv": 3}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_chain_f64_naca4412_20x80_rk4_traffic.json")
 
 
-def cpu_baseline(sim, case, seconds):
-    """Oracle flow velocity ((-ff + bound) + wake, advect.h:16-24) at a sample
+def cpu_baseline(sim, seconds):
+    """Oracle flow velocity of bound + wake rings (advect.h:16-24) at a sample
     of wake nodes of the final GPU state, timed on the host cores."""
     import oracle
     oracle.build()
@@ -57,35 +66,52 @@ def cpu_baseline(sim, case, seconds):
             oracle.induced_velocity(x, wn, wg)
         return time.perf_counter() - t0, 4.0 * nt * n_src
 
-    dt, pairs = run(16)
-    rate = pairs / max(dt, 1e-9)
-    nt = int(min(targets.shape[0], max(16, seconds * rate / (4.0 * n_src))))
+    nt = 16
     dt, pairs = run(nt)
+    while dt < 1.0 and nt < targets.shape[0]:  # calibrate on >= 1 s of work
+        nt = min(targets.shape[0], nt * 4)
+        dt, pairs = run(nt)
+    if dt < seconds and nt < targets.shape[0]:
+        nt = int(min(targets.shape[0], nt * seconds / dt))
+        dt, pairs = run(nt)
     return {"value": pairs / dt, "unit": "pairs/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": (f"oracle flow velocity at {nt} of {targets.shape[0]} wake nodes "
-                       f"against all {n_src} rings of the final state "
+            "sample": (f"oracle sum of all {n_src} bound and wake rings at {nt} of the "
+                       f"{targets.shape[0]} wake nodes of the final state "
                        f"({pairs:.3e} pairs in {dt:.1f} s)")}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed UVLM timesteps")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed steps before")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
-    ap.add_argument("--precision", default="f64exact", choices=sorted(PRECISIONS))
-    ap.add_argument("--solve", default="f64", choices=sorted(SOLVES))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
+def traffic_from_profile():
+    """HBM bytes per launch of the N-body from the committed PMC summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
-    if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: the UVLM step of one case is one "
-              "GPU's work here (multi-GPU sharding by targets is a design note, "
-              "not a measured configuration)", file=sys.stderr)
-        sys.exit(2)
 
+def sync_all():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def calibrate_copy(nbytes=1 << 28):
+    """A plain device-to-device copy of known size (for FETCH/WRITE_SIZE)."""
+    import torch
+    x = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+
+
+def run_single(args):
     from uvlm_amd import Simulation
     from uvlm_amd import workloads as W
 
@@ -94,10 +120,10 @@ def main():
     case["precision"] = PRECISIONS[args.precision]
     case["solve_mode"] = SOLVES[args.solve]
     sim = Simulation(case, total - 1)
-
     for s in range(args.warmup):
         sim.step(s)
     sim.sync()
+    sync_all()
     e0, r0 = sim.pairs()
     sim.profile(True)
     sim.profile_read()
@@ -107,17 +133,19 @@ def main():
         if (s - args.warmup) % 100 == 99:
             print(f"# step {s}", file=sys.stderr, flush=True)
     sim.sync()
+    sync_all()
     T = time.perf_counter() - t0
     launches, kms, kpairs = sim.profile_read()
     sim.profile(False)
     e1, r1 = sim.pairs()
-
-    pairs = e1 - e0
-    achieved = FLOP_PER_PAIR * kpairs / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     loads = sim.loads()
+    achieved = FLOP_PER_PAIR * kpairs / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    traffic = traffic_from_profile() if (args.workload == "c2" and args.precision == "f64"
+                                         and args.steps == 1000 and args.warmup == 1) else None
+    nw = len(case["wings"])
     out = {
-        "metric": "Biot-Savart pairs/s & timesteps/s vs plain FP64 GPU, CL/CT rel err <=1e-4",
-        "value": pairs / T,
+        "metric": METRIC,
+        "value": (r1 - r0) / T,
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -128,24 +156,90 @@ def main():
         "vs_baseline": None,
         "dtype": {"f64exact": "f64", "f64": "f64", "f32c": "f32"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": case["name"], "wings": len(case["wings"]),
+        "config": {"workload": f"{case['name']}_{total - 1}steps", "wings": nw,
                    "rings_per_wing": [case["rows"], case["cols"]],
                    "advect": case["advect"], "dt": case["dt"],
                    "timed_steps": [args.warmup, total - 1],
-                   "final_wake_rings": len(case["wings"]) * (total - 1) * case["cols"],
+                   "final_wake_rings": nw * (total - 1) * case["cols"],
                    "precision": args.precision, "solve": args.solve},
         "timesteps_per_s": args.steps / T,
-        "reference_model_pairs_per_s": (r1 - r0) / T,
+        "gpu_evaluated_pairs_per_s": (e1 - e0) / T,
         "final_CD_CL": loads[-1].tolist() if len(loads) else None,
-        "roofline": {"bound": "valu", "kernel": "k_nbody (Biot-Savart N-body)",
+        "roofline": {"bound": "valu", "kernel": "Biot-Savart N-body (k_chain_f64)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "launches": launches, "kernel_ms": kms,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "launches": launches, "avg_launch_us": 1e3 * kms / max(launches, 1),
+                     "flop_per_launch": FLOP_PER_PAIR * kpairs / max(launches, 1),
                      "kernel_share_of_step_time": kms / 1e3 / T},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sim, case, args.cpu_seconds)
-        out["cpu_baseline"]["gpu_over_cpu_pairs_per_s"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"] = cpu_baseline(sim, args.cpu_seconds)
+    return out
+
+
+def run_batch(args):
+    from uvlm_amd import workloads as W
+    from uvlm_amd.gpu import Batch
+
+    total = args.warmup + args.steps
+    cases = W.batch_c4(steps=total - 1)
+    b = Batch(cases, max_steps=total - 1, advect="RK2")
+    b.run(args.warmup)
+    chosen, times = b.autotune()
+    b.sync()
+    p0 = b.pairs()
+    sync_all()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(50, args.steps - done)
+        b.run(k)
+        done += k
+        print(f"# step {args.warmup + done - 1}", file=sys.stderr, flush=True)
+    b.sync()
+    sync_all()
+    T = time.perf_counter() - t0
+    p1 = b.pairs()
+    ref = 0.0
+    for c in cases:
+        ref += W.pair_model(c, steps=total - 1, stages=2) - W.pair_model(c, steps=args.warmup - 1,
+                                                                          stages=2)
+    return {
+        "metric": METRIC, "value": ref / T, "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * T / args.steps,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batch_256_cases_rk2_{total - 1}steps", "cases": len(cases),
+                   "timed_steps": [args.warmup, total - 1], "autotuned_variant": chosen},
+        "case_steps_per_s": len(cases) * args.steps / T,
+        "gpu_evaluated_pairs_per_s": (p1 - p0) / T,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000, help="timed UVLM timesteps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed steps before")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--precision", default="f64", choices=sorted(PRECISIONS))
+    ap.add_argument("--solve", default="f64", choices=sorted(SOLVES))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the UVLM step of a case "
+              "on a single MI355X (sharding over GPUs is a design note, not a measured "
+              "configuration)", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.warmup < 1:
+        args.warmup = 1
+    out = run_batch(args) if args.workload == "c4" else run_single(args)
     print(json.dumps(out), flush=True)
 
 
