@@ -184,7 +184,7 @@ def run_batch(args):
     total = args.warmup + args.steps
     cases = W.batch_c4(steps=total - 1)
     b = Batch(cases, max_steps=total - 1, advect="RK2")
-    b.run(args.warmup)
+    b.run(args.warmup, graph=not args.no_graph)
     chosen, times = b.autotune()
     b.sync()
     p0 = b.pairs()
@@ -193,7 +193,7 @@ def run_batch(args):
     done = 0
     while done < args.steps:
         k = min(50, args.steps - done)
-        b.run(k)
+        b.run(k, graph=not args.no_graph)
         done += k
         print(f"# step {args.warmup + done - 1}", file=sys.stderr, flush=True)
     b.sync()
@@ -210,7 +210,8 @@ def run_batch(args):
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"batch_256_cases_rk2_{total - 1}steps", "cases": len(cases),
-                   "timed_steps": [args.warmup, total - 1], "autotuned_variant": chosen},
+                   "timed_steps": [args.warmup, total - 1], "autotuned_variant": chosen,
+                   "hipgraph": not args.no_graph},
         "case_steps_per_s": len(cases) * args.steps / T,
         "gpu_evaluated_pairs_per_s": (p1 - p0) / T,
     }
@@ -227,6 +228,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch: launch each step's kernels directly instead of a hipGraph")
     args = ap.parse_args()
 
     if args.gpus != 1:

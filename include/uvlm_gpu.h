@@ -228,6 +228,11 @@ int uvlm_batch_get_wake(uvlm_ctx* ctx, size_t c, double* nodes, double* gamma);
 int uvlm_batch_get_loads(uvlm_ctx* ctx, size_t c, double* out, size_t* count_inout);
 /* Segment evaluations of all steps run so far. */
 int uvlm_batch_pairs(uvlm_ctx* ctx, double* pairs);
+/* Options.  UVLM_BATCH_FORCE_PIVOTED (value 0/1): solve every case through
+ * the partially pivoted fallback instead of the checked no-pivot path (the
+ * results agree; used to test the fallback). */
+#define UVLM_BATCH_FORCE_PIVOTED 1
+int uvlm_batch_set_option(uvlm_ctx* ctx, int option, int value);
 int uvlm_batch_free(uvlm_ctx* ctx);
 
 /* --- measurement hook (no reference counterpart) ------------------------

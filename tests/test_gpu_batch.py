@@ -3,9 +3,10 @@
 Every case of a batch is an independent single-wing SimpleSimulator
 (simulator/simple_simulator.cpp:288-313); the batch must reproduce each
 case's circulations, wake and CD/CL.  The batch sums with FP64 segment
-chains (FMA, refined rsqrt) and solves with partially pivoted LU, the oracle
-with the reference's order and full pivoting, so agreement is to 1e-9
-relative (the largest magnitude of each quantity), not bitwise.
+chains (FMA, refined rsqrt), evaluates the kinematics with the device's
+sin/cos and solves with LU (partial pivoting's choices), the oracle with the
+reference's order, glibc and full pivoting, so agreement is to 1e-9 relative
+(the largest magnitude of each quantity), not bitwise.
 """
 import math
 
@@ -142,3 +143,15 @@ def test_capacity_and_arguments(ctx):
     with pytest.raises(UVLMError) as e:
         Batch(cases, max_steps=2, advect="AB2", ctx=ctx)
     assert e.value.code == N.ERR_ARG
+
+
+def test_pivoted_fallback_agrees(orc):
+    """The partially pivoted fallback of the batched solve (normally taken
+    only by cases whose pivoting checks fail) gives the same runs."""
+    cases = mixed_cases("RK2")
+    b = Batch(cases, max_steps=8, advect="RK2", ignore_steps=2)
+    b.force_pivoted(True)
+    b.run(8)
+    b.sync()
+    for k, case in enumerate(cases):
+        check_case(b, k, *oracle_run(orc, case, 8))

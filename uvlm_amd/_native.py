@@ -20,6 +20,7 @@ PREC_F64_EXACT, PREC_F64, PREC_F32C = 0x0, 0x1, 0x2
 SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
 SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV = 0, 1, 2, 3
 ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
+BATCH_FORCE_PIVOTED = 1
 SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
            "AB2": ADVECT_AB2, "RK4classic": ADVECT_RK4_CLASSIC}
 GEN = {"rect": 0, "naca4": 1, "naca00": 2}
@@ -37,7 +38,7 @@ GPU_SYMBOLS = [
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
-    "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_free",
+    "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_set_option", "uvlm_batch_free",
 ]
 HOST_SYMBOLS = [
     "uvlm_host_sim_create", "uvlm_host_sim_destroy", "uvlm_host_last_error",
@@ -143,6 +144,7 @@ def gpu_lib():
         L.uvlm_batch_get_wake.argtypes = [_vp, _sz, _dp, _dp]
         L.uvlm_batch_get_loads.argtypes = [_vp, _sz, _dp, C.POINTER(_sz)]
         L.uvlm_batch_pairs.argtypes = [_vp, _dp]
+        L.uvlm_batch_set_option.argtypes = [_vp, C.c_int, C.c_int]
         L.uvlm_batch_free.argtypes = [_vp]
         _gpu = L
     return _gpu

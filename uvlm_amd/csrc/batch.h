@@ -54,6 +54,8 @@ class BatchEngine {
   int get_loads(int c, double* out, size_t* count, std::string* err);
   int get_wing(int c, double* out, std::string* err);
   double pairs_per_step_at(int step) const;  // segment evaluations of one step
+  // Run every case's solve through the partially pivoted path (test hook).
+  void set_force_pivoted(bool on);
 
  private:
   void release();
@@ -86,6 +88,10 @@ class BatchEngine {
          *d_wake_ = nullptr, *d_wgam_ = nullptr, *d_pos_ = nullptr, *d_pst_ = nullptr,
          *d_k1_ = nullptr, *d_k2_ = nullptr, *d_k3_ = nullptr, *d_hist_ = nullptr;
   int* d_piv_ = nullptr;
+  int* d_flag_ = nullptr;
+  double* d_Asave_ = nullptr;
+  long tot_A_ = 0;
+  bool force_pivoted_ = false;
   RingRec* d_rings_ = nullptr;
   ChainRec *d_bch_ = nullptr, *d_wch_ = nullptr;
   int2 *d_it_rhs_ = nullptr, *d_it_loads_ = nullptr, *d_it_adv_ = nullptr;

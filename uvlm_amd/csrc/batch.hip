@@ -15,8 +15,10 @@
 //                                        (case, target tile) items fetched with an
 //                                        atomic counter; sources are the case's
 //                                        segment chains staged through LDS
-//   solve        k_b_getf2/laswp/trsm/gemm per panel over all cases (static
-//                item lists), k_b_getrs one workgroup per case
+//   solve        checked no-pivot panels (k_b_panel_np) + k_b_trsm + k_b_gemm
+//                per panel over all cases (static item lists); cases whose
+//                pivoting checks failed are restored and refactored with
+//                partial pivoting (k_b_getf2/laswp); k_b_getrs per case
 //   loads        k_b_loads               one workgroup per case
 //   advection    Euler / RK2 / RK4 (reference formula) stages, k_b_update
 //
@@ -33,6 +35,7 @@
 #include "../../include/uvlm_gpu.h"
 #include "batch.h"
 #include "devmath.h"
+#include "lu_dev.h"
 
 namespace uvlm {
 namespace {
@@ -397,7 +400,9 @@ constexpr int kNB = 32;
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(1024) void k_b_getf2(const BCase* __restrict__ cases, int k0,
-                                                  double* __restrict__ Ab, int* __restrict__ piv) {
+                                                  double* __restrict__ Ab, int* __restrict__ piv,
+                                                  const int* __restrict__ only) {
+  if (!only[blockIdx.x]) return;
   const BCase& c = cases[blockIdx.x];
   const int n = c.N;
   if (k0 >= n) return;
@@ -468,7 +473,8 @@ __global__ __launch_bounds__(1024) void k_b_getf2(const BCase* __restrict__ case
 }
 
 __global__ void k_b_laswp(const BCase* __restrict__ cases, int k0, double* __restrict__ Ab,
-                          const int* __restrict__ piv) {
+                          const int* __restrict__ piv, const int* __restrict__ only) {
+  if (!only[blockIdx.y]) return;
   const BCase& c = cases[blockIdx.y];
   const int n = c.N;
   if (k0 >= n) return;
@@ -489,7 +495,9 @@ __global__ void k_b_laswp(const BCase* __restrict__ cases, int k0, double* __res
 }
 
 __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, int k0,
-                                               double* __restrict__ Ab) {
+                                               double* __restrict__ Ab,
+                                               const int* __restrict__ only) {
+  if (only && !only[blockIdx.y]) return;
   const BCase& c = cases[blockIdx.y];
   const int n = c.N;
   if (k0 >= n) return;
@@ -518,8 +526,10 @@ __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, 
 // items: (case, row tile, col tile, -) of the trailing update after panel k0
 __global__ __launch_bounds__(256) void k_b_gemm(const BCase* __restrict__ cases,
                                                 const int4* __restrict__ items, int k0,
-                                                double* __restrict__ Ab) {
+                                                double* __restrict__ Ab,
+                                                const int* __restrict__ only) {
   const int4 it = items[blockIdx.x];
+  if (only && !only[it.x]) return;
   const BCase& c = cases[it.x];
   const int n = c.N;
   const int kb = min(kNB, n - k0);
@@ -574,67 +584,113 @@ __global__ __launch_bounds__(256) void k_b_gemm(const BCase* __restrict__ cases,
       }
 }
 
-// Substitution per case (one workgroup each): x = U^-1 L^-1 P rhs -> gamma.
+// Checked no-pivot panel (the single-case path's k_panel_np, per case):
+// grid (row blocks of 64, cases); every workgroup factors the case's top
+// block itself, workgroup 0 stores it; flag[case] <- 1 when a comparison of
+// partial pivoting would have chosen an interchange.
+__global__ __launch_bounds__(64) void k_b_panel_np(const BCase* __restrict__ cases, int k0,
+                                                   double* __restrict__ Ab,
+                                                   int* __restrict__ piv,
+                                                   int* __restrict__ flag) {
+  const BCase& c = cases[blockIdx.y];
+  const int n = c.N;
+  if (k0 >= n) return;
+  const int kb = min(kNB, n - k0);
+  if ((int)blockIdx.x * 64 >= max(n - k0 - kb, 1)) return;
+  double* A = Ab + c.A_off;
+  __shared__ double D[kNB][kNB + 1];
+  __shared__ double X[kNB][64 + 1];
+  __shared__ int s_flag;
+  const int t = threadIdx.x;
+  if (t == 0) s_flag = 0;
+  for (int q = t; q < kNB * kNB; q += 64) {
+    const int r = q % kNB, cc = q / kNB;
+    D[r][cc] = (r < kb && cc < kb) ? A[(long)(k0 + cc) * n + k0 + r] : 0.0;
+  }
+  const int row = k0 + kb + blockIdx.x * 64 + t;
+  if (row < n)
+    for (int cc = 0; cc < kb; ++cc) X[cc][t] = A[(long)(k0 + cc) * n + row];
+  for (int j = 0; j < kb; ++j) {
+    __syncthreads();
+    const double d = D[j][j];
+    if (t > j && t < kb) {
+      double l = D[t][j];
+      if (fabs(l) > fabs(d) || d == 0.0) s_flag = 1;
+      if (d != 0.0) l = l / d;
+      D[t][j] = l;
+#pragma unroll 8
+      for (int cc = j + 1; cc < kb; ++cc) D[t][cc] = D[t][cc] - l * D[j][cc];
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int q = t; q < kb * kb; q += 64) {
+      const int r = q % kb, cc = q / kb;
+      A[(long)(k0 + cc) * n + k0 + r] = D[r][cc];
+    }
+    if (t < kb) piv[c.ring_off + k0 + t] = k0 + t;
+  }
+  if (row < n) {
+    bool bad = false;
+    for (int j = 0; j < kb; ++j) {
+      const double d = D[j][j];
+      double xj = X[j][t];
+      bad |= fabs(xj) > fabs(d);
+      if (d != 0.0) xj = xj / d;
+      X[j][t] = xj;
+#pragma unroll 8
+      for (int cc = j + 1; cc < kb; ++cc) X[cc][t] = X[cc][t] - xj * D[j][cc];
+    }
+    if (bad) s_flag = 1;
+    for (int cc = 0; cc < kb; ++cc) A[(long)(k0 + cc) * n + row] = X[cc][t];
+  }
+  __syncthreads();
+  if (t == 0 && s_flag) atomicOr(&flag[blockIdx.y], 1);
+}
+
+__global__ void k_b_init_flags(int* flag, int n, int value) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = value;
+}
+
+// Flagged cases only: A <- saved copy (the pivoted factorisation restarts).
+__global__ void k_b_restore(const BCase* __restrict__ cases, const int* __restrict__ flag,
+                            const double* __restrict__ saved, double* __restrict__ Ab) {
+  const BCase& c = cases[blockIdx.y];
+  if (!flag[blockIdx.y]) return;
+  const long nn = (long)c.N * c.N;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nn;
+       q += (long)gridDim.x * blockDim.x)
+    Ab[c.A_off + q] = saved[c.A_off + q];
+}
+
+// Substitution per case (one workgroup each): x = U^-1 L^-1 P rhs -> gamma,
+// x resident in LDS (dynamic, maxN doubles) or, for large cases, in gamma.
 __global__ __launch_bounds__(1024) void k_b_getrs(const BCase* __restrict__ cases,
                                                   const double* __restrict__ Ab,
                                                   const int* __restrict__ piv,
                                                   const double* __restrict__ rhs,
-                                                  double* __restrict__ gamma) {
+                                                  double* __restrict__ gamma, int lds_n) {
+  extern __shared__ unsigned char s_raw[];
+  __shared__ double D[64][65];
   const BCase& c = cases[blockIdx.x];
   const int n = c.N;
-  const double* LU = Ab + c.A_off;
+  double* x = n <= lds_n ? reinterpret_cast<double*>(s_raw) : gamma + c.ring_off;
   const int* ipiv = piv + c.ring_off;
-  double* b = gamma + c.ring_off;
-  const double* r0 = rhs + c.ring_off;
-  __shared__ double xs[64];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += 1024) b[i] = r0[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = rhs[c.ring_off + i];
   __syncthreads();
-  if (tid == 0)
+  if (threadIdx.x == 0)
     for (int i = 0; i < n; ++i) {
       const int p = ipiv[i];
       if (p != i) {
-        const double t = b[i];
-        b[i] = b[p];
-        b[p] = t;
+        const double t = x[i];
+        x[i] = x[p];
+        x[p] = t;
       }
     }
-  __syncthreads();
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int ib = min(64, n - i0);
-    if (tid < ib) xs[tid] = b[i0 + tid];
-    __syncthreads();
-    for (int i = 0; i < ib; ++i) {
-      if (tid > i && tid < ib) xs[tid] = xs[tid] - LU[(long)(i0 + i) * n + i0 + tid] * xs[i];
-      __syncthreads();
-    }
-    if (tid < ib) b[i0 + tid] = xs[tid];
-    for (int r = i0 + ib + tid; r < n; r += 1024) {
-      double sacc = b[r];
-      for (int i = 0; i < ib; ++i) sacc = sacc - LU[(long)(i0 + i) * n + r] * xs[i];
-      b[r] = sacc;
-    }
-    __syncthreads();
-  }
-  for (int iend = n; iend > 0; iend -= 64) {
-    const int i0 = max(0, iend - 64);
-    const int ib = iend - i0;
-    if (tid < ib) xs[tid] = b[i0 + tid];
-    __syncthreads();
-    for (int i = ib - 1; i >= 0; --i) {
-      if (tid == i) xs[i] = xs[i] / LU[(long)(i0 + i) * n + i0 + i];
-      __syncthreads();
-      if (tid < i) xs[tid] = xs[tid] - LU[(long)(i0 + i) * n + i0 + tid] * xs[i];
-      __syncthreads();
-    }
-    if (tid < ib) b[i0 + tid] = xs[tid];
-    for (int r = tid; r < i0; r += 1024) {
-      double sacc = b[r];
-      for (int i = 0; i < ib; ++i) sacc = sacc - LU[(long)(i0 + i) * n + r] * xs[i];
-      b[r] = sacc;
-    }
-    __syncthreads();
-  }
+  lu_substitute(Ab + c.A_off, n, x, D);
+  if (n <= lds_n)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) gamma[c.ring_off + i] = x[i];
 }
 
 __global__ void k_b_edges(const BCase* __restrict__ cases, const int* __restrict__ edge_case,
@@ -1006,6 +1062,9 @@ int BatchEngine::init(hipStream_t s, const BatchCaseSpec* specs, const double* w
   BTRY(dev_alloc(allocs_, &d_mid_, 3 * edges, err));
   BTRY(dev_alloc(allocs_, &d_uflow_, 3 * edges, err));
   BTRY(dev_alloc(allocs_, &d_A_, Aoff, err));
+  BTRY(dev_alloc(allocs_, &d_Asave_, Aoff, err));
+  BTRY(dev_alloc(allocs_, &d_flag_, n, err));
+  tot_A_ = Aoff;
   BTRY(dev_alloc(allocs_, &d_piv_, rings, err));
   BTRY(dev_alloc(allocs_, &d_wake_, 3 * woff, err));
   BTRY(dev_alloc(allocs_, &d_wgam_, goff, err));
@@ -1131,20 +1190,39 @@ int BatchEngine::step_once(std::string* err) {
   BTRY(nbody(0, d_cp_, d_uw_, 0));
   hipLaunchKernelGGL(k_b_rhs, g(tot_rings_), b256, 0, s, d_cases_, d_ring_case_, d_uw_, d_vm_,
                      d_nrm_, d_rhs_, tot_rings_);
-  // batched LU, panel by panel over all cases
+  // batched LU: checked no-pivot panels for every case; cases whose checks
+  // failed are restored and refactored with partial pivoting (those
+  // launches return at once for the other cases)
+  hipLaunchKernelGGL(k_b_init_flags, g(n_), b256, 0, s, d_flag_, n_, force_pivoted_ ? 1 : 0);
+  hipLaunchKernelGGL(k_b_copy, g(tot_A_), b256, 0, s, d_A_, d_Asave_, tot_A_);
   for (size_t p = 0; p + 1 < gemm_off_.size(); ++p) {
     const int k0 = (int)p * kNB;
-    hipLaunchKernelGGL(k_b_getf2, dim3(n_), dim3(1024), 0, s, d_cases_, k0, d_A_, d_piv_);
-    hipLaunchKernelGGL(k_b_laswp, dim3(cdiv(maxN_, 256), n_), b256, 0, s, d_cases_, k0, d_A_,
-                       d_piv_);
-    hipLaunchKernelGGL(k_b_trsm, dim3(cdiv(maxN_, 64), n_), dim3(64), 0, s, d_cases_, k0, d_A_);
+    hipLaunchKernelGGL(k_b_panel_np, dim3(std::max(1, cdiv(maxN_ - k0, 64)), n_), dim3(64), 0, s,
+                       d_cases_, k0, d_A_, d_piv_, d_flag_);
+    hipLaunchKernelGGL(k_b_trsm, dim3(cdiv(maxN_, 64), n_), dim3(64), 0, s, d_cases_, k0, d_A_,
+                       (const int*)nullptr);
     const int ng = gemm_off_[p + 1] - gemm_off_[p];
     if (ng > 0)
       hipLaunchKernelGGL(k_b_gemm, dim3(ng), b256, 0, s, d_cases_, d_it_gemm_ + gemm_off_[p], k0,
-                         d_A_);
+                         d_A_, (const int*)nullptr);
   }
-  hipLaunchKernelGGL(k_b_getrs, dim3(n_), dim3(1024), 0, s, d_cases_, d_A_, d_piv_, d_rhs_,
-                     d_gamma_);
+  hipLaunchKernelGGL(k_b_restore, dim3(64, n_), b256, 0, s, d_cases_, d_flag_, d_Asave_, d_A_);
+  for (size_t p = 0; p + 1 < gemm_off_.size(); ++p) {
+    const int k0 = (int)p * kNB;
+    hipLaunchKernelGGL(k_b_getf2, dim3(n_), dim3(1024), 0, s, d_cases_, k0, d_A_, d_piv_,
+                       (const int*)d_flag_);
+    hipLaunchKernelGGL(k_b_laswp, dim3(cdiv(maxN_, 256), n_), b256, 0, s, d_cases_, k0, d_A_,
+                       d_piv_, (const int*)d_flag_);
+    hipLaunchKernelGGL(k_b_trsm, dim3(cdiv(maxN_, 64), n_), dim3(64), 0, s, d_cases_, k0, d_A_,
+                       (const int*)d_flag_);
+    const int ng = gemm_off_[p + 1] - gemm_off_[p];
+    if (ng > 0)
+      hipLaunchKernelGGL(k_b_gemm, dim3(ng), b256, 0, s, d_cases_, d_it_gemm_ + gemm_off_[p], k0,
+                         d_A_, (const int*)d_flag_);
+  }
+  const int lds_n = maxN_ <= 4096 ? maxN_ : 0;
+  hipLaunchKernelGGL(k_b_getrs, dim3(n_), dim3(1024), sizeof(double) * lds_n, s, d_cases_, d_A_,
+                     d_piv_, d_rhs_, d_gamma_, lds_n);
   hipLaunchKernelGGL((k_b_pack_chains<false>), g(tot_cb_), b256, 0, s, d_cases_, d_state_,
                      d_chb_start_, n_, d_wing_, 0, d_gamma_, d_bch_, tot_cb_);
   // loads
@@ -1177,6 +1255,16 @@ int BatchEngine::step_once(std::string* err) {
   hipLaunchKernelGGL(k_b_end, dim3(1), dim3(64), 0, s, d_state_);
   BHIP(hipGetLastError());
   return UVLM_OK;
+}
+
+void BatchEngine::set_force_pivoted(bool on) {
+  if (on != force_pivoted_ && graph_exec_) {
+    (void)hipGraphExecDestroy(graph_exec_);
+    (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+  }
+  force_pivoted_ = on;
 }
 
 int BatchEngine::capture(std::string* err) {

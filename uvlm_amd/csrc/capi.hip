@@ -1132,6 +1132,14 @@ int uvlm_batch_pairs(uvlm_ctx* c, double* pairs) {
   return UVLM_OK;
 }
 
+int uvlm_batch_set_option(uvlm_ctx* c, int option, int value) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(need_batch(c));
+  if (option != UVLM_BATCH_FORCE_PIVOTED) return fail(c, UVLM_ERR_ARG, "unknown option %d", option);
+  c->batch->set_force_pivoted(value != 0);
+  return UVLM_OK;
+}
+
 int uvlm_batch_free(uvlm_ctx* c) {
   if (!c) return UVLM_ERR_ARG;
   TRY(set_dev(c));

@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "internal.h"
+#include "lu_dev.h"
 
 namespace uvlm {
 namespace {
@@ -543,60 +544,8 @@ constexpr int kGetrsLds = 4096;  // FP64 x of up to 32 KB in LDS (+ 33 KB block)
 template <typename T>
 __device__ void getrs_body(const T* __restrict__ LU, int n, const int* __restrict__ perm,
                            const T* __restrict__ b, T* x, T (*D)[65]) {
-  const int tid = threadIdx.x, lane = tid & 63;
-  for (int i = tid; i < n; i += 1024) x[i] = b[perm[i]];
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int ib = min(64, n - i0);
-    for (int q = tid; q < 64 * 64; q += 1024) {
-      const int r = q % 64, c = q / 64;
-      D[r][c] = (r < ib && c < ib) ? LU[(size_t)(i0 + c) * n + i0 + r] : T(0);
-    }
-    __syncthreads();
-    if (tid < 64) {
-      T v = lane < ib ? x[i0 + lane] : T(0);
-      for (int i = 0; i < ib; ++i) {
-        const T xi = __shfl(v, i);
-        if (lane > i && lane < ib) v = v - D[lane][i] * xi;
-      }
-      if (lane < ib) x[i0 + lane] = v;
-    }
-    __syncthreads();
-    for (int r = i0 + ib + tid; r < n; r += 1024) {
-      T s = x[r];
-      const T* row = LU + (size_t)i0 * n + r;
-#pragma unroll 16
-      for (int i = 0; i < ib; ++i) s = s - row[(size_t)i * n] * x[i0 + i];
-      x[r] = s;
-    }
-    __syncthreads();
-  }
-  for (int iend = n; iend > 0; iend -= 64) {
-    const int i0 = max(0, iend - 64), ib = iend - i0;
-    for (int q = tid; q < 64 * 64; q += 1024) {
-      const int r = q % 64, c = q / 64;
-      D[r][c] = (r < ib && c < ib) ? LU[(size_t)(i0 + c) * n + i0 + r] : T(0);
-    }
-    __syncthreads();
-    if (tid < 64) {
-      T v = lane < ib ? x[i0 + lane] : T(0);
-      const T dl = lane < ib ? D[lane][lane] : T(1);
-      for (int i = ib - 1; i >= 0; --i) {
-        if (lane == i) v = v / dl;
-        const T xi = __shfl(v, i);
-        if (lane < i) v = v - D[lane][i] * xi;
-      }
-      if (lane < ib) x[i0 + lane] = v;
-    }
-    __syncthreads();
-    for (int r = tid; r < i0; r += 1024) {
-      T s = x[r];
-      const T* row = LU + (size_t)i0 * n + r;
-#pragma unroll 16
-      for (int i = 0; i < ib; ++i) s = s - row[(size_t)i * n] * x[i0 + i];
-      x[r] = s;
-    }
-    __syncthreads();
-  }
+  for (int i = threadIdx.x; i < n; i += 1024) x[i] = b[perm[i]];
+  lu_substitute(LU, n, x, D);
 }
 
 template <typename T>

@@ -404,3 +404,8 @@ class Batch:
         p = C.c_double()
         self.ctx._check(self._L.uvlm_batch_pairs(self.ctx._h, C.byref(p)), "pairs")
         return p.value
+
+    def force_pivoted(self, on: bool = True):
+        """Solve through the partially pivoted fallback (test hook)."""
+        self.ctx._check(self._L.uvlm_batch_set_option(self.ctx._h, N.BATCH_FORCE_PIVOTED,
+                                                      int(on)), "set_option")
