@@ -120,6 +120,8 @@ def run_single(args):
     case["precision"] = PRECISIONS[args.precision]
     case["solve_mode"] = SOLVES[args.solve]
     sim = Simulation(case, total - 1)
+    if args.farfield > 0:
+        sim.set_farfield(args.farfield, args.audit, args.ff_tol)
     for s in range(args.warmup):
         sim.step(s)
     sim.sync()
@@ -161,7 +163,8 @@ def run_single(args):
                    "advect": case["advect"], "dt": case["dt"],
                    "timed_steps": [args.warmup, total - 1],
                    "final_wake_rings": nw * (total - 1) * case["cols"],
-                   "precision": args.precision, "solve": args.solve},
+                   "precision": args.precision, "solve": args.solve,
+                   "farfield_theta": args.farfield if args.farfield > 0 else None},
         "timesteps_per_s": args.steps / T,
         "gpu_evaluated_pairs_per_s": (e1 - e0) / T,
         "final_CD_CL": loads[-1].tolist() if len(loads) else None,
@@ -172,6 +175,10 @@ def run_single(args):
                      "flop_per_launch": FLOP_PER_PAIR * kpairs / max(launches, 1),
                      "kernel_share_of_step_time": kms / 1e3 / T},
     }
+    if args.farfield > 0:
+        err, audits, active = sim.farfield_stats()
+        out["farfield"] = {"theta": args.farfield, "audit_every": args.audit,
+                           "last_audit_rel_err": err, "audits": audits, "active": active}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sim, args.cpu_seconds)
     return out
@@ -228,6 +235,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--farfield", type=float, default=0.0,
+                    help="near/far split of the wake sum with this theta (0: exact sums)")
+    ap.add_argument("--audit", type=int, default=50, help="far field: exact audit every N steps")
+    ap.add_argument("--ff-tol", type=float, default=1e-3, help="far field: audit tolerance")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch: launch each step's kernels directly instead of a hipGraph")
     args = ap.parse_args()

@@ -36,6 +36,7 @@ GPU_SYMBOLS = [
     "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
+    "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_farfield_stats",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
     "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_set_option", "uvlm_batch_free",
@@ -135,6 +136,10 @@ def gpu_lib():
         L.uvlm_sim_pair_counts.argtypes = [_vp, _dp, _dp]
         L.uvlm_profile_enable.argtypes = [_vp, C.c_int]
         L.uvlm_profile_read.argtypes = [_vp, C.POINTER(_sz), _dp, _dp]
+        L.uvlm_farfield_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
+                                             _dp]
+        L.uvlm_sim_set_farfield.argtypes = [_vp, C.c_double, C.c_int, C.c_double]
+        L.uvlm_sim_farfield_stats.argtypes = [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
         L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
         L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]

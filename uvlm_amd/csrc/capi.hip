@@ -16,6 +16,7 @@
 
 #include "../../include/uvlm_gpu.h"
 #include "batch.h"
+#include "farfield.h"
 #include "internal.h"
 #include "step.h"
 
@@ -65,6 +66,12 @@ struct uvlm_ctx {
   // batched cases
   uvlm::BatchEngine* batch = nullptr;
   double batch_pairs = 0.0;
+  // far-field split of the wake sum in the advection stages
+  uvlm::FarField* ff = nullptr;
+  double ff_theta = 0.0, ff_tol = 0.0, ff_last_err = -1.0;
+  int ff_audit_every = 0, ff_counter = 0, ff_audits = 0;
+  bool ff_off = false;
+  Buf ffg, ffv, ffx, ffstat;
 };
 
 namespace {
@@ -284,13 +291,15 @@ int uvlm_destroy(uvlm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   delete c->batch;
   c->batch = nullptr;
+  delete c->ff;
+  c->ff = nullptr;
   Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
                  &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
                  &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
                  &c->area, &c->recs, &c->chains, &c->partial, &c->A, &c->rhs, &c->ipiv,
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
-                 &c->ref_area, &c->work, &c->tmp};
+                 &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat};
   for (Buf* b : bufs) release(*b);
   free_staging(c->st_wing);
   free_staging(c->st_vm);
@@ -839,12 +848,66 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
   launch_gather_rows(c->stream, P<double>(c->wake), c->cap, c->wstart, c->num, rows, C1,
                      P<double>(c->pos));
   const int te_period = rows * C1;
-  auto stage = [&](const double* targets, Buf& out) -> int {
+  auto exact_stage = [&](const double* targets, double* out) -> int {
     ChunkPlan plan;
     TRY(sum_sources(c, targets, V, true, true, &plan));
-    launch_flow_velocity(c->stream, P<double>(c->partial), V, V, plan.cb, plan.cw, ff,
-                         P<double>(out), te_period, C1);
+    launch_flow_velocity(c->stream, P<double>(c->partial), V, V, plan.cb, plan.cw, ff, out,
+                         te_period, C1);
     return check_launch(c);
+  };
+  // Far-field stage: bound sheets exactly, the wake (whose nodes are the
+  // targets) through the near/far split; every ff_audit_every steps the
+  // first stage is also summed exactly, compared, and the split switched
+  // off when the difference exceeds ff_tol of the largest velocity.
+  const bool use_ff = c->ff_theta > 0.0 && !c->ff_off && c->wgrows > 0;
+  bool audit = false;
+  if (use_ff) {
+    audit = c->ff_audit_every > 0 && (c->ff_counter % c->ff_audit_every) == 0;
+    ++c->ff_counter;
+    TRY(ensure(c, c->ffg, sizeof(double) * (size_t)c->num * c->wgrows * c->C));
+    TRY(ensure(c, c->ffv, vb));
+    TRY(ensure(c, c->ffstat, 2 * sizeof(unsigned long long)));
+    if (audit) TRY(ensure(c, c->ffx, vb));
+    launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
+                         c->C, P<double>(c->ffg));
+  }
+  auto ff_stage = [&](const double* targets, double* out) -> int {
+    ChunkPlan plan;
+    TRY(sum_sources(c, targets, V, true, false, &plan));
+    std::string e;
+    int ntiles = 0;
+    const int* map = c->ff->lattice_tiles(c->stream, c->num, c->wgrows, c->C, &ntiles, &e);
+    if (!map) return fail(c, UVLM_ERR_OOM, "%s", e.c_str());
+    const int rc = c->ff->wake_velocity(c->stream, targets, P<double>(c->ffg), c->num,
+                                        c->wgrows, c->C, targets, V, map, ntiles, c->ff_theta,
+                                        P<double>(c->ffv), &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    launch_ff_combine(c->stream, P<double>(c->partial), V, plan.cb, P<double>(c->ffv), V, ff,
+                      te_period, C1, out);
+    return check_launch(c);
+  };
+  bool first = true;
+  auto stage = [&](const double* targets, Buf& out) -> int {
+    if (!use_ff) return exact_stage(targets, P<double>(out));
+    TRY(ff_stage(targets, P<double>(out)));
+    if (audit && first) {
+      first = false;
+      TRY(exact_stage(targets, P<double>(c->ffx)));
+      launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), V,
+                        P<unsigned long long>(c->ffstat));
+      unsigned long long h[2] = {0, 0};
+      HIPCHK(c, hipMemcpyAsync(h, c->ffstat.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      double e, m;
+      std::memcpy(&e, &h[0], sizeof(double));
+      std::memcpy(&m, &h[1], sizeof(double));
+      c->ff_last_err = m > 0.0 ? e / m : e;
+      ++c->ff_audits;
+      if (c->ff_last_err > c->ff_tol) c->ff_off = true;
+      // the audited stage continues with the exact velocities
+      HIPCHK(c, hipMemcpyAsync(out.p, c->ffx.p, vb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return UVLM_OK;
   };
   const size_t total = (size_t)V;
   const int ign = (int)std::min<size_t>(ignore_steps, (size_t)INT32_MAX);
@@ -1026,6 +1089,62 @@ int uvlm_sim_pair_counts(uvlm_ctx* c, double* evaluated, double* reference) {
   if (!c) return UVLM_ERR_ARG;
   if (evaluated) *evaluated = c->pairs_eval;
   if (reference) *reference = c->pairs_ref;
+  return UVLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const double* nodes,
+                           const double* gamma, size_t num, size_t rows, size_t cols,
+                           double theta, double* out) {
+  if (!c || (nt && (!targets || !out)) || !nodes || !gamma) return UVLM_ERR_ARG;
+  if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (nt > (1u << 26) || num * (rows + 1) * (cols + 1) > (1u << 26))
+    return fail(c, UVLM_ERR_ARG, "problem too large");
+  TRY(set_dev(c));
+  if (nt == 0) return UVLM_OK;
+  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
+  const size_t nb = 3 * num * (rows + 1) * (cols + 1), gb = num * rows * cols;
+  TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
+  TRY(ensure(c, c->t_b, sizeof(double) * nb));
+  TRY(ensure(c, c->t_c, sizeof(double) * std::max<size_t>(gb, 1)));
+  TRY(ensure(c, c->t_out, sizeof(double) * 3 * nt));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->t_b.p, nodes, sizeof(double) * nb, hipMemcpyHostToDevice,
+                           c->stream));
+  if (gb)
+    HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * gb, hipMemcpyHostToDevice,
+                             c->stream));
+  std::string e;
+  const int rc = c->ff->wake_velocity(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)num,
+                                      (int)rows, (int)cols, P<double>(c->t_a), (int)nt, nullptr,
+                                      0, theta, P<double>(c->t_out), &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol) {
+  if (!c) return UVLM_ERR_ARG;
+  if (theta > 0.0 && !(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (theta > 0.0 && !c->ff) c->ff = new (std::nothrow) FarField();
+  if (theta > 0.0 && !c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
+  c->ff_theta = theta > 0.0 ? theta : 0.0;
+  c->ff_audit_every = audit_every > 0 ? audit_every : 0;
+  c->ff_tol = tol > 0.0 ? tol : 1e-3;
+  c->ff_counter = 0;
+  c->ff_off = false;
+  return UVLM_OK;
+}
+
+int uvlm_sim_farfield_stats(uvlm_ctx* c, double* last_audit_err, int* audits, int* active) {
+  if (!c) return UVLM_ERR_ARG;
+  if (last_audit_err) *last_audit_err = c->ff_last_err;
+  if (audits) *audits = c->ff_audits;
+  if (active) *active = (c->ff_theta > 0.0 && !c->ff_off) ? 1 : 0;
   return UVLM_OK;
 }
 

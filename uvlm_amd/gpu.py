@@ -107,6 +107,16 @@ class Context:
                                        max_iter, C.byref(it)), "uvlm_solve")
         return x, it.value
 
+    def farfield_velocity(self, targets, nodes, gamma, theta):
+        """Wake-induced velocity with the near/far split (uvlm_farfield_velocity)."""
+        x, nd, g = _f64(targets).reshape(-1, 3), _f64(nodes), _f64(gamma)
+        num, R, Cc = g.shape
+        out = np.zeros_like(x)
+        self._check(self._L.uvlm_farfield_velocity(self._h, _p(x), x.shape[0], _p(nd), _p(g),
+                                                   num, R, Cc, float(theta), _p(out)),
+                    "uvlm_farfield_velocity")
+        return out
+
     # --- raw simulation entry points ---------------------------------------
     def sim_init(self, num, rows, cols, max_steps, flags=N.PREC_F64_EXACT):
         self._check(self._L.uvlm_sim_init(self._h, num, rows, cols, max_steps, flags),
@@ -310,6 +320,20 @@ class Simulation:
         e, r = C.c_double(), C.c_double()
         self._check(self._L.uvlm_host_sim_pairs(self._h, C.byref(e), C.byref(r)), "pairs")
         return e.value, r.value
+
+    def set_farfield(self, theta, audit_every=50, tol=1e-3):
+        """Near/far split of the wake sum in the advection stages."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        self._check(N.gpu_lib().uvlm_sim_set_farfield(ctx, float(theta), int(audit_every),
+                                                      float(tol)), "set_farfield")
+
+    def farfield_stats(self):
+        """(relative difference of the last audit, audits run, split active)."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        e, a, on = C.c_double(), C.c_int(), C.c_int()
+        self._check(N.gpu_lib().uvlm_sim_farfield_stats(ctx, C.byref(e), C.byref(a), C.byref(on)),
+                    "farfield_stats")
+        return e.value, a.value, bool(on.value)
 
 
 class Batch:
