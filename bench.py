@@ -42,7 +42,7 @@ METRIC = "Biot-Savart pairs/s & timesteps/s vs plain FP64 GPU, CL/CT rel err ≤
 FLOP_PER_PAIR = 56.0
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD published)
 PRECISIONS = {"f64exact": 0x0, "f64": 0x1, "f32c": 0x2}
-SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2, "fullpiv": 3}
+SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2, "fullpiv": 3, "inv": 4}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_chain_f64_naca4412_20x80_rk4_traffic.json")
 
 

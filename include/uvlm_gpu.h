@@ -55,6 +55,10 @@ typedef struct uvlm_ctx uvlm_ctx;
 #define UVLM_SOLVE_BF16_IR 2  /* BF16-input MFMA LU updates + FP64 refinement */
 #define UVLM_SOLVE_F64_FULLPIV 3 /* FP64, full pivoting in Eigen::FullPivLU's elimination
                                    order: bitwise equal to a sequential FullPivLU */
+#define UVLM_SOLVE_F64_INV 4 /* FP64 iterative refinement x += X (b - A x) on a kept
+                               approximate inverse X (Newton-Schulz refreshed); the
+                               simulation warm-starts from the previous step's
+                               circulations.  Stops at max|dx| <= tol max|x|. */
 
 /* wake advection schemes (AdvectFactory names, advect_factory.cpp:22-27) */
 #define UVLM_ADVECT_EULER 0
@@ -201,6 +205,13 @@ int uvlm_sim_set_farfield(uvlm_ctx* ctx, double theta, int audit_every, double t
 /* Relative difference found by the last audit (-1 before the first), audits
  * run, and whether the split is active. */
 int uvlm_sim_farfield_stats(uvlm_ctx* ctx, double* last_audit_err, int* audits, int* active);
+/* Counters of UVLM_SOLVE_F64_INV in the simulation: refinement sweeps and
+ * Newton-Schulz refreshes since the context was made, rebuilds of the
+ * inverse after a diverging refinement, sweeps of the last solve.  No
+ * reference counterpart (the reference factors the matrix every step with
+ * Eigen::FullPivLU, src/simulator/simple_simulator.cpp:265-266). */
+int uvlm_sim_inverse_stats(uvlm_ctx* ctx, long long* sweeps, long long* refreshes,
+                           long long* rebuilds, int* last_sweeps);
 
 /* --- batched cases (no reference counterpart) ---------------------------
  * The reference runs one SimpleSimulator per process (simple_simulator.cpp

@@ -229,6 +229,40 @@ def test_solve_fullpiv_ties_and_rank(ctx, orc):
     np.testing.assert_array_equal(x, xr)
 
 
+@pytest.mark.parametrize("n", [1, 33, 600])
+def test_solve_inverse_refinement(ctx, orc, n):
+    """UVLM_SOLVE_F64_INV from a cold start (inverse built by Newton-Schulz
+    from A^T / (||A||_1 ||A||_inf)) agrees with the full-pivoting oracle."""
+    rng = np.random.default_rng(300 + n)
+    A = rng.normal(size=(n, n)) + np.eye(n) * n ** 0.5 * 2
+    b = rng.normal(size=n)
+    x, sweeps = ctx.solve(A, b, mode=N.SOLVE_F64_INV)
+    xr, _ = orc.fullpiv_solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=1e-10, atol=1e-12 * np.abs(xr).max())
+    assert 1 <= sweeps <= 30
+
+
+def test_solve_inverse_influence_matrix(ctx, orc):
+    nodes = orc.build_wing(orc.GEN_NACA4, 44, 1.0, 6.0, 10, 40)[None]
+    cp, nrm, _ = orc.lattice_geometry(nodes)
+    A = orc.influence_matrix(nodes, cp, nrm)
+    b = -nrm @ np.array([1.0, 0.0, 0.0]) + 0.1
+    x, sweeps = ctx.solve(A, b, mode=N.SOLVE_F64_INV)
+    xr, _ = orc.fullpiv_solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=1e-11, atol=1e-13 * np.abs(xr).max())
+
+
+def test_solve_inverse_singular_reported(ctx):
+    from uvlm_amd import UVLMError
+    with pytest.raises(UVLMError) as e:
+        ctx.solve(np.zeros((4, 4)), np.ones(4), mode=N.SOLVE_F64_INV)
+    assert e.value.code == N.ERR_SINGULAR
+    R = np.outer([1.0, 2.0, -1.0, 0.5], [2.0, -1.0, 1.0, 3.0])  # rank 1
+    with pytest.raises(UVLMError) as e:
+        ctx.solve(R, np.ones(4), mode=N.SOLVE_F64_INV)
+    assert e.value.code == N.ERR_NOCONV
+
+
 @pytest.mark.parametrize("n", [130, 700, 2100])
 def test_solve_f64_needs_pivoting(ctx, orc, n):
     """Unstructured matrices: the tournament pivoting tree (1, 2 and 3

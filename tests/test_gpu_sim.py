@@ -97,6 +97,23 @@ def test_f32c_mode_loads_within_1e4(orc):
     assert rel_err(g.gamma(), o.gamma()) < 1e-4
 
 
+@pytest.mark.parametrize("make", [W.naca4412_c2, W.tandem_c5])
+def test_inverse_solve_in_simulation(orc, make):
+    """UVLM_SOLVE_F64_INV in a flapping run: the kept inverse, warm-started
+    from the previous step's circulations, gives the LU run's results to
+    1e-10; each sweep gains 1.5 to 2 digits in these runs (8-10 sweeps).
+    """
+    steps = 20
+    case = make(steps=steps, rows=4, cols=12)
+    case["solve_mode"] = N.SOLVE_F64_INV
+    g, o, ol = run_both(orc, case, steps)
+    compare(g, o, ol)
+    sweeps, refreshes, rebuilds, last = g.inverse_stats()
+    print(f"[{case['name']}] {sweeps} sweeps, {refreshes} refreshes, {rebuilds} rebuilds")
+    assert rebuilds == 0 and 1 <= last <= 14
+    assert sweeps <= 14 * (steps + 1)
+
+
 def test_plate_c1_full_run_bitwise_vs_golden():
     """The CPU-runnable workload (8x16 plate, 200 steps, Euler) in the parity
     configuration: reference operation order in every Biot-Savart sum

@@ -18,7 +18,7 @@ OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_SINGULAR, ERR_STATE, ERR_CAPACITY, ERR_NOCONV
     0, -1, -2, -3, -4, -5, -6, -7)
 PREC_F64_EXACT, PREC_F64, PREC_F32C = 0x0, 0x1, 0x2
 SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
-SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV = 0, 1, 2, 3
+SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV, SOLVE_F64_INV = 0, 1, 2, 3, 4
 ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
 BATCH_FORCE_PIVOTED = 1
 SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
@@ -37,6 +37,7 @@ GPU_SYMBOLS = [
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
     "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_farfield_stats",
+    "uvlm_sim_inverse_stats",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
     "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_set_option", "uvlm_batch_free",
@@ -140,6 +141,8 @@ def gpu_lib():
                                              _dp]
         L.uvlm_sim_set_farfield.argtypes = [_vp, C.c_double, C.c_int, C.c_double]
         L.uvlm_sim_farfield_stats.argtypes = [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _ll = C.POINTER(C.c_longlong)
+        L.uvlm_sim_inverse_stats.argtypes = [_vp, _ll, _ll, _ll, C.POINTER(C.c_int)]
         L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
         L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
         L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]

@@ -72,6 +72,11 @@ struct uvlm_ctx {
   int ff_audit_every = 0, ff_counter = 0, ff_audits = 0;
   bool ff_off = false;
   Buf ffg, ffv, ffx, ffstat;
+  // kept-inverse solver (UVLM_SOLVE_F64_INV); one for the simulation, one
+  // for the stateless uvlm_solve
+  uvlm::InvSolver* inv = nullptr;
+  uvlm::InvSolver* t_inv = nullptr;
+  int inv_last_sweeps = 0;
 };
 
 namespace {
@@ -293,6 +298,10 @@ int uvlm_destroy(uvlm_ctx* c) {
   c->batch = nullptr;
   delete c->ff;
   c->ff = nullptr;
+  delete c->inv;
+  c->inv = nullptr;
+  delete c->t_inv;
+  c->t_inv = nullptr;
   Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
                  &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
                  &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
@@ -580,6 +589,21 @@ int uvlm_solve(uvlm_ctx* c, size_t n, const double* A, const double* rhs,
     if (iters_out) *iters_out = rank;
     return UVLM_OK;
   }
+  if (mode == UVLM_SOLVE_F64_INV) {
+    if (!c->t_inv && !(c->t_inv = new (std::nothrow) InvSolver()))
+      return fail(c, UVLM_ERR_OOM, "host allocation");
+    c->t_inv->invalidate();  // an unrelated matrix: build its inverse afresh
+    int it = 0;
+    std::string msg;
+    const int rc = c->t_inv->solve(c->stream, P<double>(c->t_A), (int)n, db, dx, false,
+                                   tol > 0 ? tol : 1e-14, max_iter > 0 ? max_iter : 30, &it,
+                                   &msg);
+    if (iters_out) *iters_out = it;
+    if (rc != UVLM_OK) return fail(c, rc, "%s", msg.c_str());
+    HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UVLM_OK;
+  }
   if (mode != UVLM_SOLVE_F32_IR && mode != UVLM_SOLVE_BF16_IR)
     return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", mode);
   TRY(ensure(c, c->t_work, dense_mixed_work_bytes((int)n)));
@@ -626,6 +650,7 @@ int uvlm_sim_init(uvlm_ctx* c, size_t num, size_t rows, size_t cols,
   c->vprev_rows = 0;
   c->loads_count = 0;
   c->pairs_eval = c->pairs_ref = 0.0;
+  if (c->inv) c->inv->invalidate();
   const size_t nn = num * (rows + 1) * (cols + 1);
   const size_t E = num * rows * (2 * cols + 1);
   const size_t V = num * cap * (cols + 1);
@@ -775,6 +800,15 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
                                      c->work.p, c->work.cap, &it);
     if (rc < 0) return fail(c, UVLM_ERR_ARG, "workspace too small");
     if (rc > 0) return fail(c, UVLM_ERR_NOCONV, "refinement did not converge");
+  } else if (solve_mode == UVLM_SOLVE_F64_INV) {
+    // warm start from the previous step's circulations (zero after init)
+    if (!c->inv && !(c->inv = new (std::nothrow) InvSolver()))
+      return fail(c, UVLM_ERR_OOM, "host allocation");
+    std::string msg;
+    const int rc = c->inv->solve(c->stream, P<double>(c->A), N, P<double>(c->rhs),
+                                 P<double>(c->gamma), true, 1e-14, 40, &c->inv_last_sweeps,
+                                 &msg);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", msg.c_str());
   } else {
     return fail(c, UVLM_ERR_ARG, "unknown solve mode %d", solve_mode);
   }
@@ -1145,6 +1179,16 @@ int uvlm_sim_farfield_stats(uvlm_ctx* c, double* last_audit_err, int* audits, in
   if (last_audit_err) *last_audit_err = c->ff_last_err;
   if (audits) *audits = c->ff_audits;
   if (active) *active = (c->ff_theta > 0.0 && !c->ff_off) ? 1 : 0;
+  return UVLM_OK;
+}
+
+int uvlm_sim_inverse_stats(uvlm_ctx* c, long long* sweeps, long long* refreshes,
+                           long long* rebuilds, int* last_sweeps) {
+  if (!c) return UVLM_ERR_ARG;
+  if (sweeps) *sweeps = c->inv ? c->inv->sweeps_total() : 0;
+  if (refreshes) *refreshes = c->inv ? c->inv->refreshes() : 0;
+  if (rebuilds) *rebuilds = c->inv ? c->inv->rebuilds() : 0;
+  if (last_sweeps) *last_sweeps = c->inv_last_sweeps;
   return UVLM_OK;
 }
 

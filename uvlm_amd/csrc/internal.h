@@ -136,4 +136,36 @@ size_t dense_fullpiv_work_bytes(int n);
 int dense_fullpiv_solve(hipStream_t s, double* A, int n, const double* b, double* x,
                         void* work, size_t work_bytes, int* rank_dev);
 
+// Iterative refinement on a kept approximate inverse X ~ A^-1 (inv_solve.hip):
+// x += X (b - A x) in FP64 until max|dx| <= tol max|x|; X is refreshed by
+// Newton-Schulz steps when the contraction slows and rebuilt when the
+// refinement diverges.  A (column-major) is not modified.
+class InvSolver {
+ public:
+  ~InvSolver();
+  // warm: start from the x passed in (e.g. the previous step's solution)
+  int solve(hipStream_t s, const double* A, int n, const double* b, double* x, bool warm,
+            double tol, int max_sweeps, int* sweeps, std::string* err);
+  void invalidate() { valid_ = false; }
+  long long sweeps_total() const { return sweeps_total_; }
+  long long refreshes() const { return refreshes_; }
+  long long rebuilds() const { return rebuilds_; }
+
+ private:
+  int prepare(hipStream_t s, int n, std::string* err);
+  int build(hipStream_t s, const double* A, std::string* err);
+  double newton_schulz(hipStream_t s, const double* A);
+  int sweep(hipStream_t s, const double* A, const double* b, double* x, double* d, double* m,
+            std::string* err);
+  void release();
+
+  void* handle_ = nullptr;
+  double *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr, *r_ = nullptr, *dx_ = nullptr;
+  unsigned long long* stat_ = nullptr;
+  int n_ = 0;
+  bool valid_ = false;
+  long long sweeps_total_ = 0, refreshes_ = 0, rebuilds_ = 0;
+  std::string last_err_;
+};
+
 }  // namespace uvlm

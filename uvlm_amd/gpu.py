@@ -335,6 +335,17 @@ class Simulation:
                     "farfield_stats")
         return e.value, a.value, bool(on.value)
 
+    def inverse_stats(self):
+        """Kept-inverse solve counters: (sweeps, Newton-Schulz refreshes,
+        rebuilds, sweeps of the last solve)."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        sw, rf, rb = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        last = C.c_int()
+        self._check(N.gpu_lib().uvlm_sim_inverse_stats(ctx, C.byref(sw), C.byref(rf),
+                                                       C.byref(rb), C.byref(last)),
+                    "inverse_stats")
+        return sw.value, rf.value, rb.value, last.value
+
 
 class Batch:
     """Many independent single-wing cases advanced in lockstep on one GPU
