@@ -1065,6 +1065,7 @@ int uvlm_sim_set_wake(uvlm_ctx* c, size_t node_rows, size_t gamma_rows,
   if (wing_gamma)
     HIPCHK(c, hipMemcpyAsync(c->gamma.p, wing_gamma, sizeof(double) * c->N,
                              hipMemcpyHostToDevice, c->stream));
+  if (c->inv) c->inv->reset_history();
   c->has_vprev = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return UVLM_OK;

@@ -143,10 +143,13 @@ int dense_fullpiv_solve(hipStream_t s, double* A, int n, const double* b, double
 class InvSolver {
  public:
   ~InvSolver();
-  // warm: start from the x passed in (e.g. the previous step's solution)
+  // warm: start from the x passed in (the previous solution of a run; after
+  // two warm solves the start is extrapolated from the last two solutions)
   int solve(hipStream_t s, const double* A, int n, const double* b, double* x, bool warm,
             double tol, int max_sweeps, int* sweeps, std::string* err);
   void invalidate() { valid_ = false; }
+  // the next warm start is not the continuation of the last solve
+  void reset_history() { streak_ = 0; }
   long long sweeps_total() const { return sweeps_total_; }
   long long refreshes() const { return refreshes_; }
   long long rebuilds() const { return rebuilds_; }
@@ -155,15 +158,17 @@ class InvSolver {
   int prepare(hipStream_t s, int n, std::string* err);
   int build(hipStream_t s, const double* A, std::string* err);
   double newton_schulz(hipStream_t s, const double* A);
-  int sweep(hipStream_t s, const double* A, const double* b, double* x, double* d, double* m,
-            std::string* err);
+  void enqueue_sweep(hipStream_t s, const double* A, const double* b, double* x, int k,
+                     double prev, double tol);
   void release();
 
   void* handle_ = nullptr;
-  double *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr, *r_ = nullptr, *dx_ = nullptr;
+  double *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr, *r_ = nullptr, *xold_ = nullptr,
+         *part_ = nullptr;
   unsigned long long* stat_ = nullptr;
-  int n_ = 0;
+  int n_ = 0, S_ = 1, cols_ = 1;
   bool valid_ = false;
+  int streak_ = 0, last_sweeps_ = 0;
   long long sweeps_total_ = 0, refreshes_ = 0, rebuilds_ = 0;
   std::string last_err_;
 };

@@ -4,13 +4,17 @@
 // The influence matrix of a time-stepping run changes little from one step
 // to the next.  The solver keeps X ~ A^-1 of a recent step and refines
 //     x_{k+1} = x_k + X (b - A x_k)
-// from the previous step's circulations, with the residual in FP64: the
-// error contracts by ||I - X A|| per sweep and the result solves the current
-// A x = b to FP64 accuracy.  Each sweep is two matrix-vector products over
-// all of HBM bandwidth instead of a latency-bound LU.  X is improved in
-// place by Newton-Schulz steps X <- X (2I - A X) (two library GEMMs) when
+// with the residual in FP64: the error contracts by rho(I - X A) per sweep
+// and the result solves the current A x = b to FP64 accuracy.  A sweep is
+// two matrix-vector products (split-column partial sums, reduced in a fixed
+// order, so results are deterministic) instead of a latency-bound LU.  The
+// start is extrapolated from the last two solutions of a run.  X is improved
+// in place by Newton-Schulz steps X <- X (2I - A X) (two library GEMMs) when
 // the contraction slows, and built initially from X0 = A^T / (||A||_1
 // ||A||_inf), for which the iteration converges for every non-singular A.
+// A batch of sweeps is queued at once; a one-thread check after each sweep
+// raises a device flag on convergence (or divergence) and the later sweeps
+// of the batch return at once, so the host usually reads back once per solve.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -24,14 +28,44 @@
 namespace uvlm {
 namespace {
 
+constexpr int kMaxBatch = 16;           // sweeps queued per host read
+constexpr int kNsSlot = 2 * kMaxBatch;  // Newton-Schulz deviation word
+constexpr int kFlagSlot = kNsSlot + 1;  // 0 running, 1 + k converged at k, -1 - k diverged
+constexpr int kStatWords = kFlagSlot + 1;
+constexpr double kRefreshRho = 0.15;    // contraction above which X is refreshed
+constexpr int kRedGroups = 16;          // partial-sum groups per row in the reductions
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-// T <- 2I - T
-__global__ void k_two_minus(double* T, int n) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (long)n * n) return;
-  const long i = k % n, j = k / n;
-  T[k] = (i == j ? 2.0 : 0.0) - T[k];
+__device__ inline void atomic_max_nonneg(unsigned long long* p, double v) {
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Max over a 256-thread workgroup of a non-negative value, then one atomic.
+__device__ inline void block_max_to(double v, unsigned long long* out, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    atomic_max_nonneg(out, v);
+  }
+}
+
+// T <- 2I - T and out <- max |I - T| (grid-stride, one atomic per workgroup)
+__global__ __launch_bounds__(256) void k_two_minus_dev(double* T, int n,
+                                                       unsigned long long* out) {
+  __shared__ double lds[4];
+  const long nn = (long)n * n;
+  double e = 0.0;
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < nn; k += (long)gridDim.x * 256) {
+    const long i = k % n, j = k / n;
+    const double id = i == j ? 1.0 : 0.0;
+    const double t = T[k];
+    e = fmax(e, fabs(id - t));
+    T[k] = 2.0 * id - t;
+  }
+  block_max_to(e, out, lds);
 }
 
 // X <- alpha A^T
@@ -43,7 +77,7 @@ __global__ void k_scaled_transpose(const double* A, int n, double alpha, double*
 }
 
 // out[0] = max_j sum_i |A_ij| (1-norm), out[1] = max_i sum_j |A_ij| (inf-norm),
-// one workgroup.
+// one workgroup (run once per inverse build).
 __global__ __launch_bounds__(1024) void k_norms1inf(const double* A, int n, double* out) {
   __shared__ double s1[16], s2[16];
   double m1 = 0.0, m2 = 0.0;
@@ -75,36 +109,125 @@ __global__ __launch_bounds__(1024) void k_norms1inf(const double* A, int n, doub
   }
 }
 
-// out[0] <- bits of max |I - T|; atomic max on non-negative doubles
-__global__ void k_max_dev_identity(const double* T, int n, unsigned long long* out) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0.0;
-  if (k < (long)n * n) {
-    const long i = k % n, j = k / n;
-    e = fabs((i == j ? 1.0 : 0.0) - T[k]);
+// part[s][i] = sum_{j in column slice s} M[j n + i] v[j] (column-major M),
+// s = blockIdx.y; one row per thread, columns in increasing order.
+__global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ M, int n,
+                                                   const double* __restrict__ v, int cols,
+                                                   double* __restrict__ part,
+                                                   const long long* flag) {
+  if (*flag != 0) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = blockIdx.y * cols;
+  const int j1 = min(n, j0 + cols);
+  const double* p = M + (long)j0 * n + i;
+  double acc = 0.0;
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    const double a0 = p[0], a1 = p[n], a2 = p[2L * n], a3 = p[3L * n];
+    acc = fma(a0, v[j], acc);
+    acc = fma(a1, v[j + 1], acc);
+    acc = fma(a2, v[j + 2], acc);
+    acc = fma(a3, v[j + 3], acc);
+    p += 4L * n;
   }
-  for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_xor(e, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+  for (; j < j1; ++j, p += n) acc = fma(*p, v[j], acc);
+  part[(long)blockIdx.y * n + i] = acc;
 }
 
-// x += dx; out[0], out[1] <- bits of max |dx|, max |x| (after the update)
-__global__ void k_add_norms(double* x, const double* dx, int n, unsigned long long* out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double a = 0.0, b = 0.0;
-  if (k < n) {
-    const double v = x[k] + dx[k];
-    x[k] = v;
-    a = fabs(dx[k]);
-    b = fabs(v);
+// Sum over s of part[s][i] in increasing s: 16 rows per workgroup, 16 groups
+// of consecutive slices per row, combined in group order.
+__device__ inline double sum_parts(const double* __restrict__ part, int S, int n, int i,
+                                   double (*lds)[17]) {
+  const int r = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int per = (S + kRedGroups - 1) / kRedGroups;
+  const int s0 = g * per, s1 = min(S, s0 + per);
+  double acc = 0.0;
+  if (i < n)
+#pragma unroll 4
+    for (int s = s0; s < s1; ++s) acc += part[(long)s * n + i];
+  lds[g][r] = acc;
+  __syncthreads();
+  double t = 0.0;
+  if (g == 0)
+    for (int q = 0; q < kRedGroups; ++q) t += lds[q][r];
+  return t;  // valid in group 0
+}
+
+// r = b - sum_s part[s]
+__global__ __launch_bounds__(256) void k_residual_sum(const double* __restrict__ part, int S,
+                                                      int n, const double* __restrict__ b,
+                                                      double* __restrict__ r,
+                                                      const long long* flag) {
+  __shared__ double lds[kRedGroups][17];
+  if (*flag != 0) return;
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const double t = sum_parts(part, S, n, i, lds);
+  if (threadIdx.x < 16 && i < n) r[i] = b[i] - t;
+}
+
+// x += sum_s part[s]; out[0], out[1] <- max |dx|, max |x| (after the update)
+__global__ __launch_bounds__(256) void k_update_sum(const double* __restrict__ part, int S,
+                                                    int n, double* __restrict__ x,
+                                                    unsigned long long* out,
+                                                    const long long* flag) {
+  __shared__ double lds[kRedGroups][17];
+  if (*flag != 0) return;
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  const double dx = sum_parts(part, S, n, i, lds);
+  if (threadIdx.x < 64) {  // wave 0; its lanes 16..63 contribute zeros
+    double a = 0.0, m = 0.0;
+    if (threadIdx.x < 16 && i < n) {
+      const double v = x[i] + dx;
+      x[i] = v;
+      a = fabs(dx);
+      m = fabs(v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      a = fmax(a, __shfl_xor(a, off));
+      m = fmax(m, __shfl_xor(m, off));
+    }
+    if (threadIdx.x == 0) {
+      atomic_max_nonneg(out, a);
+      atomic_max_nonneg(out + 1, m);
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    a = fmax(a, __shfl_xor(a, off));
-    b = fmax(b, __shfl_xor(b, off));
+}
+
+// After sweep k: raise the flag on convergence (max|dx| <= tol max|x|, or a
+// stall at the rounding level: contraction above 1/2 with max|dx| <= 1e3 tol
+// max|x|) or divergence (a growing or non-finite correction).
+__global__ void k_check(const unsigned long long* norms, int k, double prev, double tol,
+                        long long* flag) {
+  if (*flag != 0) return;
+  const double d = __longlong_as_double((long long)norms[2 * k]);
+  const double m = __longlong_as_double((long long)norms[2 * k + 1]);
+  const double p = k > 0 ? __longlong_as_double((long long)norms[2 * k - 2]) : prev;
+  if (!(d <= 1e300) || !(m <= 1e300) || (p > 0.0 && d > p)) {
+    *flag = -1 - k;
+    return;
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(out, (unsigned long long)__double_as_longlong(a));
-    atomicMax(out + 1, (unsigned long long)__double_as_longlong(b));
-  }
+  const double rho = p > 0.0 ? d / p : -1.0;
+  if (d <= tol * m || (rho > 0.5 && d <= 1e3 * tol * m)) *flag = 1 + k;
+}
+
+// x <- 2 x - xold, xold <- x (linear extrapolation of the start)
+__global__ void k_extrapolate(double* x, double* xold, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  x[i] = 2.0 * v - xold[i];
+  xold[i] = v;
+}
+
+__global__ void k_copy_vec(const double* a, double* b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_fill_vec(double* a, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -130,18 +253,20 @@ double bits_to_double(unsigned long long u) {
 
 }  // namespace
 
-InvSolver::~InvSolver() { release(); }
+InvSolver::~InvSolver() {
+  release();
+  if (handle_) (void)rocblas_destroy_handle((rocblas_handle)handle_);
+}
 
 void InvSolver::release() {
-  if (handle_) (void)rocblas_destroy_handle((rocblas_handle)handle_);
-  handle_ = nullptr;
-  void* bufs[] = {X_, T_, X2_, r_, dx_, stat_};
+  void* bufs[] = {X_, T_, X2_, r_, xold_, part_, stat_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
-  X_ = T_ = X2_ = r_ = dx_ = nullptr;
+  X_ = T_ = X2_ = r_ = xold_ = part_ = nullptr;
   stat_ = nullptr;
   n_ = 0;
   valid_ = false;
+  streak_ = 0;
 }
 
 int InvSolver::prepare(hipStream_t s, int n, std::string* err) {
@@ -158,28 +283,31 @@ int InvSolver::prepare(hipStream_t s, int n, std::string* err) {
     return UVLM_ERR_HIP;
   }
   if (n == n_) return UVLM_OK;
-  void* h = handle_;
-  handle_ = nullptr;
   release();
-  handle_ = h;
+  // split-column shape of the matrix-vector products: ~1024 workgroups
+  const int rb = cdiv(n, 256);
+  const int want = std::max(1, std::min(n, 1024 / rb));
+  cols_ = cdiv(n, want);
+  S_ = cdiv(n, cols_);
   const size_t nn = (size_t)n * n;
   if (hipMalloc(&X_, sizeof(double) * nn) != hipSuccess ||
       hipMalloc(&T_, sizeof(double) * nn) != hipSuccess ||
       hipMalloc(&X2_, sizeof(double) * nn) != hipSuccess ||
       hipMalloc(&r_, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&dx_, sizeof(double) * n) != hipSuccess ||
-      hipMalloc((void**)&stat_, sizeof(unsigned long long) * 4) != hipSuccess) {
+      hipMalloc(&xold_, sizeof(double) * n) != hipSuccess ||
+      hipMalloc(&part_, sizeof(double) * (size_t)S_ * n) != hipSuccess ||
+      hipMalloc((void**)&stat_, sizeof(unsigned long long) * kStatWords) != hipSuccess) {
     (void)hipGetLastError();
+    release();
     *err = "inverse solver allocation failed";
     return UVLM_ERR_OOM;
   }
   n_ = n;
-  valid_ = false;
   return UVLM_OK;
 }
 
 // One Newton-Schulz step from the current X; returns max |I - A X_old| (or
-// < 0 on error).  The step synchronises the stream.
+// < 0 on error, reason in last_err_).  The step synchronises the stream.
 double InvSolver::newton_schulz(hipStream_t s, const double* A) {
   last_err_.clear();
   rocblas_handle h = (rocblas_handle)handle_;
@@ -189,22 +317,23 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A) {
                     n, &zero, T_, n) != rocblas_status_success)
     return -1.0;
   const long nn = (long)n * n;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_, 4);
-  hipLaunchKernelGGL(k_max_dev_identity, dim3(cdiv(nn, 256)), dim3(256), 0, s, T_, n, stat_);
-  hipLaunchKernelGGL(k_two_minus, dim3(cdiv(nn, 256)), dim3(256), 0, s, T_, n);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_ + kNsSlot, 1);
+  hipLaunchKernelGGL(k_two_minus_dev, dim3(std::min(1024, cdiv(nn, 256))), dim3(256), 0, s,
+                     T_, n, stat_ + kNsSlot);
   if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X_, n, T_,
                     n, &zero, X2_, n) != rocblas_status_success)
     return -1.0;
   std::swap(X_, X2_);
   unsigned long long u = 0;
-  if (!read_back(s, &u, stat_, sizeof(u), "Newton-Schulz step", &last_err_)) return -1.0;
+  if (!read_back(s, &u, stat_ + kNsSlot, sizeof(u), "Newton-Schulz step", &last_err_))
+    return -1.0;
   ++refreshes_;
   return bits_to_double(u);
 }
 
 int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
   const int n = n_;
-  double* nrm = reinterpret_cast<double*>(stat_);  // 2 of its 4 words
+  double* nrm = reinterpret_cast<double*>(stat_);  // words 0 and 1
   hipLaunchKernelGGL(k_norms1inf, dim3(1), dim3(1024), 0, s, A, n, nrm);
   double h[2] = {0, 0};
   if (!read_back(s, h, nrm, sizeof(h), "matrix norms", err)) return UVLM_ERR_HIP;
@@ -221,7 +350,7 @@ int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
       *err = last_err_.empty() ? "inverse solver: Newton-Schulz GEMM failed" : last_err_;
       return UVLM_ERR_HIP;
     }
-    if (e < 1e-3) {  // X was already good; the step just taken squares that
+    if (e < 1e-3) {  // the step just taken squared a deviation below 1e-3
       valid_ = true;
       return UVLM_OK;
     }
@@ -231,28 +360,24 @@ int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
   return UVLM_ERR_NOCONV;
 }
 
-int InvSolver::sweep(hipStream_t s, const double* A, const double* b, double* x, double* d,
-                     double* m, std::string* err) {
-  rocblas_handle h = (rocblas_handle)handle_;
+// Enqueues sweep k of a batch: norms of its correction to stat_[2k], [2k+1],
+// then the convergence check (prev: last correction of the previous batch).
+void InvSolver::enqueue_sweep(hipStream_t s, const double* A, const double* b, double* x,
+                              int k, double prev, double tol) {
   const int n = n_;
-  const double one = 1.0, mone = -1.0, zero = 0.0;
-  if (hipMemcpyAsync(r_, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-      rocblas_dgemv(h, rocblas_operation_none, n, n, &mone, A, n, x, 1, &one, r_, 1) !=
-          rocblas_status_success ||
-      rocblas_dgemv(h, rocblas_operation_none, n, n, &one, X_, n, r_, 1, &zero, dx_, 1) !=
-          rocblas_status_success) {
-    *err = "inverse solver: matrix-vector product failed";
-    return UVLM_ERR_HIP;
-  }
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_, 2);
-  hipLaunchKernelGGL(k_add_norms, dim3(cdiv(n, 256)), dim3(256), 0, s, x, (const double*)dx_,
-                     n, stat_);
-  unsigned long long u[2] = {0, 0};
-  if (!read_back(s, u, stat_, sizeof(u), "correction norms", err)) return UVLM_ERR_HIP;
-  *d = bits_to_double(u[0]);
-  *m = bits_to_double(u[1]);
-  ++sweeps_total_;
-  return UVLM_OK;
+  const long long* flag = reinterpret_cast<const long long*>(stat_ + kFlagSlot);
+  const dim3 g(cdiv(n, 256), S_);
+  const int rb = cdiv(n, 16);
+  hipLaunchKernelGGL(k_gemv_part, g, dim3(256), 0, s, A, n, (const double*)x, cols_, part_,
+                     flag);
+  hipLaunchKernelGGL(k_residual_sum, dim3(rb), dim3(256), 0, s, (const double*)part_, S_, n, b,
+                     r_, flag);
+  hipLaunchKernelGGL(k_gemv_part, g, dim3(256), 0, s, (const double*)X_, n, (const double*)r_,
+                     cols_, part_, flag);
+  hipLaunchKernelGGL(k_update_sum, dim3(rb), dim3(256), 0, s, (const double*)part_, S_, n, x,
+                     stat_ + 2 * k, flag);
+  hipLaunchKernelGGL(k_check, dim3(1), dim3(1), 0, s, (const unsigned long long*)stat_, k, prev,
+                     tol, reinterpret_cast<long long*>(stat_ + kFlagSlot));
 }
 
 int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, double* x,
@@ -260,45 +385,80 @@ int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, dou
   int rc = prepare(s, n, err);
   if (rc != UVLM_OK) return rc;
   if (sweeps) *sweeps = 0;
-  if (!warm && hipMemsetAsync(x, 0, sizeof(double) * n, s) != hipSuccess) {
-    *err = "hipMemsetAsync failed";
-    return UVLM_ERR_HIP;
+  const int nb = cdiv(n, 256);
+  if (!warm) {
+    streak_ = 0;
+    hipLaunchKernelGGL(k_fill_vec, dim3(nb), dim3(256), 0, s, x, n, 0.0);
+  } else if (streak_ >= 2) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(256), 0, s, x, xold_, n);
+  } else if (streak_ == 1) {
+    hipLaunchKernelGGL(k_copy_vec, dim3(nb), dim3(256), 0, s, (const double*)x, xold_, n);
   }
   if (!valid_ && (rc = build(s, A, err)) != UVLM_OK) return rc;
   double prev = -1.0;
-  int refreshed = 0;
+  int refreshed = 0, done = 0;
   bool rebuilt = false;
-  for (int k = 0; k < max_sweeps; ++k) {
-    double d = 0.0, m = 0.0;
-    if ((rc = sweep(s, A, b, x, &d, &m, err)) != UVLM_OK) return rc;
-    if (sweeps) *sweeps = k + 1;
-    if (!(d <= 1e300) || (prev > 0.0 && d > prev)) {
+  // first batch: the last solve's count and two more (sweeps after the
+  // converged one are skipped on the device)
+  int batch = std::min(kMaxBatch, std::max(4, last_sweeps_ + 2));
+  while (done < max_sweeps) {
+    batch = std::max(1, std::min(batch, max_sweeps - done));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_, 2 * kMaxBatch);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_ + kFlagSlot, 1);
+    for (int k = 0; k < batch; ++k) enqueue_sweep(s, A, b, x, k, prev, tol);
+    unsigned long long u[kStatWords];
+    if (!read_back(s, u, stat_, sizeof(u), "correction norms", err)) return UVLM_ERR_HIP;
+    long long flag;
+    std::memcpy(&flag, &u[kFlagSlot], sizeof(flag));
+    const int ran = flag > 0 ? (int)flag : flag < 0 ? (int)(-flag) : batch;
+    done += ran;
+    sweeps_total_ += ran;
+    if (sweeps) *sweeps = done;
+    if (flag > 0) {
+      last_sweeps_ = done;
+      if (warm) ++streak_;
+      return UVLM_OK;
+    }
+    if (flag < 0) {
       // growing corrections: X no longer inverts A closely enough
       if (rebuilt) break;
       rebuilt = true;
       ++rebuilds_;
-      if (hipMemsetAsync(x, 0, sizeof(double) * n, s) != hipSuccess) {
-        *err = "hipMemsetAsync failed";
-        return UVLM_ERR_HIP;
-      }
+      streak_ = 0;
+      hipLaunchKernelGGL(k_fill_vec, dim3(nb), dim3(256), 0, s, x, n, 0.0);
       valid_ = false;
       if ((rc = build(s, A, err)) != UVLM_OK) return rc;
       prev = -1.0;
+      batch = 4;
       continue;
     }
-    if (d <= tol * m) return UVLM_OK;
-    // stalled at the rounding level of the residual
-    if (prev > 0.0 && d > 0.5 * prev && d <= 1e3 * tol * m) return UVLM_OK;
+    const double d = bits_to_double(u[2 * (batch - 1)]);
+    const double m = bits_to_double(u[2 * (batch - 1) + 1]);
+    const double dp = batch > 1 ? bits_to_double(u[2 * (batch - 2)]) : prev;
+    const double rho = dp > 0.0 ? d / dp : -1.0;
+    prev = d;
     // slow contraction: improve X (each step squares I - A X)
-    if (prev > 0.0 && d > 0.05 * prev && refreshed < 3) {
+    if (rho > kRefreshRho && refreshed < 3) {
       if (newton_schulz(s, A) < 0.0) {
         *err = last_err_.empty() ? "inverse solver: Newton-Schulz GEMM failed" : last_err_;
         return UVLM_ERR_HIP;
       }
       ++refreshed;
+      prev = -1.0;  // the contraction changes with X
+      batch = 4;
+      continue;
     }
-    prev = d;
+    // sweeps still needed at the observed contraction, and two more
+    if (rho > 0.0) {
+      const double need = std::log(std::max(tol * m, 1e-300) / d) / std::log(rho);
+      batch = (int)std::ceil(std::max(1.0, need)) + 2;
+    } else {
+      batch = 4;
+    }
+    batch = std::min(batch, kMaxBatch);
   }
+  streak_ = 0;
+  last_sweeps_ = 0;
   *err = "inverse solver: refinement did not converge";
   return UVLM_ERR_NOCONV;
 }

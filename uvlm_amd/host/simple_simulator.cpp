@@ -137,15 +137,20 @@ void SimpleSimulator::MainLoop(const std::size_t step, const double dt) {
 #pragma omp parallel for schedule(static)
   for (std::size_t K = 0; K < vm_cp_.size(); ++K)
     vm_cp_[K] = morphings_[K / Nper].Velocity(cpos_init_[K], t);
-  gpu::Check(uvlm_sim_solve(ctx_, vm_cp_[0].data(), forward_flight_.data(), solve_mode_,
-                            nullptr),
-             "uvlm_sim_solve", ctx_);
-  // CalcLoad (:279-282)
-  if (compute_loads_ || !load_path_.empty()) {
+  // the loads' kinematic velocities do not depend on the solve: computed
+  // here, while the device works, since a solve may wait for the device
+  const bool loads = compute_loads_ || !load_path_.empty();
+  if (loads) {
     const std::size_t Eper = rows_ * (2 * cols_ + 1);
 #pragma omp parallel for schedule(static)
     for (std::size_t e = 0; e < vm_edges_.size(); ++e)
       vm_edges_[e] = morphings_[e / Eper].Velocity(edge_mid0_[e], t);
+  }
+  gpu::Check(uvlm_sim_solve(ctx_, vm_cp_[0].data(), forward_flight_.data(), solve_mode_,
+                            nullptr),
+             "uvlm_sim_solve", ctx_);
+  // CalcLoad (:279-282)
+  if (loads) {
     gpu::Check(uvlm_sim_loads(ctx_, vm_edges_[0].data(), forward_flight_.data(), dt,
                               ref_area_.data()),
                "uvlm_sim_loads", ctx_);
