@@ -1,0 +1,193 @@
+"""Far-field correction network on the GPU against the numpy oracle
+(oracle/mlp.py): FP32 forward and gradient, Adam, BF16 MFMA inference
+(against an emulation of its roundings), training samples of the split
+(against the oracle's moments, expansion and direct sum), the far kernel's
+correction and exact fallback, and what a trained network gains."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from uvlm_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round to BF16 (nearest even) and back to float64."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+def forward_bf16(w, X):
+    p = M.unpack(np.asarray(w, dtype=np.float32))
+    h = bf16(X)
+    for k in (1, 2, 3):
+        h = bf16(M.silu(h @ bf16(p[f"W{k}"]).T + p[f"b{k}"].astype(np.float32)))
+    return h @ bf16(p["W4"]).T + p["b4"].astype(np.float32)
+
+
+def net_and_data(seed, n):
+    rng = np.random.default_rng(seed)
+    w = M.init(seed)
+    w[:-1] += (0.02 * rng.normal(size=w.size - 1)).astype(np.float32)
+    X = rng.normal(size=(n, M.IN)).astype(np.float32)
+    X[:, 31] = 1.0
+    V = rng.normal(size=(n, 3)).astype(np.float32)
+    return w, X, V
+
+
+def test_fp32_forward(ctx):
+    w, X, _ = net_and_data(1, 300)
+    y = ctx.mlp_forward(w, X, N.MLP_FP32)
+    ref, _ = M.forward(w.astype(np.float64), X.astype(np.float64))
+    assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n", [1, 64, 200])
+def test_bf16_inference_matches_emulation(ctx, n):
+    """The MFMA path (also the far kernel's) against BF16 roundings of
+    weights and activations emulated in numpy; a wrong operand layout
+    would be off by O(1)."""
+    w, X, _ = net_and_data(2, n)
+    y = ctx.mlp_forward(w, X, N.MLP_BF16)
+    emu = forward_bf16(w, X)
+    ref, _ = M.forward(w.astype(np.float64), X.astype(np.float64))
+    scale = np.abs(ref).max()
+    assert np.abs(y - emu).max() <= 1e-2 * scale
+    assert np.abs(y - emu).mean() <= 2e-3 * scale
+    assert np.abs(y - ref).max() <= 5e-2 * scale
+
+
+def test_gradient_matches_oracle(ctx):
+    w, X, V = net_and_data(3, 500)
+    loss, g = ctx.mlp_grad(w, X, V)
+    lref, gref = M.loss_and_grad(w.astype(np.float64), X.astype(np.float64),
+                                 V.astype(np.float64))
+    assert loss == pytest.approx(lref, rel=1e-5)
+    o = 0
+    for name, shp in M.SHAPES:
+        k = int(np.prod(shp))
+        a, b = g[o:o + k], gref[o:o + k]
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b), name
+        o += k
+    assert g[-1] == 0.0
+
+
+def test_adam_first_step(ctx):
+    w, X, V = net_and_data(4, 256)
+    lr = 1e-3
+    w1, losses = ctx.mlp_train(w, X, V, iters=1, batch=256, lr=lr, seed=5)
+    _, gref = M.loss_and_grad(w.astype(np.float64), X.astype(np.float64), V.astype(np.float64))
+    step = w1.astype(np.float64) - w
+    want = -lr * gref / (np.abs(gref) + 1e-8)
+    big = np.abs(gref) > 1e-3 * np.abs(gref).max()
+    assert np.abs(step - want)[big].max() <= 1e-3 * lr
+    assert losses[0] == pytest.approx(M.loss_and_grad(w.astype(np.float64), X, V)[0], rel=1e-5)
+
+
+def test_training_fits_and_is_deterministic(ctx):
+    """A teacher network's outputs are learned (loss falls 10x); the same
+    seed gives bitwise the same weights."""
+    teacher = M.init(11)
+    teacher[:-1] *= 3.0
+    _, X, _ = net_and_data(6, 4096)
+    V, _ = M.forward(teacher.astype(np.float64), X.astype(np.float64))
+    w0 = M.init(12)
+    wa, la = ctx.mlp_train(w0, X, V, iters=400, batch=256, lr=2e-3, seed=7)
+    wb, lb = ctx.mlp_train(w0, X, V, iters=400, batch=256, lr=2e-3, seed=7)
+    np.testing.assert_array_equal(wa, wb)
+    np.testing.assert_array_equal(la, lb)
+    assert la[-20:].mean() < 0.1 * la[:5].mean()
+
+
+def block_case(seed=3):
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(5), np.arange(17), indexing="ij")
+    nodes = np.stack([0.3 * i, 0.1 * j, 0.05 * np.sin(0.5 * i + 0.3 * j)], -1)
+    nodes = nodes + rng.normal(scale=0.01, size=nodes.shape)
+    gamma = 0.1 + 0.03 * rng.normal(size=(4, 16))
+    c, r, Q = M.block_moments(nodes, gamma)
+    # 64 targets in a tight cluster ~5 radii away: one tile, the block far
+    d = np.array([0.3, -0.5, 0.8])
+    x = c + 5.0 * r * d / np.linalg.norm(d) + 0.05 * r * rng.normal(size=(64, 3))
+    return nodes, gamma, c, r, Q, x
+
+
+def test_samples_match_oracle(ctx, orc):
+    nodes, gamma, c, r, Q, x = block_case()
+    F, R = ctx.farfield_samples(x, nodes[None], gamma[None], theta=2.0, max_samples=40)
+    assert len(F) == 40
+    ref_f = np.array([M.features(xx, c, r, Q) for xx in x])
+    exact = orc.induced_velocity(x, nodes[None], gamma[None])
+    for f, res in zip(F, R):
+        k = int(np.argmin(np.abs(ref_f - f).max(axis=1)))  # the sample's target
+        np.testing.assert_allclose(f, ref_f[k], rtol=1e-5, atol=1e-6)
+        scale = M.target_scale(x[k], c, r, Q)
+        want = (exact[k] - M.expansion(x[k], c, Q)) / scale
+        np.testing.assert_allclose(res, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+
+
+def test_far_kernel_correction_and_exact_fallback(ctx, orc):
+    """A network with zero output leaves the split unchanged; one whose
+    output leaves the trained range trips every pair to the exact sum."""
+    nodes, gamma, c, r, Q, x = block_case()
+    exact = orc.induced_velocity(x, nodes[None], gamma[None])
+    ctx.farfield_set_mlp(None)
+    plain = ctx.farfield_velocity(x, nodes[None], gamma[None], theta=2.0)
+    assert np.abs(plain - exact).max() > 1e-8 * np.abs(exact).max()  # expansion in use
+    w = M.init(1)
+    p = M.unpack(w)
+    p["W4"][:] = 0.0
+    p["b4"][:] = 0.0
+    wz = M.pack(p).astype(np.float32)
+    ctx.farfield_set_mlp(wz, ymax=1.0)
+    np.testing.assert_array_equal(ctx.farfield_velocity(x, nodes[None], gamma[None], 2.0), plain)
+    pairs, trips = ctx.farfield_mlp_stats()
+    assert pairs == 64 and trips == 0
+    p["b4"][:] = 1e6
+    ctx.farfield_set_mlp(M.pack(p).astype(np.float32), ymax=10.0)
+    got = ctx.farfield_velocity(x, nodes[None], gamma[None], 2.0)
+    pairs, trips = ctx.farfield_mlp_stats()
+    assert pairs == 64 and trips == 64
+    np.testing.assert_allclose(got, exact, rtol=0, atol=1e-12 * np.abs(exact).max())
+    ctx.farfield_set_mlp(None)
+
+
+def test_trained_network_gain(ctx, orc):
+    """Trained on samples of one rolled-up tandem wake state (oracle, step
+    60), the network lowers the split's velocity error at the same theta on
+    that state and on a later one (step 66: the gain fades over ~15 steps,
+    so a run retrains it from its current wake)."""
+    from uvlm_amd import workloads as W
+    case = W.tandem_c5(steps=66, rows=4, cols=32)
+    sim = orc.Sim(case)
+    states = {}
+    for s in range(67):
+        sim.step(s)
+        if s in (60, 66):
+            states[s] = sim.wake()
+    theta = 2.5
+    nodes, gamma = states[60]
+    x = nodes.reshape(-1, 3)
+    F, R = ctx.farfield_samples(x, nodes, gamma, theta, max_samples=60000, seed=63)
+    assert len(F) > 20000
+    ymax = float(np.abs(R).max()) * 2.0
+    w, losses = ctx.mlp_train(M.init(5), F, R, iters=3000, batch=512, lr=1e-3, seed=9)
+    print(f"samples {len(F)}, loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
+    assert losses[-50:].mean() < 0.3 * losses[:10].mean()
+    for st in (60, 66):
+        nodes, gamma = states[st]
+        x = nodes.reshape(-1, 3)
+        ref = orc.induced_velocity(x, nodes, gamma)
+        ctx.farfield_set_mlp(None)
+        e0 = np.abs(ctx.farfield_velocity(x, nodes, gamma, theta) - ref)
+        ctx.farfield_set_mlp(w, ymax=ymax)
+        e1 = np.abs(ctx.farfield_velocity(x, nodes, gamma, theta) - ref)
+        pairs, trips = ctx.farfield_mlp_stats()
+        ctx.farfield_set_mlp(None)
+        s = np.abs(ref).max()
+        print(f"step {st}, theta {theta}: max error {e0.max() / s:.3e} -> {e1.max() / s:.3e}, "
+              f"mean {e0.mean() / s:.3e} -> {e1.mean() / s:.3e}; {pairs} pairs, {trips} trips")
+        assert e1.mean() < 0.75 * e0.mean()
+        assert e1.max() < 0.9 * e0.max()

@@ -1,0 +1,81 @@
+"""The network oracle (oracle/mlp.py) pinned by mathematics: the gradient
+against central differences, Adam's first step against its closed form,
+the moment expansion against the direct Biot-Savart sum (convergence at
+its order), and the features of a unit ring against hand values."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+
+def test_gradient_matches_central_differences():
+    rng = np.random.default_rng(0)
+    w = M.init(3).astype(np.float64)
+    w[:-1] += 0.05 * rng.normal(size=w.size - 1)  # non-zero biases too
+    X = rng.normal(size=(7, M.IN))
+    V = rng.normal(size=(7, 3))
+    _, g = M.loss_and_grad(w, X, V)
+    idx = rng.choice(w.size - 1, size=60, replace=False)
+    h = 1e-6
+    for k in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (M.loss_and_grad(wp, X, V)[0] - M.loss_and_grad(wm, X, V)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), k
+    assert g[-1] == 0.0
+
+
+def test_adam_first_step_closed_form():
+    rng = np.random.default_rng(1)
+    w, g = rng.normal(size=50), rng.normal(size=50)
+    w1, m, v = M.adam_step(w, g, np.zeros(50), np.zeros(50), 1, 1e-3)
+    # t = 1: m_hat = g, v_hat = g^2
+    np.testing.assert_allclose(w1, w - 1e-3 * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m, 0.1 * g)
+    np.testing.assert_allclose(v, 0.001 * g * g)
+
+
+def test_unit_ring_features():
+    """A unit square ring of the lattice in the xy-plane (node row i along x,
+    column j along y: the ring runs (0,0) -> (0,1) -> (1,1) -> (1,0), clockwise
+    seen from +z): Q1[i][j] = Gamma int x_j dx_i around the loop."""
+    nodes = np.array([[[0.0, 0.0, 0.0], [0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0], [1.0, 1.0, 0.0]]])
+    c, r, Q = M.block_moments(nodes, np.array([[2.0]]))
+    np.testing.assert_allclose(c, [0.5, 0.5, 0.0])
+    assert r == pytest.approx(np.sqrt(0.5))
+    Q1 = Q[:9].reshape(3, 3)
+    # lattice segments (i, j) -> (i, j+1) run along +y, (i, j) -> (i+1, j) along +x
+    want = np.zeros((3, 3))
+    want[1, 0] = -2.0  # int x dy around the loop
+    want[0, 1] = 2.0   # int y dx
+    np.testing.assert_allclose(Q1, want, atol=1e-15)
+    f = M.features([0.5, 0.5, 10.0], c, r, Q)
+    np.testing.assert_allclose(f[:3], [0, 0, 1], atol=1e-15)
+    assert f[3] == pytest.approx(np.sqrt(0.5) / 10.0)
+    s = M.scale_of(Q, r)
+    assert s > np.sqrt(8.0)  # |Q1| plus the higher moments' terms
+    np.testing.assert_allclose(f[4:13], (want / s).reshape(-1), atol=1e-15)
+    assert np.abs(f[4:31]).max() <= 1.0
+    assert f[31] == 1.0
+
+
+def test_expansion_converges_to_direct_sum(orc):
+    """Block moments + expansion against the oracle's direct sum: the error
+    falls ~8x per doubling of the distance (first omitted order 4)."""
+    rng = np.random.default_rng(2)
+    i, j = np.meshgrid(np.arange(5), np.arange(17), indexing="ij")
+    nodes = np.stack([0.3 * i, 0.1 * j, 0.05 * np.sin(0.5 * i + 0.3 * j)], -1)
+    nodes = nodes + rng.normal(scale=0.01, size=nodes.shape)
+    gamma = 0.1 + 0.03 * rng.normal(size=(4, 16))
+    c, r, Q = M.block_moments(nodes, gamma)
+    dirs = rng.normal(size=(20, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    errs = []
+    for ratio in (4.0, 8.0, 16.0):
+        x = c + ratio * r * dirs
+        ref = orc.induced_velocity(x, nodes[None], gamma[None])
+        got = np.array([M.expansion(xx, c, Q) for xx in x])
+        errs.append(np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+    assert errs[0] < 2e-2
+    assert errs[1] < errs[0] / 6 and errs[2] < errs[1] / 6

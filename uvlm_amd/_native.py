@@ -21,6 +21,7 @@ SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
 SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV, SOLVE_F64_INV = 0, 1, 2, 3, 4
 ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
 BATCH_FORCE_PIVOTED = 1
+MLP_IN, MLP_PARAMS, MLP_FP32, MLP_BF16 = 32, 37636, 0, 1
 SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
            "AB2": ADVECT_AB2, "RK4classic": ADVECT_RK4_CLASSIC}
 GEN = {"rect": 0, "naca4": 1, "naca00": 2}
@@ -38,6 +39,8 @@ GPU_SYMBOLS = [
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
     "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_farfield_stats",
     "uvlm_sim_inverse_stats",
+    "uvlm_mlp_forward", "uvlm_mlp_grad", "uvlm_mlp_train", "uvlm_farfield_set_mlp",
+    "uvlm_farfield_mlp_stats", "uvlm_farfield_samples", "uvlm_sim_farfield_samples",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
     "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_set_option", "uvlm_batch_free",
@@ -88,6 +91,7 @@ class BatchCase(C.Structure):
 _gpu = None
 _host = None
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 _sz = C.c_size_t
 
@@ -143,6 +147,17 @@ def gpu_lib():
         L.uvlm_sim_farfield_stats.argtypes = [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _ll = C.POINTER(C.c_longlong)
         L.uvlm_sim_inverse_stats.argtypes = [_vp, _ll, _ll, _ll, C.POINTER(C.c_int)]
+        L.uvlm_mlp_forward.argtypes = [_vp, _fp, _fp, _sz, _fp, C.c_int]
+        L.uvlm_mlp_grad.argtypes = [_vp, _fp, _fp, _fp, _sz, _fp, _fp]
+        L.uvlm_mlp_train.argtypes = [_vp, _fp, _fp, _fp, _sz, _sz, _sz, C.c_float, C.c_uint,
+                                     _fp]
+        L.uvlm_farfield_set_mlp.argtypes = [_vp, _fp, C.c_double]
+        _ull = C.POINTER(C.c_ulonglong)
+        L.uvlm_farfield_mlp_stats.argtypes = [_vp, _ull, _ull]
+        L.uvlm_farfield_samples.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
+                                            C.c_uint, _fp, _fp, C.POINTER(_sz)]
+        L.uvlm_sim_farfield_samples.argtypes = [_vp, C.c_double, C.c_uint, _fp, _fp,
+                                                C.POINTER(_sz)]
         L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
         L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
         L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]

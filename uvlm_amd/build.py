@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 GPU_LIB = os.path.join(LIBDIR, "libuvlm_gpu.so")
 HOST_LIB = os.path.join(LIBDIR, "libuvlm.so")
 SOURCES = ["capi.hip", "nbody.hip", "chain.hip", "dense.hip", "step.hip", "batch.hip",
-           "farfield.hip", "inv_solve.hip"]
+           "farfield.hip", "inv_solve.hip", "mlp.hip"]
 ARCH = os.environ.get("UVLM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -18,6 +18,7 @@
 #include "batch.h"
 #include "farfield.h"
 #include "internal.h"
+#include "mlp.h"
 #include "step.h"
 
 using namespace uvlm;
@@ -77,6 +78,9 @@ struct uvlm_ctx {
   uvlm::InvSolver* inv = nullptr;
   uvlm::InvSolver* t_inv = nullptr;
   int inv_last_sweeps = 0;
+  // far-field correction network
+  uvlm::MlpTrainer* mlp = nullptr;
+  Buf mw, mx, mv, my, mg, mdev;
 };
 
 namespace {
@@ -302,13 +306,16 @@ int uvlm_destroy(uvlm_ctx* c) {
   c->inv = nullptr;
   delete c->t_inv;
   c->t_inv = nullptr;
+  delete c->mlp;
+  c->mlp = nullptr;
   Buf* bufs[] = {&c->t_a, &c->t_b, &c->t_c, &c->t_out, &c->t_recs, &c->t_partial,
                  &c->t_A, &c->t_x, &c->t_ipiv, &c->t_info, &c->t_work, &c->wing,
                  &c->gamma, &c->gamma_prev, &c->wake, &c->wgam, &c->cp, &c->nrm,
                  &c->area, &c->recs, &c->chains, &c->partial, &c->A, &c->rhs, &c->ipiv,
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
-                 &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat};
+                 &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat,
+                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev};
   for (Buf* b : bufs) release(*b);
   free_staging(c->st_wing);
   free_staging(c->st_vm);
@@ -1162,6 +1169,104 @@ int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const 
   return UVLM_OK;
 }
 
+int uvlm_farfield_set_mlp(uvlm_ctx* c, const float* weights, double ymax) {
+  if (!c) return UVLM_ERR_ARG;
+  TRY(set_dev(c));
+  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
+  if (weights && !(ymax > 0.0)) return fail(c, UVLM_ERR_ARG, "ymax must be positive");
+  std::string e;
+  const int rc = c->ff->set_mlp(c->stream, weights, weights != nullptr, (float)ymax, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  return UVLM_OK;
+}
+
+int uvlm_farfield_mlp_stats(uvlm_ctx* c, unsigned long long* pairs, unsigned long long* trips) {
+  if (!c) return UVLM_ERR_ARG;
+  unsigned long long p = 0, t = 0;
+  if (c->ff) {
+    TRY(set_dev(c));
+    std::string e;
+    const int rc = c->ff->mlp_stats(c->stream, &p, &t, &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  }
+  if (pairs) *pairs = p;
+  if (trips) *trips = t;
+  return UVLM_OK;
+}
+
+namespace {
+int copy_samples(uvlm_ctx* c, const std::vector<float>& f, const std::vector<float>& r,
+                 float* feats, float* resid, size_t* count) {
+  const size_t k = std::min(*count, f.size() / kMlpIn);
+  if (k) {
+    std::memcpy(feats, f.data(), sizeof(float) * kMlpIn * k);
+    std::memcpy(resid, r.data(), sizeof(float) * 3 * k);
+  }
+  *count = k;
+  (void)c;
+  return UVLM_OK;
+}
+}  // namespace
+
+int uvlm_farfield_samples(uvlm_ctx* c, const double* targets, size_t nt, const double* nodes,
+                          const double* gamma, size_t num, size_t rows, size_t cols,
+                          double theta, unsigned seed, float* feats, float* resid,
+                          size_t* count) {
+  if (!c || !count || (*count && (!feats || !resid))) return UVLM_ERR_ARG;
+  if (*count > (1u << 22)) return fail(c, UVLM_ERR_ARG, "too many samples");
+  // the split builds the interaction lists the samples are drawn from
+  const size_t want = *count;
+  *count = 0;
+  TRY(set_dev(c));
+  if (nt == 0 || want == 0) return UVLM_OK;
+  std::vector<double> tmp(3 * nt);
+  TRY(uvlm_farfield_velocity(c, targets, nt, nodes, gamma, num, rows, cols, theta, tmp.data()));
+  std::vector<float> f, r;
+  std::string e;
+  const int rc = c->ff->sample_residuals(c->stream, P<double>(c->t_a), (int)nt, (int)want, seed,
+                                         &f, &r, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  *count = want;
+  return copy_samples(c, f, r, feats, resid, count);
+}
+
+int uvlm_sim_farfield_samples(uvlm_ctx* c, double theta, unsigned seed, float* feats,
+                              float* resid, size_t* count) {
+  if (!c || !count || (*count && (!feats || !resid))) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (*count > (1u << 22)) return fail(c, UVLM_ERR_ARG, "too many samples");
+  const size_t want = *count;
+  *count = 0;
+  TRY(set_dev(c));
+  if (c->wgrows == 0 || want == 0) return UVLM_OK;
+  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
+  const int C1 = c->C + 1, rows = c->wrows, V = c->num * rows * C1;
+  const size_t vb = sizeof(double) * 3 * (size_t)V;
+  TRY(ensure(c, c->pos, vb));
+  TRY(ensure(c, c->ffv, vb));
+  TRY(ensure(c, c->ffg, sizeof(double) * (size_t)c->num * c->wgrows * c->C));
+  launch_gather_rows(c->stream, P<double>(c->wake), c->cap, c->wstart, c->num, rows, C1,
+                     P<double>(c->pos));
+  launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
+                       c->C, P<double>(c->ffg));
+  std::string e;
+  int ntiles = 0;
+  const int* map = c->ff->lattice_tiles(c->stream, c->num, c->wgrows, c->C, &ntiles, &e);
+  if (!map) return fail(c, UVLM_ERR_OOM, "%s", e.c_str());
+  int rc = c->ff->wake_velocity(c->stream, P<double>(c->pos), P<double>(c->ffg), c->num,
+                                c->wgrows, c->C, P<double>(c->pos), V, map, ntiles, theta,
+                                P<double>(c->ffv), &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  std::vector<float> f, r;
+  rc = c->ff->sample_residuals(c->stream, P<double>(c->pos), V, (int)want, seed, &f, &r, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  *count = want;
+  return copy_samples(c, f, r, feats, resid, count);
+}
+
 int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol) {
   if (!c) return UVLM_ERR_ARG;
   if (theta > 0.0 && !(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
@@ -1180,6 +1285,100 @@ int uvlm_sim_farfield_stats(uvlm_ctx* c, double* last_audit_err, int* audits, in
   if (last_audit_err) *last_audit_err = c->ff_last_err;
   if (audits) *audits = c->ff_audits;
   if (active) *active = (c->ff_theta > 0.0 && !c->ff_off) ? 1 : 0;
+  return UVLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+int mlp_upload(uvlm_ctx* c, const float* w, const float* X, const float* V, size_t n) {
+  if (!c->mlp && !(c->mlp = new (std::nothrow) MlpTrainer()))
+    return fail(c, UVLM_ERR_OOM, "network state");
+  TRY(ensure(c, c->mw, sizeof(float) * UVLM_MLP_PARAMS));
+  TRY(ensure(c, c->mx, sizeof(float) * kMlpIn * std::max<size_t>(n, 1)));
+  TRY(ensure(c, c->my, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+  HIPCHK(c, hipMemcpyAsync(c->mw.p, w, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyHostToDevice,
+                           c->stream));
+  if (n)
+    HIPCHK(c, hipMemcpyAsync(c->mx.p, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice,
+                             c->stream));
+  if (V) {
+    TRY(ensure(c, c->mv, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+    if (n)
+      HIPCHK(c, hipMemcpyAsync(c->mv.p, V, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+                               c->stream));
+  }
+  return UVLM_OK;
+}
+
+}  // namespace
+
+int uvlm_mlp_forward(uvlm_ctx* c, const float* w, const float* feats, size_t n, float* y,
+                     int mode) {
+  if (!c || !w || (n && (!feats || !y))) return UVLM_ERR_ARG;
+  if (mode != UVLM_MLP_FP32 && mode != UVLM_MLP_BF16)
+    return fail(c, UVLM_ERR_ARG, "unknown network mode %d", mode);
+  if (n > (1u << 24)) return fail(c, UVLM_ERR_ARG, "too many samples");
+  TRY(set_dev(c));
+  if (n == 0) return UVLM_OK;
+  TRY(mlp_upload(c, w, feats, nullptr, n));
+  std::string e;
+  if (mode == UVLM_MLP_FP32) {
+    const int rc = c->mlp->forward(c->stream, P<float>(c->mw), P<float>(c->mx), (int)n,
+                                   P<float>(c->my), &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  } else {
+    TRY(ensure(c, c->mdev, mlp_dev_bytes()));
+    launch_mlp_to_dev(c->stream, P<float>(c->mw), c->mdev.p);
+    launch_mlp_infer(c->stream, mlp_dev_view(c->mdev.p), P<float>(c->mx), (int)n,
+                     P<float>(c->my));
+    TRY(check_launch(c));
+  }
+  HIPCHK(c, hipMemcpyAsync(y, c->my.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
+
+int uvlm_mlp_grad(uvlm_ctx* c, const float* w, const float* feats, const float* targets,
+                  size_t n, float* grad, float* loss) {
+  if (!c || !w || !feats || !targets || !grad || n == 0) return UVLM_ERR_ARG;
+  if (n > (1u << 20)) return fail(c, UVLM_ERR_ARG, "too many samples");
+  TRY(set_dev(c));
+  TRY(mlp_upload(c, w, feats, targets, n));
+  TRY(ensure(c, c->mg, sizeof(float) * (UVLM_MLP_PARAMS + 4)));
+  std::string e;
+  const int rc = c->mlp->grad(c->stream, P<float>(c->mw), P<float>(c->mx), P<float>(c->mv),
+                              (int)n, P<float>(c->mg), P<float>(c->mg) + UVLM_MLP_PARAMS, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  TRY(check_launch(c));
+  HIPCHK(c, hipMemcpyAsync(grad, c->mg.p, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyDeviceToHost,
+                           c->stream));
+  float l = 0.0f;
+  HIPCHK(c, hipMemcpyAsync(&l, P<float>(c->mg) + UVLM_MLP_PARAMS, sizeof(float),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = l;
+  return UVLM_OK;
+}
+
+int uvlm_mlp_train(uvlm_ctx* c, float* w, const float* feats, const float* targets, size_t n,
+                   size_t iters, size_t batch, float lr, unsigned seed, float* losses) {
+  if (!c || !w || !feats || !targets || n == 0 || iters == 0 || batch == 0) return UVLM_ERR_ARG;
+  if (n > (1u << 24) || iters > (1u << 20) || batch > (1u << 16))
+    return fail(c, UVLM_ERR_ARG, "training problem too large");
+  if (!(lr > 0.0f)) return fail(c, UVLM_ERR_ARG, "learning rate must be positive");
+  TRY(set_dev(c));
+  TRY(mlp_upload(c, w, feats, targets, n));
+  std::string e;
+  std::vector<float> lh;
+  const int rc = c->mlp->train(c->stream, P<float>(c->mw), P<float>(c->mx), P<float>(c->mv),
+                               (int)n, (int)iters, (int)batch, lr, seed, losses ? &lh : nullptr,
+                               &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  HIPCHK(c, hipMemcpyAsync(w, c->mw.p, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (losses) std::copy(lh.begin(), lh.end(), losses);
   return UVLM_OK;
 }
 

@@ -18,6 +18,8 @@ constexpr int kFFCols = 16;
 // MLP correction of the expansion: inputs, hidden width, hidden layers.
 constexpr int kMlpIn = 32;
 constexpr int kMlpHidden = 128;
+// Highest cluster level the network corrects (and is trained on).
+constexpr int kMlpMaxLevel = 3;
 
 struct alignas(16) FFCluster {
   double c[3];  // expansion centre
@@ -83,16 +85,21 @@ class FarField {
   // cluster) evaluations.
   double last_near_pairs() const { return near_pairs_; }
   double last_far_pairs() const { return far_pairs_; }
-  // Network: set FP32 weights (host, MlpShape::total floats) and the output
-  // scale; enable/disable the correction.
-  int set_mlp(hipStream_t s, const float* w, bool enable, std::string* err);
+  // Network: FP32 weights (host, MlpShape::total floats) and the trained
+  // output range ymax (outputs beyond it trip to the exact cluster sum);
+  // enable = false switches the correction off.
+  int set_mlp(hipStream_t s, const float* w, bool enable, float ymax, std::string* err);
   bool mlp_enabled() const { return mlp_on_; }
-  // Training samples: for the clusters of the last call, exact minus
-  // expansion at up to max_samples far (target, cluster) pairs; features
-  // [k][kMlpIn] and normalised residuals [k][3] to host.
-  int sample_residuals(hipStream_t s, const double* nodes, const double* gamma,
-                       const double* targets, int nt, int max_samples, unsigned seed,
-                       std::vector<float>* feats, std::vector<float>* resid,
+  // Far pairs corrected by the network and pairs tripped to the exact sum,
+  // since the network was set.
+  int mlp_stats(hipStream_t s, unsigned long long* pairs, unsigned long long* trips,
+                std::string* err);
+  // Training samples from the interaction lists of the last wake_velocity
+  // call (same targets): up to max_samples far (target, cluster) pairs of
+  // clusters up to kMlpMaxLevel, drawn with a seeded generator; features
+  // [k][kMlpIn] and normalised remainders [k][3] (exact - expansion) to host.
+  int sample_residuals(hipStream_t s, const double* targets, int nt, int max_samples,
+                       unsigned seed, std::vector<float>* feats, std::vector<float>* resid,
                        std::string* err);
 
  private:
@@ -102,12 +109,15 @@ class FarField {
   FFTree tree_;
   double near_pairs_ = 0.0, far_pairs_ = 0.0;
   bool mlp_on_ = false;
+  float ymax_ = 0.0f;
   // device buffers
   void *d_cl_ = nullptr, *d_mom_ = nullptr, *d_tiles_ = nullptr, *d_near_ = nullptr,
        *d_far_ = nullptr, *d_cnt_ = nullptr, *d_part_ = nullptr, *d_map_ = nullptr,
-       *d_w_ = nullptr, *d_samp_ = nullptr;
+       *d_w_ = nullptr, *d_samp_ = nullptr, *d_front_ = nullptr, *d_rec_ = nullptr,
+       *d_wf_ = nullptr, *d_mstat_ = nullptr;
   size_t c_cl_ = 0, c_mom_ = 0, c_tiles_ = 0, c_near_ = 0, c_far_ = 0, c_cnt_ = 0,
-         c_part_ = 0, c_map_ = 0;
+         c_part_ = 0, c_map_ = 0, c_front_ = 0, c_rec_ = 0, c_w_ = 0, c_samp_ = 0, c_wf_ = 0,
+         c_mstat_ = 0;
   int map_num_ = -1, map_rows_ = -1, map_cols_ = -1, map_tiles_ = 0;
   // state of the last wake_velocity call (for sampling)
   int last_ntiles_ = 0, last_nt_ = 0;

@@ -27,6 +27,23 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _samples(L, check, call, max_samples):
+    """Run a sample-drawing entry point; (features [k][32], targets [k][3])."""
+    f = np.zeros((max_samples, N.MLP_IN), dtype=np.float32)
+    r = np.zeros((max_samples, 3), dtype=np.float32)
+    cnt = C.c_size_t(max_samples)
+    check(call(_pf(f), _pf(r), C.byref(cnt)))
+    return f[:cnt.value].copy(), r[:cnt.value].copy()
+
+
 class Context:
     """One uvlm_ctx (a HIP stream plus device buffers) on ``device``."""
 
@@ -116,6 +133,51 @@ class Context:
                                                    num, R, Cc, float(theta), _p(out)),
                     "uvlm_farfield_velocity")
         return out
+
+    # --- far-field correction network ---------------------------------------
+    def mlp_forward(self, w, X, mode=N.MLP_FP32):
+        w, X = _f32(w), _f32(X).reshape(-1, N.MLP_IN)
+        y = np.zeros((X.shape[0], 3), dtype=np.float32)
+        self._check(self._L.uvlm_mlp_forward(self._h, _pf(w), _pf(X), X.shape[0], _pf(y), mode),
+                    "uvlm_mlp_forward")
+        return y
+
+    def mlp_grad(self, w, X, V):
+        w, X, V = _f32(w), _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, 3)
+        g = np.zeros(N.MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self._L.uvlm_mlp_grad(self._h, _pf(w), _pf(X), _pf(V), X.shape[0], _pf(g),
+                                          C.byref(loss)), "uvlm_mlp_grad")
+        return loss.value, g
+
+    def mlp_train(self, w, X, V, iters, batch, lr, seed=1):
+        w = _f32(w).copy()
+        X, V = _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, 3)
+        losses = np.zeros(iters, dtype=np.float32)
+        self._check(self._L.uvlm_mlp_train(self._h, _pf(w), _pf(X), _pf(V), X.shape[0], iters,
+                                           batch, float(lr), seed, _pf(losses)),
+                    "uvlm_mlp_train")
+        return w, losses
+
+    def farfield_set_mlp(self, w, ymax=0.0):
+        """Network correction of the split (w None switches it off)."""
+        wf = None if w is None else _f32(w)
+        self._check(self._L.uvlm_farfield_set_mlp(self._h, _pf(wf), float(ymax)),
+                    "uvlm_farfield_set_mlp")
+
+    def farfield_mlp_stats(self):
+        p, t = C.c_ulonglong(), C.c_ulonglong()
+        self._check(self._L.uvlm_farfield_mlp_stats(self._h, C.byref(p), C.byref(t)),
+                    "uvlm_farfield_mlp_stats")
+        return p.value, t.value
+
+    def farfield_samples(self, targets, nodes, gamma, theta, max_samples, seed=1):
+        x, nd, g = _f64(targets).reshape(-1, 3), _f64(nodes), _f64(gamma)
+        num, R, Cc = g.shape
+        return _samples(self._L, lambda rc: self._check(rc, "uvlm_farfield_samples"),
+                        lambda f, r, c: self._L.uvlm_farfield_samples(
+                            self._h, _p(x), x.shape[0], _p(nd), _p(g), num, R, Cc, float(theta),
+                            seed, f, r, c), max_samples)
 
     # --- raw simulation entry points ---------------------------------------
     def sim_init(self, num, rows, cols, max_steps, flags=N.PREC_F64_EXACT):
@@ -334,6 +396,29 @@ class Simulation:
         self._check(N.gpu_lib().uvlm_sim_farfield_stats(ctx, C.byref(e), C.byref(a), C.byref(on)),
                     "farfield_stats")
         return e.value, a.value, bool(on.value)
+
+    def _gpu_ctx(self):
+        return self._L.uvlm_host_sim_context(self._h)
+
+    def set_farfield_mlp(self, w, ymax=0.0):
+        """Network correction of the split in the advection stages (None: off)."""
+        wf = None if w is None else _f32(w)
+        self._check(N.gpu_lib().uvlm_farfield_set_mlp(self._gpu_ctx(), _pf(wf), float(ymax)),
+                    "set_farfield_mlp")
+
+    def farfield_mlp_stats(self):
+        p, t = C.c_ulonglong(), C.c_ulonglong()
+        self._check(N.gpu_lib().uvlm_farfield_mlp_stats(self._gpu_ctx(), C.byref(p), C.byref(t)),
+                    "farfield_mlp_stats")
+        return p.value, t.value
+
+    def farfield_samples(self, theta, max_samples, seed=1):
+        """Network training samples drawn from the current wake."""
+        L = N.gpu_lib()
+        ctx = self._gpu_ctx()
+        return _samples(L, lambda rc: self._check(rc, "farfield_samples"),
+                        lambda f, r, c: L.uvlm_sim_farfield_samples(ctx, float(theta), seed, f,
+                                                                    r, c), max_samples)
 
     def inverse_stats(self):
         """Kept-inverse solve counters: (sweeps, Newton-Schulz refreshes,
