@@ -21,8 +21,12 @@ OpenMP over targets like the reference) on a bounded sample of the same
 sums at the final state, on the host's cores.
 
 Other modes: --workload c1|c3|c5 (single-case workloads), --workload c4
-(256 batched cases), --precision, --solve; --calibrate-copy runs a plain
-device copy of known size for the PMC traffic calibration.
+(256 batched cases), --precision, --solve, --farfield THETA (near/far split
+of the wake sum, audited), --mlp-every K (in-run training of the far-field
+correction network every K steps); --compare also runs the plain FP64
+method (exact sums, LU) over the same steps and reports the speed-up and
+the CL/CT differences; --calibrate-copy runs a plain device copy of known
+size for the PMC traffic calibration.
 """
 from __future__ import annotations
 
@@ -111,9 +115,52 @@ def calibrate_copy(nbytes=1 << 28):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
+def plain_run(args):
+    """The plain FP64 method (exact sums, LU) over the same steps: wall time
+    of the timed steps and the CL/CD history."""
+    from uvlm_amd import Simulation
+    from uvlm_amd import workloads as W
+
+    total = args.warmup + args.steps
+    case = W.WORKLOADS[args.workload](steps=total - 1)
+    case["precision"] = PRECISIONS["f64"]
+    case["solve_mode"] = SOLVES["f64"]
+    sim = Simulation(case, total - 1)
+    for s in range(args.warmup):
+        sim.step(s)
+    sim.sync()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, total):
+        sim.step(s)
+        if (s - args.warmup) % 100 == 99:
+            print(f"# plain step {s}", file=sys.stderr, flush=True)
+    sim.sync()
+    T = time.perf_counter() - t0
+    loads = sim.loads()
+    sim.close()
+    return T, loads
+
+
+def load_differences(loads, ref, early=30):
+    """CL/CT differences relative to each wing's largest |CL|, |CT| of the
+    reference run: pointwise over steps 0..early (before the wake's
+    sensitivity to rounding has grown) and of the means over the run."""
+    n = min(len(loads), len(ref))
+    out = {}
+    for q, name in ((1, "CL"), (0, "CT")):
+        a, b = loads[:n, :, q], ref[:n, :, q]
+        scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+        out[f"{name}_max_rel_err_steps_0_{early}"] = (
+            np.abs(a[:early + 1] - b[:early + 1]).max(axis=0) / scale).tolist()
+        out[f"{name}_mean_rel_err"] = (np.abs(a.mean(axis=0) - b.mean(axis=0)) / scale).tolist()
+        out[f"{name}_max_rel_err_all_steps"] = (np.abs(a - b).max(axis=0) / scale).tolist()
+    return out
+
+
 def run_single(args):
     from uvlm_amd import Simulation
     from uvlm_amd import workloads as W
+    from uvlm_amd.gpu import mlp_init
 
     total = args.warmup + args.steps
     case = W.WORKLOADS[args.workload](steps=total - 1)
@@ -129,9 +176,16 @@ def run_single(args):
     e0, r0 = sim.pairs()
     sim.profile(True)
     sim.profile_read()
+    w = mlp_init(5) if args.mlp_every > 0 else None
+    learned = 0
     t0 = time.perf_counter()
     for s in range(args.warmup, total):
         sim.step(s)
+        if (w is not None and args.farfield > 0 and s >= args.mlp_start
+                and (s - args.mlp_start) % args.mlp_every == 0):
+            w, _ = sim.farfield_learn(w, args.farfield, args.mlp_samples, args.mlp_iters,
+                                      512, 1e-3, seed=s)
+            learned += 1
         if (s - args.warmup) % 100 == 99:
             print(f"# step {s}", file=sys.stderr, flush=True)
     sim.sync()
@@ -179,6 +233,20 @@ def run_single(args):
         err, audits, active = sim.farfield_stats()
         out["farfield"] = {"theta": args.farfield, "audit_every": args.audit,
                            "last_audit_rel_err": err, "audits": audits, "active": active}
+        if w is not None:
+            pairs, trips = sim.farfield_mlp_stats()
+            out["farfield"]["network"] = {"trained": learned, "every": args.mlp_every,
+                                          "samples": args.mlp_samples,
+                                          "iters": args.mlp_iters, "corrected_pairs": pairs,
+                                          "exact_fallbacks": trips}
+    if args.solve == "inv":
+        sw, rf, rb, _ = sim.inverse_stats()
+        out["inverse_solve"] = {"sweeps": sw, "newton_schulz_refreshes": rf, "rebuilds": rb}
+    if args.compare:
+        Tp, lp = plain_run(args)
+        out["vs_plain_fp64"] = {"plain_ms_per_step": 1e3 * Tp / args.steps,
+                                "timesteps_per_s_ratio": Tp / T,
+                                **load_differences(loads, lp)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sim, args.cpu_seconds)
     return out
@@ -239,6 +307,13 @@ def main():
                     help="near/far split of the wake sum with this theta (0: exact sums)")
     ap.add_argument("--audit", type=int, default=50, help="far field: exact audit every N steps")
     ap.add_argument("--ff-tol", type=float, default=1e-3, help="far field: audit tolerance")
+    ap.add_argument("--mlp-every", type=int, default=0,
+                    help="far field: train the correction network every K steps (0: off)")
+    ap.add_argument("--mlp-start", type=int, default=20, help="first training step")
+    ap.add_argument("--mlp-samples", type=int, default=60000)
+    ap.add_argument("--mlp-iters", type=int, default=1000)
+    ap.add_argument("--compare", action="store_true",
+                    help="also time the plain FP64 method over the same steps")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch: launch each step's kernels directly instead of a hipGraph")
     args = ap.parse_args()

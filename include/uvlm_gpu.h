@@ -261,6 +261,14 @@ int uvlm_farfield_samples(uvlm_ctx* ctx, const double* targets, size_t nt,
                           size_t* count);
 int uvlm_sim_farfield_samples(uvlm_ctx* ctx, double theta, unsigned seed, float* feats,
                               float* resid, size_t* count);
+/* In-run learning: draw up to max_samples samples of the split at theta
+ * from the simulation's current wake, train the network on them from
+ * `weights` (in/out, warm start; Adam as uvlm_mlp_train) and switch the
+ * correction on with ymax = twice the largest sample target; last_loss
+ * (nullable) <- the final mini-batch loss. */
+int uvlm_sim_farfield_learn(uvlm_ctx* ctx, double theta, size_t max_samples, size_t iters,
+                            size_t batch, float lr, unsigned seed, float* weights,
+                            float* last_loss);
 
 /* --- batched cases (no reference counterpart) ---------------------------
  * The reference runs one SimpleSimulator per process (simple_simulator.cpp

@@ -1,23 +1,29 @@
-"""Numpy restatement of the far-field correction network: input features,
-forward pass, loss gradient and Adam step, in float64.
+"""Numpy restatement of the far-field split's source tree and of the
+correction network: Morton-ordered segment leaves, vorticity moments, the
+moment expansion, network features, forward pass, loss gradient and Adam
+step, in float64.
 
-TEST INFRASTRUCTURE ONLY.  This is the checker for the GPU network kernels
-(feature extraction, FP32 training forward/backward, BF16 inference): only
-``tests/`` import it.  The network has no counterpart in the reference
-(xeno14/UVLM sums every wake ring exactly, src/uvlm/advect.h:16-46); its
-definition is this module's:
+TEST INFRASTRUCTURE ONLY.  This is the checker for the GPU far-field and
+network kernels: only ``tests/`` import it.  Neither has a counterpart in
+the reference (xeno14/UVLM sums every wake ring exactly,
+src/uvlm/advect.h:16-46); their definitions are this module's:
 
-* features of a (target x, block) pair, the block with centre c, radius r and
-  vorticity moments Q1 [3][3], Q2 [3][3][3], Q3 [3][3][3][3] (Q_n[i, J] = sum
-  over its segments of Gamma e_i int_0^1 (a + t e - c)^(x)n dt):
+* segments of a lattice: per sheet, spanwise (i, j) -> (i, j+1) by node
+  row, then chordwise (i, j) -> (i+1, j) by node column, strength = the
+  difference of the ring circulations either side (zero outside);
+* moments of a set of segments about c: Q_n[i, J] = sum Gamma e_i
+  int_0^1 (a + t e - c)^(x)n dt, n = 0..3 (3 + 9 + 27 + 81 values);
+* expansion: u_a = 1/4pi sum_n (-1)^n / n! eps_abc Q_n[c, J] T_{n+1}[b, J];
+* features of a (target x, cluster) pair, cluster centre c, radius r:
   d = x - c, D = |d|, dh = d / D, rho = r / D,
-  s = ||Q1||_F + ||Q2||_F / r + ||Q3||_F / r^2 (every feature is at most 1),
-  f = [dh (3), rho, Q1 / s (9), Q2 . dh / (s r) (9), Q3 : dh dh / (s r^2) (9), 1];
+  s = |Q0| + |Q1| / r + |Q2| / r^2 + |Q3| / r^3 (Frobenius norms),
+  f = [dh (3), rho, Q0 / s (3), Q1 / (s r) (9), Q2 . dh / (s r^2) (9),
+       Q2 : dh dh / (s r^2) (3), Q3 : dh dh dh / (s r^3) (3), 1];
 * network: h1 = silu(W1 f + b1), h2 = silu(W2 h1 + b2), h3 = silu(W3 h2 + b3),
   y = W4 h3 + b4 (32 -> 128 -> 128 -> 128 -> 3);
 * target: the expansion's remainder in units of its leading omitted term,
-  y* = (u_exact - u_expansion) 4 pi D^3 / (s rho^3); the correction added to
-  the expansion is y s rho^3 / (4 pi D^3);
+  y* = (u_exact - u_expansion) 4 pi D^2 / (s rho^4); the correction added to
+  the expansion is y s rho^4 / (4 pi D^2);
 * loss: mean over the batch of 1/2 |y - y*|^2.
 
 Parameter vector (FP32 on the device): W1 [128][32], b1, W2 [128][128], b2,
@@ -111,13 +117,109 @@ def adam_step(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     return w - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def features(x, c, r, Q):
-    """Features [32] of target x and a block (centre c, radius r, moments Q
-    [117]); None when the block carries no circulation."""
+def lattice_segments(nodes, gamma):
+    """Segments (a [n][3], e [n][3], g [n]) of lattices nodes [num][R+1][C+1][3],
+    gamma [num][R][C], in the order above."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    gamma = np.asarray(gamma, dtype=np.float64)
+    A, E, Gs = [], [], []
+    for n in range(gamma.shape[0]):
+        P, g = nodes[n], gamma[n]
+        R, C = g.shape
+        G = np.zeros((R + 2, C + 2))
+        G[1:-1, 1:-1] = g
+        for i in range(R + 1):
+            for j in range(C):
+                A.append(P[i, j]); E.append(P[i, j + 1] - P[i, j])
+                Gs.append(G[i + 1, j + 1] - G[i, j + 1])
+        for j in range(C + 1):
+            for i in range(R):
+                A.append(P[i, j]); E.append(P[i + 1, j] - P[i, j])
+                Gs.append(G[i + 1, j] - G[i + 1, j + 1])
+    return np.array(A), np.array(E), np.array(Gs)
+
+
+def morton_order(points):
+    """Stable order of points [n][3] by their 63-bit Morton key in the
+    points' bounding box (21 bits per axis, t = (p - min) / (max - min)
+    truncated to 21 bits)."""
+    lo, hi = points.min(axis=0), points.max(axis=0)
+    key = np.zeros(len(points), dtype=np.uint64)
+    for d in range(3):
+        w = hi[d] - lo[d]
+        t = (points[:, d] - lo[d]) / w if w > 0 else np.zeros(len(points))
+        q = np.floor(np.clip(t, 0.0, 1.0) * 2097151.0).astype(np.uint64)
+        v = q & np.uint64(0x1fffff)
+        for shift, mask in ((32, 0x1f00000000ffff), (16, 0x1f0000ff0000ff),
+                            (8, 0x100f00f00f00f00f), (4, 0x10c30c30c30c30c3),
+                            (2, 0x1249249249249249)):
+            v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+        key |= v << np.uint64(d)
+    return np.argsort(key, kind="stable")
+
+
+def segment_moments(A, E, G, c):
+    """Moments [120] (Q0, Q1, Q2, Q3 flattened) of segments about c."""
+    Q = [np.zeros(3), np.zeros((3, 3)), np.zeros((3, 3, 3)), np.zeros((3, 3, 3, 3))]
+    for a, e, g in zip(A, E, G):
+        if g == 0.0:
+            continue
+        Ar = a - c
+        m1 = Ar + e / 2
+        m2 = np.outer(Ar, Ar) + (np.outer(Ar, e) + np.outer(e, Ar)) / 2 + np.outer(e, e) / 3
+        ein = lambda *v: np.einsum("j,k,l->jkl", *v)
+        m3 = (ein(Ar, Ar, Ar) + (ein(Ar, Ar, e) + ein(Ar, e, Ar) + ein(e, Ar, Ar)) / 2
+              + (ein(Ar, e, e) + ein(e, Ar, e) + ein(e, e, Ar)) / 3 + ein(e, e, e) / 4)
+        Q[0] += g * e
+        Q[1] += g * np.outer(e, m1)
+        Q[2] += g * np.einsum("i,jk->ijk", e, m2)
+        Q[3] += g * np.einsum("i,jkl->ijkl", e, m3)
+    return np.concatenate([q.reshape(-1) for q in Q])
+
+
+def leaf_tree(A, E, G, leaf=64, branch=4):
+    """Centres, radii and segment ranges of the split's clusters, level by
+    level (Morton-ordered leaves of `leaf` segments, `branch` merged per
+    level), with the segments permuted into Morton order."""
+    mid = A + 0.5 * E
+    p = morton_order(mid)
+    A, E, G = A[p], E[p], G[p]
+    levels = []
+    nodes = []
+    for s0 in range(0, len(A), leaf):
+        a, e = A[s0:s0 + leaf], E[s0:s0 + leaf]
+        c = np.zeros(3)
+        for q in range(len(a)):
+            c += a[q] + 0.5 * e[q]
+        c /= len(a)
+        r = np.sqrt(max(np.max(np.sum((a - c) ** 2, 1)), np.max(np.sum((a + e - c) ** 2, 1))))
+        nodes.append((c, r, s0, s0 + len(a)))
+    levels.append(nodes)
+    while len(levels[-1]) > 1:
+        prev, nodes = levels[-1], []
+        for q0 in range(0, len(prev), branch):
+            ch = prev[q0:q0 + branch]
+            w = np.array([x[3] - x[2] for x in ch], dtype=np.float64)
+            c = sum(wi * x[0] for wi, x in zip(w, ch)) / w.sum()
+            r = max(np.linalg.norm(x[0] - c) + x[1] for x in ch)
+            nodes.append((c, r, ch[0][2], ch[-1][3]))
+        levels.append(nodes)
+    return levels, (A, E, G)
+
+
+def scale_of(Q, r):
+    """s = |Q0| + |Q1| / r + |Q2| / r^2 + |Q3| / r^3 of a cluster of radius r."""
     Q = np.asarray(Q, dtype=np.float64)
-    Q1 = Q[:9].reshape(3, 3)
-    Q2 = Q[9:36].reshape(3, 3, 3)
-    Q3 = Q[36:117].reshape(3, 3, 3, 3)
+    return (np.linalg.norm(Q[:3]) + np.linalg.norm(Q[3:12]) / r
+            + np.linalg.norm(Q[12:39]) / r ** 2 + np.linalg.norm(Q[39:120]) / r ** 3)
+
+
+def features(x, c, r, Q):
+    """Features [32] of target x and a cluster (centre c, radius r, moments
+    Q [120]); None when the cluster carries no circulation."""
+    Q = np.asarray(Q, dtype=np.float64)
+    Q0, Q1 = Q[:3], Q[3:12].reshape(3, 3)
+    Q2, Q3 = Q[12:39].reshape(3, 3, 3), Q[39:120].reshape(3, 3, 3, 3)
     d = np.asarray(x, dtype=np.float64) - np.asarray(c, dtype=np.float64)
     D = np.linalg.norm(d)
     dh = d / D
@@ -127,38 +229,32 @@ def features(x, c, r, Q):
     f = np.zeros(IN)
     f[0:3] = dh
     f[3] = r / D
-    f[4:13] = (Q1 / s).reshape(-1)
-    f[13:22] = (Q2 @ dh).reshape(-1) / (s * r)
-    f[22:31] = ((Q3 @ dh) @ dh).reshape(-1) / (s * r * r)
+    f[4:7] = Q0 / s
+    f[7:16] = Q1.reshape(-1) / (s * r)
+    q2d = Q2 @ dh
+    f[16:25] = q2d.reshape(-1) / (s * r * r)
+    f[25:28] = q2d @ dh / (s * r * r)
+    f[28:31] = ((Q3 @ dh) @ dh) @ dh / (s * r ** 3)
     f[31] = 1.0
     return f
 
 
-def scale_of(Q, r):
-    """s = ||Q1|| + ||Q2|| / r + ||Q3|| / r^2 of a block of radius r."""
-    Q = np.asarray(Q, dtype=np.float64)
-    return (np.linalg.norm(Q[:9]) + np.linalg.norm(Q[9:36]) / r
-            + np.linalg.norm(Q[36:117]) / (r * r))
-
-
 def target_scale(x, c, r, Q):
-    """s rho^3 / (4 pi D^3): the correction per unit network output."""
+    """s rho^4 / (4 pi D^2): the correction per unit network output."""
     d = np.asarray(x, dtype=np.float64) - np.asarray(c, dtype=np.float64)
     D = np.linalg.norm(d)
-    rho = r / D
-    return scale_of(Q, r) * rho ** 3 / (4.0 * np.pi * D ** 3)
+    return scale_of(Q, r) * (r / D) ** 4 / (4.0 * np.pi * D ** 2)
 
 
 def expansion(x, c, Q):
-    """Velocity of the moment expansion (orders 1..3) of a block at x."""
+    """Velocity of the moment expansion (orders 0..3) of a cluster at x."""
     Q = np.asarray(Q, dtype=np.float64)
-    Q1 = Q[:9].reshape(3, 3)
-    Q2 = Q[9:36].reshape(3, 3, 3)
-    Q3 = Q[36:117].reshape(3, 3, 3, 3)
+    Q0, Q1 = Q[:3], Q[3:12].reshape(3, 3)
+    Q2, Q3 = Q[12:39].reshape(3, 3, 3), Q[39:120].reshape(3, 3, 3, 3)
     d = np.asarray(x, dtype=np.float64) - np.asarray(c, dtype=np.float64)
-    r2 = d @ d
-    D = np.sqrt(r2)
+    D = np.sqrt(d @ d)
     I = np.eye(3)
+    T1 = -d / D ** 3
     T2 = 3 * np.einsum("b,j->bj", d, d) / D ** 5 - I / D ** 3
     T3 = (-15 * np.einsum("b,j,k->bjk", d, d, d) / D ** 7
           + 3 * (np.einsum("jb,k->bjk", I, d) + np.einsum("kb,j->bjk", I, d)
@@ -170,47 +266,17 @@ def expansion(x, c, Q):
     three = (np.einsum("jb,kl->bjkl", I, I) + np.einsum("kb,jl->bjkl", I, I)
              + np.einsum("lb,jk->bjkl", I, I))
     T4 = 105 * dddd / D ** 9 - 15 * six / D ** 7 + 3 * three / D ** 5
-    S = (-np.einsum("cj,bj->cb", Q1, T2) + 0.5 * np.einsum("cjk,bjk->cb", Q2, T3)
-         - np.einsum("cjkl,bjkl->cb", Q3, T4) / 6.0)
+    S = (np.einsum("c,b->cb", Q0, T1) - np.einsum("cj,bj->cb", Q1, T2)
+         + 0.5 * np.einsum("cjk,bjk->cb", Q2, T3) - np.einsum("cjkl,bjkl->cb", Q3, T4) / 6.0)
     u = np.array([S[2, 1] - S[1, 2], S[0, 2] - S[2, 0], S[1, 0] - S[0, 1]])
     return u / (4.0 * np.pi)
 
 
 def block_moments(nodes, gamma):
-    """Centre (node mean), radius and moments [117] of one block: a lattice
-    of nodes [R+1][C+1][3] with ring circulations gamma [R][C] (zero
-    outside), integrated segment by segment."""
+    """Centre (node mean), radius and moments [120] of one lattice block:
+    nodes [R+1][C+1][3], ring circulations gamma [R][C] (zero outside)."""
     nodes = np.asarray(nodes, dtype=np.float64)
-    R, C = gamma.shape
     c = nodes.reshape(-1, 3).mean(0)
     r = np.linalg.norm(nodes.reshape(-1, 3) - c, axis=1).max()
-    G = np.zeros((R + 2, C + 2))
-    G[1:-1, 1:-1] = gamma
-    Q1, Q2, Q3 = np.zeros((3, 3)), np.zeros((3, 3, 3)), np.zeros((3, 3, 3, 3))
-
-    def seg(a, b, g):
-        nonlocal Q1, Q2, Q3
-        if g == 0.0:
-            return
-        A, E = a - c, b - a
-        # int_0^1 (A + tE)^(x)n dt, n = 1, 2, 3
-        m1 = A + E / 2
-        m2 = (np.outer(A, A) + (np.outer(A, E) + np.outer(E, A)) / 2 + np.outer(E, E) / 3)
-        AAA = np.einsum("j,k,l->jkl", A, A, A)
-        AAE = (np.einsum("j,k,l->jkl", A, A, E) + np.einsum("j,k,l->jkl", A, E, A)
-               + np.einsum("j,k,l->jkl", E, A, A))
-        AEE = (np.einsum("j,k,l->jkl", A, E, E) + np.einsum("j,k,l->jkl", E, A, E)
-               + np.einsum("j,k,l->jkl", E, E, A))
-        EEE = np.einsum("j,k,l->jkl", E, E, E)
-        m3 = AAA + AAE / 2 + AEE / 3 + EEE / 4
-        Q1 += g * np.outer(E, m1)
-        Q2 += g * np.einsum("i,jk->ijk", E, m2)
-        Q3 += g * np.einsum("i,jkl->ijkl", E, m3)
-
-    for i in range(R + 1):
-        for j in range(C):
-            seg(nodes[i, j], nodes[i, j + 1], G[i + 1, j + 1] - G[i, j + 1])
-    for j in range(C + 1):
-        for i in range(R):
-            seg(nodes[i, j], nodes[i + 1, j], G[i + 1, j] - G[i + 1, j + 1])
-    return c, r, np.concatenate([Q1.reshape(-1), Q2.reshape(-1), Q3.reshape(-1)])
+    A, E, G = lattice_segments(nodes[None], np.asarray(gamma)[None])
+    return c, r, segment_moments(A, E, G, c)

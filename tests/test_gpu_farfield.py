@@ -113,3 +113,31 @@ def test_audit_switches_the_split_off(orc):
     b.sync()
     err, audits, active = b.farfield_stats()
     assert not active and audits >= 1 and err > 1e-15
+
+
+def test_simulation_learns_the_correction(orc):
+    """In-run learning on the tandem workload (reduced lattice): the
+    network trained on the run's own wake corrects the far pairs of the
+    following steps; loads stay within the split's tolerance of the exact
+    run and the audits pass."""
+    from oracle import mlp as M
+    steps = 80
+    case = W.tandem_c5(steps=steps, rows=4, cols=32)
+    a = Simulation(case, steps)
+    b = Simulation(case, steps)
+    b.set_farfield(theta=2.5, audit_every=10, tol=1e-3)
+    w = M.init(5)
+    for s in range(steps + 1):
+        a.step(s)
+        b.step(s)
+        if s in (50, 65):
+            w, loss = b.farfield_learn(w, theta=2.5, max_samples=20000, iters=500, seed=s)
+            assert np.isfinite(loss)
+    a.sync()
+    b.sync()
+    pairs, trips = b.farfield_mlp_stats()
+    err, audits, active = b.farfield_stats()
+    assert pairs > 0 and active and err < 1e-3
+    la, lb = a.loads(), b.loads()
+    scale = np.abs(la).max(axis=0)
+    assert np.all(np.abs(la - lb).max(axis=0) / scale < 1e-3)

@@ -102,15 +102,20 @@ def test_training_fits_and_is_deterministic(ctx):
 
 
 def block_case(seed=3):
+    """One 4 x 16 ring block and 64 targets in a tight cluster 8 block radii
+    away (one tile; the root of the split's tree, which covers the block's
+    148 segments, is far from it)."""
     rng = np.random.default_rng(seed)
     i, j = np.meshgrid(np.arange(5), np.arange(17), indexing="ij")
     nodes = np.stack([0.3 * i, 0.1 * j, 0.05 * np.sin(0.5 * i + 0.3 * j)], -1)
     nodes = nodes + rng.normal(scale=0.01, size=nodes.shape)
     gamma = 0.1 + 0.03 * rng.normal(size=(4, 16))
-    c, r, Q = M.block_moments(nodes, gamma)
-    # 64 targets in a tight cluster ~5 radii away: one tile, the block far
+    A, E, G = M.lattice_segments(nodes[None], gamma[None])
+    levels, _ = M.leaf_tree(A, E, G)
+    c, r = levels[-1][0][0], levels[-1][0][1]  # the root's centre and radius
+    Q = M.segment_moments(A, E, G, c)
     d = np.array([0.3, -0.5, 0.8])
-    x = c + 5.0 * r * d / np.linalg.norm(d) + 0.05 * r * rng.normal(size=(64, 3))
+    x = c + 8.0 * r * d / np.linalg.norm(d) + 0.05 * r * rng.normal(size=(64, 3))
     return nodes, gamma, c, r, Q, x
 
 
@@ -135,7 +140,7 @@ def test_far_kernel_correction_and_exact_fallback(ctx, orc):
     exact = orc.induced_velocity(x, nodes[None], gamma[None])
     ctx.farfield_set_mlp(None)
     plain = ctx.farfield_velocity(x, nodes[None], gamma[None], theta=2.0)
-    assert np.abs(plain - exact).max() > 1e-8 * np.abs(exact).max()  # expansion in use
+    assert np.abs(plain - exact).max() > 1e-9 * np.abs(exact).max()  # expansion in use
     w = M.init(1)
     p = M.unpack(w)
     p["W4"][:] = 0.0

@@ -39,13 +39,13 @@ def test_adam_first_step_closed_form():
 def test_unit_ring_features():
     """A unit square ring of the lattice in the xy-plane (node row i along x,
     column j along y: the ring runs (0,0) -> (0,1) -> (1,1) -> (1,0), clockwise
-    seen from +z): Q1[i][j] = Gamma int x_j dx_i around the loop."""
+    seen from +z): Q0 = 0 (a closed loop), Q1[i][j] = Gamma int x_j dx_i."""
     nodes = np.array([[[0.0, 0.0, 0.0], [0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0], [1.0, 1.0, 0.0]]])
     c, r, Q = M.block_moments(nodes, np.array([[2.0]]))
     np.testing.assert_allclose(c, [0.5, 0.5, 0.0])
     assert r == pytest.approx(np.sqrt(0.5))
-    Q1 = Q[:9].reshape(3, 3)
-    # lattice segments (i, j) -> (i, j+1) run along +y, (i, j) -> (i+1, j) along +x
+    np.testing.assert_allclose(Q[:3], 0.0, atol=1e-15)
+    Q1 = Q[3:12].reshape(3, 3)
     want = np.zeros((3, 3))
     want[1, 0] = -2.0  # int x dy around the loop
     want[0, 1] = 2.0   # int y dx
@@ -54,10 +54,55 @@ def test_unit_ring_features():
     np.testing.assert_allclose(f[:3], [0, 0, 1], atol=1e-15)
     assert f[3] == pytest.approx(np.sqrt(0.5) / 10.0)
     s = M.scale_of(Q, r)
-    assert s > np.sqrt(8.0)  # |Q1| plus the higher moments' terms
-    np.testing.assert_allclose(f[4:13], (want / s).reshape(-1), atol=1e-15)
+    assert s > np.sqrt(8.0) / r  # |Q1| / r plus the higher moments' terms
+    np.testing.assert_allclose(f[7:16], (want / (s * r)).reshape(-1), atol=1e-15)
     assert np.abs(f[4:31]).max() <= 1.0
-    assert f[31] == 1.0
+
+
+def test_open_cluster_expansion_order(orc):
+    """64 random open segments: the expansion (Q0..Q3) against the direct
+    sum of BiotSavartLaw over the segments; the error falls ~16x per
+    doubling of the distance (first omitted order 4 against the Q0 term)."""
+    rng = np.random.default_rng(5)
+    A = rng.normal(scale=0.5, size=(64, 3))
+    E = rng.normal(scale=0.3, size=(64, 3))
+    G = rng.normal(size=64)
+    c = (A + 0.5 * E).mean(0)
+    r = max(np.linalg.norm(A - c, axis=1).max(), np.linalg.norm(A + E - c, axis=1).max())
+    Q = M.segment_moments(A, E, G, c)
+    dirs = rng.normal(size=(16, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    errs = []
+    for ratio in (4.0, 8.0, 16.0):
+        x = c + ratio * r * dirs
+        ref = np.zeros_like(x)
+        for k in range(len(x)):
+            for q in range(64):
+                ref[k] += G[q] * orc.biot_savart(A[q], A[q] + E[q], x[k])
+        got = np.array([M.expansion(xx, c, Q) for xx in x])
+        errs.append(np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+    assert errs[0] < 2e-2
+    assert errs[1] < errs[0] / 10 and errs[2] < errs[1] / 10
+
+
+def test_moments_translate_consistently():
+    """Moments about c2 equal those about c1 moved by the parent translation
+    (Q1' = Q1 + Q0 t, ... with t = c1 - c2)."""
+    rng = np.random.default_rng(6)
+    A, E, G = rng.normal(size=(20, 3)), rng.normal(size=(20, 3)), rng.normal(size=20)
+    c1, c2 = rng.normal(size=3), rng.normal(size=3)
+    q = M.segment_moments(A, E, G, c1)
+    t = c1 - c2
+    Q0, Q1 = q[:3], q[3:12].reshape(3, 3)
+    Q2, Q3 = q[12:39].reshape(3, 3, 3), q[39:].reshape(3, 3, 3, 3)
+    e = np.einsum
+    T1 = Q1 + e("i,j->ij", Q0, t)
+    T2 = Q2 + e("ij,k->ijk", Q1, t) + e("ik,j->ijk", Q1, t) + e("i,j,k->ijk", Q0, t, t)
+    T3 = (Q3 + e("ijk,l->ijkl", Q2, t) + e("ijl,k->ijkl", Q2, t) + e("ikl,j->ijkl", Q2, t)
+          + e("ij,k,l->ijkl", Q1, t, t) + e("ik,j,l->ijkl", Q1, t, t)
+          + e("il,j,k->ijkl", Q1, t, t) + e("i,j,k,l->ijkl", Q0, t, t, t))
+    moved = np.concatenate([Q0, T1.reshape(-1), T2.reshape(-1), T3.reshape(-1)])
+    np.testing.assert_allclose(moved, M.segment_moments(A, E, G, c2), rtol=1e-12, atol=1e-12)
 
 
 def test_expansion_converges_to_direct_sum(orc):

@@ -41,6 +41,7 @@ GPU_SYMBOLS = [
     "uvlm_sim_inverse_stats",
     "uvlm_mlp_forward", "uvlm_mlp_grad", "uvlm_mlp_train", "uvlm_farfield_set_mlp",
     "uvlm_farfield_mlp_stats", "uvlm_farfield_samples", "uvlm_sim_farfield_samples",
+    "uvlm_sim_farfield_learn",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
     "uvlm_batch_get_loads", "uvlm_batch_pairs", "uvlm_batch_set_option", "uvlm_batch_free",
@@ -158,6 +159,8 @@ def gpu_lib():
                                             C.c_uint, _fp, _fp, C.POINTER(_sz)]
         L.uvlm_sim_farfield_samples.argtypes = [_vp, C.c_double, C.c_uint, _fp, _fp,
                                                 C.POINTER(_sz)]
+        L.uvlm_sim_farfield_learn.argtypes = [_vp, C.c_double, _sz, _sz, _sz, C.c_float,
+                                              C.c_uint, _fp, _fp]
         L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
         L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
         L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]

@@ -72,6 +72,9 @@ struct uvlm_ctx {
   double ff_theta = 0.0, ff_tol = 0.0, ff_last_err = -1.0;
   int ff_audit_every = 0, ff_counter = 0, ff_audits = 0;
   bool ff_off = false;
+  // the split's sources are rebuilt only when the wake changed
+  long wake_version = 0, ff_version = -1;
+  Buf ffm;
   Buf ffg, ffv, ffx, ffstat;
   // kept-inverse solver (UVLM_SOLVE_F64_INV); one for the simulation, one
   // for the stateless uvlm_solve
@@ -267,6 +270,30 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
   return check_launch(c);
 }
 
+// The near/far split applies: set, not switched off by an audit, a wake.
+bool ff_active(const uvlm_ctx* c) {
+  return c->ff && c->ff_theta > 0.0 && !c->ff_off && c->wgrows > 0;
+}
+
+// Far-field sources from the current wake (positions gathered into pos,
+// circulations into ffg) unless already built for this wake state.
+int ff_current(uvlm_ctx* c) {
+  if (c->ff_version == c->wake_version) return UVLM_OK;
+  const int C1 = c->C + 1, rows = c->wrows, V = c->num * rows * C1;
+  TRY(ensure(c, c->pos, sizeof(double) * 3 * (size_t)V));
+  TRY(ensure(c, c->ffg, sizeof(double) * (size_t)c->num * c->wgrows * c->C));
+  launch_gather_rows(c->stream, P<double>(c->wake), c->cap, c->wstart, c->num, rows, C1,
+                     P<double>(c->pos));
+  launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
+                       c->C, P<double>(c->ffg));
+  std::string e;
+  const int rc = c->ff->build(c->stream, P<double>(c->pos), P<double>(c->ffg), c->num,
+                              c->wgrows, c->C, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  c->ff_version = c->wake_version;
+  return UVLM_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -315,7 +342,7 @@ int uvlm_destroy(uvlm_ctx* c) {
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
                  &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat,
-                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev};
+                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev, &c->ffm};
   for (Buf* b : bufs) release(*b);
   free_staging(c->st_wing);
   free_staging(c->st_vm);
@@ -657,6 +684,7 @@ int uvlm_sim_init(uvlm_ctx* c, size_t num, size_t rows, size_t cols,
   c->vprev_rows = 0;
   c->loads_count = 0;
   c->pairs_eval = c->pairs_ref = 0.0;
+  c->wake_version += 1;
   if (c->inv) c->inv->invalidate();
   const size_t nn = num * (rows + 1) * (cols + 1);
   const size_t E = num * rows * (2 * cols + 1);
@@ -765,6 +793,7 @@ int uvlm_sim_shed(uvlm_ctx* c, size_t step) {
   launch_shed(c->stream, P<double>(c->wing), P<double>(c->gamma), c->num, c->R, c->C,
               P<double>(c->wake), c->cap, c->wstart, P<double>(c->wgam), c->capg, grow);
   pack_wake(c, P<double>(c->wake), c->cap, c->wstart);
+  c->wake_version += 1;
   return check_launch(c);
 }
 
@@ -778,11 +807,19 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   launch_geometry(c->stream, P<double>(c->wing), c->num, c->R, c->C, P<double>(c->cp),
                   P<double>(c->nrm), P<double>(c->area));
   TRY(stage_upload(c, c->st_vm, c->vm.p, vmorph_cp, sizeof(double) * 3 * N));
-  // wake velocity at the collocation points
-  ChunkPlan plan;
-  TRY(sum_sources(c, P<double>(c->cp), N, false, true, &plan));
-  launch_sum_chunks(c->stream, P<double>(c->partial), N, N, plan.total(),
-                    P<double>(c->uw));
+  // wake velocity at the collocation points (through the split when set)
+  if (ff_active(c)) {
+    TRY(ff_current(c));
+    std::string e;
+    const int rc = c->ff->velocity(c->stream, P<double>(c->cp), N, c->ff_theta,
+                                   P<double>(c->uw), &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  } else {
+    ChunkPlan plan;
+    TRY(sum_sources(c, P<double>(c->cp), N, false, true, &plan));
+    launch_sum_chunks(c->stream, P<double>(c->partial), N, N, plan.total(),
+                      P<double>(c->uw));
+  }
   launch_rhs(c->stream, P<double>(c->uw), P<double>(c->vm), P<double>(c->nrm), N, ff,
              P<double>(c->rhs));
   launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
@@ -854,9 +891,21 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
   TRY(stage_upload(c, c->st_ve, c->vedge.p, vmorph_edges, sizeof(double) * 3 * E));
   TRY(stage_upload(c, c->st_area, c->ref_area.p, ref_area, sizeof(double) * c->num));
   ChunkPlan plan;
-  TRY(sum_sources(c, P<double>(c->mid), E, true, true, &plan));
-  launch_flow_velocity(c->stream, P<double>(c->partial), E, E, plan.cb, plan.cw, ff,
-                       P<double>(c->uflow), 0, 0);
+  if (ff_active(c)) {  // bound sheets exactly, the wake through the split
+    TRY(ff_current(c));
+    TRY(sum_sources(c, P<double>(c->mid), E, true, false, &plan));
+    TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
+    std::string e;
+    const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E, c->ff_theta,
+                                   P<double>(c->ffm), &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    launch_ff_combine(c->stream, P<double>(c->partial), E, plan.cb, P<double>(c->ffm), E, ff, 0,
+                      0, P<double>(c->uflow));
+  } else {
+    TRY(sum_sources(c, P<double>(c->mid), E, true, true, &plan));
+    launch_flow_velocity(c->stream, P<double>(c->partial), E, E, plan.cb, plan.cw, ff,
+                         P<double>(c->uflow), 0, 0);
+  }
   const double Q = std::sqrt((ff[0] * ff[0] + ff[1] * ff[1]) + ff[2] * ff[2]);
   launch_loads(c->stream, P<double>(c->wing), P<double>(c->gamma),
                P<double>(c->gamma_prev), P<double>(c->nrm), P<double>(c->area),
@@ -900,7 +949,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
   // targets) through the near/far split; every ff_audit_every steps the
   // first stage is also summed exactly, compared, and the split switched
   // off when the difference exceeds ff_tol of the largest velocity.
-  const bool use_ff = c->ff_theta > 0.0 && !c->ff_off && c->wgrows > 0;
+  const bool use_ff = ff_active(c);
   bool audit = false;
   if (use_ff) {
     audit = c->ff_audit_every > 0 && (c->ff_counter % c->ff_audit_every) == 0;
@@ -916,12 +965,15 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     ChunkPlan plan;
     TRY(sum_sources(c, targets, V, true, false, &plan));
     std::string e;
-    int ntiles = 0;
-    const int* map = c->ff->lattice_tiles(c->stream, c->num, c->wgrows, c->C, &ntiles, &e);
-    if (!map) return fail(c, UVLM_ERR_OOM, "%s", e.c_str());
-    const int rc = c->ff->wake_velocity(c->stream, targets, P<double>(c->ffg), c->num,
-                                        c->wgrows, c->C, targets, V, map, ntiles, c->ff_theta,
-                                        P<double>(c->ffv), &e);
+    int rc;
+    if (targets == P<double>(c->pos)) {  // the current wake: built for the RHS already
+      TRY(ff_current(c));
+    } else {  // an intermediate stage's positions
+      rc = c->ff->build(c->stream, targets, P<double>(c->ffg), c->num, c->wgrows, c->C, &e);
+      if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+      c->ff_version = -1;
+    }
+    rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     launch_ff_combine(c->stream, P<double>(c->partial), V, plan.cb, P<double>(c->ffv), V, ff,
                       te_period, C1, out);
@@ -993,6 +1045,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
   launch_update(c->stream, scheme, P<double>(c->pos), P<double>(c->k1), k2, k3, k4, vprev,
                 c->vprev_rows, ign, dt, c->num, rows, C1, P<double>(c->wake), c->cap,
                 c->wstart);
+  c->wake_version += 1;
   TRY(check_launch(c));
   if (scheme == UVLM_ADVECT_AB2) {
     TRY(ensure(c, c->vprev, vb));
@@ -1009,6 +1062,7 @@ int uvlm_sim_erase_oldest_wake(uvlm_ctx* c) {
   if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
   if (c->wrows > 0) c->wrows -= 1;
   if (c->wgrows > 0) c->wgrows -= 1;
+  c->wake_version += 1;
   return UVLM_OK;
 }
 
@@ -1055,6 +1109,7 @@ int uvlm_sim_set_wake(uvlm_ctx* c, size_t node_rows, size_t gamma_rows,
   const int C1 = c->C + 1;
   c->wrows = (int)node_rows;
   c->wstart = c->cap - c->wrows;
+  c->wake_version += 1;
   c->wgrows = (int)gamma_rows;
   c->gstart = c->capg - c->wgrows;
   for (int n = 0; n < c->num; ++n) {
@@ -1159,15 +1214,40 @@ int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const 
     HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * gb, hipMemcpyHostToDevice,
                              c->stream));
   std::string e;
+  c->ff_version = -1;  // the simulation's sources are replaced
   const int rc = c->ff->wake_velocity(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)num,
-                                      (int)rows, (int)cols, P<double>(c->t_a), (int)nt, nullptr,
-                                      0, theta, P<double>(c->t_out), &e);
+                                      (int)rows, (int)cols, P<double>(c->t_a), (int)nt, theta,
+                                      P<double>(c->t_out), &e);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
   HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return UVLM_OK;
 }
+
+namespace {
+
+int mlp_upload(uvlm_ctx* c, const float* w, const float* X, const float* V, size_t n) {
+  if (!c->mlp && !(c->mlp = new (std::nothrow) MlpTrainer()))
+    return fail(c, UVLM_ERR_OOM, "network state");
+  TRY(ensure(c, c->mw, sizeof(float) * UVLM_MLP_PARAMS));
+  TRY(ensure(c, c->mx, sizeof(float) * kMlpIn * std::max<size_t>(n, 1)));
+  TRY(ensure(c, c->my, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+  HIPCHK(c, hipMemcpyAsync(c->mw.p, w, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyHostToDevice,
+                           c->stream));
+  if (n)
+    HIPCHK(c, hipMemcpyAsync(c->mx.p, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice,
+                             c->stream));
+  if (V) {
+    TRY(ensure(c, c->mv, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+    if (n)
+      HIPCHK(c, hipMemcpyAsync(c->mv.p, V, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+                               c->stream));
+  }
+  return UVLM_OK;
+}
+
+}  // namespace
 
 int uvlm_farfield_set_mlp(uvlm_ctx* c, const float* weights, double ymax) {
   if (!c) return UVLM_ERR_ARG;
@@ -1231,15 +1311,12 @@ int uvlm_farfield_samples(uvlm_ctx* c, const double* targets, size_t nt, const d
   return copy_samples(c, f, r, feats, resid, count);
 }
 
-int uvlm_sim_farfield_samples(uvlm_ctx* c, double theta, unsigned seed, float* feats,
-                              float* resid, size_t* count) {
-  if (!c || !count || (*count && (!feats || !resid))) return UVLM_ERR_ARG;
-  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
-  if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
-  if (*count > (1u << 22)) return fail(c, UVLM_ERR_ARG, "too many samples");
-  const size_t want = *count;
-  *count = 0;
-  TRY(set_dev(c));
+namespace {
+// Samples of the split at theta drawn from the simulation's current wake.
+int sim_samples(uvlm_ctx* c, double theta, size_t want, unsigned seed, std::vector<float>* f,
+                std::vector<float>* r) {
+  f->clear();
+  r->clear();
   if (c->wgrows == 0 || want == 0) return UVLM_OK;
   if (!c->ff) c->ff = new (std::nothrow) FarField();
   if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
@@ -1253,18 +1330,61 @@ int uvlm_sim_farfield_samples(uvlm_ctx* c, double theta, unsigned seed, float* f
   launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
                        c->C, P<double>(c->ffg));
   std::string e;
-  int ntiles = 0;
-  const int* map = c->ff->lattice_tiles(c->stream, c->num, c->wgrows, c->C, &ntiles, &e);
-  if (!map) return fail(c, UVLM_ERR_OOM, "%s", e.c_str());
+  c->ff_version = -1;
   int rc = c->ff->wake_velocity(c->stream, P<double>(c->pos), P<double>(c->ffg), c->num,
-                                c->wgrows, c->C, P<double>(c->pos), V, map, ntiles, theta,
+                                c->wgrows, c->C, P<double>(c->pos), V, theta,
                                 P<double>(c->ffv), &e);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
-  std::vector<float> f, r;
-  rc = c->ff->sample_residuals(c->stream, P<double>(c->pos), V, (int)want, seed, &f, &r, &e);
+  rc = c->ff->sample_residuals(c->stream, P<double>(c->pos), V, (int)want, seed, f, r, &e);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  return UVLM_OK;
+}
+}  // namespace
+
+int uvlm_sim_farfield_samples(uvlm_ctx* c, double theta, unsigned seed, float* feats,
+                              float* resid, size_t* count) {
+  if (!c || !count || (*count && (!feats || !resid))) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (*count > (1u << 22)) return fail(c, UVLM_ERR_ARG, "too many samples");
+  const size_t want = *count;
+  *count = 0;
+  TRY(set_dev(c));
+  std::vector<float> f, r;
+  TRY(sim_samples(c, theta, want, seed, &f, &r));
   *count = want;
   return copy_samples(c, f, r, feats, resid, count);
+}
+
+int uvlm_sim_farfield_learn(uvlm_ctx* c, double theta, size_t max_samples, size_t iters,
+                            size_t batch, float lr, unsigned seed, float* weights,
+                            float* last_loss) {
+  if (!c || !weights || max_samples == 0 || iters == 0 || batch == 0) return UVLM_ERR_ARG;
+  if (!c->sim) return fail(c, UVLM_ERR_STATE, "uvlm_sim_init not called");
+  if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (max_samples > (1u << 22) || iters > (1u << 20) || batch > (1u << 16))
+    return fail(c, UVLM_ERR_ARG, "training problem too large");
+  if (!(lr > 0.0f)) return fail(c, UVLM_ERR_ARG, "learning rate must be positive");
+  TRY(set_dev(c));
+  std::vector<float> f, r;
+  TRY(sim_samples(c, theta, max_samples, seed, &f, &r));
+  const size_t n = f.size() / kMlpIn;
+  if (n == 0) return fail(c, UVLM_ERR_STATE, "no far pairs to learn from");
+  float ymax = 0.0f;
+  for (float v : r) ymax = std::max(ymax, std::fabs(v));
+  TRY(mlp_upload(c, weights, f.data(), r.data(), n));
+  std::string e;
+  std::vector<float> lh;
+  int rc = c->mlp->train(c->stream, P<float>(c->mw), P<float>(c->mx), P<float>(c->mv), (int)n,
+                         (int)iters, (int)batch, lr, seed, &lh, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  HIPCHK(c, hipMemcpyAsync(weights, c->mw.p, sizeof(float) * UVLM_MLP_PARAMS,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (last_loss) *last_loss = lh.empty() ? 0.0f : lh.back();
+  rc = c->ff->set_mlp(c->stream, weights, true, 2.0f * ymax, &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  return UVLM_OK;
 }
 
 int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol) {
@@ -1289,29 +1409,6 @@ int uvlm_sim_farfield_stats(uvlm_ctx* c, double* last_audit_err, int* audits, in
 }
 
 // ---------------------------------------------------------------------------
-namespace {
-
-int mlp_upload(uvlm_ctx* c, const float* w, const float* X, const float* V, size_t n) {
-  if (!c->mlp && !(c->mlp = new (std::nothrow) MlpTrainer()))
-    return fail(c, UVLM_ERR_OOM, "network state");
-  TRY(ensure(c, c->mw, sizeof(float) * UVLM_MLP_PARAMS));
-  TRY(ensure(c, c->mx, sizeof(float) * kMlpIn * std::max<size_t>(n, 1)));
-  TRY(ensure(c, c->my, sizeof(float) * 3 * std::max<size_t>(n, 1)));
-  HIPCHK(c, hipMemcpyAsync(c->mw.p, w, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyHostToDevice,
-                           c->stream));
-  if (n)
-    HIPCHK(c, hipMemcpyAsync(c->mx.p, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice,
-                             c->stream));
-  if (V) {
-    TRY(ensure(c, c->mv, sizeof(float) * 3 * std::max<size_t>(n, 1)));
-    if (n)
-      HIPCHK(c, hipMemcpyAsync(c->mv.p, V, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
-                               c->stream));
-  }
-  return UVLM_OK;
-}
-
-}  // namespace
 
 int uvlm_mlp_forward(uvlm_ctx* c, const float* w, const float* feats, size_t n, float* y,
                      int mode) {

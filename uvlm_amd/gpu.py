@@ -44,6 +44,22 @@ def _samples(L, check, call, max_samples):
     return f[:cnt.value].copy(), r[:cnt.value].copy()
 
 
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """Initial parameters of the far-field correction network (layout of
+    UVLM_MLP_PARAMS): uniform(-a, a) weights with a = sqrt(3 / fan_in) (x 0.1
+    for the output layer), zero biases."""
+    rng = np.random.default_rng(seed)
+    shapes = [(128, 32), (128,), (128, 128), (128,), (128, 128), (128,), (3, 128), (3,)]
+    parts = []
+    for k, shp in enumerate(shapes):
+        if len(shp) == 2:
+            a = np.sqrt(3.0 / shp[1]) * (0.1 if k == 6 else 1.0)
+            parts.append(rng.uniform(-a, a, size=shp).reshape(-1))
+        else:
+            parts.append(np.zeros(shp))
+    return np.concatenate(parts + [np.zeros(1)]).astype(np.float32)
+
+
 class Context:
     """One uvlm_ctx (a HIP stream plus device buffers) on ``device``."""
 
@@ -419,6 +435,20 @@ class Simulation:
         return _samples(L, lambda rc: self._check(rc, "farfield_samples"),
                         lambda f, r, c: L.uvlm_sim_farfield_samples(ctx, float(theta), seed, f,
                                                                     r, c), max_samples)
+
+    def farfield_learn(self, w, theta, max_samples=60000, iters=3000, batch=512, lr=1e-3,
+                       seed=1):
+        """Train the correction network on samples of the current wake (warm
+        start from w) and switch it on; returns (weights, final loss)."""
+        w = _f32(w).copy()
+        loss = C.c_float()
+        L, ctx = N.gpu_lib(), self._gpu_ctx()
+        rc = L.uvlm_sim_farfield_learn(ctx, float(theta), max_samples, iters, batch, float(lr),
+                                       seed, _pf(w), C.byref(loss))
+        if rc != N.OK:
+            msg = L.uvlm_last_error(ctx)
+            raise UVLMError(rc, f"farfield_learn: {msg.decode() if msg else ''}")
+        return w, loss.value
 
     def inverse_stats(self):
         """Kept-inverse solve counters: (sweeps, Newton-Schulz refreshes,
