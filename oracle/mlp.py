@@ -1,16 +1,17 @@
-"""Numpy restatement of the far-field split's source tree and of the
-correction network: Morton-ordered segment leaves, vorticity moments, the
-moment expansion, network features, forward pass, loss gradient and Adam
-step, in float64.
+"""Numpy restatement of the far-field split's block moments and expansion
+and of the correction network: vorticity moments, the moment expansion,
+network features, forward pass, loss gradient and Adam step, in float64.
 
 TEST INFRASTRUCTURE ONLY.  This is the checker for the GPU far-field and
 network kernels: only ``tests/`` import it.  Neither has a counterpart in
 the reference (xeno14/UVLM sums every wake ring exactly,
 src/uvlm/advect.h:16-46); their definitions are this module's:
 
-* segments of a lattice: per sheet, spanwise (i, j) -> (i, j+1) by node
-  row, then chordwise (i, j) -> (i+1, j) by node column, strength = the
-  difference of the ring circulations either side (zero outside);
+* rings of a lattice in VORING order (vortex.cpp:145-160); segments of a
+  lattice (another partition of the same field): per sheet,
+  spanwise (i, j) -> (i, j+1) by node row, then chordwise (i, j) -> (i+1, j)
+  by node column, strength = the difference of the ring circulations either
+  side (zero outside);
 * moments of a set of segments about c: Q_n[i, J] = sum Gamma e_i
   int_0^1 (a + t e - c)^(x)n dt, n = 0..3 (3 + 9 + 27 + 81 values);
 * expansion: u_a = 1/4pi sum_n (-1)^n / n! eps_abc Q_n[c, J] T_{n+1}[b, J];
@@ -139,25 +140,6 @@ def lattice_segments(nodes, gamma):
     return np.array(A), np.array(E), np.array(Gs)
 
 
-def morton_order(points):
-    """Stable order of points [n][3] by their 63-bit Morton key in the
-    points' bounding box (21 bits per axis, t = (p - min) / (max - min)
-    truncated to 21 bits)."""
-    lo, hi = points.min(axis=0), points.max(axis=0)
-    key = np.zeros(len(points), dtype=np.uint64)
-    for d in range(3):
-        w = hi[d] - lo[d]
-        t = (points[:, d] - lo[d]) / w if w > 0 else np.zeros(len(points))
-        q = np.floor(np.clip(t, 0.0, 1.0) * 2097151.0).astype(np.uint64)
-        v = q & np.uint64(0x1fffff)
-        for shift, mask in ((32, 0x1f00000000ffff), (16, 0x1f0000ff0000ff),
-                            (8, 0x100f00f00f00f00f), (4, 0x10c30c30c30c30c3),
-                            (2, 0x1249249249249249)):
-            v = (v | (v << np.uint64(shift))) & np.uint64(mask)
-        key |= v << np.uint64(d)
-    return np.argsort(key, kind="stable")
-
-
 def segment_moments(A, E, G, c):
     """Moments [120] (Q0, Q1, Q2, Q3 flattened) of segments about c."""
     Q = [np.zeros(3), np.zeros((3, 3)), np.zeros((3, 3, 3)), np.zeros((3, 3, 3, 3))]
@@ -177,34 +159,21 @@ def segment_moments(A, E, G, c):
     return np.concatenate([q.reshape(-1) for q in Q])
 
 
-def leaf_tree(A, E, G, leaf=64, branch=4):
-    """Centres, radii and segment ranges of the split's clusters, level by
-    level (Morton-ordered leaves of `leaf` segments, `branch` merged per
-    level), with the segments permuted into Morton order."""
-    mid = A + 0.5 * E
-    p = morton_order(mid)
-    A, E, G = A[p], E[p], G[p]
-    levels = []
-    nodes = []
-    for s0 in range(0, len(A), leaf):
-        a, e = A[s0:s0 + leaf], E[s0:s0 + leaf]
-        c = np.zeros(3)
-        for q in range(len(a)):
-            c += a[q] + 0.5 * e[q]
-        c /= len(a)
-        r = np.sqrt(max(np.max(np.sum((a - c) ** 2, 1)), np.max(np.sum((a + e - c) ** 2, 1))))
-        nodes.append((c, r, s0, s0 + len(a)))
-    levels.append(nodes)
-    while len(levels[-1]) > 1:
-        prev, nodes = levels[-1], []
-        for q0 in range(0, len(prev), branch):
-            ch = prev[q0:q0 + branch]
-            w = np.array([x[3] - x[2] for x in ch], dtype=np.float64)
-            c = sum(wi * x[0] for wi, x in zip(w, ch)) / w.sum()
-            r = max(np.linalg.norm(x[0] - c) + x[1] for x in ch)
-            nodes.append((c, r, ch[0][2], ch[-1][3]))
-        levels.append(nodes)
-    return levels, (A, E, G)
+def lattice_rings(nodes, gamma):
+    """Rings (nodes P [n][4][3] in VORING order p0 = (i,j), p1 = (i,j+1),
+    p2 = (i+1,j+1), p3 = (i+1,j), circulations G [n]) of lattices nodes
+    [num][R+1][C+1][3], gamma [num][R][C], ordered (n, i, j)."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    gamma = np.asarray(gamma, dtype=np.float64)
+    P = np.stack([nodes[:, :-1, :-1], nodes[:, :-1, 1:], nodes[:, 1:, 1:], nodes[:, 1:, :-1]], -2)
+    return P.reshape(-1, 4, 3), gamma.reshape(-1)
+
+
+def ring_segments(P, G):
+    """The rings' segments p0 -> p1 -> p2 -> p3 -> p0 as (A, E, G)."""
+    A = P.reshape(-1, 3)
+    E = (np.roll(P, -1, axis=1) - P).reshape(-1, 3)
+    return A, E, np.repeat(G, 4)
 
 
 def scale_of(Q, r):

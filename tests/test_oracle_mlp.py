@@ -124,3 +124,16 @@ def test_expansion_converges_to_direct_sum(orc):
         errs.append(np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)))
     assert errs[0] < 2e-2
     assert errs[1] < errs[0] / 6 and errs[2] < errs[1] / 6
+
+
+def test_ring_and_segment_partitions_agree():
+    """The rings of a lattice and its segments (Gamma differences) carry the
+    same moments: the ring tree's sources are the segment form's field."""
+    rng = np.random.default_rng(7)
+    nodes = rng.normal(size=(2, 4, 6, 3))
+    gamma = rng.normal(size=(2, 3, 5))
+    c = rng.normal(size=3)
+    qs = M.segment_moments(*M.lattice_segments(nodes, gamma), c)
+    qr = M.segment_moments(*M.ring_segments(*M.lattice_rings(nodes, gamma)), c)
+    np.testing.assert_allclose(qr, qs, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(qr[:3], 0.0, atol=1e-12)  # closed loops

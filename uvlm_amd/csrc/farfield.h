@@ -8,17 +8,17 @@
 
 namespace uvlm {
 
-// Vorticity moments of a cluster of segments about its centre c:
+// Vorticity moments of a block of wake rings about its centre c:
 // Q0[i], Q1[i][j], Q2[i][j][k], Q3[i][j][k][l] = sum over its segments of
 // Gamma e_i int_0^1 (a + t e - c)^{(x)n} dt (e = segment vector, a = its
-// start), n = 0..3; 3 + 9 + 27 + 81 doubles.  Q0 vanishes for closed loops
-// but not for a spatial cluster of segments.
+// start), n = 0..3; 3 + 9 + 27 + 81 doubles.  A block with circulation zero
+// outside it is a set of closed loops: Q0 vanishes up to rounding.
 constexpr int kFFMom = 120;
 constexpr int kFFQ1 = 3, kFFQ2 = 12, kFFQ3 = 39;  // offsets of Q1, Q2, Q3
-// Leaves: runs of kFFLeaf segments in Morton order; a node of level l
-// merges kFFBranch consecutive nodes of level l - 1.
-constexpr int kFFLeaf = 64;
-constexpr int kFFBranch = 4;
+// Level-0 clusters: blocks of kFFRows ring rows x kFFCols ring columns of
+// one sheet; level l merges 2 x 2 blocks of level l - 1.
+constexpr int kFFRows = 4;
+constexpr int kFFCols = 16;
 // MLP correction of the expansion: inputs, hidden width, hidden layers.
 constexpr int kMlpIn = 32;
 constexpr int kMlpHidden = 128;
@@ -27,23 +27,16 @@ constexpr int kMlpMaxLevel = 2;
 
 struct alignas(16) FFCluster {
   double c[3];  // expansion centre
-  double r;     // radius: largest distance of its segments' end points from c
-  int lo, hi;   // leaves [lo, hi) it covers
-  int level, nseg;
+  double r;     // radius: largest distance of its nodes from c
+  int n, r0, r1, c0;  // sheet, ring rows [r0, r1), ring columns [c0, c1)
+  int c1, level, pad0, pad1;
 };
-static_assert(sizeof(FFCluster) == 48, "FFCluster must be 48 bytes");
+static_assert(sizeof(FFCluster) == 64, "FFCluster must be 64 bytes");
 
-// One wake segment a -> a + e of circulation g (gs = g / 4pi).
-struct alignas(16) FFSeg {
-  double a[3];
-  double e[3];
-  double g, gs;
-};
-static_assert(sizeof(FFSeg) == 64, "FFSeg must be 64 bytes");
-
+// Layout of a lattice whose ring blocks form the cluster tree.
 struct FFTree {
-  int nseg = 0, nleaf = 0, levels = 0, total = 0;
-  int count[32], off[32];
+  int num = 0, rows = 0, cols = 0, levels = 0, total = 0;
+  int nbr[32], nbc[32], off[32], hr[32], hc[32];
 };
 
 // Weights of the correction network: h1 = silu(W1 x + b1), h2, h3 likewise
@@ -77,26 +70,26 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 class FarField {
  public:
   ~FarField();
-  // Sources: the segments of the lattice nodes [num][rows+1][cols+1][3] with
-  // ring circulations gamma [num][rows][cols] (device, compact), sorted in
-  // Morton order of their midpoints, cut into leaves of kFFLeaf and merged
-  // kFFBranch at a time up to one root; moments of every node.
+  // Sources: the wake lattice nodes [num][rows+1][cols+1][3] with ring
+  // circulations gamma [num][rows][cols] (device, compact), cut into blocks
+  // of kFFRows x kFFCols rings merged 2 x 2 up to one block per sheet;
+  // moments of every block and the near-sum record of every level-0 block.
   int build(hipStream_t s, const double* nodes, const double* gamma, int num, int rows, int cols,
             std::string* err);
   // Velocity of the built sources at nt targets (device [nt][3]): targets
   // are tiled 64 at a time in Morton order; a cluster whose distance from a
   // tile exceeds theta times its radius (beyond the tile's radius) uses its
   // moment expansion (plus the network's correction when enabled), the
-  // leaves of the others are summed exactly.  out [nt][3] overwritten.
+  // level-0 blocks of the others are summed exactly.  out [nt][3] overwritten.
   int velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
                std::string* err);
   // build + velocity.
   int wake_velocity(hipStream_t s, const double* nodes, const double* gamma, int num, int rows,
                     int cols, const double* targets, int nt, double theta, double* out,
                     std::string* err);
-  // Leaves summed exactly and clusters expanded, summed over the tiles of
-  // the last velocity call (reads the list counts back).
-  int list_stats(hipStream_t s, double* near_leaves, double* far_clusters, std::string* err);
+  // Level-0 blocks summed exactly and clusters expanded, summed over the
+  // tiles of the last velocity call (reads the list counts back).
+  int list_stats(hipStream_t s, double* near_blocks, double* far_clusters, std::string* err);
   // Network: FP32 weights (host, MlpShape::total floats) and the trained
   // output range ymax (outputs beyond it trip to the exact cluster sum);
   // enable = false switches the correction off.
@@ -122,13 +115,13 @@ class FarField {
   bool built_ = false, mlp_on_ = false;
   float ymax_ = 0.0f;
   // device buffers
-  void *d_raw_ = nullptr, *d_seg_ = nullptr, *d_key_ = nullptr, *d_key2_ = nullptr,
+  void *d_rec_ = nullptr, *d_key_ = nullptr, *d_key2_ = nullptr,
        *d_idx_ = nullptr, *d_idx2_ = nullptr, *d_sort_tmp_ = nullptr, *d_box_ = nullptr,
        *d_cl_ = nullptr, *d_mom_ = nullptr, *d_map_ = nullptr, *d_tiles_ = nullptr,
        *d_near_ = nullptr, *d_far_ = nullptr, *d_cnt_ = nullptr, *d_front_ = nullptr,
        *d_part_ = nullptr, *d_w_ = nullptr, *d_wf_ = nullptr, *d_mstat_ = nullptr,
        *d_samp_ = nullptr;
-  size_t c_raw_ = 0, c_seg_ = 0, c_key_ = 0, c_key2_ = 0, c_idx_ = 0, c_idx2_ = 0,
+  size_t c_rec_ = 0, c_key_ = 0, c_key2_ = 0, c_idx_ = 0, c_idx2_ = 0,
          c_sort_tmp_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
          c_near_ = 0, c_far_ = 0, c_cnt_ = 0, c_front_ = 0, c_part_ = 0, c_w_ = 0, c_wf_ = 0,
          c_mstat_ = 0, c_samp_ = 0;

@@ -1,23 +1,23 @@
 // farfield.hip -- near/far split of the wake's Biot-Savart sum on gfx950.
 //
-// Sources are the wake lattice's segments (strength = difference of the
-// circulations of the rings either side), sorted in Morton order of their
-// midpoints and cut into leaves of kFFLeaf segments; a node of level l
-// merges kFFBranch consecutive nodes of level l - 1 up to one root.  The
-// order follows space, not the lattice, so clusters stay compact however
-// the wake rolls up, folds or stretches.
+// Sources: the wake lattice of every sheet is cut into blocks of kFFRows x
+// kFFCols rings (level 0); level l merges 2 x 2 blocks of level l - 1
+// (columns stop doubling at the sheet width).  A block with circulation zero
+// outside it is a set of closed loops, so the blocks of any partition sum
+// to the whole lattice and a block's field falls like a dipole's.  Inside a
+// block, lattice neighbours share nodes and segments: the exact sum walks
+// its segments once (circulation differences) as chains with 1/|x - p|
+// carried from node to node.
 //
-// Targets are sorted the same way and tiled 64 at a time.  A breadth-first
-// traversal per tile classifies clusters: far when |c - c_tile| - r_tile >
-// theta r, near at leaf level, otherwise opened.  Near leaves are summed
-// segment by segment (the reference's BiotSavartLaw with its cut-offs, FP64
-// with refined rsqrt / rcp); far clusters use the multipole expansion of
-// the vector potential A = 1/4pi sum Gamma int dl / |x - y| in the
-// cluster's moments Q0..Q3:
+// Targets (any points) are sorted in Morton order and tiled 64 at a time.
+// A breadth-first traversal per tile classifies blocks: far when
+// |c - c_tile| - r_tile > theta r, near at level 0, otherwise opened.  Far
+// blocks use the multipole expansion of the vector potential
+// A = 1/4pi sum Gamma int dl / |x - y| in the block's moments Q0..Q3:
 //   u_a = 1/4pi sum_{n=0..3} (-1)^n / n! eps_abc Q_n[c, J] T_{n+1}[b, J],
-// T_m = the m-th derivative tensor of 1/r (u = curl A holds for open
-// segments too).  Leaf moments are integrated segment by segment (exact
-// polynomials in the segment parameter); parents translate their children's.
+// T_m = the m-th derivative tensor of 1/r.  Level-0 moments are integrated
+// segment by segment (exact polynomials in the segment parameter); parents
+// translate their children's.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -39,64 +39,28 @@ constexpr int kTileT = 64;        // targets per tile
 constexpr int kGroups = 8;        // partial-sum groups per tile (near and far each)
 constexpr int kMomThreads = 128;  // >= kFFMom: one thread per moment component
 
-// --- sources -----------------------------------------------------------------
-// Segment q of the lattice: per sheet, spanwise (i, j) -> (i, j+1) by node
-// row, then chordwise (i, j) -> (i+1, j) by node column.
-__global__ void k_seg_emit(const double* __restrict__ nodes, const double* __restrict__ gamma,
-                           int num, int rows, int cols, FFSeg* __restrict__ out) {
-  const long per = (long)(rows + 1) * cols + (long)(cols + 1) * rows;
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= per * num) return;
-  const int n = (int)(q / per);
-  const long u = q % per;
-  const int C1 = cols + 1;
-  const double* P = nodes + 3 * (long)n * (rows + 1) * C1;
-  const double* G = gamma + (long)n * rows * cols;
-  auto gl = [&](int i, int j) -> double {
-    return (i >= 0 && i < rows && j >= 0 && j < cols) ? G[(long)i * cols + j] : 0.0;
-  };
-  int ia, ja, ib, jb;
-  double g;
-  if (u < (long)(rows + 1) * cols) {
-    ia = ib = (int)(u / cols);
-    ja = (int)(u % cols);
-    jb = ja + 1;
-    g = gl(ia, ja) - gl(ia - 1, ja);
-  } else {
-    const long v = u - (long)(rows + 1) * cols;
-    ja = jb = (int)(v / rows);
-    ia = (int)(v % rows);
-    ib = ia + 1;
-    g = gl(ia, ja - 1) - gl(ia, ja);
-  }
-  const double* pa = P + 3 * ((long)ia * C1 + ja);
-  const double* pb = P + 3 * ((long)ib * C1 + jb);
-  FFSeg s;
-  s.a[0] = pa[0];
-  s.a[1] = pa[1];
-  s.a[2] = pa[2];
-  s.e[0] = pb[0] - pa[0];
-  s.e[1] = pb[1] - pa[1];
-  s.e[2] = pb[2] - pa[2];
-  s.g = g;
-  s.gs = g * kInv4Pi;
-  out[q] = s;
+// --- tree layout ------------------------------------------------------------
+__host__ __device__ inline int ff_id(const FFTree& t, int l, int n, int bi, int bj) {
+  return t.off[l] + (n * t.nbr[l] + bi) * t.nbc[l] + bj;
 }
+// Level-0 block shapes: nodes and segments of a full block.
+constexpr int kL0Nodes = (kFFRows + 1) * (kFFCols + 1);
+constexpr int kL0Segs = (kFFRows + 1) * kFFCols + (kFFCols + 1) * kFFRows;
+// Near-sum record of a level-0 block: its nodes (row-major, nc + 1 per row)
+// and its segment strengths / 4pi (spanwise segments by node row, then
+// chordwise by node column), contiguous so one wavefront loads it coalesced.
+constexpr int kRecStr = 3 * kL0Nodes;
+constexpr int kRec = kRecStr + kL0Segs;
 
-// Bounding box of n points (segment midpoints when segs, else x[n][3]):
-// box[0..2] min, box[3..5] max (one workgroup).
-__global__ __launch_bounds__(1024) void k_bbox(const FFSeg* __restrict__ segs,
-                                               const double* __restrict__ x, int n,
+// Bounding box of n points x[n][3]: box[0..2] min, box[3..5] max (one
+// workgroup).
+__global__ __launch_bounds__(1024) void k_bbox(const double* __restrict__ x, int n,
                                                double* __restrict__ box) {
   __shared__ double s[6][16];
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int q = threadIdx.x; q < n; q += 1024) {
     double p[3];
-    if (segs) {
-      for (int d = 0; d < 3; ++d) p[d] = segs[q].a[d] + 0.5 * segs[q].e[d];
-    } else {
-      for (int d = 0; d < 3; ++d) p[d] = x[3 * (long)q + d];
-    }
+    for (int d = 0; d < 3; ++d) p[d] = x[3 * (long)q + d];
     for (int d = 0; d < 3; ++d) {
       lo[d] = fmin(lo[d], p[d]);
       hi[d] = fmax(hi[d], p[d]);
@@ -132,14 +96,14 @@ __device__ inline unsigned long long spread21(unsigned long long v) {
 }
 
 // 63-bit Morton key of each point in its box (21 bits per axis).
-__global__ void k_morton(const FFSeg* __restrict__ segs, const double* __restrict__ x, int n,
+__global__ void k_morton(const double* __restrict__ x, int n,
                          const double* __restrict__ box, unsigned long long* __restrict__ key,
                          int* __restrict__ idx) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   unsigned long long k = 0;
   for (int d = 0; d < 3; ++d) {
-    const double p = segs ? segs[q].a[d] + 0.5 * segs[q].e[d] : x[3 * (long)q + d];
+    const double p = x[3 * (long)q + d];
     const double w = box[3 + d] - box[d];
     double t = w > 0.0 ? (p - box[d]) / w : 0.0;
     t = fmin(fmax(t, 0.0), 1.0);
@@ -148,12 +112,6 @@ __global__ void k_morton(const FFSeg* __restrict__ segs, const double* __restric
   }
   key[q] = k;
   idx[q] = q;
-}
-
-__global__ void k_seg_gather(const FFSeg* __restrict__ raw, const int* __restrict__ perm, int n,
-                             FFSeg* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) out[q] = raw[perm[q]];
 }
 
 // Component m of the moments: type 0 (m < 3), 1 (< 12), 2 (< 39), 3 (< 120)
@@ -194,97 +152,153 @@ __device__ __forceinline__ double segment_moment(const MomIndex& x, double g, co
   return ge * p3;
 }
 
-// One workgroup per leaf: centre = mean of its segments' midpoints (in
-// order), radius over their end points, one thread per moment component.
-__global__ __launch_bounds__(kMomThreads) void k_ff_leaf(const FFSeg* __restrict__ seg, int nseg,
-                                                         FFCluster* __restrict__ cl,
-                                                         double* __restrict__ mom) {
-  __shared__ double sA[kFFLeaf * 3], sE[kFFLeaf * 3], sg[kFFLeaf];
+// One workgroup per level-0 block: nodes staged in LDS, centre (node mean,
+// in node order) and radius, the block's segments (spanwise node rows, then
+// chordwise node columns; strength = difference of the in-block
+// circulations either side), then one thread per moment component summing
+// over the segments in order; the block's near record is written on the way.
+__global__ __launch_bounds__(kMomThreads) void k_ff_level0(FFTree t,
+                                                           const double* __restrict__ nodes,
+                                                           const double* __restrict__ gamma,
+                                                           FFCluster* __restrict__ cl,
+                                                           double* __restrict__ mom,
+                                                           double* __restrict__ rec) {
+  __shared__ double sp[kL0Nodes * 3];
+  __shared__ double sA[kL0Segs * 3], sE[kL0Segs * 3], sg[kL0Segs];
   __shared__ double sc[3], srad[kMomThreads / 64];
-  const int leaf = blockIdx.x, tid = threadIdx.x;
-  const int s0 = leaf * kFFLeaf, ns = min(kFFLeaf, nseg - s0);
-  if (tid < ns) {
-    const FFSeg& s = seg[s0 + tid];
+  const int q = blockIdx.x, tid = threadIdx.x;
+  const int bj = q % t.nbc[0], bi = (q / t.nbc[0]) % t.nbr[0], n = q / (t.nbc[0] * t.nbr[0]);
+  const int C1 = t.cols + 1;
+  const int r0 = bi * kFFRows, r1 = min(t.rows, r0 + kFFRows);
+  const int c0 = bj * kFFCols, c1 = min(t.cols, c0 + kFFCols);
+  const int nr = r1 - r0, nc = c1 - c0, nn = (nr + 1) * (nc + 1);
+  const double* P = nodes + 3 * (long)n * (t.rows + 1) * C1;
+  const double* G = gamma + (long)n * t.rows * t.cols;
+  double* R = rec + (long)q * kRec;
+  for (int k = tid; k < nn; k += kMomThreads) {
+    const double* p = P + 3 * ((long)(r0 + k / (nc + 1)) * C1 + c0 + k % (nc + 1));
     for (int d = 0; d < 3; ++d) {
-      sA[3 * tid + d] = s.a[d];
-      sE[3 * tid + d] = s.e[d];
+      sp[3 * k + d] = p[d];
+      R[3 * k + d] = p[d];
     }
-    sg[tid] = s.g;
   }
   __syncthreads();
   if (tid == 0) {
-    double c[3] = {0.0, 0.0, 0.0};
-    for (int q = 0; q < ns; ++q)
-      for (int d = 0; d < 3; ++d) c[d] += sA[3 * q + d] + 0.5 * sE[3 * q + d];
-    for (int d = 0; d < 3; ++d) sc[d] = c[d] / ns;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int k = 0; k < nn; ++k) {
+      cx += sp[3 * k];
+      cy += sp[3 * k + 1];
+      cz += sp[3 * k + 2];
+    }
+    const double inv = 1.0 / nn;
+    sc[0] = cx * inv;
+    sc[1] = cy * inv;
+    sc[2] = cz * inv;
   }
   __syncthreads();
   const double cx = sc[0], cy = sc[1], cz = sc[2];
   double rr = 0.0;
-  if (tid < ns) {
-    const double ax = sA[3 * tid] - cx, ay = sA[3 * tid + 1] - cy, az = sA[3 * tid + 2] - cz;
-    const double bx = ax + sE[3 * tid], by = ay + sE[3 * tid + 1], bz = az + sE[3 * tid + 2];
-    rr = fmax(ax * ax + ay * ay + az * az, bx * bx + by * by + bz * bz);
+  for (int k = tid; k < nn; k += kMomThreads) {
+    const double dx = sp[3 * k] - cx, dy = sp[3 * k + 1] - cy, dz = sp[3 * k + 2] - cz;
+    rr = fmax(rr, dx * dx + dy * dy + dz * dz);
   }
   for (int off = 32; off > 0; off >>= 1) rr = fmax(rr, __shfl_xor(rr, off));
   if ((tid & 63) == 0) srad[tid >> 6] = rr;
-  __syncthreads();
-  // positions relative to the centre
-  if (tid < ns) {
-    sA[3 * tid] -= cx;
-    sA[3 * tid + 1] -= cy;
-    sA[3 * tid + 2] -= cz;
+  auto gl = [&](int i, int j) -> double {
+    return (i >= 0 && i < nr && j >= 0 && j < nc) ? G[(long)(r0 + i) * t.cols + c0 + j] : 0.0;
+  };
+  const int s1 = (nr + 1) * nc, ns = s1 + (nc + 1) * nr;
+  for (int s = tid; s < ns; s += kMomThreads) {
+    int ia, ja, ib, jb;
+    double ga, gb;
+    if (s < s1) {
+      ia = ib = s / nc;
+      ja = s % nc;
+      jb = ja + 1;
+      ga = gl(ia, ja);
+      gb = gl(ia - 1, ja);
+    } else {
+      const int u = s - s1;
+      ja = jb = u / nr;
+      ia = u % nr;
+      ib = ia + 1;
+      ga = gl(ia, ja - 1);
+      gb = gl(ia, ja);
+    }
+    R[kRecStr + s] = ga * kInv4Pi - gb * kInv4Pi;
+    const double* pa = sp + 3 * (ia * (nc + 1) + ja);
+    const double* pb = sp + 3 * (ib * (nc + 1) + jb);
+    sA[3 * s] = pa[0] - cx;
+    sA[3 * s + 1] = pa[1] - cy;
+    sA[3 * s + 2] = pa[2] - cz;
+    sE[3 * s] = pb[0] - pa[0];
+    sE[3 * s + 1] = pb[1] - pa[1];
+    sE[3 * s + 2] = pb[2] - pa[2];
+    sg[s] = ga - gb;
   }
   __syncthreads();
   if (tid < kFFMom) {
     const MomIndex x = mom_index(tid);
     double acc = 0.0;
-    for (int q = 0; q < ns; ++q) acc += segment_moment(x, sg[q], sA + 3 * q, sE + 3 * q);
-    mom[(long)leaf * kFFMom + tid] = acc;
+    for (int s = 0; s < ns; ++s) {
+      const double g = sg[s];
+      if (g == 0.0) continue;
+      acc += segment_moment(x, g, sA + 3 * s, sE + 3 * s);
+    }
+    mom[(long)q * kFFMom + tid] = acc;
   }
   if (tid == 0) {
-    double r2 = srad[0];
-    for (int w = 1; w < kMomThreads / 64; ++w) r2 = fmax(r2, srad[w]);
     FFCluster o;
     o.c[0] = cx;
     o.c[1] = cy;
     o.c[2] = cz;
+    double r2 = srad[0];
+    for (int w = 1; w < kMomThreads / 64; ++w) r2 = fmax(r2, srad[w]);
     o.r = sqrt(r2);
-    o.lo = leaf;
-    o.hi = leaf + 1;
+    o.n = n;
+    o.r0 = r0;
+    o.r1 = r1;
+    o.c0 = c0;
+    o.c1 = c1;
     o.level = 0;
-    o.nseg = ns;
-    cl[leaf] = o;
+    o.pad0 = o.pad1 = 0;
+    cl[q] = o;
   }
 }
 
-// One workgroup per node of level l: children kFFBranch q .. at level l - 1;
-// centre = segment-count weighted mean of theirs, radius bound from theirs;
-// thread m translates component m of every child to the new centre
+// One workgroup per block of level l (at most 2 x 2 children): centre =
+// ring-count weighted mean of the children's centres, radius bound from
+// theirs; thread m translates component m of every child to the new centre
 // (t = c_child - c):  Q1' = Q1 + Q0 t,  Q2' = Q2 + Q1 t + t Q1 + Q0 t t, ...
-__global__ __launch_bounds__(kMomThreads) void k_ff_node(FFTree t, int l,
-                                                         FFCluster* __restrict__ cl,
-                                                         double* __restrict__ mom) {
-  __shared__ int sid[kFFBranch];
-  __shared__ double stv[kFFBranch][3];
+// and sums over the children in order.
+__global__ __launch_bounds__(kMomThreads) void k_ff_parent(FFTree t, int l,
+                                                           FFCluster* __restrict__ cl,
+                                                           double* __restrict__ mom) {
+  __shared__ int sid[4];
+  __shared__ double stv[4][3];
   __shared__ int snch;
   const int q = blockIdx.x, tid = threadIdx.x;
-  const int id = t.off[l] + q;
+  const int bj = q % t.nbc[l], bi = (q / t.nbc[l]) % t.nbr[l], n = q / (t.nbc[l] * t.nbr[l]);
+  const int r0 = bi * t.hr[l], r1 = min(t.rows, r0 + t.hr[l]);
+  const int c0 = bj * t.hc[l], c1 = min(t.cols, c0 + t.hc[l]);
+  const int id = ff_id(t, l, n, bi, bj);
   if (tid == 0) {
-    const int c0 = kFFBranch * q, c1 = min(c0 + kFFBranch, t.count[l - 1]);
+    const int lc = l - 1;
+    const int ci0 = r0 / t.hr[lc], ci1 = min(t.nbr[lc], cdiv(r1, t.hr[lc]));
+    const int cj0 = c0 / t.hc[lc], cj1 = min(t.nbc[lc], cdiv(c1, t.hc[lc]));
     double cx = 0.0, cy = 0.0, cz = 0.0, w = 0.0;
-    int nch = 0, nseg = 0;
-    for (int c = c0; c < c1; ++c) {
-      const int cid = t.off[l - 1] + c;
-      const FFCluster& ch = cl[cid];
-      const double wc = (double)ch.nseg;
-      cx += wc * ch.c[0];
-      cy += wc * ch.c[1];
-      cz += wc * ch.c[2];
-      w += wc;
-      nseg += ch.nseg;
-      sid[nch++] = cid;
-    }
+    int nch = 0;
+    for (int ci = ci0; ci < ci1; ++ci)
+      for (int cj = cj0; cj < cj1; ++cj) {
+        const int cid = ff_id(t, lc, n, ci, cj);
+        const FFCluster& ch = cl[cid];
+        const double wc = (double)(ch.r1 - ch.r0) * (ch.c1 - ch.c0);
+        cx += wc * ch.c[0];
+        cy += wc * ch.c[1];
+        cz += wc * ch.c[2];
+        w += wc;
+        if (nch < 4) sid[nch++] = cid;
+      }
     cx /= w;
     cy /= w;
     cz /= w;
@@ -303,10 +317,13 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_node(FFTree t, int l,
     o.c[1] = cy;
     o.c[2] = cz;
     o.r = rad;
-    o.lo = cl[sid[0]].lo;
-    o.hi = cl[sid[nch - 1]].hi;
+    o.n = n;
+    o.r0 = r0;
+    o.r1 = r1;
+    o.c0 = c0;
+    o.c1 = c1;
     o.level = l;
-    o.nseg = nseg;
+    o.pad0 = o.pad1 = 0;
     cl[id] = o;
   }
   __syncthreads();
@@ -368,27 +385,27 @@ __global__ __launch_bounds__(64) void k_ff_tiles(const double* __restrict__ x,
   if (lane == 0) tiles[tile] = double4{cx, cy, cz, sqrt(r2)};
 }
 
-// One wavefront per tile: breadth-first traversal from the root.  Each
-// level's frontier is classified lane-parallel: far (the cluster's
-// expansion serves the whole tile), near at leaf level (summed exactly), or
-// opened (its children form the next frontier).  Lists are appended in
-// frontier order (ballot prefix counts), so they are deterministic.
-// Capacities: near the leaf count, far all clusters, each frontier the leaf
-// count (front: 2 of those per tile).
+// One wavefront per tile: breadth-first traversal of the block tree from the
+// top level.  Each level's frontier is classified lane-parallel: far (the
+// block's expansion serves the whole tile), near at level 0 (summed
+// exactly), or opened (its children form the next frontier).  Lists are
+// appended in frontier order (ballot prefix counts), so they are
+// deterministic.  Capacities: near all level-0 blocks, far all blocks, each
+// frontier the level-0 count (front: 2 of those per tile).
 __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __restrict__ cl,
                                                  const double4* __restrict__ tiles, double theta,
                                                  int* __restrict__ near, int* __restrict__ far,
                                                  int2* __restrict__ cnt, int* __restrict__ front) {
   const int tile = blockIdx.x, lane = threadIdx.x;
   const double4 tc = tiles[tile];
-  const int nl = t.nleaf;
-  int* my_near = near + (long)tile * nl;
+  const int n0 = t.num * t.nbr[0] * t.nbc[0];
+  int* my_near = near + (long)tile * n0;
   int* my_far = far + (long)tile * t.total;
-  int* F = front + (long)tile * 2 * nl;
-  int* Gn = F + nl;
-  int level = t.levels - 1;
-  int nF = t.count[level];
-  for (int k = lane; k < nF; k += 64) F[k] = k;  // level-local indices
+  int* F = front + (long)tile * 2 * n0;
+  int* Gn = F + n0;
+  const int L = t.levels - 1;
+  int nF = t.num * t.nbr[L] * t.nbc[L];
+  for (int k = lane; k < nF; k += 64) F[k] = t.off[L] + k;
   const unsigned long long below = (1ull << lane) - 1ull;
   int nn = 0, nf = 0;
   __syncthreads();
@@ -396,24 +413,31 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
     int nG = 0;
     for (int base = 0; base < nF; base += 64) {
       const int k = base + lane;
-      int q = -1, kind = 0, nch = 0;
+      int id = -1, kind = 0, nch = 0, ci0 = 0, cj0 = 0, ncj = 1, lc = 0, cn = 0;
       if (k < nF) {
-        q = F[k];
-        const FFCluster c = cl[t.off[level] + q];
+        id = F[k];
+        const FFCluster c = cl[id];
         const double dx = c.c[0] - tc.x, dy = c.c[1] - tc.y, dz = c.c[2] - tc.z;
         const double dist = sqrt(dx * dx + dy * dy + dz * dz);
         if (dist - tc.w > theta * c.r) {
           kind = 1;
-        } else if (level == 0) {
+        } else if (c.level == 0) {
           kind = 2;
         } else {
           kind = 3;
-          nch = min(kFFBranch, t.count[level - 1] - kFFBranch * q);
+          lc = c.level - 1;
+          cn = c.n;
+          ci0 = c.r0 / t.hr[lc];
+          const int ci1 = min(t.nbr[lc], cdiv(c.r1, t.hr[lc]));
+          cj0 = c.c0 / t.hc[lc];
+          const int cj1 = min(t.nbc[lc], cdiv(c.c1, t.hc[lc]));
+          ncj = cj1 - cj0;
+          nch = (ci1 - ci0) * ncj;
         }
       }
       const unsigned long long mf = __ballot(kind == 1), mn = __ballot(kind == 2);
-      if (kind == 1) my_far[nf + __popcll(mf & below)] = t.off[level] + q;
-      if (kind == 2) my_near[nn + __popcll(mn & below)] = q;
+      if (kind == 1) my_far[nf + __popcll(mf & below)] = id;
+      if (kind == 2) my_near[nn + __popcll(mn & below)] = id - t.off[0];
       nf += __popcll(mf);
       nn += __popcll(mn);
       int inc = nch;
@@ -422,7 +446,8 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
         if (lane >= off) inc += v;
       }
       const int excl = inc - nch;
-      for (int c = 0; c < nch; ++c) Gn[nG + excl + c] = kFFBranch * q + c;
+      for (int c = 0; c < nch; ++c)
+        Gn[nG + excl + c] = ff_id(t, lc, cn, ci0 + c / ncj, cj0 + c % ncj);
       nG += __shfl(inc, 63);
     }
     __syncthreads();
@@ -430,87 +455,123 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
     F = Gn;
     Gn = tmp;
     nF = nG;
-    --level;
   }
   if (lane == 0) cnt[tile] = int2{nn, nf};
 }
 
-// --- exact segments ---------------------------------------------------------------
-// u += gs * BiotSavartLaw(a -> a + e, x) with the reference's cut-offs
-// (|d|^2, |r1|, |r2| against 1e-10), refined hardware rsqrt / rcp.
-__device__ __forceinline__ void seg_bs(const double* a, const double* e, double gs, double x,
-                                       double y, double z, double& ux, double& uy, double& uz) {
-  const double r1x = x - a[0], r1y = y - a[1], r1z = z - a[2];
-  const double r2x = r1x - e[0], r2y = r1y - e[1], r2z = r1z - e[2];
-  const double s1 = __builtin_fma(r1z, r1z, __builtin_fma(r1y, r1y, r1x * r1x));
-  const double s2 = __builtin_fma(r2z, r2z, __builtin_fma(r2y, r2y, r2x * r2x));
-  const double dx = __builtin_fma(e[1], r1z, -e[2] * r1y);
-  const double dy = __builtin_fma(e[2], r1x, -e[0] * r1z);
-  const double dz = __builtin_fma(e[0], r1y, -e[1] * r1x);
-  const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-  const double oa = __builtin_fma(e[2], r1z, __builtin_fma(e[1], r1y, e[0] * r1x));
-  const double ob = __builtin_fma(e[2], r2z, __builtin_fma(e[1], r2y, e[0] * r2x));
-  const double tt = __builtin_fma(oa, dev::rsq64(s1), -ob * dev::rsq64(s2));
-  const bool cut = (dsq < kCutoff) | (s1 < kCutoff * kCutoff) | (s2 < kCutoff * kCutoff);
-  double K = tt * gs * dev::rcp64(dsq);
-  K = cut ? 0.0 : K;
-  ux = __builtin_fma(dx, K, ux);
-  uy = __builtin_fma(dy, K, uy);
-  uz = __builtin_fma(dz, K, uz);
+// --- exact sums of blocks ---------------------------------------------------------
+// Chains of a level-0 block's record (nr x nc rings) at one target: node
+// rows (spanwise segments), then node columns (chordwise), 1/|x - p|
+// carried along each chain (devmath.h); adds to u (with 1/4pi).
+__device__ __forceinline__ void walk_record(const double* R, int nr, int nc, double px, double py,
+                                            double pz, double& ux, double& uy, double& uz) {
+  const double* S = R + kRecStr;
+  double e[8];
+  e[7] = 0.0;
+  for (int i = 0; i <= nr; ++i) {
+    dev::Carry64 cr;
+    dev::carry_reset(cr);
+    const double* p = R + 3 * i * (nc + 1);
+    e[0] = p[0];
+    e[1] = p[1];
+    e[2] = p[2];
+    e[3] = e[4] = e[5] = e[6] = 0.0;
+    dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
+    for (int j = 1; j <= nc; ++j) {
+      p += 3;
+      e[0] = p[0];
+      e[1] = p[1];
+      e[2] = p[2];
+      e[3] = S[i * nc + j - 1];
+      e[4] = p[0] - p[-3];
+      e[5] = p[1] - p[-2];
+      e[6] = p[2] - p[-1];
+      dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
+    }
+  }
+  const int s1 = (nr + 1) * nc, st = 3 * (nc + 1);
+  for (int j = 0; j <= nc; ++j) {
+    dev::Carry64 cr;
+    dev::carry_reset(cr);
+    const double* p = R + 3 * j;
+    e[0] = p[0];
+    e[1] = p[1];
+    e[2] = p[2];
+    e[3] = e[4] = e[5] = e[6] = 0.0;
+    dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
+    for (int i = 1; i <= nr; ++i) {
+      p += st;
+      e[0] = p[0];
+      e[1] = p[1];
+      e[2] = p[2];
+      e[3] = S[s1 + j * nr + i - 1];
+      e[4] = p[0] - p[-st];
+      e[5] = p[1] - p[1 - st];
+      e[6] = p[2] - p[2 - st];
+      dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
+    }
+  }
 }
 
-// Exact velocity (with 1/4pi) of the segments of cluster c at one target.
-__device__ void exact_cluster(const FFCluster& c, const FFSeg* __restrict__ seg, int nseg,
+// Exact velocity (with 1/4pi) at one target of the level-0 blocks under
+// cluster c, blocks in row order.
+__device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double* __restrict__ rec,
                               double px, double py, double pz, double u[3]) {
   double ux = 0.0, uy = 0.0, uz = 0.0;
-  const int s1 = min(nseg, c.hi * kFFLeaf);
-  for (int q = c.lo * kFFLeaf; q < s1; ++q)
-    seg_bs(seg[q].a, seg[q].e, seg[q].gs, px, py, pz, ux, uy, uz);
+  const int bi0 = c.r0 / kFFRows, bi1 = cdiv(c.r1, kFFRows);
+  const int bj0 = c.c0 / kFFCols, bj1 = cdiv(c.c1, kFFCols);
+  for (int bi = bi0; bi < bi1; ++bi)
+    for (int bj = bj0; bj < bj1; ++bj) {
+      const int nr = min(t.rows, (bi + 1) * kFFRows) - bi * kFFRows;
+      const int nc = min(t.cols, (bj + 1) * kFFCols) - bj * kFFCols;
+      walk_record(rec + (long)ff_id(t, 0, c.n, bi, bj) * kRec, nr, nc, px, py, pz, ux, uy, uz);
+    }
   u[0] = ux;
   u[1] = uy;
   u[2] = uz;
 }
 
-// Group g of a tile sums its near leaves g, g + kGroups, ...: the leaf's
-// segments are staged in LDS (the next leaf is loaded into registers
-// meanwhile) and every lane sums them for its target.
-__global__ __launch_bounds__(64) void k_ff_near(const FFSeg* __restrict__ seg, int nseg,
+// Group g of a tile sums the tile's near blocks g, g + kGroups, ...: each
+// block's record is staged in LDS (the next one is loaded into registers
+// meanwhile) and walked for every lane's target.
+__global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl,
+                                                const double* __restrict__ rec,
                                                 const double* __restrict__ x, int nt,
-                                                const int* __restrict__ map, int nleaf,
+                                                const int* __restrict__ map, int n0,
                                                 const int* __restrict__ near,
                                                 const int2* __restrict__ cnt,
                                                 double* __restrict__ part) {
-  __shared__ double ss[kFFLeaf * 8];
+  constexpr int kPer = (kRec + 63) / 64;
+  __shared__ double sr[kPer * 64];
   const int tile = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
   const int tg = tile_target(map, tile, lane);
   const double px = tg >= 0 ? x[3 * (long)tg] : 0.0, py = tg >= 0 ? x[3 * (long)tg + 1] : 0.0,
                pz = tg >= 0 ? x[3 * (long)tg + 2] : 0.0;
   const int count = cnt[tile].x;
-  const int* list = near + (long)tile * nleaf;
+  const int* list = near + (long)tile * n0;
   double ux = 0.0, uy = 0.0, uz = 0.0;
-  const double* S = reinterpret_cast<const double*>(seg);
-  double pre[8];
-  int leaf = g < count ? list[g] : 0;
-  auto load = [&](int lf) {
-    const long q = (long)lf * kFFLeaf + lane;
+  double pre[kPer];
+  int id = g < count ? list[g] : 0;
+  auto load = [&](int b) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) pre[u] = q < nseg ? S[q * 8 + u] : 0.0;
+    for (int u = 0; u < kPer; ++u) {
+      const int k = lane + 64 * u;
+      pre[u] = k < kRec ? rec[(long)b * kRec + k] : 0.0;
+    }
   };
-  if (g < count) load(leaf);
+  if (g < count) load(id);
   for (int k = g; k < count; k += kGroups) {
-    const int ns = min(kFFLeaf, nseg - leaf * kFFLeaf);
+    const FFCluster c = cl[id];
+    const int nr = c.r1 - c.r0, nc = c.c1 - c.c0;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 8; ++u) ss[lane * 8 + u] = pre[u];
+    for (int u = 0; u < kPer; ++u) sr[lane + 64 * u] = pre[u];
     __syncthreads();
-    if (k + kGroups < count) {  // next leaf, in flight during the sums
-      leaf = list[k + kGroups];
-      load(leaf);
+    if (k + kGroups < count) {  // next block's record, in flight during the walk
+      id = list[k + kGroups];
+      load(id);
     }
-    for (int q = 0; q < ns; ++q) {
-      const double* s = ss + 8 * q;
-      seg_bs(s, s + 3, s[7], px, py, pz, ux, uy, uz);
-    }
+    walk_record(sr, nr, nc, px, py, pz, ux, uy, uz);
   }
   if (tg >= 0) {
     part[((long)g * 3 + 0) * nt + tg] = ux;
@@ -520,47 +581,61 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFSeg* __restrict__ seg, i
 }
 
 // --- far clusters: moment expansion ------------------------------------------------
-// u (without 1/4pi) of the moments Q at d = x - c, orders 0..3.
-__device__ void expand_moments(const double* Q, double dx, double dy, double dz, double u[3]) {
+// Traces of the symmetric position indices: tr[c] = Q2[c,j,j] (c < 3),
+// tr[3 + 3c + b] = Q3[c,b,k,k].
+__device__ inline void moment_traces(const double* Q, double tr[12]) {
+  const double* Q2 = Q + kFFQ2;
+  const double* Q3 = Q + kFFQ3;
+  for (int c = 0; c < 3; ++c) {
+    tr[c] = Q2[(c * 3 + 0) * 3 + 0] + Q2[(c * 3 + 1) * 3 + 1] + Q2[(c * 3 + 2) * 3 + 2];
+    for (int b = 0; b < 3; ++b) {
+      const double* q = Q3 + ((c * 3 + b) * 3) * 3;
+      tr[3 + 3 * c + b] = q[0] + q[4] + q[8];
+    }
+  }
+}
+
+// u (without 1/4pi) of the moments Q at d = x - c, orders 0..3.  With the
+// position indices of Q2, Q3 symmetric, the derivative tensors of 1/r
+// contract to
+//   S[c][b] = d_b alpha[c] + beta[c][b],  u_a = eps_abc S[c][b],
+//   alpha = -Q0/r^3 - 3 Q1d/r^5 - 15/2 Q2dd/r^7 + 3/2 tr2/r^5
+//           - 35/2 Q3ddd/r^9 + 15/2 tr3 d/r^7,
+//   beta  = Q1/r^3 + 3 Q2d/r^5 + 15/2 Q3dd/r^7 - 3/2 tr3/r^5
+// (Qd = Q contracted with d over its last position index).
+__device__ void expand_moments(const double* Q, const double* tr, double dx, double dy, double dz,
+                               double u[3]) {
   const double d[3] = {dx, dy, dz};
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double ir = rsqrt(r2);
   const double i2 = ir * ir, i3 = i2 * ir, i5 = i3 * i2, i7 = i5 * i2, i9 = i7 * i2;
-  // S[c][b] = Q0[c] T1[b] - Q1[c,j] T2[b,j] + 1/2 Q2[c,jk] T3[b,jk]
-  //           - 1/6 Q3[c,jkl] T4[b,jkl]
-  double S[3][3];
-  for (int c = 0; c < 3; ++c)
-    for (int b = 0; b < 3; ++b) S[c][b] = -Q[c] * d[b] * i3;
   const double* Q1 = Q + kFFQ1;
   const double* Q2 = Q + kFFQ2;
   const double* Q3 = Q + kFFQ3;
-  for (int b = 0; b < 3; ++b)
-    for (int j = 0; j < 3; ++j) {
-      const double djb = (j == b) ? 1.0 : 0.0;
-      const double t2 = 3.0 * d[b] * d[j] * i5 - djb * i3;
-      for (int c = 0; c < 3; ++c) S[c][b] -= Q1[c * 3 + j] * t2;
+  double al[3], be[3][3];
+  for (int c = 0; c < 3; ++c) {
+    double q1d = 0.0, q2dd = 0.0, q3ddd = 0.0, trd = 0.0;
+    for (int b = 0; b < 3; ++b) {
+      double q2d = 0.0, q3dd = 0.0;
       for (int k = 0; k < 3; ++k) {
-        const double dkb = (k == b) ? 1.0 : 0.0, djk = (j == k) ? 1.0 : 0.0;
-        const double t3 = -15.0 * d[b] * d[j] * d[k] * i7 +
-                          3.0 * (djb * d[k] + dkb * d[j] + djk * d[b]) * i5;
-        for (int c = 0; c < 3; ++c) S[c][b] += 0.5 * Q2[(c * 3 + j) * 3 + k] * t3;
-        for (int l = 0; l < 3; ++l) {
-          const double dlb = (l == b) ? 1.0 : 0.0, djl = (j == l) ? 1.0 : 0.0,
-                       dkl = (k == l) ? 1.0 : 0.0;
-          const double t4 =
-              105.0 * d[b] * d[j] * d[k] * d[l] * i9 -
-              15.0 * (djb * d[k] * d[l] + dkb * d[j] * d[l] + dlb * d[j] * d[k] +
-                      djk * d[b] * d[l] + djl * d[b] * d[k] + dkl * d[b] * d[j]) * i7 +
-              3.0 * (djb * dkl + dkb * djl + dlb * djk) * i5;
-          for (int c = 0; c < 3; ++c)
-            S[c][b] -= (1.0 / 6.0) * Q3[((c * 3 + j) * 3 + k) * 3 + l] * t4;
-        }
+        const double* q3 = Q3 + ((c * 3 + b) * 3 + k) * 3;
+        const double q3d = q3[0] * d[0] + q3[1] * d[1] + q3[2] * d[2];
+        q2d += Q2[(c * 3 + b) * 3 + k] * d[k];
+        q3dd += q3d * d[k];
       }
+      q1d += Q1[c * 3 + b] * d[b];
+      q2dd += q2d * d[b];
+      q3ddd += q3dd * d[b];
+      trd += tr[3 + 3 * c + b] * d[b];
+      be[c][b] = Q1[c * 3 + b] * i3 + 3.0 * q2d * i5 + 7.5 * q3dd * i7 -
+                 1.5 * tr[3 + 3 * c + b] * i5;
     }
-  // u_a = eps_abc S[c][b]
-  u[0] = S[2][1] - S[1][2];
-  u[1] = S[0][2] - S[2][0];
-  u[2] = S[1][0] - S[0][1];
+    al[c] = -Q[c] * i3 - 3.0 * q1d * i5 - 7.5 * q2dd * i7 + 1.5 * tr[c] * i5 -
+            17.5 * q3ddd * i9 + 7.5 * trd * i7;
+  }
+  u[0] = (d[1] * al[2] - d[2] * al[1]) + (be[2][1] - be[1][2]);
+  u[1] = (d[2] * al[0] - d[0] * al[2]) + (be[0][2] - be[2][0]);
+  u[2] = (d[0] * al[1] - d[1] * al[0]) + (be[1][0] - be[0][1]);
 }
 
 // Network features of (target, cluster) from its moments Q, offset d = x - c
@@ -619,7 +694,7 @@ __device__ double ff_features(const double* Q, double dx, double dy, double dz, 
 template <bool MLP>
 __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __restrict__ cl,
                                                const double* __restrict__ mom,
-                                               const FFSeg* __restrict__ seg,
+                                               const double* __restrict__ rec,
                                                const double* __restrict__ x, int nt,
                                                const int* __restrict__ map,
                                                const int* __restrict__ far,
@@ -627,7 +702,7 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
                                                float ymax, unsigned long long* __restrict__ stats,
                                                double* __restrict__ part) {
   __shared__ double sq[kFFMom];
-  __shared__ double sc[5];
+  __shared__ double sc[5], str_[12];
   __shared__ unsigned short lds[MLP ? 2 * kMlpLdsHalfs : 1];
   __shared__ float ylds[MLP ? 64 * 4 : 1];
   const int tile = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
@@ -646,9 +721,11 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
     if (lane == 3) sc[3] = cl[id].r;
     if (lane == 4) sc[4] = (double)cl[id].level;
     __syncthreads();
+    if (lane == 0) moment_traces(sq, str_);
+    __syncthreads();
     const double dx = px - sc[0], dy = py - sc[1], dz = pz - sc[2];
     double u[3];
-    expand_moments(sq, dx, dy, dz, u);
+    expand_moments(sq, str_, dx, dy, dz, u);
     u[0] *= kInv4Pi;
     u[1] *= kInv4Pi;
     u[2] *= kInv4Pi;
@@ -666,7 +743,7 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
           u[2] += (double)y[2] * scale;
         } else {  // outside the trained range: the exact sum of the cluster
           ++ntrips;
-          exact_cluster(cl[id], seg, t.nseg, px, py, pz, u);
+          exact_cluster(t, cl[id], rec, px, py, pz, u);
         }
       }
     }
@@ -696,7 +773,7 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
 // units of s rho^4 / (4pi D^2)); valid[q] = 0 for an empty lane, a cluster
 // above kMlpMaxLevel or one without circulation.
 __global__ void k_ff_samples(FFTree t, const FFCluster* __restrict__ cl,
-                             const double* __restrict__ mom, const FFSeg* __restrict__ seg,
+                             const double* __restrict__ mom, const double* __restrict__ rec,
                              const double* __restrict__ x, const int* __restrict__ map,
                              const int* __restrict__ far, const int4* __restrict__ pick, int n,
                              float* __restrict__ feats, float* __restrict__ resid,
@@ -715,9 +792,10 @@ __global__ void k_ff_samples(FFTree t, const FFCluster* __restrict__ cl,
   float f[kMlpIn];
   const double scale = ff_features(Q, dx, dy, dz, c.r, f);
   if (!(scale > 0.0)) return;
-  double ue[3], ux[3];
-  expand_moments(Q, dx, dy, dz, ue);
-  exact_cluster(c, seg, t.nseg, px, py, pz, ux);
+  double ue[3], ux[3], tr[12];
+  moment_traces(Q, tr);
+  expand_moments(Q, tr, dx, dy, dz, ue);
+  exact_cluster(t, c, rec, px, py, pz, ux);
   for (int k = 0; k < kMlpIn; ++k) feats[(long)q * kMlpIn + k] = f[k];
   for (int d = 0; d < 3; ++d) resid[(long)q * 3 + d] = (float)((ux[d] - ue[d] * kInv4Pi) / scale);
   valid[q] = 1;
@@ -815,7 +893,7 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 
 // ===========================================================================
 FarField::~FarField() {
-  void* bufs[] = {d_raw_,  d_seg_,   d_key_,  d_key2_, d_idx_,   d_idx2_, d_sort_tmp_,
+  void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_sort_tmp_,
                   d_box_,  d_cl_,    d_mom_,  d_map_,  d_tiles_, d_near_, d_far_,
                   d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_};
   for (void* p : bufs)
@@ -869,54 +947,39 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
     *err = "far field: empty source lattice";
     return UVLM_ERR_ARG;
   }
-  const long per = (long)(rows + 1) * cols + (long)(cols + 1) * rows;
-  const long nseg = per * num;
-  if (nseg > (1l << 28)) {
-    *err = "far field: too many segments";
+  if ((long)num * rows * cols > (1l << 28)) {
+    *err = "far field: too many rings";
     return UVLM_ERR_ARG;
   }
   FFTree& t = tree_;
-  t.nseg = (int)nseg;
-  t.nleaf = cdiv(nseg, kFFLeaf);
-  int l = 0, off = 0, cnt = t.nleaf;
+  t.num = num;
+  t.rows = rows;
+  t.cols = cols;
+  int l = 0, off = 0;
   for (;;) {
-    t.count[l] = cnt;
+    t.hr[l] = kFFRows << l;
+    t.hc[l] = std::min(kFFCols << l, cols);
+    t.nbr[l] = cdiv(rows, t.hr[l]);
+    t.nbc[l] = cdiv(cols, t.hc[l]);
     t.off[l] = off;
-    off += cnt;
-    if (cnt == 1 || l == 31) break;
-    cnt = cdiv(cnt, kFFBranch);
+    off += num * t.nbr[l] * t.nbc[l];
+    if ((t.nbr[l] == 1 && t.nbc[l] == 1) || l == 31) break;
     ++l;
   }
   t.levels = l + 1;
   t.total = off;
+  const int n0 = num * t.nbr[0] * t.nbc[0];
   int rc;
-  const size_t n = (size_t)nseg;
-  if ((rc = ensure(&d_raw_, &c_raw_, sizeof(FFSeg) * n, err)) ||
-      (rc = ensure(&d_seg_, &c_seg_, sizeof(FFSeg) * n, err)) ||
-      (rc = ensure(&d_key_, &c_key_, sizeof(unsigned long long) * n, err)) ||
-      (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned long long) * n, err)) ||
-      (rc = ensure(&d_idx_, &c_idx_, sizeof(int) * n, err)) ||
-      (rc = ensure(&d_idx2_, &c_idx2_, sizeof(int) * n, err)) ||
-      (rc = ensure(&d_box_, &c_box_, sizeof(double) * 8, err)) ||
+  if ((rc = ensure(&d_box_, &c_box_, sizeof(double) * 8, err)) ||
       (rc = ensure(&d_cl_, &c_cl_, sizeof(FFCluster) * (size_t)t.total, err)) ||
-      (rc = ensure(&d_mom_, &c_mom_, sizeof(double) * kFFMom * (size_t)t.total, err)))
+      (rc = ensure(&d_mom_, &c_mom_, sizeof(double) * kFFMom * (size_t)t.total, err)) ||
+      (rc = ensure(&d_rec_, &c_rec_, sizeof(double) * kRec * (size_t)n0, err)))
     return rc;
-  FFSeg* raw = (FFSeg*)d_raw_;
-  hipLaunchKernelGGL(k_seg_emit, dim3(cdiv(nseg, 256)), dim3(256), 0, s, nodes, gamma, num, rows,
-                     cols, raw);
-  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, (const FFSeg*)raw, (const double*)nullptr,
-                     (int)nseg, (double*)d_box_);
-  hipLaunchKernelGGL(k_morton, dim3(cdiv(nseg, 256)), dim3(256), 0, s, (const FFSeg*)raw,
-                     (const double*)nullptr, (int)nseg, (const double*)d_box_,
-                     (unsigned long long*)d_key_, (int*)d_idx_);
-  if ((rc = sort_pairs(s, (int)nseg, err)) != UVLM_OK) return rc;
-  hipLaunchKernelGGL(k_seg_gather, dim3(cdiv(nseg, 256)), dim3(256), 0, s, (const FFSeg*)raw,
-                     (const int*)d_idx2_, (int)nseg, (FFSeg*)d_seg_);
-  hipLaunchKernelGGL(k_ff_leaf, dim3(t.nleaf), dim3(kMomThreads), 0, s, (const FFSeg*)d_seg_,
-                     (int)nseg, (FFCluster*)d_cl_, (double*)d_mom_);
+  hipLaunchKernelGGL(k_ff_level0, dim3(n0), dim3(kMomThreads), 0, s, t, nodes, gamma,
+                     (FFCluster*)d_cl_, (double*)d_mom_, (double*)d_rec_);
   for (int lv = 1; lv < t.levels; ++lv)
-    hipLaunchKernelGGL(k_ff_node, dim3(t.count[lv]), dim3(kMomThreads), 0, s, t, lv,
-                       (FFCluster*)d_cl_, (double*)d_mom_);
+    hipLaunchKernelGGL(k_ff_parent, dim3(num * t.nbr[lv] * t.nbc[lv]), dim3(kMomThreads), 0, s,
+                       t, lv, (FFCluster*)d_cl_, (double*)d_mom_);
   if (hipGetLastError() != hipSuccess) {
     *err = "far-field build launch failed";
     return UVLM_ERR_HIP;
@@ -935,25 +998,25 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
   const FFTree& t = tree_;
   const int ntiles = cdiv(nt, kTileT);
   int rc;
-  const size_t nk = (size_t)std::max(nt, t.nseg);
+  const size_t nk = (size_t)nt;
+  const int n0 = t.num * t.nbr[0] * t.nbc[0];
   if ((rc = ensure(&d_key_, &c_key_, sizeof(unsigned long long) * nk, err)) ||
       (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned long long) * nk, err)) ||
       (rc = ensure(&d_idx_, &c_idx_, sizeof(int) * nk, err)) ||
       (rc = ensure(&d_idx2_, &c_idx2_, sizeof(int) * nk, err)) ||
       (rc = ensure(&d_map_, &c_map_, sizeof(int) * (size_t)ntiles * kTileT, err)) ||
       (rc = ensure(&d_tiles_, &c_tiles_, sizeof(double4) * (size_t)ntiles, err)) ||
-      (rc = ensure(&d_near_, &c_near_, sizeof(int) * (size_t)ntiles * t.nleaf, err)) ||
+      (rc = ensure(&d_near_, &c_near_, sizeof(int) * (size_t)ntiles * n0, err)) ||
       (rc = ensure(&d_far_, &c_far_, sizeof(int) * (size_t)ntiles * t.total, err)) ||
       (rc = ensure(&d_cnt_, &c_cnt_, sizeof(int2) * (size_t)ntiles, err)) ||
-      (rc = ensure(&d_front_, &c_front_, sizeof(int) * 2 * (size_t)ntiles * t.nleaf, err)) ||
+      (rc = ensure(&d_front_, &c_front_, sizeof(int) * 2 * (size_t)ntiles * n0, err)) ||
       (rc = ensure(&d_part_, &c_part_, sizeof(double) * 3 * 2 * kGroups * (size_t)nt, err)))
     return rc;
   // target tiles: the targets in Morton order of their own box
   double* tbox = (double*)d_box_;
-  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, (const FFSeg*)nullptr, targets, nt, tbox);
-  hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, (const FFSeg*)nullptr,
-                     targets, nt, (const double*)tbox, (unsigned long long*)d_key_,
-                     (int*)d_idx_);
+  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, targets, nt, tbox);
+  hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
+                     (const double*)tbox, (unsigned long long*)d_key_, (int*)d_idx_);
   if ((rc = sort_pairs(s, nt, err)) != UVLM_OK) return rc;
   int* map = (int*)d_map_;
   hipLaunchKernelGGL(k_tile_map, dim3(cdiv((long)ntiles * kTileT, 256)), dim3(256), 0, s,
@@ -963,18 +1026,18 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
                      (const double4*)d_tiles_, theta, (int*)d_near_, (int*)d_far_,
                      (int2*)d_cnt_, (int*)d_front_);
-  hipLaunchKernelGGL(k_ff_near, dim3(ntiles, kGroups), dim3(64), 0, s, (const FFSeg*)d_seg_,
-                     t.nseg, targets, nt, (const int*)map, t.nleaf, (const int*)d_near_,
+  hipLaunchKernelGGL(k_ff_near, dim3(ntiles, kGroups), dim3(64), 0, s, (const FFCluster*)d_cl_,
+                     (const double*)d_rec_, targets, nt, (const int*)map, n0, (const int*)d_near_,
                      (const int2*)d_cnt_, (double*)d_part_);
   if (mlp_on_) {
     hipLaunchKernelGGL(k_ff_far<true>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
-                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const FFSeg*)d_seg_,
+                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
                        targets, nt, (const int*)map, (const int*)d_far_, (const int2*)d_cnt_,
                        mlp_dev_view(d_w_), ymax_, (unsigned long long*)d_mstat_,
                        (double*)d_part_);
   } else {
     hipLaunchKernelGGL(k_ff_far<false>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
-                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const FFSeg*)d_seg_,
+                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
                        targets, nt, (const int*)map, (const int*)d_far_, (const int2*)d_cnt_,
                        MlpDev{}, 0.0f, (unsigned long long*)nullptr, (double*)d_part_);
   }
@@ -1115,7 +1178,7 @@ int FarField::sample_residuals(hipStream_t s, const double* targets, int nt, int
     return UVLM_ERR_HIP;
   }
   hipLaunchKernelGGL(k_ff_samples, dim3(cdiv(max_samples, 64)), dim3(64), 0, s, tree_,
-                     (const FFCluster*)d_cl_, (const double*)d_mom_, (const FFSeg*)d_seg_,
+                     (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
                      targets, (const int*)d_map_, (const int*)d_far_, (const int4*)dpick,
                      max_samples, df, dr, dv);
   std::vector<float> hf(sb * kMlpIn), hr(sb * 3);
