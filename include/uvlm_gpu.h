@@ -187,20 +187,25 @@ int uvlm_sim_pair_counts(uvlm_ctx* ctx, double* evaluated, double* reference);
 
 /* --- far-field split of the wake sum (no reference counterpart) -----------
  * The reference sums every wake ring at every target (advect.h:16-46).  The
- * split keeps that sum exact for the blocks of rings near a target tile and
- * replaces it, for blocks farther than theta times their radius, by the
- * multipole expansion of the Biot-Savart law in the block's vorticity
- * moments (orders 1..3). */
+ * split cuts the wake lattice into blocks of 4 x 16 rings merged 2 x 2 into
+ * a tree, tiles the targets 64 at a time in Morton order, keeps the sum
+ * exact for the blocks near a tile and replaces it, for blocks farther than
+ * theta times their radius (beyond the tile's radius), by the multipole
+ * expansion of the block's vector potential in its vorticity moments
+ * (orders 0..3). */
 
 /* Wake-induced velocity at n_targets targets of the lattice nodes
  * [num][rows+1][cols+1][3], gamma [num][rows][cols] (theta > 1). */
 int uvlm_farfield_velocity(uvlm_ctx* ctx, const double* targets, size_t n_targets,
                            const double* nodes, const double* gamma, size_t num,
                            size_t rows, size_t cols, double theta, double* out);
-/* Use the split for the wake in the simulation's advection stages (theta <= 0
- * switches it off).  Every audit_every steps (0: never) the first stage is
- * also summed exactly; if the largest difference exceeds tol times the
- * largest velocity, the split is switched off for the rest of the run. */
+/* Use the split for the wake's velocity in the simulation (theta <= 0
+ * switches it off): in the advection stages at the wake's nodes with theta,
+ * at the collocation points (RHS) and the edge midpoints (loads) with
+ * 2 theta; the tree is built once per wake state.  Every audit_every steps
+ * (0: never) the first advection stage is also summed exactly; if the
+ * largest difference exceeds tol times the largest velocity, the split is
+ * switched off for the rest of the run. */
 int uvlm_sim_set_farfield(uvlm_ctx* ctx, double theta, int audit_every, double tol);
 /* Relative difference found by the last audit (-1 before the first), audits
  * run, and whether the split is active. */

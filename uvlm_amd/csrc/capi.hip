@@ -270,6 +270,12 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
   return check_launch(c);
 }
 
+// The wing's few targets (collocation points for the RHS, edge midpoints
+// for the loads) go through the split with a tighter criterion than the
+// wake's own nodes: their velocities set the circulations and the loads
+// directly, and they cost little.
+constexpr double kWingThetaFactor = 2.0;
+
 // The near/far split applies: set, not switched off by an audit, a wake.
 bool ff_active(const uvlm_ctx* c) {
   return c->ff && c->ff_theta > 0.0 && !c->ff_off && c->wgrows > 0;
@@ -278,6 +284,7 @@ bool ff_active(const uvlm_ctx* c) {
 // Far-field sources from the current wake (positions gathered into pos,
 // circulations into ffg) unless already built for this wake state.
 int ff_current(uvlm_ctx* c) {
+  c->ff->set_f32c(prec_of(c->flags) == kF32C);
   if (c->ff_version == c->wake_version) return UVLM_OK;
   const int C1 = c->C + 1, rows = c->wrows, V = c->num * rows * C1;
   TRY(ensure(c, c->pos, sizeof(double) * 3 * (size_t)V));
@@ -811,7 +818,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   if (ff_active(c)) {
     TRY(ff_current(c));
     std::string e;
-    const int rc = c->ff->velocity(c->stream, P<double>(c->cp), N, c->ff_theta,
+    const int rc = c->ff->velocity(c->stream, P<double>(c->cp), N, kWingThetaFactor * c->ff_theta,
                                    P<double>(c->uw), &e);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
   } else {
@@ -896,8 +903,8 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
     TRY(sum_sources(c, P<double>(c->mid), E, true, false, &plan));
     TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
     std::string e;
-    const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E, c->ff_theta,
-                                   P<double>(c->ffm), &e);
+    const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E,
+                                   kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     launch_ff_combine(c->stream, P<double>(c->partial), E, plan.cb, P<double>(c->ffm), E, ff, 0,
                       0, P<double>(c->uflow));
@@ -969,6 +976,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     if (targets == P<double>(c->pos)) {  // the current wake: built for the RHS already
       TRY(ff_current(c));
     } else {  // an intermediate stage's positions
+      c->ff->set_f32c(prec_of(c->flags) == kF32C);
       rc = c->ff->build(c->stream, targets, P<double>(c->ffg), c->num, c->wgrows, c->C, &e);
       if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
       c->ff_version = -1;
@@ -1215,6 +1223,7 @@ int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const 
                              c->stream));
   std::string e;
   c->ff_version = -1;  // the simulation's sources are replaced
+  c->ff->set_f32c(false);
   const int rc = c->ff->wake_velocity(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)num,
                                       (int)rows, (int)cols, P<double>(c->t_a), (int)nt, theta,
                                       P<double>(c->t_out), &e);

@@ -30,32 +30,6 @@ constexpr int kTile = 128;  // chain elements per LDS tile (8 KB)
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-// BiotSavartLaw of segment a->b at x, times g (vortex.cpp:21-39 formula,
-// FP64, used for the near-core pairs of F32C).
-__device__ __forceinline__ void seg_ref(double ax, double ay, double az, double bx,
-                                        double by, double bz, double x, double y,
-                                        double z, double g, double& ux, double& uy,
-                                        double& uz) {
-  const double r0x = bx - ax, r0y = by - ay, r0z = bz - az;
-  const double r1x = x - ax, r1y = y - ay, r1z = z - az;
-  const double r2x = x - bx, r2y = y - by, r2z = z - bz;
-  const double dx = r1y * r2z - r1z * r2y;
-  const double dy = r1z * r2x - r1x * r2z;
-  const double dz = r1x * r2y - r1y * r2x;
-  const double dsq = (dx * dx + dy * dy) + dz * dz;
-  const double r1n = sqrt((r1x * r1x + r1y * r1y) + r1z * r1z);
-  const double r2n = sqrt((r2x * r2x + r2y * r2y) + r2z * r2z);
-  const bool cut = (sqrt(dsq) * sqrt(dsq) < kCutoff) | (r1n < kCutoff) | (r2n < kCutoff);
-  const double K = ((r0x * (r1x / r1n - r2x / r2n) + r0y * (r1y / r1n - r2y / r2n)) +
-                    r0z * (r1z / r1n - r2z / r2n)) /
-                   dsq;
-  if (!cut) {
-    ux += dx * K * g;
-    uy += dy * K * g;
-    uz += dz * K * g;
-  }
-}
-
 __device__ __forceinline__ void chunk_range(const ChunkPlan& p, int c, int& b, int& e) {
   if (c < p.cb) {
     b = c * p.szb;
@@ -173,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         fz = fmaf(dz, K, fz);
         if (__any(near)) {
           if (near && e[3] != 0.0)
-            seg_ref(pax, pay, paz, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
+            dev::seg_ref(pax, pay, paz, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
         }
         pax = e[0];
         pay = e[1];

@@ -95,6 +95,9 @@ class FarField {
   // enable = false switches the correction off.
   int set_mlp(hipStream_t s, const float* w, bool enable, float ymax, std::string* err);
   bool mlp_enabled() const { return mlp_on_; }
+  // Near blocks in FP32 with FP64 compensation and near-core re-evaluation
+  // (the F32C precision) instead of FP64.
+  void set_f32c(bool on) { f32c_ = on; }
   // Far pairs corrected by the network and pairs tripped to the exact sum,
   // since the network was set.
   int mlp_stats(hipStream_t s, unsigned long long* pairs, unsigned long long* trips,
@@ -112,7 +115,7 @@ class FarField {
   int sort_pairs(hipStream_t s, int n, std::string* err);
 
   FFTree tree_;
-  bool built_ = false, mlp_on_ = false;
+  bool built_ = false, mlp_on_ = false, f32c_ = false;
   float ymax_ = 0.0f;
   // device buffers
   void *d_rec_ = nullptr, *d_key_ = nullptr, *d_key2_ = nullptr,

@@ -513,6 +513,71 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
   }
 }
 
+// The same walk in FP32 with compensation: x - p in FP64 (exact to FP32
+// rounding relative to |x - p|), the segment law in FP32 with the hardware
+// rsqrt / reciprocal, each chain (<= 17 elements) summed in FP32 and added
+// to the FP64 sums; pairs within a band around the reference's cut-off
+// (|r| < 1e-9, |d|^2 < 2e-9 plus the FP32 rounding bound of d) are
+// re-evaluated in FP64 exactly as the reference does.
+constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
+__device__ __forceinline__ void chain32(const double* R, int n0, int stride_nodes, int count,
+                                        const double* S, int s0, double x, double y, double z,
+                                        double& ux, double& uy, double& uz) {
+  const double* p = R + 3 * n0;
+  double pax = p[0], pay = p[1], paz = p[2];
+  float rax = (float)(x - pax), ray = (float)(y - pay), raz = (float)(z - paz);
+  float ras = fmaf(raz, raz, fmaf(ray, ray, rax * rax));
+  float ia = __builtin_amdgcn_rsqf(ras);
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int k = 1; k <= count; ++k) {
+    p += 3 * stride_nodes;
+    const double bx = p[0], by = p[1], bz = p[2];
+    const double g = S[s0 + k - 1];
+    const float rbx = (float)(x - bx), rby = (float)(y - by), rbz = (float)(z - bz);
+    const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
+    const float ib = __builtin_amdgcn_rsqf(rbs);
+    const float r0x = (float)(bx - pax), r0y = (float)(by - pay), r0z = (float)(bz - paz);
+    const float dx = fmaf(r0y, raz, -r0z * ray);
+    const float dy = fmaf(r0z, rax, -r0x * raz);
+    const float dz = fmaf(r0x, ray, -r0y * rax);
+    const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
+    const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
+    const float tt = fmaf(oa, ia, -ob * ib);
+    const float r0s = fmaf(r0z, r0z, fmaf(r0y, r0y, r0x * r0x));
+    const float derr2 = 64.f * kEps32 * kEps32 * r0s * ras;
+    const bool nearcore = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
+    float K = tt * (float)g * __builtin_amdgcn_rcpf(dsq);
+    K = nearcore ? 0.f : K;
+    fx = fmaf(dx, K, fx);
+    fy = fmaf(dy, K, fy);
+    fz = fmaf(dz, K, fz);
+    if (nearcore && g != 0.0) dev::seg_ref(pax, pay, paz, bx, by, bz, x, y, z, g, ux, uy, uz);
+    pax = bx;
+    pay = by;
+    paz = bz;
+    rax = rbx;
+    ray = rby;
+    raz = rbz;
+    ras = rbs;
+    ia = ib;
+  }
+  ux += (double)fx;
+  uy += (double)fy;
+  uz += (double)fz;
+}
+
+__device__ __forceinline__ void walk_record32(const double* R, int nr, int nc, double px,
+                                              double py, double pz, double& ux, double& uy,
+                                              double& uz) {
+  const double* S = R + kRecStr;
+  for (int i = 0; i <= nr; ++i)  // node rows: spanwise segments
+    chain32(R, i * (nc + 1), 1, nc, S, i * nc, px, py, pz, ux, uy, uz);
+  const int s1 = (nr + 1) * nc;
+  for (int j = 0; j <= nc; ++j)  // node columns: chordwise segments
+    chain32(R, j, nc + 1, nr, S, s1 + j * nr, px, py, pz, ux, uy, uz);
+}
+
 // Exact velocity (with 1/4pi) at one target of the level-0 blocks under
 // cluster c, blocks in row order.
 __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double* __restrict__ rec,
@@ -533,7 +598,8 @@ __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double*
 
 // Group g of a tile sums the tile's near blocks g, g + kGroups, ...: each
 // block's record is staged in LDS (the next one is loaded into registers
-// meanwhile) and walked for every lane's target.
+// meanwhile) and walked for every lane's target (FP64, or FP32 compensated).
+template <bool F32>
 __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl,
                                                 const double* __restrict__ rec,
                                                 const double* __restrict__ x, int nt,
@@ -571,7 +637,10 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
       id = list[k + kGroups];
       load(id);
     }
-    walk_record(sr, nr, nc, px, py, pz, ux, uy, uz);
+    if (F32)
+      walk_record32(sr, nr, nc, px, py, pz, ux, uy, uz);
+    else
+      walk_record(sr, nr, nc, px, py, pz, ux, uy, uz);
   }
   if (tg >= 0) {
     part[((long)g * 3 + 0) * nt + tg] = ux;
@@ -1026,9 +1095,16 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
                      (const double4*)d_tiles_, theta, (int*)d_near_, (int*)d_far_,
                      (int2*)d_cnt_, (int*)d_front_);
-  hipLaunchKernelGGL(k_ff_near, dim3(ntiles, kGroups), dim3(64), 0, s, (const FFCluster*)d_cl_,
-                     (const double*)d_rec_, targets, nt, (const int*)map, n0, (const int*)d_near_,
-                     (const int2*)d_cnt_, (double*)d_part_);
+  if (f32c_)
+    hipLaunchKernelGGL(k_ff_near<true>, dim3(ntiles, kGroups), dim3(64), 0, s,
+                       (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
+                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_,
+                       (double*)d_part_);
+  else
+    hipLaunchKernelGGL(k_ff_near<false>, dim3(ntiles, kGroups), dim3(64), 0, s,
+                       (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
+                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_,
+                       (double*)d_part_);
   if (mlp_on_) {
     hipLaunchKernelGGL(k_ff_far<true>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
                        (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
