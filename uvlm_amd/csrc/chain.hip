@@ -63,7 +63,8 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     cr.ry = y - p[1];
     cr.rz = z - p[2];
     cr.rs = __builtin_fma(cr.rz, cr.rz, __builtin_fma(cr.ry, cr.ry, cr.rx * cr.rx));
-    cr.inv = dev::rsq64(cr.rs);
+    cr.inv = dev::rsq64_1(cr.rs);
+    cr.small = cr.rs < kCutoff * kCutoff;
   }
   double ux = 0.0, uy = 0.0, uz = 0.0;
   const double2* src = reinterpret_cast<const double2*>(recs);
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     for (int q = threadIdx.x; q < cnt * 4; q += kBlock) tile[q] = src[(size_t)base * 4 + q];
     __syncthreads();
     const double* tl = reinterpret_cast<const double*>(tile);
+#pragma unroll 2
     for (int k = 0; k < cnt; ++k) dev::chain_step64(tl + 8 * k, x, y, z, cr, ux, uy, uz);
   }
   if (t < nt) {

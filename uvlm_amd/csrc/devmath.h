@@ -25,36 +25,53 @@ __device__ __forceinline__ double rcp64(double a) {
   return __builtin_fma(r, e, r);
 }
 
+// One Newton step each from the hardware seed: relative error below 2^-47
+// (rsqrt, ~20 ulp) and 2^-49 (reciprocal), for the fast chains whose pair
+// sums carry rounding of that order anyway.
+__device__ __forceinline__ double rsq64_1(double a) {
+  const double y = __builtin_amdgcn_rsq(a);
+  const double e = __builtin_fma(-(a * y), y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+__device__ __forceinline__ double rcp64_1(double a) {
+  const double r = __builtin_amdgcn_rcp(a);
+  return __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
+}
+
 // Carried state of a chain walk for one target: x - p of the previous node,
-// its squared norm and 1/|x - p|.
+// its squared norm, 1/|x - p| and whether |x - p| is below the cut-off.
 struct Carry64 {
   double rx, ry, rz, rs, inv;
+  bool small;
 };
 
 __device__ __forceinline__ void carry_reset(Carry64& c) {
   c.rx = c.ry = c.rz = c.rs = c.inv = 0.0;
+  c.small = true;
 }
 
 // One chain element e (ChainRec as 8 doubles: p[3], g, r0[3], r0sq) at
 // target (x, y, z): adds g * BiotSavartLaw(prev -> p) to u and advances the
-// carry to p.  d = r1 x r2 = r0 x (x - prev); the cut-off set is the
-// reference's (|d|^2, |r1|, |r2| against 1e-10) up to rounding.
+// carry to p.  d = r1 x r2 = r0 x (x - prev), r0 . r2 = r0 . r1 - |r0|^2;
+// the cut-off set is the reference's (|d|^2, |r1|, |r2| against 1e-10) up
+// to rounding.
 __device__ __forceinline__ void chain_step64(const double* e, double x, double y, double z,
                                              Carry64& c, double& ux, double& uy,
                                              double& uz) {
   const double rbx = x - e[0], rby = y - e[1], rbz = z - e[2];
   const double rbs = __builtin_fma(rbz, rbz, __builtin_fma(rby, rby, rbx * rbx));
-  const double ib = rsq64(rbs);
+  const double ib = rsq64_1(rbs);
   const double r0x = e[4], r0y = e[5], r0z = e[6];
   const double dx = __builtin_fma(r0y, c.rz, -r0z * c.ry);
   const double dy = __builtin_fma(r0z, c.rx, -r0x * c.rz);
   const double dz = __builtin_fma(r0x, c.ry, -r0y * c.rx);
   const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
   const double oa = __builtin_fma(r0z, c.rz, __builtin_fma(r0y, c.ry, r0x * c.rx));
-  const double ob = __builtin_fma(r0z, rbz, __builtin_fma(r0y, rby, r0x * rbx));
+  const double ob = oa - e[7];
   const double tt = __builtin_fma(oa, c.inv, -ob * ib);
-  const bool cut = (dsq < kCutoff) | (c.rs < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
-  double K = tt * e[3] * rcp64(dsq);
+  const bool small = rbs < kCutoff * kCutoff;
+  const bool cut = (dsq < kCutoff) | c.small | small;
+  double K = tt * e[3] * rcp64_1(dsq);
   K = cut ? 0.0 : K;
   ux = __builtin_fma(dx, K, ux);
   uy = __builtin_fma(dy, K, uy);
@@ -64,6 +81,7 @@ __device__ __forceinline__ void chain_step64(const double* e, double x, double y
   c.rz = rbz;
   c.rs = rbs;
   c.inv = ib;
+  c.small = small;
 }
 
 // BiotSavartLaw of segment a->b at x, times g (vortex.cpp:21-39 formula,

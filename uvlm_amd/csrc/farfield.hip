@@ -475,7 +475,7 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
     e[0] = p[0];
     e[1] = p[1];
     e[2] = p[2];
-    e[3] = e[4] = e[5] = e[6] = 0.0;
+    e[3] = e[4] = e[5] = e[6] = e[7] = 0.0;
     dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
     for (int j = 1; j <= nc; ++j) {
       p += 3;
@@ -486,6 +486,7 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
       e[4] = p[0] - p[-3];
       e[5] = p[1] - p[-2];
       e[6] = p[2] - p[-1];
+      e[7] = __builtin_fma(e[6], e[6], __builtin_fma(e[5], e[5], e[4] * e[4]));
       dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
     }
   }
@@ -497,7 +498,7 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
     e[0] = p[0];
     e[1] = p[1];
     e[2] = p[2];
-    e[3] = e[4] = e[5] = e[6] = 0.0;
+    e[3] = e[4] = e[5] = e[6] = e[7] = 0.0;
     dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
     for (int i = 1; i <= nr; ++i) {
       p += st;
@@ -508,6 +509,7 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
       e[4] = p[0] - p[-st];
       e[5] = p[1] - p[1 - st];
       e[6] = p[2] - p[2 - st];
+      e[7] = __builtin_fma(e[6], e[6], __builtin_fma(e[5], e[5], e[4] * e[4]));
       dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
     }
   }
