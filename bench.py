@@ -7,7 +7,11 @@ SimpleSimulator on the gfx950 kernels.  Default workload: NACA4412-cambered
 wing, 20x80 rings, root-hinged flapping +/-30 deg at k = 0.3 with 5 deg
 pitch, reference "RK4" wake advection, 1000 steps (the wake grows to 80,000
 rings).  `--warmup W` untimed steps (from step 0), then `--steps K` timed
-steps; the defaults time steps 1..1000 of the 1001-iteration run.
+steps; the defaults time steps 1..1000 of the 1001-iteration run.  The
+default method is FP64 throughout: segment-chain N-body in FP64 and the
+kept-inverse FP64 solve (iterative refinement from the previous step's
+circulations to max|dx| <= 1e-14 max|x|); `--solve f64` selects the blocked
+LU instead.
 
 Prints one JSON line.  `value` is Biot-Savart pairs per second in the
 reference's counting: the pairs the reference loop evaluates for the timed
@@ -169,6 +173,8 @@ def run_single(args):
     sim = Simulation(case, total - 1)
     if args.farfield > 0:
         sim.set_farfield(args.farfield, args.audit, args.ff_tol)
+        if args.ff_reach > 0:
+            sim.set_farfield_focus(args.ff_reach, args.ff_theta_min)
     for s in range(args.warmup):
         sim.step(s)
     sim.sync()
@@ -197,7 +203,8 @@ def run_single(args):
     loads = sim.loads()
     achieved = FLOP_PER_PAIR * kpairs / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     traffic = traffic_from_profile() if (args.workload == "c2" and args.precision == "f64"
-                                         and args.steps == 1000 and args.warmup == 1) else None
+                                         and args.solve == "inv" and args.steps == 1000
+                                         and args.warmup == 1) else None
     nw = len(case["wings"])
     out = {
         "metric": METRIC,
@@ -233,6 +240,8 @@ def run_single(args):
         err, audits, active = sim.farfield_stats()
         out["farfield"] = {"theta": args.farfield, "audit_every": args.audit,
                            "last_audit_rel_err": err, "audits": audits, "active": active}
+        if args.ff_reach > 0:
+            out["farfield"]["graded"] = {"reach": args.ff_reach, "theta_min": args.ff_theta_min}
         if w is not None:
             pairs, trips = sim.farfield_mlp_stats()
             out["farfield"]["network"] = {"trained": learned, "every": args.mlp_every,
@@ -299,7 +308,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed steps before")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--precision", default="f64", choices=sorted(PRECISIONS))
-    ap.add_argument("--solve", default="f64", choices=sorted(SOLVES))
+    ap.add_argument("--solve", default="inv", choices=sorted(SOLVES))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -307,6 +316,10 @@ def main():
                     help="near/far split of the wake sum with this theta (0: exact sums)")
     ap.add_argument("--audit", type=int, default=50, help="far field: exact audit every N steps")
     ap.add_argument("--ff-tol", type=float, default=1e-3, help="far field: audit tolerance")
+    ap.add_argument("--ff-reach", type=float, default=0.0,
+                    help="far field: graded criterion beyond this many wing radii (0: off)")
+    ap.add_argument("--ff-theta-min", type=float, default=2.0,
+                    help="far field: smallest theta of the graded criterion")
     ap.add_argument("--mlp-every", type=int, default=0,
                     help="far field: train the correction network every K steps (0: off)")
     ap.add_argument("--mlp-start", type=int, default=20, help="first training step")

@@ -207,6 +207,13 @@ int uvlm_farfield_velocity(uvlm_ctx* ctx, const double* targets, size_t n_target
  * largest difference exceeds tol times the largest velocity, the split is
  * switched off for the rest of the run. */
 int uvlm_sim_set_farfield(uvlm_ctx* ctx, double theta, int audit_every, double tol);
+/* Graded criterion for the advection stages (reach <= 0 switches it off):
+ * a tile of wake nodes whose gap g to the bounding sphere of the wings
+ * (radius rw) exceeds reach * rw uses theta * reach * rw / g, not below
+ * theta_min (> 1).  Velocity errors far downstream move wake whose
+ * influence on the wings falls like 1/g^3; the audit weighs the difference
+ * at each node by min(1, reach rw / g)^3 accordingly. */
+int uvlm_sim_set_farfield_focus(uvlm_ctx* ctx, double reach, double theta_min);
 /* Relative difference found by the last audit (-1 before the first), audits
  * run, and whether the split is active. */
 int uvlm_sim_farfield_stats(uvlm_ctx* ctx, double* last_audit_err, int* audits, int* active);

@@ -141,3 +141,34 @@ def test_simulation_learns_the_correction(orc):
     la, lb = a.loads(), b.loads()
     scale = np.abs(la).max(axis=0)
     assert np.all(np.abs(la - lb).max(axis=0) / scale < 1e-3)
+
+
+def test_graded_criterion_away_from_the_wings(orc):
+    """The graded criterion (theta falls towards theta_min beyond reach wing
+    radii): in effect on the wake away from the wings, loads within the
+    split's tolerance of the exact run, the weighted audits pass; a reach
+    beyond the whole wake gives the plain split bitwise; theta_min <= 1 is
+    refused."""
+    steps = 80
+    case = W.tandem_c5(steps=steps, rows=4, cols=16)
+    exact = Simulation(case, steps)
+    plain, graded, wide = (Simulation(case, steps) for _ in range(3))
+    for sim in (plain, graded, wide):
+        sim.set_farfield(theta=4.0, audit_every=20, tol=1e-3)
+    graded.set_farfield_focus(reach=0.5, theta_min=1.5)
+    wide.set_farfield_focus(reach=1e6, theta_min=1.5)
+    with pytest.raises(Exception):
+        graded.set_farfield_focus(reach=1.0, theta_min=1.0)
+    sims = (exact, plain, graded, wide)
+    for s in range(steps + 1):
+        for sim in sims:
+            sim.step(s)
+    for sim in sims:
+        sim.sync()
+    le, lp, lg, lw = (sim.loads() for sim in sims)
+    np.testing.assert_array_equal(lw, lp)
+    assert np.abs(lg - lp).max() > 0.0
+    scale = np.abs(le).max(axis=0)
+    assert np.all(np.abs(lg - le).max(axis=0) / scale < 1e-3)
+    err, audits, active = graded.farfield_stats()
+    assert audits == 4 and active and 0 <= err < 1e-3

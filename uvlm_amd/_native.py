@@ -37,7 +37,8 @@ GPU_SYMBOLS = [
     "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
-    "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_farfield_stats",
+    "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_set_farfield_focus",
+    "uvlm_sim_farfield_stats",
     "uvlm_sim_inverse_stats",
     "uvlm_mlp_forward", "uvlm_mlp_grad", "uvlm_mlp_train", "uvlm_farfield_set_mlp",
     "uvlm_farfield_mlp_stats", "uvlm_farfield_samples", "uvlm_sim_farfield_samples",
@@ -145,6 +146,7 @@ def gpu_lib():
         L.uvlm_farfield_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
                                              _dp]
         L.uvlm_sim_set_farfield.argtypes = [_vp, C.c_double, C.c_int, C.c_double]
+        L.uvlm_sim_set_farfield_focus.argtypes = [_vp, C.c_double, C.c_double]
         L.uvlm_sim_farfield_stats.argtypes = [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _ll = C.POINTER(C.c_longlong)
         L.uvlm_sim_inverse_stats.argtypes = [_vp, _ll, _ll, _ll, C.POINTER(C.c_int)]

@@ -70,12 +70,13 @@ struct uvlm_ctx {
   // far-field split of the wake sum in the advection stages
   uvlm::FarField* ff = nullptr;
   double ff_theta = 0.0, ff_tol = 0.0, ff_last_err = -1.0;
+  double ff_reach = 0.0, ff_theta_min = 0.0;  // graded criterion (reach 0: off)
   int ff_audit_every = 0, ff_counter = 0, ff_audits = 0;
   bool ff_off = false;
   // the split's sources are rebuilt only when the wake changed
   long wake_version = 0, ff_version = -1;
   Buf ffm;
-  Buf ffg, ffv, ffx, ffstat;
+  Buf ffg, ffv, ffx, ffstat, ffbox;
   // kept-inverse solver (UVLM_SOLVE_F64_INV); one for the simulation, one
   // for the stateless uvlm_solve
   uvlm::InvSolver* inv = nullptr;
@@ -349,7 +350,7 @@ int uvlm_destroy(uvlm_ctx* c) {
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
                  &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat,
-                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev, &c->ffm};
+                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev, &c->ffm, &c->ffbox};
   for (Buf* b : bufs) release(*b);
   free_staging(c->st_wing);
   free_staging(c->st_vm);
@@ -968,6 +969,15 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
                          c->C, P<double>(c->ffg));
   }
+  FFFocus focus;
+  if (use_ff && c->ff_reach > 0.0) {  // bounding box of the wings' nodes
+    TRY(ensure(c, c->ffbox, 6 * sizeof(double)));
+    launch_ff_bbox(c->stream, P<double>(c->wing), c->num * (c->R + 1) * (c->C + 1),
+                   P<double>(c->ffbox));
+    focus.box = P<double>(c->ffbox);
+    focus.reach = c->ff_reach;
+    focus.theta_min = c->ff_theta_min;
+  }
   auto ff_stage = [&](const double* targets, double* out) -> int {
     ChunkPlan plan;
     TRY(sum_sources(c, targets, V, true, false, &plan));
@@ -981,7 +991,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
       if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
       c->ff_version = -1;
     }
-    rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e);
+    rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e, &focus);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     launch_ff_combine(c->stream, P<double>(c->partial), V, plan.cb, P<double>(c->ffv), V, ff,
                       te_period, C1, out);
@@ -995,7 +1005,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
       first = false;
       TRY(exact_stage(targets, P<double>(c->ffx)));
       launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), V,
-                        P<unsigned long long>(c->ffstat));
+                        P<unsigned long long>(c->ffstat), targets, &focus);
       unsigned long long h[2] = {0, 0};
       HIPCHK(c, hipMemcpyAsync(h, c->ffstat.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1406,6 +1416,15 @@ int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol
   c->ff_tol = tol > 0.0 ? tol : 1e-3;
   c->ff_counter = 0;
   c->ff_off = false;
+  return UVLM_OK;
+}
+
+int uvlm_sim_set_farfield_focus(uvlm_ctx* c, double reach, double theta_min) {
+  if (!c) return UVLM_ERR_ARG;
+  if (reach > 0.0 && !(theta_min > 1.0))
+    return fail(c, UVLM_ERR_ARG, "theta_min must exceed 1");
+  c->ff_reach = reach > 0.0 ? reach : 0.0;
+  c->ff_theta_min = theta_min;
   return UVLM_OK;
 }
 

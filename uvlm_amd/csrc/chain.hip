@@ -64,18 +64,27 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     cr.rz = z - p[2];
     cr.rs = __builtin_fma(cr.rz, cr.rz, __builtin_fma(cr.ry, cr.ry, cr.rx * cr.rx));
     cr.inv = dev::rsq64_1(cr.rs);
-    cr.small = cr.rs < kCutoff * kCutoff;
   }
   double ux = 0.0, uy = 0.0, uz = 0.0;
   const double2* src = reinterpret_cast<const double2*>(recs);
   for (int base = rb; base < re; base += kTile) {
     const int cnt = min(kTile, re - base);
     __syncthreads();
-    for (int q = threadIdx.x; q < cnt * 4; q += kBlock) tile[q] = src[(size_t)base * 4 + q];
-    __syncthreads();
+    int lng = 0;  // a segment of the tile too long for the short form
+    for (int q = threadIdx.x; q < cnt * 4; q += kBlock) {
+      const double2 v = src[(size_t)base * 4 + q];
+      tile[q] = v;
+      lng |= (q & 3) == 3 && !(v.y < dev::kShortSeg2);
+    }
+    const bool any_long = __syncthreads_or(lng);
     const double* tl = reinterpret_cast<const double*>(tile);
+    if (any_long) {
+#pragma unroll 1
+      for (int k = 0; k < cnt; ++k) dev::chain_step64<true>(tl + 8 * k, x, y, z, cr, ux, uy, uz);
+    } else {
 #pragma unroll 2
-    for (int k = 0; k < cnt; ++k) dev::chain_step64(tl + 8 * k, x, y, z, cr, ux, uy, uz);
+      for (int k = 0; k < cnt; ++k) dev::chain_step64<false>(tl + 8 * k, x, y, z, cr, ux, uy, uz);
+    }
   }
   if (t < nt) {
     const size_t c = blockIdx.y;

@@ -39,22 +39,28 @@ __device__ __forceinline__ double rcp64_1(double a) {
 }
 
 // Carried state of a chain walk for one target: x - p of the previous node,
-// its squared norm, 1/|x - p| and whether |x - p| is below the cut-off.
+// its squared norm and 1/|x - p|.
 struct Carry64 {
   double rx, ry, rz, rs, inv;
-  bool small;
 };
 
 __device__ __forceinline__ void carry_reset(Carry64& c) {
   c.rx = c.ry = c.rz = c.rs = c.inv = 0.0;
-  c.small = true;
 }
+
+// Squared segment length below which the end-point cut-offs are implied by
+// the |d|^2 cut-off: |r1| < 1e-10 gives |d| <= |r1| |r2| < 1e-10 (|r0| +
+// 1e-10), far below sqrt(1e-10) while |r0| < 3e4 (likewise for r2).
+constexpr double kShortSeg2 = 1e9;
 
 // One chain element e (ChainRec as 8 doubles: p[3], g, r0[3], r0sq) at
 // target (x, y, z): adds g * BiotSavartLaw(prev -> p) to u and advances the
 // carry to p.  d = r1 x r2 = r0 x (x - prev), r0 . r2 = r0 . r1 - |r0|^2;
 // the cut-off set is the reference's (|d|^2, |r1|, |r2| against 1e-10) up
-// to rounding.
+// to rounding.  Ends = false drops the end-point tests, exact when the
+// segment is shorter than sqrt(kShortSeg2) (a NaN from a target on a node is
+// then removed by the |d|^2 test, since d = 0 there).
+template <bool Ends = true>
 __device__ __forceinline__ void chain_step64(const double* e, double x, double y, double z,
                                              Carry64& c, double& ux, double& uy,
                                              double& uz) {
@@ -69,8 +75,8 @@ __device__ __forceinline__ void chain_step64(const double* e, double x, double y
   const double oa = __builtin_fma(r0z, c.rz, __builtin_fma(r0y, c.ry, r0x * c.rx));
   const double ob = oa - e[7];
   const double tt = __builtin_fma(oa, c.inv, -ob * ib);
-  const bool small = rbs < kCutoff * kCutoff;
-  const bool cut = (dsq < kCutoff) | c.small | small;
+  bool cut = dsq < kCutoff;
+  if (Ends) cut = cut | (c.rs < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
   double K = tt * e[3] * rcp64_1(dsq);
   K = cut ? 0.0 : K;
   ux = __builtin_fma(dx, K, ux);
@@ -81,7 +87,6 @@ __device__ __forceinline__ void chain_step64(const double* e, double x, double y
   c.rz = rbz;
   c.rs = rbs;
   c.inv = ib;
-  c.small = small;
 }
 
 // BiotSavartLaw of segment a->b at x, times g (vortex.cpp:21-39 formula,

@@ -57,12 +57,29 @@ struct MlpShape {
   static constexpr int total = off_b4 + 4;
 };
 
+// Graded criterion for targets away from the bodies.  The focus is the
+// bounding sphere of the bodies (centre fc, radius fr, from the device box
+// min[3], max[3]); a target tile whose gap g to it (distance of the spheres'
+// surfaces) exceeds L = reach fr uses theta L / g instead of theta, but not
+// below theta_min.  A velocity error at distance g moves wake that acts on
+// the bodies like a dipole, ~1/g^3, so accuracy there is spent where it
+// matters.  reach <= 0: the plain criterion.
+struct FFFocus {
+  const double* box = nullptr;
+  double reach = 0.0, theta_min = 0.0;
+};
+
 // out = (-ff + bound chunks in order) + wake (AoS), trailing-edge rows -ff.
 void launch_ff_combine(hipStream_t s, const double* bound, int ld, int cb, const double* wake,
                        int nt, const double* ff, int te_period, int te_width, double* out);
 // res[0] <- bits of max |a - b|, res[1] <- bits of max |b| over n 3-vectors.
+// With a focus (reach > 0), |a - b| at point x[i] is weighted by
+// min(1, L / g)^3 (g, L as in FFFocus: the error the graded criterion allows).
 void launch_ff_maxdiff(hipStream_t s, const double* a, const double* b, int n,
-                       unsigned long long* res);
+                       unsigned long long* res, const double* x = nullptr,
+                       const FFFocus* focus = nullptr);
+// Bounding box min[3], max[3] of n points x[n][3] (device).
+void launch_ff_bbox(hipStream_t s, const double* x, int n, double* box);
 // Rows [row0, row0+rows) of a ring buffer [num][cap][w] -> compact [num][rows][w].
 void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, int num, int rows,
                           int w, double* out);
@@ -81,8 +98,9 @@ class FarField {
   // tile exceeds theta times its radius (beyond the tile's radius) uses its
   // moment expansion (plus the network's correction when enabled), the
   // level-0 blocks of the others are summed exactly.  out [nt][3] overwritten.
+  // focus (optional): the graded criterion of FFFocus.
   int velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
-               std::string* err);
+               std::string* err, const FFFocus* focus = nullptr);
   // build + velocity.
   int wake_velocity(hipStream_t s, const double* nodes, const double* gamma, int num, int rows,
                     int cols, const double* targets, int nt, double theta, double* out,

@@ -392,12 +392,27 @@ __global__ __launch_bounds__(64) void k_ff_tiles(const double* __restrict__ x,
 // appended in frontier order (ballot prefix counts), so they are
 // deterministic.  Capacities: near all level-0 blocks, far all blocks, each
 // frontier the level-0 count (front: 2 of those per tile).
+// Focus sphere (centre, radius) of a bounding box min[3], max[3].
+__device__ inline double4 focus_sphere(const double* box) {
+  const double hx = 0.5 * (box[3] - box[0]), hy = 0.5 * (box[4] - box[1]),
+               hz = 0.5 * (box[5] - box[2]);
+  return double4{box[0] + hx, box[1] + hy, box[2] + hz, sqrt(hx * hx + hy * hy + hz * hz)};
+}
+
 __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __restrict__ cl,
                                                  const double4* __restrict__ tiles, double theta,
-                                                 int* __restrict__ near, int* __restrict__ far,
-                                                 int2* __restrict__ cnt, int* __restrict__ front) {
+                                                 const double* __restrict__ fbox, double reach,
+                                                 double theta_min, int* __restrict__ near,
+                                                 int* __restrict__ far, int2* __restrict__ cnt,
+                                                 int* __restrict__ front) {
   const int tile = blockIdx.x, lane = threadIdx.x;
   const double4 tc = tiles[tile];
+  if (fbox) {  // graded criterion away from the focus
+    const double4 f = focus_sphere(fbox);
+    const double fx = tc.x - f.x, fy = tc.y - f.y, fz = tc.z - f.z;
+    const double gap = sqrt(fx * fx + fy * fy + fz * fz) - tc.w - f.w, L = reach * f.w;
+    if (gap > L) theta = fmax(theta_min, theta * (L / gap));
+  }
   const int n0 = t.num * t.nbr[0] * t.nbc[0];
   int* my_near = near + (long)tile * n0;
   int* my_far = far + (long)tile * t.total;
@@ -460,60 +475,53 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
 }
 
 // --- exact sums of blocks ---------------------------------------------------------
-// Chains of a level-0 block's record (nr x nc rings) at one target: node
-// rows (spanwise segments), then node columns (chordwise), 1/|x - p|
-// carried along each chain (devmath.h); adds to u (with 1/4pi).
+// One chain of a level-0 block's record at one target: count segments from
+// node n0 in steps of stride nodes, strengths S[s0 ...]; 1/|x - p| carried
+// along the chain (devmath.h); adds to u (strengths carry 1/4pi).  Ends as
+// in dev::chain_step64.
+template <bool Ends>
+__device__ __forceinline__ void chain64(const double* R, int n0, int stride, int count,
+                                        const double* S, int s0, double px, double py,
+                                        double pz, double& ux, double& uy, double& uz) {
+  const double* p = R + 3 * n0;
+  dev::Carry64 cr;
+  cr.rx = px - p[0];
+  cr.ry = py - p[1];
+  cr.rz = pz - p[2];
+  cr.rs = __builtin_fma(cr.rz, cr.rz, __builtin_fma(cr.ry, cr.ry, cr.rx * cr.rx));
+  cr.inv = dev::rsq64_1(cr.rs);
+  double e[8];
+  for (int k = 0; k < count; ++k) {
+    const double* q = p + 3 * stride;
+    e[0] = q[0];
+    e[1] = q[1];
+    e[2] = q[2];
+    e[3] = S[s0 + k];
+    e[4] = q[0] - p[0];
+    e[5] = q[1] - p[1];
+    e[6] = q[2] - p[2];
+    e[7] = __builtin_fma(e[6], e[6], __builtin_fma(e[5], e[5], e[4] * e[4]));
+    dev::chain_step64<Ends>(e, px, py, pz, cr, ux, uy, uz);
+    p = q;
+  }
+}
+
+// All chains of a record (nr x nc rings): node rows (spanwise segments),
+// then node columns (chordwise).
+template <bool Ends>
 __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, double px, double py,
                                             double pz, double& ux, double& uy, double& uz) {
   const double* S = R + kRecStr;
-  double e[8];
-  e[7] = 0.0;
-  for (int i = 0; i <= nr; ++i) {
-    dev::Carry64 cr;
-    dev::carry_reset(cr);
-    const double* p = R + 3 * i * (nc + 1);
-    e[0] = p[0];
-    e[1] = p[1];
-    e[2] = p[2];
-    e[3] = e[4] = e[5] = e[6] = e[7] = 0.0;
-    dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
-    for (int j = 1; j <= nc; ++j) {
-      p += 3;
-      e[0] = p[0];
-      e[1] = p[1];
-      e[2] = p[2];
-      e[3] = S[i * nc + j - 1];
-      e[4] = p[0] - p[-3];
-      e[5] = p[1] - p[-2];
-      e[6] = p[2] - p[-1];
-      e[7] = __builtin_fma(e[6], e[6], __builtin_fma(e[5], e[5], e[4] * e[4]));
-      dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
-    }
-  }
-  const int s1 = (nr + 1) * nc, st = 3 * (nc + 1);
-  for (int j = 0; j <= nc; ++j) {
-    dev::Carry64 cr;
-    dev::carry_reset(cr);
-    const double* p = R + 3 * j;
-    e[0] = p[0];
-    e[1] = p[1];
-    e[2] = p[2];
-    e[3] = e[4] = e[5] = e[6] = e[7] = 0.0;
-    dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
-    for (int i = 1; i <= nr; ++i) {
-      p += st;
-      e[0] = p[0];
-      e[1] = p[1];
-      e[2] = p[2];
-      e[3] = S[s1 + j * nr + i - 1];
-      e[4] = p[0] - p[-st];
-      e[5] = p[1] - p[1 - st];
-      e[6] = p[2] - p[2 - st];
-      e[7] = __builtin_fma(e[6], e[6], __builtin_fma(e[5], e[5], e[4] * e[4]));
-      dev::chain_step64(e, px, py, pz, cr, ux, uy, uz);
-    }
-  }
+  for (int i = 0; i <= nr; ++i)
+    chain64<Ends>(R, i * (nc + 1), 1, nc, S, i * nc, px, py, pz, ux, uy, uz);
+  const int s1 = (nr + 1) * nc;
+  for (int j = 0; j <= nc; ++j)
+    chain64<Ends>(R, j, nc + 1, nr, S, s1 + j * nr, px, py, pz, ux, uy, uz);
 }
+
+// Every segment of a block lies within its radius r of the centre, so it is
+// shorter than 2 r: below this radius the end-point cut-offs are implied.
+constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 
 // The same walk in FP32 with compensation: x - p in FP64 (exact to FP32
 // rounding relative to |x - p|), the segment law in FP32 with the hardware
@@ -591,7 +599,8 @@ __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double*
     for (int bj = bj0; bj < bj1; ++bj) {
       const int nr = min(t.rows, (bi + 1) * kFFRows) - bi * kFFRows;
       const int nc = min(t.cols, (bj + 1) * kFFCols) - bj * kFFCols;
-      walk_record(rec + (long)ff_id(t, 0, c.n, bi, bj) * kRec, nr, nc, px, py, pz, ux, uy, uz);
+      walk_record<true>(rec + (long)ff_id(t, 0, c.n, bi, bj) * kRec, nr, nc, px, py, pz, ux, uy,
+                        uz);
     }
   u[0] = ux;
   u[1] = uy;
@@ -641,8 +650,10 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
     }
     if (F32)
       walk_record32(sr, nr, nc, px, py, pz, ux, uy, uz);
+    else if (c.r < kShortRadius)
+      walk_record<false>(sr, nr, nc, px, py, pz, ux, uy, uz);
     else
-      walk_record(sr, nr, nc, px, py, pz, ux, uy, uz);
+      walk_record<true>(sr, nr, nc, px, py, pz, ux, uy, uz);
   }
   if (tg >= 0) {
     part[((long)g * 3 + 0) * nt + tg] = ux;
@@ -905,13 +916,24 @@ __global__ void k_ff_combine(const double* __restrict__ bound, int ld, int cb,
 // Largest |a - b| and |b| over n vectors (atomic max on the bit patterns of
 // non-negative doubles).
 __global__ void k_ff_maxdiff(const double* __restrict__ a, const double* __restrict__ b, int n,
-                             unsigned long long* __restrict__ res) {
+                             const double* __restrict__ x, const double* __restrict__ fbox,
+                             double reach, unsigned long long* __restrict__ res) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0, m = 0.0;
   if (t < n) {
     for (int d = 0; d < 3; ++d) {
       e = fmax(e, fabs(a[3 * (long)t + d] - b[3 * (long)t + d]));
       m = fmax(m, fabs(b[3 * (long)t + d]));
+    }
+    if (fbox) {  // the error the graded criterion allows at this point
+      const double4 f = focus_sphere(fbox);
+      const double fx = x[3 * (long)t] - f.x, fy = x[3 * (long)t + 1] - f.y,
+                   fz = x[3 * (long)t + 2] - f.z;
+      const double gap = sqrt(fx * fx + fy * fy + fz * fz) - f.w, L = reach * f.w;
+      if (gap > L) {
+        const double w = L / gap;
+        e *= w * w * w;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -948,10 +970,17 @@ void launch_ff_combine(hipStream_t s, const double* bound, int ld, int cb, const
 }
 
 void launch_ff_maxdiff(hipStream_t s, const double* a, const double* b, int n,
-                       unsigned long long* res) {
+                       unsigned long long* res, const double* x, const FFFocus* focus) {
+  const bool graded = focus && focus->box && focus->reach > 0.0 && x;
   hipLaunchKernelGGL(k_ff_zero2, dim3(1), dim3(1), 0, s, res);
   if (n > 0)
-    hipLaunchKernelGGL(k_ff_maxdiff, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, res);
+    hipLaunchKernelGGL(k_ff_maxdiff, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n,
+                       graded ? x : nullptr, graded ? focus->box : nullptr,
+                       graded ? focus->reach : 0.0, res);
+}
+
+void launch_ff_bbox(hipStream_t s, const double* x, int n, double* box) {
+  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, x, n, box);
 }
 
 void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, int num, int rows,
@@ -1060,7 +1089,7 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
 }
 
 int FarField::velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
-                       std::string* err) {
+                       std::string* err, const FFFocus* focus) {
   if (nt <= 0) return UVLM_OK;
   if (!built_) {
     *err = "far field: no sources built";
@@ -1094,9 +1123,11 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (const int*)d_idx2_, nt, ntiles, map);
   hipLaunchKernelGGL(k_ff_tiles, dim3(ntiles), dim3(64), 0, s, targets, (const int*)map,
                      (double4*)d_tiles_);
+  const bool graded = focus && focus->box && focus->reach > 0.0;
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
-                     (const double4*)d_tiles_, theta, (int*)d_near_, (int*)d_far_,
-                     (int2*)d_cnt_, (int*)d_front_);
+                     (const double4*)d_tiles_, theta, graded ? focus->box : nullptr,
+                     graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
+                     (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
   if (f32c_)
     hipLaunchKernelGGL(k_ff_near<true>, dim3(ntiles, kGroups), dim3(64), 0, s,
                        (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,

@@ -405,6 +405,12 @@ class Simulation:
         self._check(N.gpu_lib().uvlm_sim_set_farfield(ctx, float(theta), int(audit_every),
                                                       float(tol)), "set_farfield")
 
+    def set_farfield_focus(self, reach, theta_min=2.0):
+        """Graded criterion for wake tiles beyond reach wing radii (0: off)."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        self._check(N.gpu_lib().uvlm_sim_set_farfield_focus(ctx, float(reach), float(theta_min)),
+                    "set_farfield_focus")
+
     def farfield_stats(self):
         """(relative difference of the last audit, audits run, split active)."""
         ctx = self._L.uvlm_host_sim_context(self._h)
