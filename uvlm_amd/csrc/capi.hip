@@ -242,7 +242,7 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
   const int eb = chain_elems(c->num, c->R, c->C);
   const int ew = chain_elems(c->num, c->wgrows, c->C);
   const int nb = bound ? eb : 0, nw = wake ? ew : 0;
-  ChunkPlan plan = plan_chunks(nt, nb, nw, seq);
+  ChunkPlan plan = plan_chunks(nt, nb, nw, seq, chain_slots(prec));
   TRY(ensure(c, c->partial, sizeof(double) * 3 * (size_t)std::max(1, plan.total()) *
                                 (size_t)std::max(1, nt)));
   const double segs = (bound ? chain_segments(c->num, c->R, c->C) : 0.0) +

@@ -256,6 +256,27 @@ void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows, int 
                      out);
 }
 
+int chain_slots(int prec) {
+  static int cached[2][64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  const int k = prec == kF32C ? 1 : 0;
+  if (cached[k][dev] == 0) {
+    int per_cu = 0, cus = 0;
+    const hipError_t e1 =
+        k ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain_f32c, kBlock, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chain_f64, kBlock, 0);
+    const hipError_t e2 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e1 != hipSuccess || e2 != hipSuccess || per_cu <= 0 || cus <= 0) {
+      (void)hipGetLastError();
+      cached[k][dev] = -1;
+    } else {
+      cached[k][dev] = per_cu * cus;
+    }
+  }
+  return cached[k][dev] > 0 ? cached[k][dev] : 0;
+}
+
 void launch_nbody_chain(hipStream_t s, int prec, const double* targets, int nt,
                         const ChainRec* recs, const ChunkPlan& plan, double* partial,
                         int ld) {

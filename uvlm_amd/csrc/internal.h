@@ -76,7 +76,13 @@ struct ChunkPlan {
   int total() const { return cb + cw; }
 };
 
-ChunkPlan plan_chunks(int n_targets, int nb, int nw, bool sequential);
+// slots > 0: workgroups the device holds at once for the kernel; the grid
+// (target blocks x chunks) is then sized to about three full rounds of them,
+// so that the last round is not left mostly empty.
+ChunkPlan plan_chunks(int n_targets, int nb, int nw, bool sequential, int slots = 0);
+// Resident workgroups of the chain N-body kernel of precision prec on the
+// current device (occupancy x compute units; cached per device).
+int chain_slots(int prec);
 
 // ---- launchers (stream-ordered, asynchronous) ----------------------------
 // Threshold T with  fl(fl(sqrt(x))^2) < kCutoff  <=>  x < T  (x >= 0).

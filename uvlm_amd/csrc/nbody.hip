@@ -404,7 +404,7 @@ double cutoff_dsq_threshold() {
   return T;
 }
 
-ChunkPlan plan_chunks(int nt, int nb, int nw, bool sequential) {
+ChunkPlan plan_chunks(int nt, int nb, int nw, bool sequential, int slots) {
   ChunkPlan p{};
   p.nb = nb;
   p.nw = nw;
@@ -413,8 +413,11 @@ ChunkPlan plan_chunks(int nt, int nb, int nw, bool sequential) {
     p.cw = nw > 0 ? 1 : 0;
   } else {
     const int tblocks = std::max(1, cdiv(nt, kBlock));
-    const int want_blocks = 4096;
-    int want = std::max(1, cdiv(want_blocks, tblocks));
+    int want;
+    if (slots > 0)  // whole rounds of resident workgroups, at most 3
+      want = std::max(1, 3 * slots / tblocks);
+    else
+      want = std::max(1, cdiv(4096, tblocks));
     const int total = nb + nw;
     want = std::min(want, std::max(1, cdiv(total, 2 * kTile)));
     want = std::min(want, 1024);
