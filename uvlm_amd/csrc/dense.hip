@@ -10,11 +10,11 @@
 //   k_tslu    panel factorisation by tournament pivoting (LDS tiles + tree)
 //   k_tslu_l21 L21 = A21 U11^-1 below the panel's pivot block
 //   k_apply_moves  the panel's row permutation applied to the other columns
-//   k_trsm    U12 = L11^-1 A12 (one lane per column, L11 and X in LDS)
+//   k_trsm    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS)
 //   k_gemm    A22 -= L21 U12 on the FP64 matrix cores
 //             (v_mfma_f64_16x16x4f64, 64x64 tile per workgroup)
-// followed by k_getrs (permutation, unit-lower and upper substitution) in
-// one workgroup.  The mixed-precision modes factor a copy in FP32 (or with
+// followed by the permutation and blocked unit-lower and upper substitution
+// (k_trsv_lower, k_trsv_upper, one launch per 256-row block column).  The mixed-precision modes factor a copy in FP32 (or with
 // BF16-rounded MFMA inputs) and refine the FP64 solution with FP64
 // residuals r = b - A x.
 #include <algorithm>
@@ -51,49 +51,50 @@ __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) /
 constexpr int kTR = 128;  // rows per tile (= workgroup size)
 constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
 
-// No-pivot LU of the kb x kb block D in LDS (one lane per row), with the
-// pivoting checks.  Returns (all lanes) the first zero pivot or -1.
-template <typename T>
-__device__ int diag_lu_np(T (*D)[kNB + 1], int kb, int* s_flag) {
-  const int t = threadIdx.x;
-  int zero = -1;
-  for (int j = 0; j < kb; ++j) {
-    __syncthreads();
-    const T d = D[j][j];
-    if (d == T(0) && zero < 0) zero = j;
-    if (t > j && t < kb) {
-      T l = D[t][j];
-      if (fabs(l) > fabs(d)) *s_flag = 1;
-      if (d != T(0)) l = l / d;
-      D[t][j] = l;
-#pragma unroll 8
-      for (int c = j + 1; c < kb; ++c) D[t][c] = D[t][c] - l * D[j][c];
-    }
-  }
-  __syncthreads();
-  return zero;
-}
-
 // Fast-path panel: every workgroup factors the top block itself (workgroup 0
-// stores it), then forms L21 = A21 U11^-1 for its 64 rows by a column sweep
-// in LDS.  flag <- 1 when a pivoting check fails; info <- first zero pivot.
+// stores it), then forms L21 = A21 U11^-1 for its 64 rows.  Rows live in
+// registers (one lane per row, the block's columns unrolled); the factored
+// rows of the diagonal block are published through LDS one step at a time.
+// Columns and rows beyond kb are padded with the identity, which the
+// elimination leaves unchanged.  flag <- 1 when a pivoting check fails;
+// info <- first zero pivot.
 template <typename T>
 __global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, int kb,
                                                  int* __restrict__ flag,
                                                  int* __restrict__ info) {
   __shared__ T D[kNB][kNB + 1];
-  __shared__ T X[kNB][64 + 1];
   __shared__ int s_flag;
   const int t = threadIdx.x;
   if (t == 0) s_flag = 0;
-  for (int q = t; q < kNB * kNB; q += 64) {
-    const int r = q % kNB, c = q / kNB;
-    D[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
-  }
   const int row = k0 + kb + blockIdx.x * 64 + t;
-  if (row < n)
-    for (int c = 0; c < kb; ++c) X[c][t] = A[(size_t)(k0 + c) * lda + row];
-  const int zero = diag_lu_np(D, kb, &s_flag);
+  T x[kNB];  // this lane's row below the panel's diagonal block
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    x[c] = (row < n && c < kb) ? A[(size_t)(k0 + c) * lda + row] : T(0);
+  T d[kNB];  // row t of the diagonal block (lanes t < kNB)
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    d[c] = (t < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + t] : (c == t ? T(1) : T(0));
+  int zero = -1;
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (t == j) {  // row j is final: publish it
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) D[j][c] = d[c];
+    }
+    __syncthreads();
+    const T piv = D[j][j];
+    if (piv == T(0) && zero < 0 && j < kb) zero = j;
+    if (t > j && t < kNB) {
+      T l = d[j];
+      bad |= fabs(l) > fabs(piv);
+      if (piv != T(0)) l = l / piv;
+      d[j] = l;
+#pragma unroll
+      for (int c = j + 1; c < kNB; ++c) d[c] = d[c] - l * D[j][c];
+    }
+  }
   if (blockIdx.x == 0) {
     for (int q = t; q < kb * kb; q += 64) {
       const int r = q % kb, c = q / kb;
@@ -101,20 +102,28 @@ __global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, i
     }
     if (t == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
   }
-  if (row < n) {
-    bool bad = false;
-    for (int j = 0; j < kb; ++j) {
-      const T d = D[j][j];
-      T xj = X[j][t];
-      bad |= fabs(xj) > fabs(d);
-      if (d != T(0)) xj = xj / d;
-      X[j][t] = xj;
-#pragma unroll 8
-      for (int c = j + 1; c < kb; ++c) X[c][t] = X[c][t] - xj * D[j][c];
-    }
-    if (bad) s_flag = 1;
-    for (int c = 0; c < kb; ++c) A[(size_t)(k0 + c) * lda + row] = X[c][t];
+  const T* Db = &D[0][0];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    // row offset passed through an empty asm that also takes x[j - 1]: the
+    // reads of row j are issued once step j - 1 starts, not all at the top
+    // (which would hold the whole block in registers)
+    int o = j * (kNB + 1);
+    asm volatile("" : "+s"(o) : "v"(x[j > 0 ? j - 1 : 0]));
+    const T piv = Db[o + j];
+    T xj = x[j];
+    bad |= fabs(xj) > fabs(piv);
+    if (piv != T(0)) xj = xj / piv;
+    x[j] = xj;
+#pragma unroll
+    for (int c = j + 1; c < kNB; ++c) x[c] = x[c] - xj * Db[o + c];
   }
+  if (row < n) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c < kb) A[(size_t)(k0 + c) * lda + row] = x[c];
+  }
+  if (bad) s_flag = 1;
   __syncthreads();
   if (t == 0 && s_flag) atomicOr(flag, 1);
 }
@@ -382,12 +391,11 @@ __global__ __launch_bounds__(256) void k_apply_moves(T* A, int lda, int n, int k
   if (act) a[dst] = v;
 }
 
-// U12 = L11^-1 A12 for the columns [k0+kb, n): one lane per column, column
-// sweep in LDS.
+// U12 = L11^-1 A12 for the columns [k0+kb, n): one lane per column held in
+// registers, L11 in LDS (zero beyond kb).
 template <typename T>
 __global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb) {
   __shared__ T L[kNB][kNB + 1];
-  __shared__ T X[kNB][64 + 1];
   const int t = threadIdx.x;
   for (int q = t; q < kNB * kNB; q += 64) {
     const int r = q % kNB, c = q / kNB;
@@ -395,16 +403,25 @@ __global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int k
   }
   const int col = k0 + kb + blockIdx.x * 64 + t;
   T* a = A + (size_t)col * lda + k0;
-  if (col < n)
-    for (int i = 0; i < kb; ++i) X[i][t] = a[i];
+  T x[kNB];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i) x[i] = (col < n && i < kb) ? a[i] : T(0);
   __syncthreads();
   if (col >= n) return;
-  for (int j = 0; j < kb; ++j) {
-    const T xj = X[j][t];
-#pragma unroll 8
-    for (int i = j + 1; i < kb; ++i) X[i][t] = X[i][t] - L[i][j] * xj;
+  const T* Lb = &L[0][0];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    // column offset through an empty asm that also takes x[j - 1]: the reads
+    // of column j are issued once step j - 1 starts, not all at the top
+    int o = j;
+    asm volatile("" : "+s"(o) : "v"(x[j > 0 ? j - 1 : 0]));
+    const T xj = x[j];
+#pragma unroll
+    for (int i = j + 1; i < kNB; ++i) x[i] = x[i] - Lb[i * (kNB + 1) + o] * xj;
   }
-  for (int i = 0; i < kb; ++i) a[i] = X[i][t];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i)
+    if (i < kb) a[i] = x[i];
 }
 
 // A22 -= L21 * U12 with the FP64 matrix cores.  The product is formed
@@ -535,38 +552,148 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float* A, int lda, int n,
       }
 }
 
-// x = U^-1 L^-1 (P b), one 1024-thread workgroup, x resident in LDS
-// (n <= kGetrsLds) or in global memory.  Blocks of 64 rows: wavefront 0
-// solves the 64x64 diagonal block with lane shuffles (no workgroup barrier
-// inside the block), then all lanes apply the block to the remaining rows.
-constexpr int kGetrsLds = 4096;  // FP64 x of up to 32 KB in LDS (+ 33 KB block)
+// x = U^-1 L^-1 (P b), blocked over launches: launch j handles block column
+// j of kTB rows.  Each workgroup solves the diagonal block itself (64-row
+// sub-blocks solved by wavefront 0 with lane shuffles, no barrier inside a
+// sub-block, then applied to the block's later rows), workgroup 0 stores
+// the block's solution, and every workgroup applies it to its own kTB rows
+// of the rest.  A launch reads the block's right-hand side from the work
+// vector w and stores the solution in another vector y, so no workgroup can
+// read a solution in place of its right-hand side (forward: w = tmp, y =
+// out; backward: w = out, y = tmp, then copied to out).  The per-row operation sequence is that of one sweep over
+// 64-row blocks (forward from row 0, backward from row n), so the result
+// does not depend on kTB.  The forward grid of 64-row blocks starts at row
+// 0 and the backward one ends at row n; kTB-row blocks follow the same
+// alignment.
+constexpr int kTB = 256;
 
 template <typename T>
-__device__ void getrs_body(const T* __restrict__ LU, int n, const int* __restrict__ perm,
-                           const T* __restrict__ b, T* x, T (*D)[65]) {
-  for (int i = threadIdx.x; i < n; i += 1024) x[i] = b[perm[i]];
-  lu_substitute(LU, n, x, D);
+__global__ void k_permute(const T* __restrict__ b, const int* __restrict__ perm, int n,
+                          T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = b[perm[i]];
 }
 
+// Forward (unit lower) step for rows [j0, j0 + jb).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_getrs(const T* __restrict__ LU, int n,
-                                               const int* __restrict__ perm,
-                                               const T* __restrict__ b, T* __restrict__ out) {
-  extern __shared__ unsigned char s_raw[];
+__global__ __launch_bounds__(kTB) void k_trsv_lower(const T* __restrict__ LU, int n,
+                                                    T* __restrict__ x, T* __restrict__ y,
+                                                    int j0, int jb) {
   __shared__ T D[64][65];
-  T* xs = reinterpret_cast<T*>(s_raw);
-  if (n <= kGetrsLds) {
-    getrs_body(LU, n, perm, b, xs, D);
-    for (int i = threadIdx.x; i < n; i += 1024) out[i] = xs[i];
-  } else {
-    getrs_body(LU, n, perm, b, out, D);
+  __shared__ T xs[kTB];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid < jb) xs[tid] = x[j0 + tid];
+  for (int s0 = 0; s0 < jb; s0 += 64) {
+    const int sb = min(64, jb - s0);
+    __syncthreads();
+    for (int q = tid; q < 64 * 64; q += kTB) {
+      const int r = q % 64, c = q / 64;
+      D[r][c] = (r < sb && c < sb) ? LU[(size_t)(j0 + s0 + c) * n + j0 + s0 + r] : T(0);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      T v = lane < sb ? xs[s0 + lane] : T(0);
+      for (int i = 0; i < sb; ++i) {
+        const T xi = __shfl(v, i);
+        if (lane > i && lane < sb) v = v - D[lane][i] * xi;
+      }
+      if (lane < sb) xs[s0 + lane] = v;
+    }
+    __syncthreads();
+    const int r = s0 + sb + tid;  // later rows of the diagonal block
+    if (r < jb) {
+      T acc = xs[r];
+      const T* row = LU + (size_t)(j0 + s0) * n + j0 + r;
+      for (int i = 0; i < sb; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
+      xs[r] = acc;
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < jb) y[j0 + tid] = xs[tid];
+  const int r = j0 + jb + blockIdx.x * kTB + tid;
+  if (r < n) {
+    T acc = x[r];
+    const T* row = LU + (size_t)j0 * n + r;
+#pragma unroll 16
+    for (int i = 0; i < jb; ++i) acc = acc - row[(size_t)i * n] * xs[i];
+    x[r] = acc;
+  }
+}
+
+// Backward (upper) step for rows [j0, j0 + jb); rows above are updated.
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_trsv_upper(const T* __restrict__ LU, int n,
+                                                    T* __restrict__ x, T* __restrict__ y,
+                                                    int j0, int jb) {
+  __shared__ T D[64][65];
+  __shared__ T xs[kTB];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid < jb) xs[tid] = x[j0 + tid];
+  for (int se = jb; se > 0; se -= 64) {
+    const int s0 = max(0, se - 64), sb = se - s0;
+    __syncthreads();
+    for (int q = tid; q < 64 * 64; q += kTB) {
+      const int r = q % 64, c = q / 64;
+      D[r][c] = (r < sb && c < sb) ? LU[(size_t)(j0 + s0 + c) * n + j0 + s0 + r] : T(0);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      T v = lane < sb ? xs[s0 + lane] : T(0);
+      const T dl = lane < sb ? D[lane][lane] : T(1);
+      for (int i = sb - 1; i >= 0; --i) {
+        if (lane == i) v = v / dl;
+        const T xi = __shfl(v, i);
+        if (lane < i) v = v - D[lane][i] * xi;
+      }
+      if (lane < sb) xs[s0 + lane] = v;
+    }
+    __syncthreads();
+    if (tid < s0) {  // earlier rows of the diagonal block
+      T acc = xs[tid];
+      const T* row = LU + (size_t)(j0 + s0) * n + j0 + tid;
+      for (int i = 0; i < sb; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
+      xs[tid] = acc;
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && tid < jb) y[j0 + tid] = xs[tid];
+  const int r = blockIdx.x * kTB + tid;
+  if (r < j0) {
+    T acc = x[r];
+    for (int se = jb; se > 0; se -= 64) {  // 64-row sub-blocks from the end
+      const int s0 = max(0, se - 64);
+      const T* row = LU + (size_t)(j0 + s0) * n + r;
+#pragma unroll 16
+      for (int i = 0; i < se - s0; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
+    }
+    x[r] = acc;
   }
 }
 
 template <typename T>
-void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* b, T* out) {
-  const size_t lds = n <= kGetrsLds ? sizeof(T) * (size_t)n : 0;
-  hipLaunchKernelGGL((k_getrs<T>), dim3(1), dim3(1024), lds, s, LU, n, perm, b, out);
+__global__ void k_copy_t(const T* __restrict__ a, int n, T* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// tmp: n values of scratch (distinct from b and out).
+template <typename T>
+void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* b, T* out,
+                  T* tmp) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_permute<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, b, perm, n, tmp);
+  for (int j0 = 0; j0 < n; j0 += kTB) {
+    const int jb = std::min(kTB, n - j0);
+    const int rest = n - j0 - jb;
+    hipLaunchKernelGGL((k_trsv_lower<T>), dim3(std::max(1, cdiv(rest, kTB))), dim3(kTB), 0, s,
+                       LU, n, tmp, out, j0, jb);
+  }
+  for (int je = n; je > 0; je -= kTB) {
+    const int j0 = std::max(0, je - kTB), jb = je - j0;
+    hipLaunchKernelGGL((k_trsv_upper<T>), dim3(std::max(1, cdiv(j0, kTB))), dim3(kTB), 0, s,
+                       LU, n, out, tmp, j0, jb);
+  }
+  hipLaunchKernelGGL((k_copy_t<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, (const T*)tmp, n, out);
 }
 
 // Workspace of the factorisation (ints): move list of the current panel,
@@ -646,20 +773,49 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
   if (i < n) b[i] = (float)a[i];
 }
 
-// r = b - A x  (FP64, column-major A, one row per lane; columns streamed).
-__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ A,
-                                                  int n,
-                                                  const double* __restrict__ x,
-                                                  const double* __restrict__ b,
-                                                  double* __restrict__ r,
-                                                  float* __restrict__ rf) {
+// r = b - A x in FP64 (column-major A): workgroup (i, s) sums columns
+// [s cols, (s + 1) cols) for 256 rows into part[s][row]; k_resid_sum adds
+// the slices in increasing s.
+__global__ __launch_bounds__(256) void k_resid_part(const double* __restrict__ A, int n,
+                                                    const double* __restrict__ x, int cols,
+                                                    double* __restrict__ part) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
-  for (int j = 0; j < n; ++j) s = __builtin_fma(A[(size_t)j * n + i], x[j], s);
-  const double v = b[i] - s;
+  const int j0 = blockIdx.y * cols, j1 = min(n, j0 + cols);
+  const double* p = A + (size_t)j0 * n + i;
+  double acc = 0.0;
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    const double a0 = p[0], a1 = p[n], a2 = p[2L * n], a3 = p[3L * n];
+    acc = __builtin_fma(a0, x[j], acc);
+    acc = __builtin_fma(a1, x[j + 1], acc);
+    acc = __builtin_fma(a2, x[j + 2], acc);
+    acc = __builtin_fma(a3, x[j + 3], acc);
+    p += 4L * n;
+  }
+  for (; j < j1; ++j, p += n) acc = __builtin_fma(*p, x[j], acc);
+  part[(size_t)blockIdx.y * n + i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_resid_sum(const double* __restrict__ part, int S,
+                                                   int n, const double* __restrict__ b,
+                                                   double* __restrict__ r,
+                                                   float* __restrict__ rf) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int k = 0; k < S; ++k) acc += part[(size_t)k * n + i];
+  const double v = b[i] - acc;
   r[i] = v;
   rf[i] = (float)v;
+}
+
+// Column slices of the residual: about 1024 workgroups, at most 64 slices.
+inline void resid_split(int n, int* S, int* cols) {
+  const int rb = cdiv(n, 256);
+  int want = std::min(64, std::max(1, cdiv(1024, rb)));
+  *cols = cdiv(n, want);
+  *S = cdiv(n, *cols);
 }
 
 // x += dx; norms |dx|_inf and |x|_inf for the convergence test.
@@ -898,16 +1054,19 @@ int dense_lu_factor(hipStream_t s, double* A, int n, void* work) {
   return lu_factor_checked<double, 0>(s, A, n, w + 64, w, w + 64 + (size_t)n, saved);
 }
 
-void dense_lu_solve(hipStream_t s, const double* LU, int n, const void* work, const double* b,
+void dense_lu_solve(hipStream_t s, const double* LU, int n, void* work, const double* b,
                     double* x) {
   const int* w = (const int*)work;
-  launch_getrs<double>(s, LU, n, w + 64, b, x);
+  // the copy of A kept for a pivoted retry is free once A is factored
+  double* tmp = (double*)((char*)work + (sizeof(int) * lu_ints(n) + 255) / 256 * 256);
+  launch_getrs<double>(s, LU, n, w + 64, b, x, tmp);
 }
 
 size_t dense_mixed_work_bytes(int n) {
   const size_t nn = (size_t)n * n;
   return 2 * nn * sizeof(float) + 512 + sizeof(int) * lu_ints(n) + (size_t)n * sizeof(double) +
-         64 * sizeof(double) + 2 * (size_t)n * sizeof(float) + 256;
+         64 * sizeof(double) + 2 * (size_t)n * sizeof(float) + 256 +
+         64 * (size_t)n * sizeof(double) + 256;
 }
 
 // Mixed precision: LU of fl32(A) (FP32 or BF16-input MFMA updates), then
@@ -931,6 +1090,11 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   float* rf = (float*)w;
   w += (size_t)n * sizeof(float);
   float* dxf = (float*)w;
+  w += (size_t)n * sizeof(float);
+  w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  double* part = (double*)w;  // residual slices [S][n], 256-byte aligned
+  int S = 1, cols = n;
+  resid_split(n, &S, &cols);
   int* perm = lw + 64;
   int* tree = lw + 64 + (size_t)n;
   hipLaunchKernelGGL(k_to_f32, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, Af, nn);
@@ -941,9 +1105,11 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   int it = 0;
   double h_norms[2] = {0, 0};
   for (it = 0; it < max_iter; ++it) {
-    hipLaunchKernelGGL(k_residual, dim3(cdiv(n, 256)), dim3(256), 0, s, A, n, x,
-                       b, r, rf);
-    launch_getrs<float>(s, Af, n, perm, rf, dxf);
+    hipLaunchKernelGGL(k_resid_part, dim3(cdiv(n, 256), S), dim3(256), 0, s, A, n,
+                       (const double*)x, cols, part);
+    hipLaunchKernelGGL(k_resid_sum, dim3(cdiv(n, 256)), dim3(256), 0, s, (const double*)part,
+                       S, n, b, r, rf);
+    launch_getrs<float>(s, Af, n, perm, rf, dxf, Asaved);  // Asaved is free after the LU
     hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, s, x, (const float*)dxf,
                        n, norms);
     if (hipMemcpyAsync(h_norms, norms, sizeof(h_norms), hipMemcpyDeviceToHost, s) !=

@@ -130,7 +130,7 @@ void launch_flow_velocity(hipStream_t s, const double* partial, int ld,
 size_t dense_lu_work_bytes(int n);
 int* dense_lu_info(void* work);
 int dense_lu_factor(hipStream_t s, double* A, int n, void* work);
-void dense_lu_solve(hipStream_t s, const double* LU, int n, const void* work, const double* b,
+void dense_lu_solve(hipStream_t s, const double* LU, int n, void* work, const double* b,
                     double* x);
 // Mixed precision solve with FP64 iterative refinement; A is not modified.
 // mode: 1 = FP32 LU, 2 = BF16-input LU updates.
