@@ -184,11 +184,16 @@ def run_single(args):
     sim.profile_read()
     w = mlp_init(5) if args.mlp_every > 0 else None
     learned = 0
+    net_pairs = net_trips = 0  # counters survive retraining (which resets them)
     t0 = time.perf_counter()
     for s in range(args.warmup, total):
         sim.step(s)
         if (w is not None and args.farfield > 0 and s >= args.mlp_start
                 and (s - args.mlp_start) % args.mlp_every == 0):
+            if learned:
+                p_, t_ = sim.farfield_mlp_stats()
+                net_pairs += p_
+                net_trips += t_
             w, _ = sim.farfield_learn(w, args.farfield, args.mlp_samples, args.mlp_iters,
                                       512, 1e-3, seed=s)
             learned += 1
@@ -244,6 +249,8 @@ def run_single(args):
             out["farfield"]["graded"] = {"reach": args.ff_reach, "theta_min": args.ff_theta_min}
         if w is not None:
             pairs, trips = sim.farfield_mlp_stats()
+            pairs += net_pairs
+            trips += net_trips
             out["farfield"]["network"] = {"trained": learned, "every": args.mlp_every,
                                           "samples": args.mlp_samples,
                                           "iters": args.mlp_iters, "corrected_pairs": pairs,
