@@ -172,3 +172,35 @@ def test_graded_criterion_away_from_the_wings(orc):
     assert np.all(np.abs(lg - le).max(axis=0) / scale < 1e-3)
     err, audits, active = graded.farfield_stats()
     assert audits == 4 and active and 0 <= err < 1e-3
+
+
+def test_f32c_near_sums_in_the_split(orc):
+    """The split with F32C near sums (the near blocks' records walked in FP32
+    from FP64 differences, FP64 re-evaluation near the cut-off): loads
+    within 1e-4 of the same split in FP64 over steps before the tandem wake
+    amplifies rounding, within the split's tolerance of the exact run, and
+    the audits pass."""
+    from uvlm_amd import _native as N
+    steps = 30
+    case = W.tandem_c5(steps=steps, rows=4, cols=16)
+    exact = Simulation(case, steps)
+    f64 = Simulation(case, steps)
+    c32 = dict(case)
+    c32["precision"] = N.PREC_F32C
+    f32 = Simulation(c32, steps)
+    for sim in (f64, f32):
+        sim.set_farfield(theta=3.0, audit_every=10, tol=1e-3)
+    sims = (exact, f64, f32)
+    for s in range(steps + 1):
+        for sim in sims:
+            sim.step(s)
+    for sim in sims:
+        sim.sync()
+    le, l64, l32 = (sim.loads() for sim in sims)
+    scale = np.abs(le).max(axis=0)
+    d32 = np.abs(l32 - l64).max(axis=0) / scale
+    print(f"F32C vs FP64 split: {d32.ravel()}")
+    assert np.all(d32 < 1e-4) and np.abs(l32 - l64).max() > 0.0
+    assert np.all(np.abs(l32 - le).max(axis=0) / scale < 1e-3)
+    err, audits, active = f32.farfield_stats()
+    assert audits == 3 and active and 0 <= err < 1e-3
