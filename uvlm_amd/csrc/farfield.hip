@@ -36,7 +36,8 @@ namespace {
 
 __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 constexpr int kTileT = 64;        // targets per tile
-constexpr int kGroups = 8;        // partial-sum groups per tile (near and far each)
+constexpr int kGroups = 8;        // partial-sum groups per tile, far clusters
+constexpr int kNearGroups = 16;   // partial-sum groups per tile, near blocks
 constexpr int kMomThreads = 128;  // >= kFFMom: one thread per moment component
 
 // --- tree layout ------------------------------------------------------------
@@ -539,6 +540,8 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
   float ras = fmaf(raz, raz, fmaf(ray, ray, rax * rax));
   float ia = __builtin_amdgcn_rsqf(ras);
   float fx = 0.f, fy = 0.f, fz = 0.f;
+  unsigned redo = 0;  // elements to re-evaluate in FP64 (bit k - 1)
+  static_assert(kFFCols <= 32 && kFFRows <= 32, "chain masks hold 32 elements");
   for (int k = 1; k <= count; ++k) {
     p += 3 * stride_nodes;
     const double bx = p[0], by = p[1], bz = p[2];
@@ -562,7 +565,7 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
     fx = fmaf(dx, K, fx);
     fy = fmaf(dy, K, fy);
     fz = fmaf(dz, K, fz);
-    if (nearcore && g != 0.0) dev::seg_ref(pax, pay, paz, bx, by, bz, x, y, z, g, ux, uy, uz);
+    redo |= (unsigned)(nearcore && g != 0.0) << (k - 1);
     pax = bx;
     pay = by;
     paz = bz;
@@ -571,6 +574,15 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
     raz = rbz;
     ras = rbs;
     ia = ib;
+  }
+  if (__any(redo != 0)) {  // rare: a second, FP64 pass over the flagged elements
+    p = R + 3 * n0;
+    for (int k = 1; k <= count; ++k) {
+      const double* b = p + 3 * stride_nodes;
+      if ((redo >> (k - 1)) & 1u)
+        dev::seg_ref(p[0], p[1], p[2], b[0], b[1], b[2], x, y, z, S[s0 + k - 1], ux, uy, uz);
+      p = b;
+    }
   }
   ux += (double)fx;
   uy += (double)fy;
@@ -607,7 +619,18 @@ __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double*
   u[2] = uz;
 }
 
-// Group g of a tile sums the tile's near blocks g, g + kGroups, ...: each
+// Near-count keys of the tiles: the near kernel takes tiles in decreasing
+// order of their near lists (heaviest first), so that the longest
+// workgroups do not start last and run alone at the end of the launch.
+__global__ void k_near_keys(const int2* __restrict__ cnt, int ntiles,
+                            unsigned long long* __restrict__ key, int* __restrict__ idx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  key[t] = (unsigned long long)(0x7fffffff - cnt[t].x);
+  idx[t] = t;
+}
+
+// Group g of a tile sums the tile's near blocks g, g + kNearGroups, ...: each
 // block's record is staged in LDS (the next one is loaded into registers
 // meanwhile) and walked for every lane's target (FP64, or FP32 compensated).
 template <bool F32>
@@ -617,10 +640,11 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
                                                 const int* __restrict__ map, int n0,
                                                 const int* __restrict__ near,
                                                 const int2* __restrict__ cnt,
+                                                const int* __restrict__ order,
                                                 double* __restrict__ part) {
   constexpr int kPer = (kRec + 63) / 64;
   __shared__ double sr[kPer * 64];
-  const int tile = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
+  const int tile = order[blockIdx.x], g = blockIdx.y, lane = threadIdx.x;
   const int tg = tile_target(map, tile, lane);
   const double px = tg >= 0 ? x[3 * (long)tg] : 0.0, py = tg >= 0 ? x[3 * (long)tg + 1] : 0.0,
                pz = tg >= 0 ? x[3 * (long)tg + 2] : 0.0;
@@ -637,15 +661,15 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
     }
   };
   if (g < count) load(id);
-  for (int k = g; k < count; k += kGroups) {
+  for (int k = g; k < count; k += kNearGroups) {
     const FFCluster c = cl[id];
     const int nr = c.r1 - c.r0, nc = c.c1 - c.c0;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kPer; ++u) sr[lane + 64 * u] = pre[u];
     __syncthreads();
-    if (k + kGroups < count) {  // next block's record, in flight during the walk
-      id = list[k + kGroups];
+    if (k + kNearGroups < count) {  // next block's record, in flight during the walk
+      id = list[k + kNearGroups];
       load(id);
     }
     if (F32)
@@ -834,9 +858,9 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
     uz += u[2];
   }
   if (tg >= 0) {
-    part[((long)(kGroups + g) * 3 + 0) * nt + tg] = ux;
-    part[((long)(kGroups + g) * 3 + 1) * nt + tg] = uy;
-    part[((long)(kGroups + g) * 3 + 2) * nt + tg] = uz;
+    part[((long)(kNearGroups + g) * 3 + 0) * nt + tg] = ux;
+    part[((long)(kNearGroups + g) * 3 + 1) * nt + tg] = uy;
+    part[((long)(kNearGroups + g) * 3 + 2) * nt + tg] = uz;
   }
   if (MLP) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -889,8 +913,8 @@ __global__ void k_ff_sum(const double* __restrict__ part, int nt, double* __rest
   if (t >= nt) return;
   for (int d = 0; d < 3; ++d) {
     double a = 0.0, b = 0.0;
-    for (int g = 0; g < kGroups; ++g) a += part[((long)g * 3 + d) * nt + t];
-    for (int g = 0; g < kGroups; ++g) b += part[((long)(kGroups + g) * 3 + d) * nt + t];
+    for (int g = 0; g < kNearGroups; ++g) a += part[((long)g * 3 + d) * nt + t];
+    for (int g = 0; g < kGroups; ++g) b += part[((long)(kNearGroups + g) * 3 + d) * nt + t];
     out[3 * (long)t + d] = a + b;
   }
 }
@@ -1110,7 +1134,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
       (rc = ensure(&d_far_, &c_far_, sizeof(int) * (size_t)ntiles * t.total, err)) ||
       (rc = ensure(&d_cnt_, &c_cnt_, sizeof(int2) * (size_t)ntiles, err)) ||
       (rc = ensure(&d_front_, &c_front_, sizeof(int) * 2 * (size_t)ntiles * n0, err)) ||
-      (rc = ensure(&d_part_, &c_part_, sizeof(double) * 3 * 2 * kGroups * (size_t)nt, err)))
+      (rc = ensure(&d_part_, &c_part_,
+                   sizeof(double) * 3 * (kNearGroups + kGroups) * (size_t)nt, err)))
     return rc;
   // target tiles: the targets in Morton order of their own box
   double* tbox = (double*)d_box_;
@@ -1128,15 +1153,20 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (const double4*)d_tiles_, theta, graded ? focus->box : nullptr,
                      graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
                      (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
+  // heaviest tiles first (the Morton sort's buffers are free again)
+  hipLaunchKernelGGL(k_near_keys, dim3(cdiv(ntiles, 256)), dim3(256), 0, s,
+                     (const int2*)d_cnt_, ntiles, (unsigned long long*)d_key_, (int*)d_idx_);
+  if ((rc = sort_pairs(s, ntiles, err)) != UVLM_OK) return rc;
+  const int* order = (const int*)d_idx2_;
   if (f32c_)
-    hipLaunchKernelGGL(k_ff_near<true>, dim3(ntiles, kGroups), dim3(64), 0, s,
+    hipLaunchKernelGGL(k_ff_near<true>, dim3(ntiles, kNearGroups), dim3(64), 0, s,
                        (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
-                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_,
+                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_, order,
                        (double*)d_part_);
   else
-    hipLaunchKernelGGL(k_ff_near<false>, dim3(ntiles, kGroups), dim3(64), 0, s,
+    hipLaunchKernelGGL(k_ff_near<false>, dim3(ntiles, kNearGroups), dim3(64), 0, s,
                        (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
-                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_,
+                       (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_, order,
                        (double*)d_part_);
   if (mlp_on_) {
     hipLaunchKernelGGL(k_ff_far<true>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
