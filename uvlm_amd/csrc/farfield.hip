@@ -37,7 +37,7 @@ namespace {
 __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 constexpr int kTileT = 64;        // targets per tile
 constexpr int kGroups = 8;        // partial-sum groups per tile, far clusters
-constexpr int kNearGroups = 16;   // partial-sum groups per tile, near blocks
+constexpr int kNearGroups = 32;   // partial-sum groups per tile, near blocks
 constexpr int kMomThreads = 128;  // >= kFFMom: one thread per moment component
 
 // --- tree layout ------------------------------------------------------------
