@@ -115,7 +115,7 @@ int ensure(uvlm_ctx* c, Buf& b, size_t bytes) {
     b.p = nullptr;
     b.cap = 0;
   }
-  const size_t want = bytes + bytes / 4;
+  const size_t want = 2 * bytes;  // doubling: O(log) reallocations for a growing wake
   if (hipMalloc(&b.p, want) != hipSuccess) {
     b.p = nullptr;
     (void)hipGetLastError();

@@ -37,7 +37,7 @@ namespace {
 __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 constexpr int kTileT = 64;        // targets per tile
 constexpr int kGroups = 8;        // partial-sum groups per tile, far clusters
-constexpr int kNearGroups = 32;   // partial-sum groups per tile, near blocks
+constexpr int kNearGroups = 64;   // partial-sum groups per tile, near blocks
 constexpr int kMomThreads = 128;  // >= kFFMom: one thread per moment component
 
 // --- tree layout ------------------------------------------------------------
@@ -1032,7 +1032,9 @@ int FarField::ensure(void** p, size_t* cap, size_t bytes, std::string* err) {
   }
   *p = nullptr;
   *cap = 0;
-  const size_t b = std::max<size_t>(bytes + bytes / 4, 256);
+  // doubling: buffers that follow a growing wake are reallocated (a device
+  // synchronisation each) O(log) times per run
+  const size_t b = std::max<size_t>(2 * bytes, 256);
   if (hipMalloc(p, b) != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
