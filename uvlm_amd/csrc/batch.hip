@@ -355,16 +355,30 @@ __global__ __launch_bounds__(BLOCK) void k_b_nbody(
       const int ne = part == 0 ? eb : ew;
       dev::Carry64 cr;
       dev::carry_reset(cr);
+      double ax = 0.0, ay = 0.0, az = 0.0;
       const double2* s2 = reinterpret_cast<const double2*>(src);
       for (int base = 0; base < ne; base += TILE) {
         const int cnt = min(TILE, ne - base);
         __syncthreads();
-        for (int q = threadIdx.x; q < cnt * 4; q += BLOCK) tile[q] = s2[(long)base * 4 + q];
-        __syncthreads();
+        int lng = 0;  // a segment of the tile too long for the short form
+        for (int q = threadIdx.x; q < cnt * 4; q += BLOCK) {
+          const double2 v = s2[(long)base * 4 + q];
+          tile[q] = v;
+          lng |= (q & 3) == 3 && !(v.y < dev::kShortSeg2);
+        }
+        const bool any_long = __syncthreads_or(lng);
         const double* tl = reinterpret_cast<const double*>(tile);
-        for (int k = 0; k < cnt; ++k)
-          dev::chain_step64(tl + 8 * k, x, y, z, cr, acc[part][0], acc[part][1], acc[part][2]);
+        if (any_long) {
+#pragma unroll 1
+          for (int k = 0; k < cnt; ++k) dev::chain_step64<true>(tl + 8 * k, x, y, z, cr, ax, ay, az);
+        } else {
+#pragma unroll 2
+          for (int k = 0; k < cnt; ++k) dev::chain_step64<false>(tl + 8 * k, x, y, z, cr, ax, ay, az);
+        }
       }
+      acc[part][0] = ax;
+      acc[part][1] = ay;
+      acc[part][2] = az;
     }
     if (t < nt) {
       double* o = out + 3 * (toff + t);
