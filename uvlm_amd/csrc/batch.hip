@@ -518,7 +518,6 @@ __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, 
   const int kb = min(kNB, n - k0);
   double* A = Ab + c.A_off;
   __shared__ double L[kNB][kNB + 1];
-  __shared__ double X[kNB][64 + 1];
   for (int q = threadIdx.x; q < kNB * kNB; q += 64) {
     const int r = q % kNB, cc = q / kNB;
     L[r][cc] = (r < kb && cc < kb) ? A[(long)(k0 + cc) * n + k0 + r] : 0.0;
@@ -526,15 +525,23 @@ __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, 
   const int t = threadIdx.x;
   const int col = k0 + kb + blockIdx.x * 64 + t;
   double* a = A + (long)col * n + k0;
-  if (col < n)
-    for (int i = 0; i < kb; ++i) X[i][t] = a[i];
+  double x[kNB];  // this lane's column, in registers (zero beyond kb)
+#pragma unroll
+  for (int i = 0; i < kNB; ++i) x[i] = (col < n && i < kb) ? a[i] : 0.0;
   __syncthreads();
   if (col >= n) return;
-  for (int j = 0; j < kb; ++j) {
-    const double xj = X[j][t];
-    for (int i = j + 1; i < kb; ++i) X[i][t] = X[i][t] - L[i][j] * xj;
+  const double* Lb = &L[0][0];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    int o = j;  // column offset through an empty asm (see k_b_panel_np)
+    asm volatile("" : "+s"(o) : "v"(x[j > 0 ? j - 1 : 0]));
+    const double xj = x[j];
+#pragma unroll
+    for (int i = j + 1; i < kNB; ++i) x[i] = x[i] - Lb[i * (kNB + 1) + o] * xj;
   }
-  for (int i = 0; i < kb; ++i) a[i] = X[i][t];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i)
+    if (i < kb) a[i] = x[i];
 }
 
 // items: (case, row tile, col tile, -) of the trailing update after panel k0
@@ -613,30 +620,38 @@ __global__ __launch_bounds__(64) void k_b_panel_np(const BCase* __restrict__ cas
   if ((int)blockIdx.x * 64 >= max(n - k0 - kb, 1)) return;
   double* A = Ab + c.A_off;
   __shared__ double D[kNB][kNB + 1];
-  __shared__ double X[kNB][64 + 1];
   __shared__ int s_flag;
   const int t = threadIdx.x;
   if (t == 0) s_flag = 0;
-  for (int q = t; q < kNB * kNB; q += 64) {
-    const int r = q % kNB, cc = q / kNB;
-    D[r][cc] = (r < kb && cc < kb) ? A[(long)(k0 + cc) * n + k0 + r] : 0.0;
-  }
+  // rows in registers (one lane per row, columns unrolled), identity padding
+  // beyond kb; the diagonal block's final rows are published through LDS
   const int row = k0 + kb + blockIdx.x * 64 + t;
-  if (row < n)
-    for (int cc = 0; cc < kb; ++cc) X[cc][t] = A[(long)(k0 + cc) * n + row];
-  for (int j = 0; j < kb; ++j) {
+  double x[kNB];
+#pragma unroll
+  for (int cc = 0; cc < kNB; ++cc)
+    x[cc] = (row < n && cc < kb) ? A[(long)(k0 + cc) * n + row] : 0.0;
+  double d[kNB];
+#pragma unroll
+  for (int cc = 0; cc < kNB; ++cc)
+    d[cc] = (t < kb && cc < kb) ? A[(long)(k0 + cc) * n + k0 + t] : (cc == t ? 1.0 : 0.0);
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (t == j) {
+#pragma unroll
+      for (int cc = 0; cc < kNB; ++cc) D[j][cc] = d[cc];
+    }
     __syncthreads();
-    const double d = D[j][j];
+    const double pv = D[j][j];
     if (t > j && t < kb) {
-      double l = D[t][j];
-      if (fabs(l) > fabs(d) || d == 0.0) s_flag = 1;
-      if (d != 0.0) l = l / d;
-      D[t][j] = l;
-#pragma unroll 8
-      for (int cc = j + 1; cc < kb; ++cc) D[t][cc] = D[t][cc] - l * D[j][cc];
+      double l = d[j];
+      bad |= fabs(l) > fabs(pv) || pv == 0.0;
+      if (pv != 0.0) l = l / pv;
+      d[j] = l;
+#pragma unroll
+      for (int cc = j + 1; cc < kNB; ++cc) d[cc] = d[cc] - l * D[j][cc];
     }
   }
-  __syncthreads();
   if (blockIdx.x == 0) {
     for (int q = t; q < kb * kb; q += 64) {
       const int r = q % kb, cc = q / kb;
@@ -644,20 +659,27 @@ __global__ __launch_bounds__(64) void k_b_panel_np(const BCase* __restrict__ cas
     }
     if (t < kb) piv[c.ring_off + k0 + t] = k0 + t;
   }
-  if (row < n) {
-    bool bad = false;
-    for (int j = 0; j < kb; ++j) {
-      const double d = D[j][j];
-      double xj = X[j][t];
-      bad |= fabs(xj) > fabs(d);
-      if (d != 0.0) xj = xj / d;
-      X[j][t] = xj;
-#pragma unroll 8
-      for (int cc = j + 1; cc < kb; ++cc) X[cc][t] = X[cc][t] - xj * D[j][cc];
-    }
-    if (bad) s_flag = 1;
-    for (int cc = 0; cc < kb; ++cc) A[(long)(k0 + cc) * n + row] = X[cc][t];
+  const double* Db = &D[0][0];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    // row offset through an empty asm that takes x[j - 1]: row j's reads are
+    // issued once step j - 1 starts, not hoisted all to the top
+    int o = j * (kNB + 1);
+    asm volatile("" : "+s"(o) : "v"(x[j > 0 ? j - 1 : 0]));
+    const double pv = Db[o + j];
+    double xj = x[j];
+    bad |= fabs(xj) > fabs(pv);
+    if (pv != 0.0) xj = xj / pv;
+    x[j] = xj;
+#pragma unroll
+    for (int cc = j + 1; cc < kNB; ++cc) x[cc] = x[cc] - xj * Db[o + cc];
   }
+  if (row < n) {
+#pragma unroll
+    for (int cc = 0; cc < kNB; ++cc)
+      if (cc < kb) A[(long)(k0 + cc) * n + row] = x[cc];
+  }
+  if (bad) s_flag = 1;
   __syncthreads();
   if (t == 0 && s_flag) atomicOr(&flag[blockIdx.y], 1);
 }
