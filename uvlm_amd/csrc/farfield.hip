@@ -47,11 +47,14 @@ __host__ __device__ inline int ff_id(const FFTree& t, int l, int n, int bi, int 
 // Level-0 block shapes: nodes and segments of a full block.
 constexpr int kL0Nodes = (kFFRows + 1) * (kFFCols + 1);
 constexpr int kL0Segs = (kFFRows + 1) * kFFCols + (kFFCols + 1) * kFFRows;
-// Near-sum record of a level-0 block: its nodes (row-major, nc + 1 per row)
-// and its segment strengths / 4pi (spanwise segments by node row, then
-// chordwise by node column), contiguous so one wavefront loads it coalesced.
+// Near-sum record of a level-0 block: its nodes (row-major, nc + 1 per row),
+// its segment strengths / 4pi (spanwise segments by node row, then
+// chordwise by node column) and, for the FP32 walk, each segment's vector
+// and squared length rounded to FP32 (4 floats per segment, same order);
+// contiguous so one wavefront loads it coalesced.
 constexpr int kRecStr = 3 * kL0Nodes;
-constexpr int kRec = kRecStr + kL0Segs;
+constexpr int kRecSeg = kRecStr + kL0Segs;  // FP32 segment vectors (as floats)
+constexpr int kRec = kRecSeg + 2 * kL0Segs;
 
 // Bounding box of n points x[n][3]: box[0..2] min, box[3..5] max (one
 // workgroup).
@@ -235,6 +238,14 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_level0(FFTree t,
     sE[3 * s] = pb[0] - pa[0];
     sE[3 * s + 1] = pb[1] - pa[1];
     sE[3 * s + 2] = pb[2] - pa[2];
+    {
+      float* F = reinterpret_cast<float*>(R + kRecSeg) + 4 * s;
+      const float ex = (float)sE[3 * s], ey = (float)sE[3 * s + 1], ez = (float)sE[3 * s + 2];
+      F[0] = ex;
+      F[1] = ey;
+      F[2] = ez;
+      F[3] = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+    }
     sg[s] = ga - gb;
   }
   __syncthreads();
@@ -532,8 +543,9 @@ constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 // re-evaluated in FP64 exactly as the reference does.
 constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
 __device__ __forceinline__ void chain32(const double* R, int n0, int stride_nodes, int count,
-                                        const double* S, int s0, double x, double y, double z,
-                                        double& ux, double& uy, double& uz) {
+                                        const double* S, const float* F, int s0, double x,
+                                        double y, double z, double& ux, double& uy,
+                                        double& uz) {
   const double* p = R + 3 * n0;
   double pax = p[0], pay = p[1], paz = p[2];
   float rax = (float)(x - pax), ray = (float)(y - pay), raz = (float)(z - paz);
@@ -549,7 +561,8 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
     const float rbx = (float)(x - bx), rby = (float)(y - by), rbz = (float)(z - bz);
     const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
     const float ib = __builtin_amdgcn_rsqf(rbs);
-    const float r0x = (float)(bx - pax), r0y = (float)(by - pay), r0z = (float)(bz - paz);
+    const float* f = F + 4 * (s0 + k - 1);  // (float)(b - a) and its squared length
+    const float r0x = f[0], r0y = f[1], r0z = f[2];
     const float dx = fmaf(r0y, raz, -r0z * ray);
     const float dy = fmaf(r0z, rax, -r0x * raz);
     const float dz = fmaf(r0x, ray, -r0y * rax);
@@ -557,8 +570,7 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
     const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
     const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
     const float tt = fmaf(oa, ia, -ob * ib);
-    const float r0s = fmaf(r0z, r0z, fmaf(r0y, r0y, r0x * r0x));
-    const float derr2 = 64.f * kEps32 * kEps32 * r0s * ras;
+    const float derr2 = 64.f * kEps32 * kEps32 * f[3] * ras;
     const bool nearcore = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
     float K = tt * (float)g * __builtin_amdgcn_rcpf(dsq);
     K = nearcore ? 0.f : K;
@@ -593,11 +605,12 @@ __device__ __forceinline__ void walk_record32(const double* R, int nr, int nc, d
                                               double py, double pz, double& ux, double& uy,
                                               double& uz) {
   const double* S = R + kRecStr;
+  const float* F = reinterpret_cast<const float*>(R + kRecSeg);
   for (int i = 0; i <= nr; ++i)  // node rows: spanwise segments
-    chain32(R, i * (nc + 1), 1, nc, S, i * nc, px, py, pz, ux, uy, uz);
+    chain32(R, i * (nc + 1), 1, nc, S, F, i * nc, px, py, pz, ux, uy, uz);
   const int s1 = (nr + 1) * nc;
   for (int j = 0; j <= nc; ++j)  // node columns: chordwise segments
-    chain32(R, j, nc + 1, nr, S, s1 + j * nr, px, py, pz, ux, uy, uz);
+    chain32(R, j, nc + 1, nr, S, F, s1 + j * nr, px, py, pz, ux, uy, uz);
 }
 
 // Exact velocity (with 1/4pi) at one target of the level-0 blocks under
@@ -642,7 +655,8 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
                                                 const int2* __restrict__ cnt,
                                                 const int* __restrict__ order,
                                                 double* __restrict__ part) {
-  constexpr int kPer = (kRec + 63) / 64;
+  constexpr int kLen = F32 ? kRec : kRecSeg;  // the FP64 walk needs no FP32 vectors
+  constexpr int kPer = (kLen + 63) / 64;
   __shared__ double sr[kPer * 64];
   const int tile = order[blockIdx.x], g = blockIdx.y, lane = threadIdx.x;
   const int tg = tile_target(map, tile, lane);
@@ -657,7 +671,7 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int k = lane + 64 * u;
-      pre[u] = k < kRec ? rec[(long)b * kRec + k] : 0.0;
+      pre[u] = k < kLen ? rec[(long)b * kRec + k] : 0.0;
     }
   };
   if (g < count) load(id);
