@@ -232,7 +232,8 @@ void SimpleSimulator::WriteLoads() const {
   for (std::size_t n = 0; n < num(); n++) ofs << "\tCD" << n << "\tCL" << n;
   ofs << "\n";
   const auto h = load_history();
-  ofs.precision(17);
+  // values in the stream's default format, as the reference writes them
+  // (util::join over a std::stringstream, simple_simulator.cpp:231-232)
   for (std::size_t s = 0; s < load_t_.size(); ++s) {
     ofs << load_t_[s];
     for (std::size_t n = 0; n < num(); n++)
