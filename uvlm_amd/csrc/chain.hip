@@ -134,6 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
     for (int k0 = 0; k0 < cnt; k0 += 16) {
       float fx = 0.f, fy = 0.f, fz = 0.f;
       const int kend = min(cnt, k0 + 16);
+      // run start: previous node for the FP64 re-evaluation pass below
+      const double qx = pax, qy = pay, qz = paz;
+      unsigned redo = 0;  // elements of the run to re-evaluate (bit k - k0)
       for (int k = k0; k < kend; ++k) {
         const double* e = tl + 8 * k;
         const float rbx = (float)(x - e[0]), rby = (float)(y - e[1]), rbz = (float)(z - e[2]);
@@ -156,10 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         fx = fmaf(dx, K, fx);
         fy = fmaf(dy, K, fy);
         fz = fmaf(dz, K, fz);
-        if (__any(near)) {
-          if (near && e[3] != 0.0)
-            dev::seg_ref(pax, pay, paz, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
-        }
+        redo |= (unsigned)(near && e[3] != 0.0) << (k - k0);
         pax = e[0];
         pay = e[1];
         paz = e[2];
@@ -168,6 +168,17 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         raz = rbz;
         ras = rbs;
         ia = ib;
+      }
+      if (__any(redo != 0)) {  // rare: pairs near the cut-off again, in FP64
+        double ax = qx, ay = qy, az = qz;
+        for (int k = k0; k < kend; ++k) {
+          const double* e = tl + 8 * k;
+          if ((redo >> (k - k0)) & 1u)
+            dev::seg_ref(ax, ay, az, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
+          ax = e[0];
+          ay = e[1];
+          az = e[2];
+        }
       }
       ux += (double)fx;
       uy += (double)fy;
