@@ -443,7 +443,7 @@ int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
     TRY(ensure(c, c->t_recs, sizeof(ChainRec) * ne));
     launch_pack_chains(c->stream, P<double>(c->t_b), (int)rows + 1, 0, P<double>(c->t_c),
                        (int)rows, 0, (int)num, (int)rows, (int)cols,
-                       P<ChainRec>(c->t_recs));
+                       P<ChainRec>(c->t_recs), prec_of(flags) == kF32C);
     plan = plan_chunks((int)nt, 0, ne, (flags & UVLM_SEQUENTIAL) != 0);
     TRY(ensure(c, c->t_partial,
                sizeof(double) * 3 * (size_t)std::max(1, plan.total()) * nt));
@@ -773,7 +773,8 @@ void pack_wake(uvlm_ctx* c, const double* nodes, int sheet_rows, int row_off) {
   else
     launch_pack_chains(c->stream, nodes, sheet_rows, row_off, P<double>(c->wgam),
                        c->capg, c->gstart, c->num, c->wgrows, c->C,
-                       P<ChainRec>(c->chains) + chain_elems(c->num, c->R, c->C));
+                       P<ChainRec>(c->chains) + chain_elems(c->num, c->R, c->C),
+                       prec_of(c->flags) == kF32C);
 }
 }  // namespace
 
@@ -867,7 +868,8 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   launch_set_gamma(c->stream, P<double>(c->gamma), N, P<RingRec>(c->recs));
   if (prec_of(c->flags) != kF64Exact)
     launch_pack_chains(c->stream, P<double>(c->wing), c->R + 1, 0, P<double>(c->gamma), c->R,
-                       0, c->num, c->R, c->C, P<ChainRec>(c->chains));
+                       0, c->num, c->R, c->C, P<ChainRec>(c->chains),
+                       prec_of(c->flags) == kF32C);
   TRY(check_launch(c));
   if (gamma_out) {
     int info = 0;

@@ -138,11 +138,12 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
       const double qx = pax, qy = pay, qz = paz;
       unsigned redo = 0;  // elements of the run to re-evaluate (bit k - k0)
       for (int k = k0; k < kend; ++k) {
-        const double* e = tl + 8 * k;
+        const double* e = tl + 8 * k;  // ChainRec32
+        const float* ef = reinterpret_cast<const float*>(e + 4);
         const float rbx = (float)(x - e[0]), rby = (float)(y - e[1]), rbz = (float)(z - e[2]);
         const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
         const float ib = __builtin_amdgcn_rsqf(rbs);
-        const float r0x = (float)e[4], r0y = (float)e[5], r0z = (float)e[6];
+        const float r0x = ef[0], r0y = ef[1], r0z = ef[2];
         const float dx = fmaf(r0y, raz, -r0z * ray);
         const float dy = fmaf(r0z, rax, -r0x * raz);
         const float dz = fmaf(r0x, ray, -r0y * rax);
@@ -152,9 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         const float tt = fmaf(oa, ia, -ob * ib);
         // Near-core band: |d| < sqrt(10) * 1e-5 plus the FP32 rounding bound
         // of d (~5 eps |r0| |x-a|), in squares: (a+b)^2 <= 2a^2 + 2b^2.
-        const float derr2 = 64.f * kEps32 * kEps32 * (float)e[7] * ras;
+        const float derr2 = 64.f * kEps32 * kEps32 * ef[3] * ras;
         const bool near = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
-        float K = tt * (float)e[3] * __builtin_amdgcn_rcpf(dsq);
+        float K = tt * ef[4] * __builtin_amdgcn_rcpf(dsq);
         K = near ? 0.f : K;
         fx = fmaf(dx, K, fx);
         fy = fmaf(dy, K, fy);
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
 __global__ void k_pack_chains(const double* __restrict__ nodes, int sheet_rows,
                               int row_off, const double* __restrict__ gamma,
                               int g_sheet_rows, int g_row_off, int num, int rows,
-                              int cols, ChainRec* __restrict__ out) {
+                              int cols, ChainRec* __restrict__ out, bool f32) {
   const int per = (rows + 1) * (cols + 1);
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= 2 * num * per) return;
@@ -252,19 +253,34 @@ __global__ void k_pack_chains(const double* __restrict__ nodes, int sheet_rows,
   }
   e.r0sq = (e.r0[0] * e.r0[0] + e.r0[1] * e.r0[1]) + e.r0[2] * e.r0[2];
   e.g = g * kInv4Pi;
-  out[q] = e;
+  if (!f32) {
+    out[q] = e;
+    return;
+  }
+  ChainRec32 f;
+  f.p[0] = e.p[0];
+  f.p[1] = e.p[1];
+  f.p[2] = e.p[2];
+  f.g = e.g;
+  f.r0[0] = (float)e.r0[0];
+  f.r0[1] = (float)e.r0[1];
+  f.r0[2] = (float)e.r0[2];
+  f.r0sq = (float)e.r0sq;
+  f.gf = (float)e.g;
+  f.pad[0] = f.pad[1] = f.pad[2] = 0.f;
+  reinterpret_cast<ChainRec32*>(out)[q] = f;
 }
 
 }  // namespace
 
 void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows, int row_off,
                         const double* gamma, int g_sheet_rows, int g_row_off, int num,
-                        int rows, int cols, ChainRec* out) {
+                        int rows, int cols, ChainRec* out, bool f32) {
   const int total = chain_elems(num, rows, cols);
   if (total <= 0) return;
   hipLaunchKernelGGL(k_pack_chains, dim3(cdiv(total, 256)), dim3(256), 0, s, nodes,
                      sheet_rows, row_off, gamma, g_sheet_rows, g_row_off, num, rows, cols,
-                     out);
+                     out, f32);
 }
 
 int chain_slots(int prec) {

@@ -46,6 +46,18 @@ struct alignas(16) ChainRec {
   double r0sq;
 };
 static_assert(sizeof(ChainRec) == 64, "ChainRec must be 64 bytes");
+// The same element for the F32C kernel: the node and g in FP64 (for the
+// FP64 x - p and the near-core re-evaluation), r0, |r0|^2 and g rounded to
+// FP32 once when packed rather than in every (target, element) pair.
+struct alignas(16) ChainRec32 {
+  double p[3];
+  double g;
+  float r0[3];
+  float r0sq;
+  float gf;
+  float pad[3];
+};
+static_assert(sizeof(ChainRec32) == 64, "ChainRec32 must be 64 bytes");
 
 // Element count of the chains of a lattice with `rows` ring rows.
 inline int chain_elems(int num, int rows, int cols) {
@@ -100,7 +112,7 @@ void launch_set_gamma(hipStream_t s, const double* gamma, int n, RingRec* out);
 void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows,
                         int row_off, const double* gamma, int g_sheet_rows,
                         int g_row_off, int num, int rows, int cols,
-                        ChainRec* out);
+                        ChainRec* out, bool f32 = false);
 // Chain N-body (prec = kF64Fast or kF32C); same partial-sum layout and chunk
 // plan semantics as launch_nbody, in units of chain elements.
 void launch_nbody_chain(hipStream_t s, int prec, const double* targets,
