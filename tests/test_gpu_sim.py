@@ -179,3 +179,48 @@ def test_restore_state_continues_identically(orc):
     a.sync()
     b.sync()
     np.testing.assert_array_equal(a.gamma(), b.gamma())
+
+
+FULL_SIZE = [("naca4412_20x80_rk4", W.naca4412_c2, 10),
+             ("tandem_2x16x64_euler", W.tandem_c5, 8),
+             ("morphing_32x128_euler", W.morphing_c3, 5)]
+
+
+@pytest.mark.parametrize("name,make,steps", FULL_SIZE)
+def test_full_size_fast_path_vs_reference_order(name, make, steps):
+    """At the workloads' full lattice sizes (1600, 2048 and 4096 bound
+    rings), the benchmark's method (FP64 segment chains, kept-inverse
+    solve) against the reference-order kernels (F64_EXACT ring sums,
+    FullPivLU elimination order; bitwise the oracle at small sizes):
+    circulations, wake and loads agree to 1e-10 over the first steps,
+    before the wake amplifies rounding.  Shedding keeps the Kutta
+    condition bitwise (the newest wake row carries the previous step's
+    trailing-edge circulations), and a repeated run is bitwise identical
+    (fixed reduction orders, no atomics in the sums)."""
+    case = make(steps=steps)
+    fast = dict(case, precision=N.PREC_F64, solve_mode=N.SOLVE_F64_INV)
+    ref = dict(case, precision=N.PREC_F64_EXACT, solve_mode=N.SOLVE_F64_FULLPIV)
+    a, b, r = Simulation(fast, steps), Simulation(fast, steps), Simulation(ref, steps)
+    num, R, C = len(case["wings"]), case["rows"], case["cols"]
+    prev_te = None
+    for s in range(steps + 1):
+        print(f"[{name}] step {s}", flush=True)
+        for sim in (a, b, r):
+            sim.step(s)
+        a.sync()
+        g = a.gamma().reshape(num, R, C)
+        if prev_te is not None:
+            _, wg = a.wake()
+            np.testing.assert_array_equal(wg[:, 0, :], prev_te)
+        prev_te = g[:, R - 1, :].copy()
+    for sim in (b, r):
+        sim.sync()
+    np.testing.assert_array_equal(a.gamma(), b.gamma())
+    np.testing.assert_array_equal(a.wake()[0], b.wake()[0])
+    np.testing.assert_array_equal(a.loads(), b.loads())
+    assert rel_err(a.gamma(), r.gamma()) < TOL
+    assert rel_err(a.wake()[0], r.wake()[0]) < TOL
+    assert rel_err(a.wake()[1], r.wake()[1]) < TOL
+    la, lr = a.loads(), r.loads()
+    assert rel_err(la[..., 0], lr[..., 0]) < TOL
+    assert rel_err(la[..., 1], lr[..., 1]) < TOL
