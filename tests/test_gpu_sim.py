@@ -224,3 +224,24 @@ def test_full_size_fast_path_vs_reference_order(name, make, steps):
     la, lr = a.loads(), r.loads()
     assert rel_err(la[..., 0], lr[..., 0]) < TOL
     assert rel_err(la[..., 1], lr[..., 1]) < TOL
+
+
+def test_full_size_f32c_loads():
+    """F32C chains at the NACA4412 workload's full size (20x80 rings, RK4)
+    against the FP64 chains: CL/CD and circulations within 1e-5 of the
+    largest magnitude over the first 10 steps (the precision's stated bound
+    is 1e-4 on CL/CT)."""
+    steps = 10
+    case = W.naca4412_c2(steps=steps)
+    a = Simulation(dict(case, precision=N.PREC_F64, solve_mode=N.SOLVE_F64_INV), steps)
+    b = Simulation(dict(case, precision=N.PREC_F32C, solve_mode=N.SOLVE_F64_INV), steps)
+    for s in range(steps + 1):
+        a.step(s)
+        b.step(s)
+    a.sync()
+    b.sync()
+    la, lb = a.loads(), b.loads()
+    err = (rel_err(lb[..., 0], la[..., 0]), rel_err(lb[..., 1], la[..., 1]),
+           rel_err(b.gamma(), a.gamma()))
+    print(f"F32C vs FP64, CD, CL, gamma: {err}")
+    assert max(err) < 1e-5 and err[2] > 0.0
