@@ -78,35 +78,38 @@ __global__ void k_scaled_transpose(const double* A, int n, double alpha, double*
 
 // out[0] = max_j sum_i |A_ij| (1-norm), out[1] = max_i sum_j |A_ij| (inf-norm),
 // one workgroup (run once per inverse build).
-__global__ __launch_bounds__(1024) void k_norms1inf(const double* A, int n, double* out) {
-  __shared__ double s1[16], s2[16];
-  double m1 = 0.0, m2 = 0.0;
-  for (int c = threadIdx.x; c < n; c += 1024) {
-    double cs = 0.0, rs = 0.0;
-    for (int r = 0; r < n; ++r) {
-      cs += fabs(A[(long)c * n + r]);
-      rs += fabs(A[(long)r * n + c]);
-    }
-    m1 = fmax(m1, cs);
-    m2 = fmax(m2, rs);
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    m1 = fmax(m1, __shfl_xor(m1, off));
-    m2 = fmax(m2, __shfl_xor(m2, off));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    s1[threadIdx.x >> 6] = m1;
-    s2[threadIdx.x >> 6] = m2;
-  }
+// |A|_1 and |A|_inf of a column-major A: out[0], out[1] hold the bit
+// patterns of the maxima (non-negative doubles order like their bits; an
+// atomic max is order-independent, so the result is deterministic).
+__global__ void k_norms_zero(unsigned long long* out) {
+  out[0] = 0ull;
+  out[1] = 0ull;
+}
+
+// One workgroup per column: the column's absolute sum (fixed tree order).
+__global__ __launch_bounds__(256) void k_colsum_max(const double* __restrict__ A, int n,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ double sw[4];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int r = threadIdx.x; r < n; r += 256) acc += fabs(A[(long)c * n + r]);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 16; ++w) {
-      m1 = fmax(m1, s1[w]);
-      m2 = fmax(m2, s2[w]);
-    }
-    out[0] = m1;
-    out[1] = m2;
-  }
+  if (threadIdx.x == 0)
+    atomicMax(out, (unsigned long long)__double_as_longlong((sw[0] + sw[1]) + (sw[2] + sw[3])));
+}
+
+// One thread per row: the row's absolute sum over the columns in order.
+__global__ __launch_bounds__(256) void k_rowsum_max(const double* __restrict__ A, int n,
+                                                    unsigned long long* __restrict__ out) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  double acc = 0.0;
+  if (r < n)
+    for (int c = 0; c < n; ++c) acc += fabs(A[(long)c * n + r]);
+  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(out + 1, (unsigned long long)__double_as_longlong(acc));
 }
 
 // part[s][i] = sum_{j in column slice s} M[j n + i] v[j] (column-major M),
@@ -334,7 +337,10 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A) {
 int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
   const int n = n_;
   double* nrm = reinterpret_cast<double*>(stat_);  // words 0 and 1
-  hipLaunchKernelGGL(k_norms1inf, dim3(1), dim3(1024), 0, s, A, n, nrm);
+  unsigned long long* nb = reinterpret_cast<unsigned long long*>(stat_);
+  hipLaunchKernelGGL(k_norms_zero, dim3(1), dim3(1), 0, s, nb);
+  hipLaunchKernelGGL(k_colsum_max, dim3(n), dim3(256), 0, s, A, n, nb);
+  hipLaunchKernelGGL(k_rowsum_max, dim3(cdiv(n, 256)), dim3(256), 0, s, A, n, nb);
   double h[2] = {0, 0};
   if (!read_back(s, h, nrm, sizeof(h), "matrix norms", err)) return UVLM_ERR_HIP;
   if (!(h[0] > 0.0) || !(h[1] > 0.0)) {
