@@ -339,8 +339,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the UVLM step of a case "
-              "on a single MI355X (sharding over GPUs is a design note, not a measured "
+        print("bench.py measures one GPU only: the UVLM step of a case runs on a "
+              "single MI355X (sharding over GPUs is a design note, not a measured "
               "configuration)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:

@@ -6,8 +6,8 @@ plunge 0.1c at k = 0.5, dt = 1/8, Euler, steps 0..200) from the oracle.
 The reference cannot be built here (Eigen/glog/gflags/protobuf and its
 multiple_sheet submodule are absent), so these vectors come from the oracle,
 which is pinned separately by the reference's known-answer tests and by
-closed forms; the fixture freezes it so that the GPU tests on a box without
-the oracle's run time still compare against the full 200-step history.
+closed forms; the fixture freezes it so that the GPU tests, without the
+oracle's run time, still compare against the full 200-step history.
 """
 import os
 import sys

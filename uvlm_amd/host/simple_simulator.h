@@ -49,8 +49,8 @@ class SimpleSimulator {
   void Run(const std::size_t steps, const double dt);
 
   void set_forward_flight(const Vector3d& v) { forward_flight_ = v; }
-  // Snapshot2 / AllVortexSheets recordio output is not part of this build:
-  // the paths are recorded and nothing is written.
+  // Snapshot2 / AllVortexSheets recordio output (protobuf) is not written
+  // by this library: the paths are recorded and nothing is written.
   void set_result_path(const std::string& path) { result_path_ = path; }
   void set_sheet_path(const std::string& path) { sheet_path_ = path; }
   // Load TSV "t CD0 CL0 CD1 CL1 ..." (simple_simulator.cpp:229-232,334-345).

@@ -1,7 +1,7 @@
 // vector3d.h -- UVLM::Vector3d, the 3-double value type of the host API.
 //
-// The reference uses Eigen::Vector3d (3 packed doubles).  Eigen is not part
-// of this build, so the host API defines a 24-byte POD with the members the
+// The reference uses Eigen::Vector3d (3 packed doubles).  This library does
+// not depend on Eigen, so the host API defines a 24-byte POD with the members the
 // reference's hot path uses.  Arithmetic conventions: dot and squaredNorm are
 // (x*x' + y*y') + z*z'; normalize divides each component by the norm when the
 // squared norm is positive; scalar division divides each component.
