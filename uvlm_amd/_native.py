@@ -52,7 +52,7 @@ HOST_SYMBOLS = [
     "uvlm_host_sim_step", "uvlm_host_sim_sync", "uvlm_host_sim_wing_nodes",
     "uvlm_host_sim_gamma", "uvlm_host_sim_wake_shape", "uvlm_host_sim_wake",
     "uvlm_host_sim_set_wake", "uvlm_host_sim_loads", "uvlm_host_sim_pairs",
-    "uvlm_host_sim_context", "uvlm_host_build_wing",
+    "uvlm_host_sim_context", "uvlm_host_build_wing", "uvlm_host_legacy_run",
 ]
 
 
@@ -198,6 +198,8 @@ def host_lib():
         L.uvlm_host_sim_pairs.argtypes = [_vp, _dp, _dp]
         L.uvlm_host_sim_context.argtypes = [_vp]
         L.uvlm_host_build_wing.argtypes = [C.POINTER(WingSpec), _sz, _sz, _dp]
+        L.uvlm_host_legacy_run.argtypes = [C.POINTER(CaseSpec), _sz, _dp, _dp, _dp, _dp,
+                                           C.POINTER(_sz)]
         L.uvlm_host_sim_context.restype = _vp
         _host = L
     return _host

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/uvlm_host.h"
+#include "legacy_simulator.h"
 #include "simple_simulator.h"
 
 using UVLM::Morphing;
@@ -193,5 +194,56 @@ int uvlm_host_sim_pairs(uvlm_host_sim* s, double* evaluated, double* reference) 
 }
 
 uvlm_ctx* uvlm_host_sim_context(uvlm_host_sim* s) { return s ? s->sim.context() : nullptr; }
+
+
+int uvlm_host_legacy_run(const uvlm_case_spec* sp, size_t steps, double* bound_nodes,
+                         double* bound_gamma, double* wake_nodes, double* wake_gamma,
+                         size_t* nw_inout) {
+  namespace sim = UVLM::simulator;
+  if (!sp) return err("null argument");
+  if (sp->num_wings < 1 || sp->num_wings > UVLM_HOST_MAX_WINGS) return err("num_wings out of range");
+  if (sp->cols & 1) return err("columns must be even.");
+  if (sp->rows == 0 || sp->cols == 0) return err("empty lattice");
+  if ((wake_nodes || wake_gamma) && !nw_inout) return err("wake capacity missing");
+  sim::ResetSimulator();
+  for (int n = 0; n < sp->num_wings; ++n) {
+    const uvlm_wing_spec& w = sp->wings[n];
+    std::unique_ptr<UVLM::wing::WingGenerator> gen;
+    if (w.generator == UVLM_GEN_NACA4)
+      gen.reset(new UVLM::wing::NACA4digitGenerator(w.digit));
+    else if (w.generator == UVLM_GEN_NACA00)
+      gen.reset(new UVLM::wing::NACA00XXGenerator(w.digit));
+    else
+      gen.reset(new UVLM::wing::RectGenerator());
+    UVLM::proto::Wing half;
+    gen->Generate(&half, w.chord, w.span / 2, sp->rows, sp->cols / 2);
+    half.mutable_origin()->set_x(w.origin[0]);
+    half.mutable_origin()->set_y(w.origin[1]);
+    half.mutable_origin()->set_z(w.origin[2]);
+    sim::AddWing(half, make_morphing(w));
+  }
+  sim::SetInlet(-sp->ff[0], -sp->ff[1], -sp->ff[2]);
+  sim::Start(steps, sp->dt);
+  const auto& v = sim::Vortices();
+  const std::size_t nb = (std::size_t)sp->num_wings * sp->rows * sp->cols;
+  if (v.size() < nb) return err("legacy loop lost its bound rings");
+  const std::size_t nw = v.size() - nb;
+  if (nw_inout) {
+    if ((wake_nodes || wake_gamma) && *nw_inout < nw) return err("wake capacity too small");
+    *nw_inout = nw;
+  }
+  auto put = [](const UVLM::VortexRing& r, double* nodes, double* g, std::size_t k) {
+    if (nodes)
+      for (int q = 0; q < 4; ++q) {
+        nodes[12 * k + 3 * q + 0] = r.nodes()[q].x();
+        nodes[12 * k + 3 * q + 1] = r.nodes()[q].y();
+        nodes[12 * k + 3 * q + 2] = r.nodes()[q].z();
+      }
+    if (g) g[k] = r.gamma();
+  };
+  for (std::size_t k = 0; k < nb; ++k) put(v[k], bound_nodes, bound_gamma, k);
+  for (std::size_t k = 0; k < nw; ++k) put(v[nb + k], wake_nodes, wake_gamma, k);
+  return UVLM_OK;
+}
 
 }  // extern "C"
