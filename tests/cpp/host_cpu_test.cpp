@@ -1,13 +1,15 @@
 // host_cpu_test.cpp -- host-API tests that need no GPU: containers, the
 // kinematics and the wing generators (bitwise against the oracle), and the
 // reference's container tests (vortex_container_test.cpp,
-// uvlm_vortex_ring_test.cpp, proto_adaptor_test.cpp index order).
+// uvlm_vortex_ring_test.cpp, wing_builder_test.cpp, proto_adaptor_test.cpp
+// index order).
 #include <cmath>
 #include <memory>
 
 #include "../../oracle/uvlm_oracle.h"
 #include "../../uvlm_amd/host/advect.h"
 #include "../../uvlm_amd/host/legacy.h"
+#include "../../uvlm_amd/host/legacy_simulator.h"
 #include "../../uvlm_amd/host/morphing.h"
 #include "../../uvlm_amd/host/multiple_sheet.h"
 #include "../../uvlm_amd/host/wing.h"
@@ -254,6 +256,70 @@ TEST(uvlm_vortex_ring_init_wing) {
     }
   const auto te = rings.TrailingEdgeRange();
   EXPECT_EQ(te.second - te.first, 2 * cols);
+}
+
+// wing_builder_test.cpp: a half wing of 2 x 3 rings with node (i, j) at
+// (i, j, 0) becomes 2 x 6 rings; TransformRow mirrors in reverse order.
+namespace {
+proto::Wing half_wing_2x3(double oz) {
+  proto::Wing w;
+  w.set_rows(2);
+  w.set_cols(3);
+  for (int i = 0; i <= 2; i++)
+    for (int j = 0; j <= 3; j++) {
+      auto* p = w.add_points();
+      p->set_x(i);
+      p->set_y(j);
+    }
+  w.mutable_origin()->set_z(oz);
+  return w;
+}
+}  // namespace
+
+TEST(wing_builder_transform_row) {
+  const std::vector<Vector3d> row{{1, 0, 2}, {1, 1, 2}, {1, 2, 2}};
+  const auto r = internal::TransformRow(row.cbegin(), row.cend());
+  EXPECT_EQ(r.size(), 5u);
+  const double ys[5] = {-2, -1, 0, 1, 2};
+  for (int k = 0; k < 5; k++) {
+    EXPECT_DOUBLE_EQ(ys[k], r[k].y());
+    EXPECT_DOUBLE_EQ(1, r[k].x());
+    EXPECT_DOUBLE_EQ(2, r[k].z());
+  }
+  std::vector<internal::WingHolder> h(3);
+  for (std::size_t k = 0; k < 3; k++) h[k].rows = h[k].cols = k + 1;
+  EXPECT_EQ(WingBuilder::CountTotalSize(h), 14u);
+}
+
+TEST(wing_builder_build) {
+  auto vortices = std::make_shared<std::vector<VortexRing>>();
+  std::vector<VortexContainer> containers;
+  WingBuilder builder(&containers, vortices);
+  EXPECT_EQ(vortices->size(), 0u);
+  builder.AddWing(half_wing_2x3(0)).AddWing(half_wing_2x3(1));
+  const auto& h = builder.holders();
+  EXPECT_EQ(h.size(), 2u);
+  EXPECT_EQ(h[1].rows, 2u);
+  EXPECT_EQ(h[1].cols, 6u);
+  EXPECT_DOUBLE_EQ(0, h[1].points[0].x());
+  EXPECT_DOUBLE_EQ(-3, h[1].points[0].y());
+  for (std::size_t k : {0u, 1u, 6u}) EXPECT_DOUBLE_EQ(1, h[1].points[k].z());
+  EXPECT_DOUBLE_EQ(0, h[0].points[0].z());
+  builder.Build();
+  EXPECT_EQ(vortices->size(), 24u);
+  EXPECT_EQ(containers.size(), 2u);
+  EXPECT_EQ(containers[0].rows(), 2u);
+  EXPECT_EQ(containers[0].cols(), 6u);
+  const VortexRing& v = containers[0].at(0, 0);
+  EXPECT_EQ(v.nodes()[0], Vector3d(0, -3, 0));
+  EXPECT_EQ(v.nodes0()[0], Vector3d(0, -3, 0));
+  // ring (1, 5) of wing 1: nodes (1,5), (2,5), (2,6), (1,6) of the 7-column lattice
+  const VortexRing& w = containers[1].at(1, 5);
+  EXPECT_EQ(w.nodes()[0], Vector3d(1, 2, 1));
+  EXPECT_EQ(w.nodes()[1], Vector3d(2, 2, 1));
+  EXPECT_EQ(w.nodes()[2], Vector3d(2, 3, 1));
+  EXPECT_EQ(w.nodes()[3], Vector3d(1, 3, 1));
+  EXPECT_EQ(&(*vortices)[12 + 6 + 5], &w);  // id R C + j + i C
 }
 
 int main(int argc, char** argv) { return check::run(argc, argv); }
