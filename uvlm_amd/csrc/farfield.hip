@@ -411,14 +411,27 @@ __device__ inline double4 focus_sphere(const double* box) {
   return double4{box[0] + hx, box[1] + hy, box[2] + hz, sqrt(hx * hx + hy * hy + hz * hz)};
 }
 
+// A cluster is far when every target of the tile lies beyond theta times
+// its radius: the tile's sphere decides when it clears the cluster or lies
+// inside its reach, the targets one by one otherwise.
 __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __restrict__ cl,
-                                                 const double4* __restrict__ tiles, double theta,
+                                                 const double4* __restrict__ tiles,
+                                                 const double* __restrict__ x,
+                                                 const int* __restrict__ map, double theta,
                                                  const double* __restrict__ fbox, double reach,
                                                  double theta_min, int* __restrict__ near,
                                                  int* __restrict__ far, int2* __restrict__ cnt,
                                                  int* __restrict__ front) {
   const int tile = blockIdx.x, lane = threadIdx.x;
   const double4 tc = tiles[tile];
+  __shared__ double tx[kTileT], ty[kTileT], tz[kTileT];
+  {  // the tile's targets (lane 0's stands in for the missing ones of the last tile)
+    int tg = tile_target(map, tile, lane);
+    if (tg < 0) tg = tile_target(map, tile, 0);
+    tx[lane] = x[3 * (long)tg];
+    ty[lane] = x[3 * (long)tg + 1];
+    tz[lane] = x[3 * (long)tg + 2];
+  }
   if (fbox) {  // graded criterion away from the focus
     const double4 f = focus_sphere(fbox);
     const double fx = tc.x - f.x, fy = tc.y - f.y, fz = tc.z - f.z;
@@ -446,7 +459,17 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
         const FFCluster c = cl[id];
         const double dx = c.c[0] - tc.x, dy = c.c[1] - tc.y, dz = c.c[2] - tc.z;
         const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        if (dist - tc.w > theta * c.r) {
+        const double reach_c = theta * c.r;
+        bool is_far = dist - tc.w > reach_c;
+        if (!is_far && dist + tc.w > reach_c) {
+          const double lim2 = reach_c * reach_c;
+          is_far = true;
+          for (int q = 0; q < kTileT && is_far; ++q) {
+            const double ex = c.c[0] - tx[q], ey = c.c[1] - ty[q], ez = c.c[2] - tz[q];
+            is_far = ex * ex + ey * ey + ez * ez > lim2;
+          }
+        }
+        if (is_far) {
           kind = 1;
         } else if (c.level == 0) {
           kind = 2;
@@ -1166,7 +1189,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (double4*)d_tiles_);
   const bool graded = focus && focus->box && focus->reach > 0.0;
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
-                     (const double4*)d_tiles_, theta, graded ? focus->box : nullptr,
+                     (const double4*)d_tiles_, targets, (const int*)map, theta,
+                     graded ? focus->box : nullptr,
                      graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
                      (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
   // heaviest tiles first (the Morton sort's buffers are free again)
