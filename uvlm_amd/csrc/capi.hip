@@ -821,7 +821,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
     TRY(ff_current(c));
     std::string e;
     const int rc = c->ff->velocity(c->stream, P<double>(c->cp), N, kWingThetaFactor * c->ff_theta,
-                                   P<double>(c->uw), &e);
+                                   P<double>(c->uw), &e, nullptr, true);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
   } else {
     ChunkPlan plan;
@@ -907,7 +907,8 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
     TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
     std::string e;
     const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E,
-                                   kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e);
+                                   kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e, nullptr,
+                                   true);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     launch_ff_combine(c->stream, P<double>(c->partial), E, plan.cb, P<double>(c->ffm), E, ff, 0,
                       0, P<double>(c->uflow));
@@ -993,7 +994,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
       if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
       c->ff_version = -1;
     }
-    rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e, &focus);
+    rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e, &focus, true);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     launch_ff_combine(c->stream, P<double>(c->partial), V, plan.cb, P<double>(c->ffv), V, ff,
                       te_period, C1, out);

@@ -55,6 +55,9 @@ constexpr int kL0Segs = (kFFRows + 1) * kFFCols + (kFFCols + 1) * kFFRows;
 constexpr int kRecStr = 3 * kL0Nodes;
 constexpr int kRecSeg = kRecStr + kL0Segs;  // FP32 segment vectors (as floats)
 constexpr int kRec = kRecSeg + 2 * kL0Segs;
+// 64 (2^-24)^2: the FP32 walk's bound on the rounding of |d|^2 is
+// kBand32 |b - a|^2 |x - a|^2; records hold kBand32 |b - a|^2 (exact scaling).
+constexpr float kBand32 = 0x1p-42f;
 
 // Bounding box of n points x[n][3]: box[0..2] min, box[3..5] max (one
 // workgroup).
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_level0(FFTree t,
       F[0] = ex;
       F[1] = ey;
       F[2] = ez;
-      F[3] = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+      F[3] = kBand32 * fmaf(ez, ez, fmaf(ey, ey, ex * ex));  // the walk's rounding band
     }
     sg[s] = ga - gb;
   }
@@ -413,11 +416,13 @@ __device__ inline double4 focus_sphere(const double* box) {
 
 // A cluster is far when every target of the tile lies beyond theta times
 // its radius: the tile's sphere decides when it clears the cluster or lies
-// inside its reach, the targets one by one otherwise.
+// inside its reach, the targets one by one otherwise (per_target; without
+// it, the sphere alone).
 __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __restrict__ cl,
                                                  const double4* __restrict__ tiles,
                                                  const double* __restrict__ x,
-                                                 const int* __restrict__ map, double theta,
+                                                 const int* __restrict__ map, bool per_target,
+                                                 double theta,
                                                  const double* __restrict__ fbox, double reach,
                                                  double theta_min, int* __restrict__ near,
                                                  int* __restrict__ far, int2* __restrict__ cnt,
@@ -461,7 +466,7 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
         const double dist = sqrt(dx * dx + dy * dy + dz * dz);
         const double reach_c = theta * c.r;
         bool is_far = dist - tc.w > reach_c;
-        if (!is_far && dist + tc.w > reach_c) {
+        if (per_target && !is_far && dist + tc.w > reach_c) {
           const double lim2 = reach_c * reach_c;
           is_far = true;
           for (int q = 0; q < kTileT && is_far; ++q) {
@@ -558,64 +563,79 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
 // shorter than 2 r: below this radius the end-point cut-offs are implied.
 constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 
+// End point of a segment seen from the target, in FP32: offsets, squared
+// distance, its reciprocal root, and whether it lies within the cut-off band.
+struct End32 {
+  float rx, ry, rz, rs, inv;
+  bool tiny;
+};
+
+__device__ __forceinline__ void end32(const double* b, double x, double y, double z, End32& e) {
+  e.rx = (float)(x - b[0]);
+  e.ry = (float)(y - b[1]);
+  e.rz = (float)(z - b[2]);
+  e.rs = fmaf(e.rz, e.rz, fmaf(e.ry, e.ry, e.rx * e.rx));
+  e.inv = __builtin_amdgcn_rsqf(e.rs);
+  e.tiny = e.rs < 1e-18f;
+}
+
+// One segment a -> b with strength g; f = (float)(b - a) and kBand32 |b - a|^2.
+// Returns whether the pair lies in the near-core band (added here as zero).
+__device__ __forceinline__ bool elem32(const End32& a, const End32& b, const float* f, double g,
+                                       float& fx, float& fy, float& fz) {
+  const float r0x = f[0], r0y = f[1], r0z = f[2];
+  const float dx = fmaf(r0y, a.rz, -r0z * a.ry);
+  const float dy = fmaf(r0z, a.rx, -r0x * a.rz);
+  const float dz = fmaf(r0x, a.ry, -r0y * a.rx);
+  const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float oa = fmaf(r0z, a.rz, fmaf(r0y, a.ry, r0x * a.rx));
+  const float ob = fmaf(r0z, b.rz, fmaf(r0y, b.ry, r0x * b.rx));
+  const float tt = fmaf(oa, a.inv, -ob * b.inv);
+  const bool nearcore = (dsq < fmaf(f[3], a.rs, 2e-9f)) | a.tiny | b.tiny;
+  float K = tt * (float)g * __builtin_amdgcn_rcpf(dsq);
+  K = nearcore ? 0.f : K;
+  fx = fmaf(dx, K, fx);
+  fy = fmaf(dy, K, fy);
+  fz = fmaf(dz, K, fz);
+  return nearcore;
+}
+
 // The same walk in FP32 with compensation: x - p in FP64 (exact to FP32
 // rounding relative to |x - p|), the segment law in FP32 with the hardware
 // rsqrt / reciprocal, each chain (<= 17 elements) summed in FP32 and added
 // to the FP64 sums; pairs within a band around the reference's cut-off
 // (|r| < 1e-9, |d|^2 < 2e-9 plus the FP32 rounding bound of d) are
-// re-evaluated in FP64 exactly as the reference does.
-constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
+// re-evaluated in FP64 exactly as the reference does.  Two elements per
+// trip, the end point alternating between two register sets.
 __device__ __forceinline__ void chain32(const double* R, int n0, int stride_nodes, int count,
                                         const double* S, const float* F, int s0, double x,
                                         double y, double z, double& ux, double& uy,
                                         double& uz) {
   const double* p = R + 3 * n0;
-  double pax = p[0], pay = p[1], paz = p[2];
-  float rax = (float)(x - pax), ray = (float)(y - pay), raz = (float)(z - paz);
-  float ras = fmaf(raz, raz, fmaf(ray, ray, rax * rax));
-  float ia = __builtin_amdgcn_rsqf(ras);
+  const int step = 3 * stride_nodes;
+  End32 A, B;
+  end32(p, x, y, z, A);
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  unsigned redo = 0;  // elements to re-evaluate in FP64 (bit k - 1)
+  unsigned redo = 0;  // elements to re-evaluate in FP64 (bit k)
   static_assert(kFFCols <= 32 && kFFRows <= 32, "chain masks hold 32 elements");
-  for (int k = 1; k <= count; ++k) {
-    p += 3 * stride_nodes;
-    const double bx = p[0], by = p[1], bz = p[2];
-    const double g = S[s0 + k - 1];
-    const float rbx = (float)(x - bx), rby = (float)(y - by), rbz = (float)(z - bz);
-    const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
-    const float ib = __builtin_amdgcn_rsqf(rbs);
-    const float* f = F + 4 * (s0 + k - 1);  // (float)(b - a) and its squared length
-    const float r0x = f[0], r0y = f[1], r0z = f[2];
-    const float dx = fmaf(r0y, raz, -r0z * ray);
-    const float dy = fmaf(r0z, rax, -r0x * raz);
-    const float dz = fmaf(r0x, ray, -r0y * rax);
-    const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
-    const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
-    const float tt = fmaf(oa, ia, -ob * ib);
-    const float derr2 = 64.f * kEps32 * kEps32 * f[3] * ras;
-    const bool nearcore = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
-    float K = tt * (float)g * __builtin_amdgcn_rcpf(dsq);
-    K = nearcore ? 0.f : K;
-    fx = fmaf(dx, K, fx);
-    fy = fmaf(dy, K, fy);
-    fz = fmaf(dz, K, fz);
-    redo |= (unsigned)(nearcore && g != 0.0) << (k - 1);
-    pax = bx;
-    pay = by;
-    paz = bz;
-    rax = rbx;
-    ray = rby;
-    raz = rbz;
-    ras = rbs;
-    ia = ib;
+  int k = 0;
+  for (; k + 2 <= count; k += 2) {
+    end32(p + step, x, y, z, B);
+    redo |= (unsigned)elem32(A, B, F + 4 * (s0 + k), S[s0 + k], fx, fy, fz) << k;
+    p += 2 * step;
+    end32(p, x, y, z, A);
+    redo |= (unsigned)elem32(B, A, F + 4 * (s0 + k + 1), S[s0 + k + 1], fx, fy, fz) << (k + 1);
+  }
+  if (k < count) {
+    end32(p + step, x, y, z, B);
+    redo |= (unsigned)elem32(A, B, F + 4 * (s0 + k), S[s0 + k], fx, fy, fz) << k;
   }
   if (__any(redo != 0)) {  // rare: a second, FP64 pass over the flagged elements
     p = R + 3 * n0;
-    for (int k = 1; k <= count; ++k) {
-      const double* b = p + 3 * stride_nodes;
-      if ((redo >> (k - 1)) & 1u)
-        dev::seg_ref(p[0], p[1], p[2], b[0], b[1], b[2], x, y, z, S[s0 + k - 1], ux, uy, uz);
+    for (int e = 0; e < count; ++e) {
+      const double* b = p + step;
+      if ((redo >> e) & 1u)
+        dev::seg_ref(p[0], p[1], p[2], b[0], b[1], b[2], x, y, z, S[s0 + e], ux, uy, uz);
       p = b;
     }
   }
@@ -1152,7 +1172,7 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
 }
 
 int FarField::velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
-                       std::string* err, const FFFocus* focus) {
+                       std::string* err, const FFFocus* focus, bool per_target) {
   if (nt <= 0) return UVLM_OK;
   if (!built_) {
     *err = "far field: no sources built";
@@ -1189,7 +1209,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (double4*)d_tiles_);
   const bool graded = focus && focus->box && focus->reach > 0.0;
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
-                     (const double4*)d_tiles_, targets, (const int*)map, theta,
+                     (const double4*)d_tiles_, targets, (const int*)map, per_target && !mlp_on_,
+                     theta,
                      graded ? focus->box : nullptr,
                      graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
                      (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
