@@ -95,8 +95,6 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
 }
 
 // ---------------------------------------------------------------------------
-constexpr float kEps32 = 5.9604645e-8f;  // 2^-24
-
 __global__ __launch_bounds__(kBlock) void k_chain_f32c(
     const double* __restrict__ targets, int nt, const ChainRec* __restrict__ recs,
     ChunkPlan plan, double* __restrict__ partial, int ld) {
@@ -111,17 +109,13 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
     z = targets[3 * t + 2];
   }
   double pax = 0.0, pay = 0.0, paz = 0.0;  // previous node (FP64, near-core path)
-  float rax = 0.f, ray = 0.f, raz = 0.f, ras = 0.f, ia = 0.f;
+  dev::End32 A = {0.f, 0.f, 0.f, 0.f, 0.f, true}, B;  // element 0 starts a chain (g = 0)
   if (rb > 0) {
     const double* p = recs[rb - 1].p;
     pax = p[0];
     pay = p[1];
     paz = p[2];
-    rax = (float)(x - pax);
-    ray = (float)(y - pay);
-    raz = (float)(z - paz);
-    ras = fmaf(raz, raz, fmaf(ray, ray, rax * rax));
-    ia = __builtin_amdgcn_rsqf(ras);
+    dev::end32(p, x, y, z, A);
   }
   double ux = 0.0, uy = 0.0, uz = 0.0;
   const double2* src = reinterpret_cast<const double2*>(recs);
@@ -137,38 +131,32 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
       // run start: previous node for the FP64 re-evaluation pass below
       const double qx = pax, qy = pay, qz = paz;
       unsigned redo = 0;  // elements of the run to re-evaluate (bit k - k0)
-      for (int k = k0; k < kend; ++k) {
-        const double* e = tl + 8 * k;  // ChainRec32
-        const float* ef = reinterpret_cast<const float*>(e + 4);
-        const float rbx = (float)(x - e[0]), rby = (float)(y - e[1]), rbz = (float)(z - e[2]);
-        const float rbs = fmaf(rbz, rbz, fmaf(rby, rby, rbx * rbx));
-        const float ib = __builtin_amdgcn_rsqf(rbs);
-        const float r0x = ef[0], r0y = ef[1], r0z = ef[2];
-        const float dx = fmaf(r0y, raz, -r0z * ray);
-        const float dy = fmaf(r0z, rax, -r0x * raz);
-        const float dz = fmaf(r0x, ray, -r0y * rax);
-        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float oa = fmaf(r0z, raz, fmaf(r0y, ray, r0x * rax));
-        const float ob = fmaf(r0z, rbz, fmaf(r0y, rby, r0x * rbx));
-        const float tt = fmaf(oa, ia, -ob * ib);
-        // Near-core band: |d| < sqrt(10) * 1e-5 plus the FP32 rounding bound
-        // of d (~5 eps |r0| |x-a|), in squares: (a+b)^2 <= 2a^2 + 2b^2.
-        const float derr2 = 64.f * kEps32 * kEps32 * ef[3] * ras;
-        const bool near = (dsq < 2e-9f + derr2) | (ras < 1e-18f) | (rbs < 1e-18f);
-        float K = tt * ef[4] * __builtin_amdgcn_rcpf(dsq);
-        K = near ? 0.f : K;
-        fx = fmaf(dx, K, fx);
-        fy = fmaf(dy, K, fy);
-        fz = fmaf(dz, K, fz);
-        redo |= (unsigned)(near && e[3] != 0.0) << (k - k0);
-        pax = e[0];
-        pay = e[1];
-        paz = e[2];
-        rax = rbx;
-        ray = rby;
-        raz = rbz;
-        ras = rbs;
-        ia = ib;
+      // Two elements per trip, the end point alternating between A and B;
+      // chain starts (g = 0, r0 = 0) fall in the band and are not re-evaluated.
+      int k = k0;
+      for (; k + 2 <= kend; k += 2) {
+        const double* e0 = tl + 8 * k;  // ChainRec32
+        const double* e1 = e0 + 8;
+        const float* f0 = reinterpret_cast<const float*>(e0 + 4);
+        const float* f1 = reinterpret_cast<const float*>(e1 + 4);
+        dev::end32(e0, x, y, z, B);
+        redo |= (unsigned)(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0) << (k - k0);
+        dev::end32(e1, x, y, z, A);
+        redo |= (unsigned)(dev::elem32(B, A, f1, f1[4], fx, fy, fz) && e1[3] != 0.0)
+                << (k + 1 - k0);
+      }
+      if (k < kend) {
+        const double* e0 = tl + 8 * k;
+        const float* f0 = reinterpret_cast<const float*>(e0 + 4);
+        dev::end32(e0, x, y, z, B);
+        redo |= (unsigned)(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0) << (k - k0);
+        A = B;
+      }
+      {
+        const double* el = tl + 8 * (kend - 1);
+        pax = el[0];
+        pay = el[1];
+        paz = el[2];
       }
       if (__any(redo != 0)) {  // rare: pairs near the cut-off again, in FP64
         double ax = qx, ay = qy, az = qz;
@@ -265,7 +253,7 @@ __global__ void k_pack_chains(const double* __restrict__ nodes, int sheet_rows,
   f.r0[0] = (float)e.r0[0];
   f.r0[1] = (float)e.r0[1];
   f.r0[2] = (float)e.r0[2];
-  f.r0sq = (float)e.r0sq;
+  f.band = dev::kBand32 * (float)e.r0sq;
   f.gf = (float)e.g;
   f.pad[0] = f.pad[1] = f.pad[2] = 0.f;
   reinterpret_cast<ChainRec32*>(out)[q] = f;

@@ -115,5 +115,51 @@ __device__ __forceinline__ void seg_ref(double ax, double ay, double az, double 
   }
 }
 
+// --- the FP32 segment law with compensation ---------------------------------
+// 64 (2^-24)^2: the FP32 law's bound on the rounding of |d|^2 is
+// kBand32 |b - a|^2 |x - a|^2; records hold kBand32 |b - a|^2 (an exact
+// scaling).
+constexpr float kBand32 = 0x1p-42f;
+
+// End point of a segment seen from the target, in FP32: offsets (x - p
+// formed in FP64, so exact to FP32 rounding relative to |x - p|), squared
+// distance, its reciprocal root, and whether it lies in the cut-off band.
+struct End32 {
+  float rx, ry, rz, rs, inv;
+  bool tiny;
+};
+
+__device__ __forceinline__ void end32(const double* p, double x, double y, double z, End32& e) {
+  e.rx = (float)(x - p[0]);
+  e.ry = (float)(y - p[1]);
+  e.rz = (float)(z - p[2]);
+  e.rs = __builtin_fmaf(e.rz, e.rz, __builtin_fmaf(e.ry, e.ry, e.rx * e.rx));
+  e.inv = __builtin_amdgcn_rsqf(e.rs);
+  e.tiny = e.rs < 1e-18f;
+}
+
+// One segment a -> b of strength g (with 1/4pi): f = (float)(b - a) and
+// kBand32 |b - a|^2.  Pairs within a band around the reference's cut-off
+// (|r| < 1e-9, |d|^2 < 2e-9 plus the FP32 rounding bound of d) add nothing
+// here and are reported, for the caller's FP64 re-evaluation.
+__device__ __forceinline__ bool elem32(const End32& a, const End32& b, const float* f, float g,
+                                       float& fx, float& fy, float& fz) {
+  const float r0x = f[0], r0y = f[1], r0z = f[2];
+  const float dx = __builtin_fmaf(r0y, a.rz, -r0z * a.ry);
+  const float dy = __builtin_fmaf(r0z, a.rx, -r0x * a.rz);
+  const float dz = __builtin_fmaf(r0x, a.ry, -r0y * a.rx);
+  const float dsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  const float oa = __builtin_fmaf(r0z, a.rz, __builtin_fmaf(r0y, a.ry, r0x * a.rx));
+  const float ob = __builtin_fmaf(r0z, b.rz, __builtin_fmaf(r0y, b.ry, r0x * b.rx));
+  const float tt = __builtin_fmaf(oa, a.inv, -ob * b.inv);
+  const bool nearcore = (dsq < __builtin_fmaf(f[3], a.rs, 2e-9f)) | a.tiny | b.tiny;
+  float K = tt * g * __builtin_amdgcn_rcpf(dsq);
+  K = nearcore ? 0.f : K;
+  fx = __builtin_fmaf(dx, K, fx);
+  fy = __builtin_fmaf(dy, K, fy);
+  fz = __builtin_fmaf(dz, K, fz);
+  return nearcore;
+}
+
 }  // namespace dev
 }  // namespace uvlm

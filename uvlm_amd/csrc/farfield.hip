@@ -55,9 +55,6 @@ constexpr int kL0Segs = (kFFRows + 1) * kFFCols + (kFFCols + 1) * kFFRows;
 constexpr int kRecStr = 3 * kL0Nodes;
 constexpr int kRecSeg = kRecStr + kL0Segs;  // FP32 segment vectors (as floats)
 constexpr int kRec = kRecSeg + 2 * kL0Segs;
-// 64 (2^-24)^2: the FP32 walk's bound on the rounding of |d|^2 is
-// kBand32 |b - a|^2 |x - a|^2; records hold kBand32 |b - a|^2 (exact scaling).
-constexpr float kBand32 = 0x1p-42f;
 
 // Bounding box of n points x[n][3]: box[0..2] min, box[3..5] max (one
 // workgroup).
@@ -247,7 +244,7 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_level0(FFTree t,
       F[0] = ex;
       F[1] = ey;
       F[2] = ez;
-      F[3] = kBand32 * fmaf(ez, ez, fmaf(ey, ey, ex * ex));  // the walk's rounding band
+      F[3] = dev::kBand32 * fmaf(ez, ez, fmaf(ey, ey, ex * ex));  // the walk's rounding band
     }
     sg[s] = ga - gb;
   }
@@ -563,43 +560,6 @@ __device__ __forceinline__ void walk_record(const double* R, int nr, int nc, dou
 // shorter than 2 r: below this radius the end-point cut-offs are implied.
 constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 
-// End point of a segment seen from the target, in FP32: offsets, squared
-// distance, its reciprocal root, and whether it lies within the cut-off band.
-struct End32 {
-  float rx, ry, rz, rs, inv;
-  bool tiny;
-};
-
-__device__ __forceinline__ void end32(const double* b, double x, double y, double z, End32& e) {
-  e.rx = (float)(x - b[0]);
-  e.ry = (float)(y - b[1]);
-  e.rz = (float)(z - b[2]);
-  e.rs = fmaf(e.rz, e.rz, fmaf(e.ry, e.ry, e.rx * e.rx));
-  e.inv = __builtin_amdgcn_rsqf(e.rs);
-  e.tiny = e.rs < 1e-18f;
-}
-
-// One segment a -> b with strength g; f = (float)(b - a) and kBand32 |b - a|^2.
-// Returns whether the pair lies in the near-core band (added here as zero).
-__device__ __forceinline__ bool elem32(const End32& a, const End32& b, const float* f, double g,
-                                       float& fx, float& fy, float& fz) {
-  const float r0x = f[0], r0y = f[1], r0z = f[2];
-  const float dx = fmaf(r0y, a.rz, -r0z * a.ry);
-  const float dy = fmaf(r0z, a.rx, -r0x * a.rz);
-  const float dz = fmaf(r0x, a.ry, -r0y * a.rx);
-  const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const float oa = fmaf(r0z, a.rz, fmaf(r0y, a.ry, r0x * a.rx));
-  const float ob = fmaf(r0z, b.rz, fmaf(r0y, b.ry, r0x * b.rx));
-  const float tt = fmaf(oa, a.inv, -ob * b.inv);
-  const bool nearcore = (dsq < fmaf(f[3], a.rs, 2e-9f)) | a.tiny | b.tiny;
-  float K = tt * (float)g * __builtin_amdgcn_rcpf(dsq);
-  K = nearcore ? 0.f : K;
-  fx = fmaf(dx, K, fx);
-  fy = fmaf(dy, K, fy);
-  fz = fmaf(dz, K, fz);
-  return nearcore;
-}
-
 // The same walk in FP32 with compensation: x - p in FP64 (exact to FP32
 // rounding relative to |x - p|), the segment law in FP32 with the hardware
 // rsqrt / reciprocal, each chain (<= 17 elements) summed in FP32 and added
@@ -613,22 +573,23 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
                                         double& uz) {
   const double* p = R + 3 * n0;
   const int step = 3 * stride_nodes;
-  End32 A, B;
-  end32(p, x, y, z, A);
+  dev::End32 A, B;
+  dev::end32(p, x, y, z, A);
   float fx = 0.f, fy = 0.f, fz = 0.f;
   unsigned redo = 0;  // elements to re-evaluate in FP64 (bit k)
   static_assert(kFFCols <= 32 && kFFRows <= 32, "chain masks hold 32 elements");
   int k = 0;
   for (; k + 2 <= count; k += 2) {
-    end32(p + step, x, y, z, B);
-    redo |= (unsigned)elem32(A, B, F + 4 * (s0 + k), S[s0 + k], fx, fy, fz) << k;
+    dev::end32(p + step, x, y, z, B);
+    redo |= (unsigned)dev::elem32(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz) << k;
     p += 2 * step;
-    end32(p, x, y, z, A);
-    redo |= (unsigned)elem32(B, A, F + 4 * (s0 + k + 1), S[s0 + k + 1], fx, fy, fz) << (k + 1);
+    dev::end32(p, x, y, z, A);
+    redo |= (unsigned)dev::elem32(B, A, F + 4 * (s0 + k + 1), (float)S[s0 + k + 1], fx, fy, fz)
+            << (k + 1);
   }
   if (k < count) {
-    end32(p + step, x, y, z, B);
-    redo |= (unsigned)elem32(A, B, F + 4 * (s0 + k), S[s0 + k], fx, fy, fz) << k;
+    dev::end32(p + step, x, y, z, B);
+    redo |= (unsigned)dev::elem32(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz) << k;
   }
   if (__any(redo != 0)) {  // rare: a second, FP64 pass over the flagged elements
     p = R + 3 * n0;

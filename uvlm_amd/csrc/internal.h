@@ -47,13 +47,14 @@ struct alignas(16) ChainRec {
 };
 static_assert(sizeof(ChainRec) == 64, "ChainRec must be 64 bytes");
 // The same element for the F32C kernel: the node and g in FP64 (for the
-// FP64 x - p and the near-core re-evaluation), r0, |r0|^2 and g rounded to
-// FP32 once when packed rather than in every (target, element) pair.
+// FP64 x - p and the near-core re-evaluation), r0, the rounding band
+// 2^-42 |r0|^2 and g rounded to FP32 once when packed rather than in every
+// (target, element) pair.
 struct alignas(16) ChainRec32 {
   double p[3];
   double g;
   float r0[3];
-  float r0sq;
+  float band;
   float gf;
   float pad[3];
 };
