@@ -203,9 +203,11 @@ int uvlm_farfield_velocity(uvlm_ctx* ctx, const double* targets, size_t n_target
  * switches it off): in the advection stages at the wake's nodes with theta,
  * at the collocation points (RHS) and the edge midpoints (loads) with
  * 2 theta; the tree is built once per wake state.  Every audit_every steps
- * (0: never) the first advection stage is also summed exactly; if the
- * largest difference exceeds tol times the largest velocity, the split is
- * switched off for the rest of the run. */
+ * (0: never) the first advection stage is also summed exactly (wakes of
+ * 32768 nodes or more: every 8th node, a different eighth each audit; the
+ * stage keeps the split's velocities, where a full audit continues with the
+ * exact ones); if the largest difference exceeds tol times the largest
+ * velocity, the split is switched off for the rest of the run. */
 int uvlm_sim_set_farfield(uvlm_ctx* ctx, double theta, int audit_every, double tol);
 /* Graded criterion for the advection stages (reach <= 0 switches it off):
  * a tile of wake nodes whose gap g to the bounding sphere of the wings

@@ -115,6 +115,30 @@ def test_audit_switches_the_split_off(orc):
     assert not active and audits >= 1 and err > 1e-15
 
 
+def test_strided_audit_of_a_large_wake(orc):
+    """Wakes of 32768 nodes or more are audited on every 8th node: the audit
+    still reports the stage difference, and a tolerance no expansion can
+    meet still switches the split off."""
+    case = W.tandem_c5(steps=273)  # 2 x 16 x 64: 130 wake nodes per step
+    a = Simulation(case, 273)
+    a.set_farfield(theta=4.0, audit_every=30, tol=1e-3)
+    b = Simulation(case, 273)
+    b.set_farfield(theta=4.0, audit_every=1000, tol=1e-3)
+    for s in range(272):
+        print(f"[strided audit] step {s}", flush=True) if s % 50 == 0 else None
+        a.step(s)
+        b.step(s)
+    a.sync()
+    err, audits, active = a.farfield_stats()
+    # audits at steps 1, 31, ..., 271; the last one's wake has 2 * 272 * 65 nodes
+    assert audits == 10 and active and 0 < err < 1e-3
+    b.set_farfield(theta=2.0, audit_every=10, tol=1e-15)
+    b.step(272)  # audited: 2 * 273 * 65 wake nodes, every 8th summed exactly
+    b.sync()
+    err, audits, active = b.farfield_stats()
+    assert audits == 2 and not active and err > 1e-15
+
+
 def test_simulation_learns_the_correction(orc):
     """In-run learning on the tandem workload (reduced lattice): the
     network trained on the run's own wake corrects the far pairs of the

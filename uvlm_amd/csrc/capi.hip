@@ -76,7 +76,7 @@ struct uvlm_ctx {
   // the split's sources are rebuilt only when the wake changed
   long wake_version = 0, ff_version = -1;
   Buf ffm;
-  Buf ffg, ffv, ffx, ffstat, ffbox;
+  Buf ffg, ffv, ffx, ffxs, ffstat, ffbox;
   // kept-inverse solver (UVLM_SOLVE_F64_INV); one for the simulation, one
   // for the stateless uvlm_solve
   uvlm::InvSolver* inv = nullptr;
@@ -276,6 +276,9 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
 // wake's own nodes: their velocities set the circulations and the loads
 // directly, and they cost little.
 constexpr double kWingThetaFactor = 2.0;
+// Audits of wakes of at least kAuditFull nodes sum every kAuditStride-th
+// node exactly (an eighth of the cost of the exact stage).
+constexpr int kAuditFull = 32768, kAuditStride = 8;
 
 // The near/far split applies: set, not switched off by an audit, a wake.
 bool ff_active(const uvlm_ctx* c) {
@@ -350,7 +353,7 @@ int uvlm_destroy(uvlm_ctx* c) {
                  &c->info, &c->uw, &c->vm, &c->mid, &c->uflow, &c->vedge, &c->pos,
                  &c->pst, &c->k1, &c->k2, &c->k3, &c->k4, &c->vprev, &c->loads,
                  &c->ref_area, &c->work, &c->tmp, &c->ffg, &c->ffv, &c->ffx, &c->ffstat,
-                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev, &c->ffm, &c->ffbox};
+                 &c->mw, &c->mx, &c->mv, &c->my, &c->mg, &c->mdev, &c->ffm, &c->ffbox, &c->ffxs};
   for (Buf* b : bufs) release(*b);
   free_staging(c->st_wing);
   free_staging(c->st_vm);
@@ -1006,9 +1009,25 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     TRY(ff_stage(targets, P<double>(out)));
     if (audit && first) {
       first = false;
-      TRY(exact_stage(targets, P<double>(c->ffx)));
-      launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), V,
-                        P<unsigned long long>(c->ffstat), targets, &focus);
+      // Large wakes are audited on every kAuditStride-th node (the subset
+      // moves along with each audit) and keep the split's velocities.
+      const int stride = V >= kAuditFull ? kAuditStride : 1;
+      if (stride == 1) {
+        TRY(exact_stage(targets, P<double>(c->ffx)));
+        launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), V,
+                          P<unsigned long long>(c->ffstat), targets, &focus);
+      } else {
+        const int off = c->ff_audits % stride, Vs = (V - off + stride - 1) / stride;
+        TRY(ensure(c, c->ffxs, sizeof(double) * 3 * (size_t)Vs));
+        launch_gather_strided(c->stream, targets, off, stride, Vs, P<double>(c->ffxs));
+        ChunkPlan plan;
+        TRY(sum_sources(c, P<double>(c->ffxs), Vs, true, true, &plan));
+        launch_flow_velocity(c->stream, P<double>(c->partial), Vs, Vs, plan.cb, plan.cw, ff,
+                             P<double>(c->ffx), 0, 0);
+        launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), Vs,
+                          P<unsigned long long>(c->ffstat), P<double>(c->ffxs), &focus, off,
+                          stride, te_period, C1);
+      }
       unsigned long long h[2] = {0, 0};
       HIPCHK(c, hipMemcpyAsync(h, c->ffstat.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1018,8 +1037,9 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
       c->ff_last_err = m > 0.0 ? e / m : e;
       ++c->ff_audits;
       if (c->ff_last_err > c->ff_tol) c->ff_off = true;
-      // the audited stage continues with the exact velocities
-      HIPCHK(c, hipMemcpyAsync(out.p, c->ffx.p, vb, hipMemcpyDeviceToDevice, c->stream));
+      // a fully audited stage continues with the exact velocities
+      if (stride == 1)
+        HIPCHK(c, hipMemcpyAsync(out.p, c->ffx.p, vb, hipMemcpyDeviceToDevice, c->stream));
     }
     return UVLM_OK;
   };

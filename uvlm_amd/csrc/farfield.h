@@ -75,9 +75,14 @@ void launch_ff_combine(hipStream_t s, const double* bound, int ld, int cb, const
 // res[0] <- bits of max |a - b|, res[1] <- bits of max |b| over n 3-vectors.
 // With a focus (reach > 0), |a - b| at point x[i] is weighted by
 // min(1, L / g)^3 (g, L as in FFFocus: the error the graded criterion allows).
+// Largest |a - b| and |b| into res (doubles as bit patterns); a may hold
+// the points at a_off + t a_stride with trailing-edge rows skipped.
 void launch_ff_maxdiff(hipStream_t s, const double* a, const double* b, int n,
                        unsigned long long* res, const double* x = nullptr,
-                       const FFFocus* focus = nullptr);
+                       const FFFocus* focus = nullptr, int a_off = 0, int a_stride = 1,
+                       int te_period = 0, int te_width = 0);
+void launch_gather_strided(hipStream_t s, const double* src, int off, int stride, int n,
+                           double* x);
 // Bounding box min[3], max[3] of n points x[n][3] (device).
 void launch_ff_bbox(hipStream_t s, const double* x, int n, double* box);
 // Rows [row0, row0+rows) of a ring buffer [num][cap][w] -> compact [num][rows][w].

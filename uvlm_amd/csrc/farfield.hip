@@ -957,14 +957,18 @@ __global__ void k_ff_combine(const double* __restrict__ bound, int ld, int cb,
 
 // Largest |a - b| and |b| over n vectors (atomic max on the bit patterns of
 // non-negative doubles).
+// b and x hold n points; a holds point t at a_off + t a_stride, and a's
+// trailing-edge rows (te_period > 0) are skipped.
 __global__ void k_ff_maxdiff(const double* __restrict__ a, const double* __restrict__ b, int n,
+                             int a_off, int a_stride, int te_period, int te_width,
                              const double* __restrict__ x, const double* __restrict__ fbox,
                              double reach, unsigned long long* __restrict__ res) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0, m = 0.0;
-  if (t < n) {
+  const long ta = a_off + (long)t * a_stride;
+  if (t < n && !(te_period > 0 && (ta % te_period) < te_width)) {
     for (int d = 0; d < 3; ++d) {
-      e = fmax(e, fabs(a[3 * (long)t + d] - b[3 * (long)t + d]));
+      e = fmax(e, fabs(a[3 * ta + d] - b[3 * (long)t + d]));
       m = fmax(m, fabs(b[3 * (long)t + d]));
     }
     if (fbox) {  // the error the graded criterion allows at this point
@@ -1011,13 +1015,33 @@ void launch_ff_combine(hipStream_t s, const double* bound, int ld, int cb, const
                      ff[0], ff[1], ff[2], te_period, te_width, out);
 }
 
+// x[i] = src[off + i stride], i < n.
+__global__ void k_gather_strided(const double* __restrict__ src, int off, int stride, int n,
+                                 double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long k = off + (long)i * stride;
+  x[3 * (long)i] = src[3 * k];
+  x[3 * (long)i + 1] = src[3 * k + 1];
+  x[3 * (long)i + 2] = src[3 * k + 2];
+}
+
+void launch_gather_strided(hipStream_t s, const double* src, int off, int stride, int n,
+                           double* x) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_strided, dim3(cdiv(n, 256)), dim3(256), 0, s, src, off, stride,
+                       n, x);
+}
+
 void launch_ff_maxdiff(hipStream_t s, const double* a, const double* b, int n,
-                       unsigned long long* res, const double* x, const FFFocus* focus) {
+                       unsigned long long* res, const double* x, const FFFocus* focus,
+                       int a_off, int a_stride, int te_period, int te_width) {
   const bool graded = focus && focus->box && focus->reach > 0.0 && x;
   hipLaunchKernelGGL(k_ff_zero2, dim3(1), dim3(1), 0, s, res);
   if (n > 0)
-    hipLaunchKernelGGL(k_ff_maxdiff, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n,
-                       graded ? x : nullptr, graded ? focus->box : nullptr,
+    hipLaunchKernelGGL(k_ff_maxdiff, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, a_off,
+                       a_stride, te_period, te_width, graded ? x : nullptr,
+                       graded ? focus->box : nullptr,
                        graded ? focus->reach : 0.0, res);
 }
 
