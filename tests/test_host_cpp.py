@@ -48,3 +48,26 @@ def test_unknown_advection_name_aborts(tmp_path):
     p = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert p.returncode != 0
     assert "Invalid advection name! Please choose from following: [AB2, RK2, RK4, RK4classic, euler]" in p.stdout
+
+
+@pytest.mark.parametrize("cols,wings,msg", [(7, 1, "columns must be even"),
+                                            (8, 0, "num_wings out of range")])
+def test_legacy_run_rejects_bad_specs(cols, wings, msg):
+    """uvlm_host_legacy_run checks the case before any device work (the
+    reference aborts on odd columns, simple_simulator.cpp:136)."""
+    from uvlm_amd import workloads as W
+    from uvlm_amd import _native as N
+    from uvlm_amd.gpu import UVLMError, case_spec, legacy_run
+    case = W.plate_c1(steps=2, rows=2, cols=cols)
+    if wings == 1:
+        with pytest.raises(UVLMError, match=msg):
+            legacy_run(case, 2)
+    else:
+        import ctypes as C
+        spec = case_spec(case)
+        spec.num_wings = 0
+        L = N.host_lib()
+        nw = C.c_size_t(0)
+        assert L.uvlm_host_legacy_run(C.byref(spec), 2, None, None, None, None,
+                                      C.byref(nw)) != N.OK
+        assert msg in L.uvlm_host_last_error().decode()
