@@ -49,6 +49,11 @@ sys.path.insert(0, ROOT)
 METRIC = "Biot-Savart pairs/s & timesteps/s vs plain FP64 GPU, CL/CT rel err ≤1e-4"
 FLOP_PER_PAIR = 56.0
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD published)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (AMD published)
+# dominant kernel and its peak per precision
+ROOFLINE_KERNEL = {"f64exact": ("Biot-Savart ring pairs (k_nbody)", PEAK_FP64_TFLOPS),
+                   "f64": ("Biot-Savart N-body (k_chain_f64)", PEAK_FP64_TFLOPS),
+                   "f32c": ("Biot-Savart N-body (k_chain_f32c)", PEAK_FP32_TFLOPS)}
 PRECISIONS = {"f64exact": 0x0, "f64": 0x1, "f32c": 0x2}
 SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2, "fullpiv": 3, "inv": 4}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_chain_f64_naca4412_20x80_rk4_traffic.json")
@@ -234,9 +239,10 @@ def run_single(args):
         "timesteps_per_s": args.steps / T,
         "gpu_evaluated_pairs_per_s": (e1 - e0) / T,
         "final_CD_CL": loads[-1].tolist() if len(loads) else None,
-        "roofline": {"bound": "valu", "kernel": "Biot-Savart N-body (k_chain_f64)",
-                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+        "roofline": {"bound": "valu", "kernel": ROOFLINE_KERNEL[args.precision][0],
+                     "achieved": achieved, "peak": ROOFLINE_KERNEL[args.precision][1],
+                     "unit": "TFLOP/s",
+                     "frac": achieved / ROOFLINE_KERNEL[args.precision][1], "traffic": traffic,
                      "launches": launches, "avg_launch_us": 1e3 * kms / max(launches, 1),
                      "flop_per_launch": FLOP_PER_PAIR * kpairs / max(launches, 1),
                      "kernel_share_of_step_time": kms / 1e3 / T},
