@@ -112,6 +112,10 @@ def lib():
         L.orc_loads.argtypes = [_dp, _dp, _dp, _dp, sz, sz, sz, _dp, _dp, sz,
                                 C.POINTER(WingSpec), _dp, C.c_double, C.c_double, _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_sim_advect.argtypes = [C.c_void_p, C.c_double]
+        L.orc_sim_replace_wake.argtypes = [C.c_void_p, sz, sz, _dp, _dp]
+        L.orc_set_cutoff.argtypes = [C.c_double]
+        L.orc_get_cutoff.restype = C.c_double
         _lib = L
     return _lib
 
@@ -126,6 +130,33 @@ def _f64(a) -> np.ndarray:
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def set_cutoff(eps: float) -> None:
+    """Core cut-off of the segment law (vortex.cpp:31, 1e-10 by default)."""
+    lib().orc_set_cutoff(float(eps))
+
+
+def get_cutoff() -> float:
+    return lib().orc_get_cutoff()
+
+
+def loads(wing_nodes, wing_nodes_init, gamma, gamma_prev, wake_nodes, wake_gamma,
+          wings, ff, t, dt):
+    """CalcLoad from an explicit state (simple_simulator.cpp:195-233,
+    joukowski.cpp:12-92): per wing (CD, CL)."""
+    wn, w0 = _f64(wing_nodes), _f64(wing_nodes_init)
+    g, gp = _f64(gamma), _f64(gamma_prev)
+    num, R, Cc = g.shape
+    wk, wg = _f64(wake_nodes), _f64(wake_gamma)
+    specs = (WingSpec * MAX_WINGS)()
+    for n, d in enumerate(wings):
+        specs[n] = wing_spec(d)
+    f = _f64(ff)
+    out = np.zeros((num, 2))
+    lib().orc_loads(_p(wn), _p(w0), _p(g), _p(gp), num, R, Cc, _p(wk), _p(wg), wg.shape[1],
+                    specs, _p(f), float(t), float(dt), _p(out))
+    return out
 
 
 # --- kernels ---------------------------------------------------------------
@@ -434,6 +465,16 @@ class Sim:
         rc = lib().orc_sim_set_state(self._h, wn.shape[1], wg.shape[1], _p(wn), _p(wg),
                                      _p(g) if g is not None else None)
         if rc != 0:
+            raise ValueError("inconsistent wake shape")
+
+    def advect(self, dt=None):
+        """One Advect call alone (advect.cpp:11-216) on the current wake."""
+        lib().orc_sim_advect(self._h, float(self.case["dt"] if dt is None else dt))
+
+    def replace_wake(self, wake_nodes, wake_gamma):
+        """New wake nodes/circulations, keeping the AB2 velocity history."""
+        wn, wg = _f64(wake_nodes), _f64(wake_gamma)
+        if lib().orc_sim_replace_wake(self._h, wn.shape[1], wg.shape[1], _p(wn), _p(wg)) != 0:
             raise ValueError("inconsistent wake shape")
 
     def loads(self):

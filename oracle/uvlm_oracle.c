@@ -68,7 +68,12 @@ int orc_num_threads(void) {
 
 /* ------------------------------------------------------------------------ */
 /* vortex.cpp:21-39 */
-static const double BS_EPS = 1e-10; /* vortex.cpp:31 cut off length */
+/* vortex.cpp:31 cut off length (1e-10 in the reference; settable so that a
+ * configured core cut-off runs with the same rule shape) */
+static double BS_EPS = 1e-10;
+
+void orc_set_cutoff(double eps) { BS_EPS = eps; }
+double orc_get_cutoff(void) { return BS_EPS; }
 
 static inline v3 bs_law(v3 start, v3 end, v3 pos) {
   v3 r0 = sub(end, start);
@@ -823,6 +828,26 @@ static void advect(orc_sim* s, double dt) {
     s->v_prev = v;
     s->v_prev_rows = s->wake_rows;
   }
+}
+
+/* One Advect call alone (advect.cpp:11-216) on the current wake, the
+ * current wing and its circulations as sources. */
+void orc_sim_advect(orc_sim* s, double dt) { advect(s, dt); }
+
+/* Replace the wake (any shape) and keep the AB2 velocity history. */
+int orc_sim_replace_wake(orc_sim* s, size_t node_rows, size_t gamma_rows,
+                         const double* wake_nodes, const double* wake_gamma) {
+  if (!(gamma_rows + 1 == node_rows || (node_rows == 0 && gamma_rows == 0)))
+    return -1;
+  free(s->wake);
+  free(s->wake_gamma);
+  s->wake = (double*)xcalloc(3 * s->num * node_rows * (s->C + 1), sizeof(double));
+  s->wake_gamma = (double*)xcalloc(s->num * gamma_rows * s->C, sizeof(double));
+  memcpy(s->wake, wake_nodes, sizeof(double) * 3 * s->num * node_rows * (s->C + 1));
+  memcpy(s->wake_gamma, wake_gamma, sizeof(double) * s->num * gamma_rows * s->C);
+  s->wake_rows = node_rows;
+  s->wake_grows = gamma_rows;
+  return 0;
 }
 
 /* joukowski.cpp:12-92 + simple_simulator.cpp:195-233 */

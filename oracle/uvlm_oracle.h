@@ -183,6 +183,20 @@ void orc_loads(const double* wing_nodes, const double* wing_nodes_init,
                const orc_wing_spec* wings, const double ff[3], double t,
                double dt, double* out);
 
+/* One Advect call alone (advect.cpp:11-216): the configured scheme moves the
+ * current wake; sources are the current wing, its circulations and the wake
+ * itself.  The AB2 history is kept between calls. */
+void orc_sim_advect(orc_sim* s, double dt);
+/* Replace the wake nodes and circulations (any row count) keeping the AB2
+ * history (orc_sim_set_state clears it). */
+int orc_sim_replace_wake(orc_sim* s, size_t node_rows, size_t gamma_rows,
+                         const double* wake_nodes, const double* wake_gamma);
+
+/* Core cut-off of the segment law (vortex.cpp:31; default 1e-10): a pair
+ * returns zero when |r1 x r2|^2 < eps, |r1| < eps or |r2| < eps. */
+void orc_set_cutoff(double eps);
+double orc_get_cutoff(void);
+
 /* Number of OpenMP threads used by the oracle (1 when built without it). */
 int orc_num_threads(void);
 
