@@ -3,40 +3,47 @@
 
 A step is one UVLM timestep (morph -> shed -> matrix -> RHS -> solve ->
 loads -> advect, simulator/simple_simulator.cpp:240-286) run by the C++
-SimpleSimulator on the gfx950 kernels.  Default workload: NACA4412-cambered
-wing, 20x80 rings, root-hinged flapping +/-30 deg at k = 0.3 with 5 deg
-pitch, reference "RK4" wake advection, 1000 steps (the wake grows to 80,000
-rings).  `--warmup W` untimed steps (from step 0), then `--steps K` timed
-steps; the defaults time steps 1..1000 of the 1001-iteration run.  The
-default method is FP64 throughout: segment-chain N-body in FP64 and the
-kept-inverse FP64 solve (iterative refinement from the previous step's
-circulations to max|dx| <= 1e-14 max|x|); `--solve f64` selects the blocked
-LU instead.
+SimpleSimulator on the gfx950 kernels.
 
-Prints one JSON line.  `value` is Biot-Savart pairs per second in the
-reference's counting: the pairs the reference loop evaluates for the timed
-steps, 4 [N^2 + N Nw + L (N + Nw) + S V (N + Nw)] per step (one pair = one
-BiotSavartLaw call, vortex.cpp:21-39), divided by the wall time of those
-steps with inputs resident on the GPU.  `roofline` describes the dominant
-kernel (the FP64 segment-chain N-body): achieved = 56 FLOP x segment pairs
-it evaluated / its summed duration from HIP events on its stream.
-`cpu_baseline` times the oracle (a C restatement of the reference's loop,
-OpenMP over targets like the reference) on a bounded sample of the same
-sums at the final state, on the host's cores.
+Default workload: two tandem NACA 8300-camber wings, 16 x 64 rings each,
+gap 1.5 c, flapping -(pi/4) cos(wt + n 90 deg) at k = 0.1, Euler wake
+advection, 2000 steps (2001 iterations; the wake grows to 256,000 rings).
+The whole run is done twice in this process:
 
-Other modes: --workload c1|c3|c5 (single-case workloads), --workload c4
-(256 batched cases), --precision, --solve, --farfield THETA (near/far split
-of the wake sum, audited), --mlp-every K (in-run training of the far-field
-correction network every K steps); --compare also runs the plain FP64
-method (exact sums, LU) over the same steps and reports the speed-up and
-the CL/CT differences; --calibrate-copy runs a plain device copy of known
-size for the PMC traffic calibration.
+* the fast form: near/far split of the wake sum (lattice-block moment
+  expansions for far clusters, theta 4 near the wings graded beyond 1.5
+  wing radii to 1.3, exact audits every 50 steps), FP32 compensated near
+  sums with FP64 near-core re-evaluation, kept-inverse FP64 solve with
+  iterative refinement;
+* the plain FP64 method: every sum exact (FP64 segment chains), blocked
+  FP64 LU.
+
+`--steps K --warmup W` choose the timed window inside the full-length run,
+never the run length: the last K steps of the run, with the W steps before
+them untimed (everything earlier is untimed too).  The window is bracketed
+by device synchronisations on both sides.  `value` is timesteps/s of the
+fast form over the window; `vs_plain_fp64` gives both runs' timesteps/s over
+the whole run (steps 1..2000) and over the same window, and the CL/CT
+differences.  `roofline` describes the fast form's dominant kernel, the
+near-block sum `k_ff_near` (56 FLOP per (target, segment) pair, pairs
+counted on the device, duration from HIP events on its stream over every
+launch of the fast run).  `c2_naca4412` is a second block: the NACA4412
+20 x 80 wing, RK4, 1000 steps, FP64 segment chains and the kept-inverse
+solve, with the roofline of `k_chain_f64`.  `cpu_baseline` times the oracle
+(a C restatement of the reference's loop, OpenMP over targets like the
+reference) on a bounded sample of the plain method's sums at the final
+state of the plain run, on the host's cores.
+
+Other modes: --workload c1|c2|c3|c5 runs one configuration alone (same
+window rules, `--precision`, `--solve`, `--farfield` choose the method),
+--workload c4 the 256 batched cases; --no-plain, --no-c2, --no-cpu skip
+legs; --calibrate-copy runs a plain device copy of known size for the PMC
+traffic calibration.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -50,67 +57,29 @@ METRIC = "Biot-Savart pairs/s & timesteps/s vs plain FP64 GPU, CL/CT rel err ≤
 FLOP_PER_PAIR = 56.0
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD published)
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (AMD published)
-# dominant kernel and its peak per precision
-ROOFLINE_KERNEL = {"f64exact": ("Biot-Savart ring pairs (k_nbody)", PEAK_FP64_TFLOPS),
-                   "f64": ("Biot-Savart N-body (k_chain_f64)", PEAK_FP64_TFLOPS),
-                   "f32c": ("Biot-Savart N-body (k_chain_f32c)", PEAK_FP32_TFLOPS)}
 PRECISIONS = {"f64exact": 0x0, "f64": 0x1, "f32c": 0x2}
 SOLVES = {"f64": 0, "f32ir": 1, "bf16ir": 2, "fullpiv": 3, "inv": 4}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_chain_f64_naca4412_20x80_rk4_traffic.json")
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC = {  # committed PMC summaries (HBM bytes per launch) of the default runs
+    "k_chain_f64": os.path.join(PROFILES, "k_chain_f64_naca4412_20x80_rk4_traffic.json"),
+    "k_ff_near": os.path.join(PROFILES, "k_ff_near_tandem_2x16x64_traffic.json"),
+}
+# The fast form of the tandem workload.
+FAST = {"precision": "f32c", "solve": "inv", "farfield": 4.0, "ff_reach": 1.5,
+        "ff_theta_min": 1.3, "audit": 50, "ff_tol": 1e-3}
+PLAIN = {"precision": "f64", "solve": "f64", "farfield": 0.0}
 
 
-def cpu_baseline(sim, seconds):
-    """Oracle flow velocity of bound + wake rings (advect.h:16-24) at a sample
-    of wake nodes of the final GPU state, timed on the host cores."""
-    import oracle
-    oracle.build()
-    wing = sim.wing_nodes()
-    g = sim.gamma()
-    wn, wg = sim.wake()
-    targets = wn.reshape(-1, 3)
-    n_src = g.size + wg.size
-
-    def run(nt):
-        idx = np.linspace(0, targets.shape[0] - 1, nt).astype(int)
-        x = targets[idx]
-        t0 = time.perf_counter()
-        oracle.induced_velocity(x, wing, g)
-        if wg.size:
-            oracle.induced_velocity(x, wn, wg)
-        return time.perf_counter() - t0, 4.0 * nt * n_src
-
-    nt = 16
-    dt, pairs = run(nt)
-    while dt < 1.0 and nt < targets.shape[0]:  # calibrate on >= 1 s of work
-        nt = min(targets.shape[0], nt * 4)
-        dt, pairs = run(nt)
-    if dt < seconds and nt < targets.shape[0]:
-        nt = int(min(targets.shape[0], nt * seconds / dt))
-        dt, pairs = run(nt)
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": (f"oracle sum of all {n_src} bound and wake rings at {nt} of the "
-                       f"{targets.shape[0]} wake nodes of the final state "
-                       f"({pairs:.3e} pairs in {dt:.1f} s)")}
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the N-body from the committed PMC summary."""
+def traffic_of(kernel):
     try:
-        with open(TRAFFIC_FILE) as f:
-            d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
+        with open(TRAFFIC[kernel]) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
-
-
-def sync_all():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except ImportError:
-        pass
 
 
 def calibrate_copy(nbytes=1 << 28):
@@ -124,36 +93,72 @@ def calibrate_copy(nbytes=1 << 28):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
-def plain_run(args):
-    """The plain FP64 method (exact sums, LU) over the same steps: wall time
-    of the timed steps and the CL/CD history."""
+def run_case(case, method, window, warmup, label):
+    """Every step 0..S of the workload (S = case["steps"]).  Returns wall
+    times of the whole run (steps 1..S) and of the window (the last
+    `window` steps), the load history, and the kernels' event profiles
+    over the whole run.  The simulation is returned open (for the CPU leg)."""
     from uvlm_amd import Simulation
-    from uvlm_amd import workloads as W
 
-    total = args.warmup + args.steps
-    case = W.WORKLOADS[args.workload](steps=total - 1)
-    case["precision"] = PRECISIONS["f64"]
-    case["solve_mode"] = SOLVES["f64"]
-    sim = Simulation(case, total - 1)
-    for s in range(args.warmup):
-        sim.step(s)
+    S = int(case["steps"])
+    case = dict(case)
+    case["precision"] = PRECISIONS[method["precision"]]
+    case["solve_mode"] = SOLVES[method["solve"]]
+    sim = Simulation(case, S)
+    if method.get("farfield", 0) > 0:
+        sim.set_farfield(method["farfield"], method.get("audit", 50), method.get("ff_tol", 1e-3))
+        if method.get("ff_reach", 0) > 0:
+            sim.set_farfield_focus(method["ff_reach"], method.get("ff_theta_min", 2.0))
+    K = max(1, min(window, S))
+    first = S - K + 1  # first timed step
+    sim.profile(True)
+    sim.profile_read()
+    sim.profile_read_near()
+    sim.step(0)
     sim.sync()
-    t0 = time.perf_counter()
-    for s in range(args.warmup, total):
+    t_run0 = time.perf_counter()
+    t_win = None
+    for s in range(1, S + 1):
+        if s == first:
+            sim.sync()
+            tw0 = time.perf_counter()
         sim.step(s)
-        if (s - args.warmup) % 100 == 99:
-            print(f"# plain step {s}", file=sys.stderr, flush=True)
+        if s % 200 == 0:
+            log(f"{label} step {s}")
     sim.sync()
-    T = time.perf_counter() - t0
-    loads = sim.loads()
-    sim.close()
-    return T, loads
+    t_end = time.perf_counter()
+    t_win = t_end - tw0
+    t_run = t_end - t_run0
+    chain = sim.profile_read()
+    near = sim.profile_read_near()
+    sim.profile(False)
+    out = {"sim": sim, "S": S, "K": K, "first": first, "warmup": min(warmup, first - 1),
+           "t_run": t_run, "t_win": t_win, "loads": sim.loads(), "chain": chain,
+           "near": near, "pairs": sim.pairs()}
+    if method.get("farfield", 0) > 0:
+        out["ff"] = sim.farfield_stats()
+    if method["solve"] == "inv":
+        out["inv"] = sim.inverse_stats()
+    return out
+
+
+def roofline(kernel, prof, t_run, peak, traffic_key=None):
+    launches, kms, kpairs = prof
+    achieved = FLOP_PER_PAIR * kpairs / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak,
+            "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": traffic_of(traffic_key) if traffic_key else None,
+            "launches": launches, "avg_launch_us": 1e3 * kms / max(launches, 1),
+            "flop_per_launch": FLOP_PER_PAIR * kpairs / max(launches, 1),
+            "pairs_per_launch": kpairs / max(launches, 1),
+            "kernel_share_of_run_time": kms / 1e3 / t_run}
 
 
 def load_differences(loads, ref, early=30):
     """CL/CT differences relative to each wing's largest |CL|, |CT| of the
     reference run: pointwise over steps 0..early (before the wake's
-    sensitivity to rounding has grown) and of the means over the run."""
+    sensitivity to rounding has grown), of the run means, and pointwise
+    over every step."""
     n = min(len(loads), len(ref))
     out = {}
     for q, name in ((1, "CL"), (0, "CT")):
@@ -166,182 +171,227 @@ def load_differences(loads, ref, early=30):
     return out
 
 
-def run_single(args):
-    from uvlm_amd import Simulation
+def cpu_baseline(sim, case, seconds):
+    """Oracle flow velocity of every bound and wake ring (advect.h:16-24) at
+    a sample of the wake nodes of the final state, timed on the host cores
+    for at least `seconds`; the plain method's timesteps/s on the CPU is
+    extrapolated from that pair rate and the run's pair model."""
+    import oracle
     from uvlm_amd import workloads as W
-    from uvlm_amd.gpu import mlp_init
+    oracle.build()
+    wing, g = sim.wing_nodes(), sim.gamma()
+    wn, wg = sim.wake()
+    targets = wn.reshape(-1, 3)
+    n_src = g.size + wg.size
+    rng = np.random.default_rng(7)
+    nt = 64
+    done_pairs, done_t, rounds = 0.0, 0.0, 0
+    while done_t < seconds:
+        x = targets[rng.integers(0, targets.shape[0], nt)]
+        t0 = time.perf_counter()
+        oracle.induced_velocity(x, wing, g)
+        if wg.size:
+            oracle.induced_velocity(x, wn, wg)
+        dt = time.perf_counter() - t0
+        done_t += dt
+        done_pairs += 4.0 * nt * n_src
+        rounds += 1
+        if dt < 0.5:
+            nt *= 2
+    rate = done_pairs / done_t
+    S = int(case["steps"])
+    cpu_run_s = (W.pair_model(case, steps=S) - W.pair_model(case, steps=0)) / rate
+    return {"value": S / cpu_run_s, "unit": "timesteps/s", "cores": oracle.num_threads(),
+            "kind": "port", "pairs_per_s": rate,
+            "sample": (f"oracle sum of all {n_src} bound and wake rings at random wake "
+                       f"nodes of the final state ({done_pairs:.3e} pairs in {done_t:.1f} s, "
+                       f"{rounds} rounds); timesteps/s of the plain method over steps 1..{S} "
+                       f"extrapolated from that rate and the run's pair model "
+                       f"({cpu_run_s:.0f} s of CPU time)")}
 
-    total = args.warmup + args.steps
-    case = W.WORKLOADS[args.workload](steps=total - 1)
-    case["precision"] = PRECISIONS[args.precision]
-    case["solve_mode"] = SOLVES[args.solve]
-    sim = Simulation(case, total - 1)
-    if args.farfield > 0:
-        sim.set_farfield(args.farfield, args.audit, args.ff_tol)
-        if args.ff_reach > 0:
-            sim.set_farfield_focus(args.ff_reach, args.ff_theta_min)
-    for s in range(args.warmup):
-        sim.step(s)
-    sim.sync()
-    sync_all()
-    e0, r0 = sim.pairs()
-    sim.profile(True)
-    sim.profile_read()
-    w = mlp_init(5) if args.mlp_every > 0 else None
-    learned = 0
-    net_pairs = net_trips = 0  # counters survive retraining (which resets them)
-    t0 = time.perf_counter()
-    for s in range(args.warmup, total):
-        sim.step(s)
-        if (w is not None and args.farfield > 0 and s >= args.mlp_start
-                and (s - args.mlp_start) % args.mlp_every == 0):
-            if learned:
-                p_, t_ = sim.farfield_mlp_stats()
-                net_pairs += p_
-                net_trips += t_
-            w, _ = sim.farfield_learn(w, args.farfield, args.mlp_samples, args.mlp_iters,
-                                      512, 1e-3, seed=s)
-            learned += 1
-        if (s - args.warmup) % 100 == 99:
-            print(f"# step {s}", file=sys.stderr, flush=True)
-    sim.sync()
-    sync_all()
-    T = time.perf_counter() - t0
-    launches, kms, kpairs = sim.profile_read()
-    sim.profile(False)
-    e1, r1 = sim.pairs()
-    loads = sim.loads()
-    achieved = FLOP_PER_PAIR * kpairs / (kms / 1e3) / 1e12 if kms > 0 else 0.0
-    traffic = traffic_from_profile() if (args.workload == "c2" and args.precision == "f64"
-                                         and args.solve == "inv" and args.steps == 1000
-                                         and args.warmup == 1) else None
-    nw = len(case["wings"])
+
+def bench_tandem(args):
+    from uvlm_amd import workloads as W
+
+    case = W.tandem_c5()
+    S = int(case["steps"])
+    fast_m = dict(FAST)
+    log("fast form")
+    fast = run_case(case, fast_m, args.steps, args.warmup, "fast")
+    fast_sim = fast.pop("sim")
+    K, first = fast["K"], fast["first"]
+    ref_pairs_win = W.pair_model(case, steps=S) - W.pair_model(case, steps=first - 1)
     out = {
-        "metric": METRIC,
-        "value": (r1 - r0) / T,
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * T / args.steps,
-        "higher_is_better": True,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": {"f64exact": "f64", "f64": "f64", "f32c": "f32"}[args.precision],
-        "data": "synthetic",
-        "config": {"workload": f"{case['name']}_{total - 1}steps", "wings": nw,
-                   "rings_per_wing": [case["rows"], case["cols"]],
-                   "advect": case["advect"], "dt": case["dt"],
-                   "timed_steps": [args.warmup, total - 1],
-                   "final_wake_rings": nw * (total - 1) * case["cols"],
-                   "precision": args.precision, "solve": args.solve,
-                   "farfield_theta": args.farfield if args.farfield > 0 else None},
-        "timesteps_per_s": args.steps / T,
-        "gpu_evaluated_pairs_per_s": (e1 - e0) / T,
-        "final_CD_CL": loads[-1].tolist() if len(loads) else None,
-        "roofline": {"bound": "valu", "kernel": ROOFLINE_KERNEL[args.precision][0],
-                     "achieved": achieved, "peak": ROOFLINE_KERNEL[args.precision][1],
-                     "unit": "TFLOP/s",
-                     "frac": achieved / ROOFLINE_KERNEL[args.precision][1], "traffic": traffic,
-                     "launches": launches, "avg_launch_us": 1e3 * kms / max(launches, 1),
-                     "flop_per_launch": FLOP_PER_PAIR * kpairs / max(launches, 1),
-                     "kernel_share_of_step_time": kms / 1e3 / T},
+        "metric": METRIC, "value": K / fast["t_win"], "unit": "timesteps/s", "n_gpus": 1,
+        "steps": K, "warmup": fast["warmup"], "ms_per_step": 1e3 * fast["t_win"] / K,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{case['name']}_{S}steps", "wings": 2,
+                   "rings_per_wing": [case["rows"], case["cols"]], "advect": case["advect"],
+                   "dt": case["dt"], "run_steps": S, "timed_steps": [first, S],
+                   "final_wake_rings": 2 * S * case["cols"],
+                   "method": {"sums": "near/far split of the wake: lattice-block moment "
+                                      "expansions (orders 0-3) for far clusters, exact "
+                                      "near blocks in FP32 with FP64 accumulation and "
+                                      "FP64 near-core re-evaluation; bound sheets exact",
+                              "theta": fast_m["farfield"],
+                              "graded": {"reach": fast_m["ff_reach"],
+                                         "theta_min": fast_m["ff_theta_min"]},
+                              "audit_every": fast_m["audit"],
+                              "solve": "kept-inverse FP64, iterative refinement to 1e-14",
+                              "loads": "FP64"}},
+        "timesteps_per_s_run": S / fast["t_run"],
+        "work_rate_reference_counting": {
+            "value": ref_pairs_win / fast["t_win"], "unit": "pairs/s",
+            "note": ("pairs the reference loop would evaluate in the window "
+                     "(4 [N^2 + N Nw + L (N + Nw) + V (N + Nw)] per step) / window time: "
+                     "a work-equivalent rate, not a hardware rate")},
+        "roofline": roofline("k_ff_near<F32C> (near-block sums of the far-field split)",
+                             fast["near"], fast["t_run"], PEAK_FP32_TFLOPS, "k_ff_near"),
     }
-    if args.farfield > 0:
-        err, audits, active = sim.farfield_stats()
-        out["farfield"] = {"theta": args.farfield, "audit_every": args.audit,
-                           "last_audit_rel_err": err, "audits": audits, "active": active}
-        if args.ff_reach > 0:
-            out["farfield"]["graded"] = {"reach": args.ff_reach, "theta_min": args.ff_theta_min}
-        if w is not None:
-            pairs, trips = sim.farfield_mlp_stats()
-            pairs += net_pairs
-            trips += net_trips
-            out["farfield"]["network"] = {"trained": learned, "every": args.mlp_every,
-                                          "samples": args.mlp_samples,
-                                          "iters": args.mlp_iters, "corrected_pairs": pairs,
-                                          "exact_fallbacks": trips}
-    if args.solve == "inv":
-        sw, rf, rb, _ = sim.inverse_stats()
+    out["roofline"]["note"] = ("averaged over every launch of the fast run (steps 0..2000), "
+                               "as rocprofv3 sees it; FP32 segment law, peak = FP32 vector")
+    if "ff" in fast:
+        err, audits, active = fast["ff"]
+        out["farfield"] = {"last_audit_rel_err": err, "audits": audits, "active": bool(active)}
+    if "inv" in fast:
+        sw, rf, rb, _ = fast["inv"]
         out["inverse_solve"] = {"sweeps": sw, "newton_schulz_refreshes": rf, "rebuilds": rb}
-    if args.compare:
-        Tp, lp = plain_run(args)
-        out["vs_plain_fp64"] = {"plain_ms_per_step": 1e3 * Tp / args.steps,
-                                "timesteps_per_s_ratio": Tp / T,
-                                **load_differences(loads, lp)}
+    del fast_sim
+    plain_sim = None
+    if not args.no_plain:
+        log("plain FP64 method")
+        plain = run_case(case, PLAIN, args.steps, args.warmup, "plain")
+        plain_sim = plain.pop("sim")
+        out["vs_plain_fp64"] = {
+            "plain_ms_per_step_run": 1e3 * plain["t_run"] / S,
+            "fast_ms_per_step_run": 1e3 * fast["t_run"] / S,
+            "timesteps_per_s_ratio_run": plain["t_run"] / fast["t_run"],
+            "plain_ms_per_step_window": 1e3 * plain["t_win"] / K,
+            "timesteps_per_s_ratio_window": plain["t_win"] / fast["t_win"],
+            **load_differences(fast["loads"], plain["loads"])}
+        out["vs_plain_fp64"]["plain_chain_roofline"] = roofline(
+            "k_chain_f64 (plain run)", plain["chain"], plain["t_run"], PEAK_FP64_TFLOPS)
+    if not args.no_c2:
+        log("NACA4412 block")
+        c2 = W.naca4412_c2()
+        r = run_case(c2, {"precision": "f64", "solve": "inv"}, 1000, 0, "c2")
+        sim2 = r.pop("sim")
+        S2 = r["S"]
+        ref2 = W.pair_model(c2, steps=S2) - W.pair_model(c2, steps=0)
+        out["c2_naca4412"] = {
+            "workload": f"{c2['name']}_{S2}steps", "ms_per_step": 1e3 * r["t_run"] / S2,
+            "timesteps_per_s": S2 / r["t_run"],
+            "work_rate_reference_counting_pairs_per_s": ref2 / r["t_run"],
+            "roofline": roofline("k_chain_f64", r["chain"], r["t_run"], PEAK_FP64_TFLOPS,
+                                 "k_chain_f64")}
+        sim2.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sim, args.cpu_seconds)
+        log("CPU baseline")
+        sim = plain_sim
+        if sim is None:
+            raise SystemExit("the CPU baseline samples the plain run's final state")
+        out["cpu_baseline"] = cpu_baseline(sim, case, args.cpu_seconds)
+    if plain_sim is not None:
+        plain_sim.close()
     return out
 
 
-def run_batch(args):
+def bench_single(args):
+    from uvlm_amd import workloads as W
+    case = W.WORKLOADS[args.workload]()
+    m = {"precision": args.precision, "solve": args.solve, "farfield": args.farfield,
+         "ff_reach": args.ff_reach, "ff_theta_min": args.ff_theta_min, "audit": args.audit,
+         "ff_tol": args.ff_tol}
+    r = run_case(case, m, args.steps, args.warmup, args.workload)
+    sim = r.pop("sim")
+    S, K, first = r["S"], r["K"], r["first"]
+    peak = PEAK_FP32_TFLOPS if args.precision == "f32c" else PEAK_FP64_TFLOPS
+    chain_name = {"f64exact": "k_nbody", "f64": "k_chain_f64", "f32c": "k_chain_f32c"}
+    out = {"metric": METRIC, "value": K / r["t_win"], "unit": "timesteps/s", "n_gpus": 1,
+           "steps": K, "warmup": r["warmup"], "ms_per_step": 1e3 * r["t_win"] / K,
+           "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": {"f64exact": "f64", "f64": "f64", "f32c": "f32+f64"}[args.precision],
+           "data": "synthetic",
+           "config": {"workload": f"{case['name']}_{S}steps", "timed_steps": [first, S],
+                      "precision": args.precision, "solve": args.solve,
+                      "farfield_theta": args.farfield or None},
+           "timesteps_per_s_run": S / r["t_run"],
+           "chain_roofline": roofline(chain_name[args.precision], r["chain"], r["t_run"], peak,
+                                      "k_chain_f64" if (args.workload == "c2"
+                                                        and args.precision == "f64") else None)}
+    if args.farfield > 0:
+        out["near_roofline"] = roofline("k_ff_near", r["near"], r["t_run"], peak)
+    sim.close()
+    return out
+
+
+def bench_batch(args):
     from uvlm_amd import workloads as W
     from uvlm_amd.gpu import Batch
 
-    total = args.warmup + args.steps
-    cases = W.batch_c4(steps=total - 1)
-    b = Batch(cases, max_steps=total - 1, advect="RK2")
-    b.run(args.warmup, graph=not args.no_graph)
-    chosen, times = b.autotune()
+    S = 300
+    cases = W.batch_c4(steps=S)
+    K = max(1, min(args.steps, S))
+    first = S - K + 1
+    b = Batch(cases, max_steps=S, advect="RK2")
+    b.run(1, graph=not args.no_graph)  # step 0
+    chosen, _ = b.autotune()
     b.sync()
-    p0 = b.pairs()
-    sync_all()
     t0 = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        k = min(50, args.steps - done)
-        b.run(k, graph=not args.no_graph)
-        done += k
-        print(f"# step {args.warmup + done - 1}", file=sys.stderr, flush=True)
+
+    def advance(lo, hi):  # steps lo..hi, in chunks of at most 50
+        s = lo
+        while s <= hi:
+            k = min(50, hi - s + 1)
+            b.run(k, graph=not args.no_graph)
+            s += k
+            log(f"batch step {s - 1}")
+
+    advance(1, first - 1)
     b.sync()
-    sync_all()
-    T = time.perf_counter() - t0
-    p1 = b.pairs()
-    ref = 0.0
-    for c in cases:
-        ref += W.pair_model(c, steps=total - 1, stages=2) - W.pair_model(c, steps=args.warmup - 1,
-                                                                          stages=2)
-    return {
-        "metric": METRIC, "value": ref / T, "unit": "pairs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * T / args.steps,
-        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"batch_256_cases_rk2_{total - 1}steps", "cases": len(cases),
-                   "timed_steps": [args.warmup, total - 1], "autotuned_variant": chosen,
-                   "hipgraph": not args.no_graph},
-        "case_steps_per_s": len(cases) * args.steps / T,
-        "gpu_evaluated_pairs_per_s": (p1 - p0) / T,
-    }
+    tw0 = time.perf_counter()
+    advance(first, S)
+    b.sync()
+    t_end = time.perf_counter()
+    t_win = t_end - tw0
+    ref = sum(W.pair_model(c, steps=S, stages=2) - W.pair_model(c, steps=first - 1, stages=2)
+              for c in cases)
+    return {"metric": METRIC, "value": len(cases) * K / t_win, "unit": "case-timesteps/s",
+            "n_gpus": 1, "steps": K, "warmup": min(args.warmup, first - 1),
+            "ms_per_step": 1e3 * t_win / K, "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"batch_256_cases_rk2_{S}steps", "cases": len(cases),
+                       "timed_steps": [first, S], "autotuned_variant": chosen,
+                       "hipgraph": not args.no_graph},
+            "timesteps_per_s_run": S / (t_end - t0),
+            "work_rate_reference_counting_pairs_per_s": ref / t_win}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed UVLM timesteps")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed steps before")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=2000,
+                    help="timed window: the last K steps of the full-length run")
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed steps just before the window")
+    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--single", action="store_true",
+                    help="c5: run one method alone (as the other workloads) instead of the "
+                         "fast-vs-plain comparison")
     ap.add_argument("--precision", default="f64", choices=sorted(PRECISIONS))
     ap.add_argument("--solve", default="inv", choices=sorted(SOLVES))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--farfield", type=float, default=0.0)
+    ap.add_argument("--audit", type=int, default=50)
+    ap.add_argument("--ff-tol", type=float, default=1e-3)
+    ap.add_argument("--ff-reach", type=float, default=0.0)
+    ap.add_argument("--ff-theta-min", type=float, default=2.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--farfield", type=float, default=0.0,
-                    help="near/far split of the wake sum with this theta (0: exact sums)")
-    ap.add_argument("--audit", type=int, default=50, help="far field: exact audit every N steps")
-    ap.add_argument("--ff-tol", type=float, default=1e-3, help="far field: audit tolerance")
-    ap.add_argument("--ff-reach", type=float, default=0.0,
-                    help="far field: graded criterion beyond this many wing radii (0: off)")
-    ap.add_argument("--ff-theta-min", type=float, default=2.0,
-                    help="far field: smallest theta of the graded criterion")
-    ap.add_argument("--mlp-every", type=int, default=0,
-                    help="far field: train the correction network every K steps (0: off)")
-    ap.add_argument("--mlp-start", type=int, default=20, help="first training step")
-    ap.add_argument("--mlp-samples", type=int, default=60000)
-    ap.add_argument("--mlp-iters", type=int, default=1000)
-    ap.add_argument("--compare", action="store_true",
-                    help="also time the plain FP64 method over the same steps")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="batch: launch each step's kernels directly instead of a hipGraph")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -352,9 +402,14 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.warmup < 1:
-        args.warmup = 1
-    out = run_batch(args) if args.workload == "c4" else run_single(args)
+    if args.workload == "c4":
+        out = bench_batch(args)
+    elif args.workload == "c5" and not args.single:
+        if args.no_plain:
+            args.no_cpu = True
+        out = bench_tandem(args)
+    else:
+        out = bench_single(args)
     print(json.dumps(out), flush=True)
 
 

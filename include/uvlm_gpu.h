@@ -226,6 +226,11 @@ int uvlm_sim_farfield_stats(uvlm_ctx* ctx, double* last_audit_err, int* audits, 
  * Eigen::FullPivLU, src/simulator/simple_simulator.cpp:265-266). */
 int uvlm_sim_inverse_stats(uvlm_ctx* ctx, long long* sweeps, long long* refreshes,
                            long long* rebuilds, int* last_sweeps);
+/* Counters of UVLM_SOLVE_F32_IR / UVLM_SOLVE_BF16_IR in the simulation:
+ * solves, FP64 refinement sweeps in total and in the last solve.  No
+ * reference counterpart. */
+int uvlm_sim_refine_stats(uvlm_ctx* ctx, long long* solves, long long* sweeps,
+                          int* last_sweeps);
 
 /* --- far-field correction network (no reference counterpart) -----------
  * A small MLP (32 -> 128 -> 128 -> 128 -> 3, SiLU) that predicts the
@@ -311,9 +316,19 @@ typedef struct uvlm_batch_case {
 int uvlm_batch_init(uvlm_ctx* ctx, const uvlm_batch_case* cases, size_t n,
                     const double* wing_nodes, size_t max_steps, int scheme,
                     size_t ignore_steps);
+/* Host-only check of a batch plan (no device work, no context): lays out
+ * the n cases as uvlm_batch_init does and replays the store and load
+ * extents of every kernel of steps 0..max_steps against the allocations.
+ * 0 when every access is in bounds, else UVLM_ERR_ARG (invalid case) or
+ * UVLM_ERR_CAPACITY with the first offending access written to msg
+ * (nullable, msg_len bytes).  uvlm_batch_init runs the same check. */
+int uvlm_batch_check_plan(const uvlm_batch_case* cases, size_t n, size_t max_steps, char* msg,
+                          size_t msg_len);
 /* Enqueue `steps` timesteps of every case (MainLoop, simple_simulator.cpp
  * :240-286, per case).  use_graph: replay one captured hipGraph per step.
- * Asynchronous. */
+ * The host waits for the device after every step, so that at most one
+ * step's launches (a few hundred on the eager path) are queued at a time.
+ * Returns after the last step completed. */
 int uvlm_batch_run(uvlm_ctx* ctx, size_t steps, int use_graph);
 /* Time every compiled launch shape of the batch's N-body on the current
  * state and keep the fastest.  chosen (nullable) receives its index,
@@ -346,6 +361,11 @@ int uvlm_batch_free(uvlm_ctx* ctx);
 int uvlm_profile_enable(uvlm_ctx* ctx, int enable);
 int uvlm_profile_read(uvlm_ctx* ctx, size_t* launches, double* kernel_ms,
                       double* pairs);
+/* The same for the near-block sums of the far-field split (k_ff_near): its
+ * launches since the last read, their summed time and their (target,
+ * segment) pairs, counted on the device. */
+int uvlm_profile_read_near(uvlm_ctx* ctx, size_t* launches, double* kernel_ms,
+                           double* pairs);
 
 #ifdef __cplusplus
 }

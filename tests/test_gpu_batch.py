@@ -155,3 +155,67 @@ def test_pivoted_fallback_agrees(orc):
     b.sync()
     for k, case in enumerate(cases):
         check_case(b, k, *oracle_run(orc, case, 8))
+
+
+# --- the 256-case workload at its full size ---------------------------------
+C4_STEPS = 300
+
+
+@pytest.fixture(scope="module")
+def c4_full():
+    cases = W.batch_c4(steps=C4_STEPS)
+    print(f"[batch c4] {len(cases)} cases, {C4_STEPS} steps, RK2, graph", flush=True)
+    b = Batch(cases, max_steps=C4_STEPS, advect="RK2")
+    b.run(C4_STEPS + 1, graph=True)  # steps 0..300
+    b.sync()
+    yield cases, b
+
+
+def c4_sample(cases, k=8):
+    """k cases spread over the lattice sizes (smallest to largest N)."""
+    order = sorted(range(len(cases)), key=lambda i: (cases[i]["rows"] * cases[i]["cols"], i))
+    return [order[round(q * (len(order) - 1) / (k - 1))] for q in range(k)]
+
+
+def test_c4_full_invariants(c4_full):
+    """Every case of the 256: 301 load rows, all finite; 300 wake circulation
+    rows; the trailing-edge wake row moved at -ff from the final trailing
+    edge (advect.h:41-45; RK2 uses Euler there), bitwise."""
+    cases, b = c4_full
+    assert b.steps == C4_STEPS + 1
+    for k, c in enumerate(cases):
+        loads = b.loads(k)
+        assert loads.shape == (C4_STEPS + 1, 2) and np.all(np.isfinite(loads)), k
+        nodes, gam = b.wake(k)
+        assert nodes.shape == (C4_STEPS + 1, c["cols"] + 1, 3), k
+        assert gam.shape == (C4_STEPS, c["cols"]), k
+        assert np.all(np.isfinite(nodes)) and np.all(np.isfinite(gam)), k
+        te = b.wing_nodes(k)[-1]
+        ff = np.asarray(c["forward_flight"], dtype=np.float64)
+        np.testing.assert_array_equal(nodes[0], te + (-ff) * c["dt"])
+
+
+def test_c4_full_early_window_vs_oracle(orc, c4_full):
+    """8 cases of different sizes: CD and CL of steps 0..40 of the full batch
+    agree with the oracle to 1e-9 (before the roll-up amplifies rounding:
+    two oracle runs that differ only in pivoting part by up to 1e-5 in
+    the loads of some cases by step 300)."""
+    cases, b = c4_full
+    early = 40
+    for k in c4_sample(cases):
+        c = cases[k]
+        print(f"[batch c4] oracle case {k}: {c['rows']}x{c['cols']}", flush=True)
+        _, ol = oracle_run(orc, c, early + 1)
+        gl = b.loads(k)[: early + 1]
+        assert rel_err(gl[:, 0], ol[:, 0]) < TOL, k
+        assert rel_err(gl[:, 1], ol[:, 1]) < TOL, k
+
+
+def test_c4_full_run_of_a_stable_case_vs_oracle(orc, c4_full):
+    """Case 245 (5 x 8 rings) is one whose oracle runs with full and with
+    partial pivoting stay within 1e-14 of each other for all 300 steps: its
+    whole state at step 300 agrees with the oracle to 1e-9."""
+    cases, b = c4_full
+    k = 245
+    o, ol = oracle_run(orc, cases[k], C4_STEPS + 1)
+    check_case(b, k, o, ol)

@@ -270,3 +270,37 @@ def test_legacy_loop_matches_oracle(orc, name, make, steps):
     bn2, bg2, wn2, wg2 = legacy_run(case, steps)
     np.testing.assert_array_equal(bg2, bg)
     np.testing.assert_array_equal(wn2, wn)
+
+
+@pytest.mark.parametrize("solve", ["f32ir", "bf16ir"])
+def test_full_size_precision_path(solve):
+    """The 32 x 128 morphing wing (4096 unknowns): F32C chains with the FP32
+    or BF16 LU and FP64 iterative refinement against the plain FP64 method
+    (FP64 chains, FP64 LU) over its first 8 steps.  The refinement drives
+    the solution to 1e-14 of the FP32/F32C system, so the errors are the
+    F32C sums' own: circulations, wake and CL/CT within 1e-6 of their
+    largest magnitude (the precision's stated bound is 1e-4 on CL/CT), and
+    every solve converges within 30 sweeps."""
+    steps = 8
+    case = W.morphing_c3(steps=steps)
+    mode = {"f32ir": N.SOLVE_F32_IR, "bf16ir": N.SOLVE_BF16_IR}[solve]
+    ref = Simulation(dict(case, precision=N.PREC_F64, solve_mode=N.SOLVE_F64), steps)
+    mix = Simulation(dict(case, precision=N.PREC_F32C, solve_mode=mode), steps)
+    for s in range(steps + 1):
+        print(f"[morphing 32x128 {solve}] step {s}", flush=True)
+        ref.step(s)
+        mix.step(s)
+    ref.sync()
+    mix.sync()
+    lr, lm = ref.loads(), mix.loads()
+    errs = {"gamma": rel_err(mix.gamma(), ref.gamma()),
+            "wake_nodes": rel_err(mix.wake()[0], ref.wake()[0]),
+            "wake_gamma": rel_err(mix.wake()[1], ref.wake()[1]),
+            "CT": rel_err(lm[..., 0], lr[..., 0]), "CL": rel_err(lm[..., 1], lr[..., 1])}
+    solves, sweeps, last = mix.refine_stats()
+    print(f"{solve}: errors {errs}, {solves} solves, {sweeps} sweeps", flush=True)
+    assert solves == steps + 1
+    assert sweeps <= 30 * solves and 1 <= last <= 30
+    for k, e in errs.items():
+        assert e < 1e-6, (k, e)
+    assert errs["gamma"] > 0.0

@@ -37,9 +37,10 @@ GPU_SYMBOLS = [
     "uvlm_sim_erase_oldest_wake", "uvlm_sim_wake_shape", "uvlm_sim_get_wake",
     "uvlm_sim_set_wake", "uvlm_sim_get_gamma", "uvlm_sim_get_loads",
     "uvlm_sim_pair_counts", "uvlm_profile_enable", "uvlm_profile_read",
+    "uvlm_profile_read_near", "uvlm_batch_check_plan",
     "uvlm_farfield_velocity", "uvlm_sim_set_farfield", "uvlm_sim_set_farfield_focus",
     "uvlm_sim_farfield_stats",
-    "uvlm_sim_inverse_stats",
+    "uvlm_sim_inverse_stats", "uvlm_sim_refine_stats",
     "uvlm_mlp_forward", "uvlm_mlp_grad", "uvlm_mlp_train", "uvlm_farfield_set_mlp",
     "uvlm_farfield_mlp_stats", "uvlm_farfield_samples", "uvlm_sim_farfield_samples",
     "uvlm_sim_farfield_learn",
@@ -143,6 +144,7 @@ def gpu_lib():
         L.uvlm_sim_pair_counts.argtypes = [_vp, _dp, _dp]
         L.uvlm_profile_enable.argtypes = [_vp, C.c_int]
         L.uvlm_profile_read.argtypes = [_vp, C.POINTER(_sz), _dp, _dp]
+        L.uvlm_profile_read_near.argtypes = [_vp, C.POINTER(_sz), _dp, _dp]
         L.uvlm_farfield_velocity.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
                                              _dp]
         L.uvlm_sim_set_farfield.argtypes = [_vp, C.c_double, C.c_int, C.c_double]
@@ -150,6 +152,7 @@ def gpu_lib():
         L.uvlm_sim_farfield_stats.argtypes = [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _ll = C.POINTER(C.c_longlong)
         L.uvlm_sim_inverse_stats.argtypes = [_vp, _ll, _ll, _ll, C.POINTER(C.c_int)]
+        L.uvlm_sim_refine_stats.argtypes = [_vp, _ll, _ll, C.POINTER(C.c_int)]
         L.uvlm_mlp_forward.argtypes = [_vp, _fp, _fp, _sz, _fp, C.c_int]
         L.uvlm_mlp_grad.argtypes = [_vp, _fp, _fp, _fp, _sz, _fp, _fp]
         L.uvlm_mlp_train.argtypes = [_vp, _fp, _fp, _fp, _sz, _sz, _sz, C.c_float, C.c_uint,
@@ -164,6 +167,7 @@ def gpu_lib():
         L.uvlm_sim_farfield_learn.argtypes = [_vp, C.c_double, _sz, _sz, _sz, C.c_float,
                                               C.c_uint, _fp, _fp]
         L.uvlm_batch_init.argtypes = [_vp, C.POINTER(BatchCase), _sz, _dp, _sz, C.c_int, _sz]
+        L.uvlm_batch_check_plan.argtypes = [C.POINTER(BatchCase), _sz, _sz, C.c_char_p, _sz]
         L.uvlm_batch_run.argtypes = [_vp, _sz, C.c_int]
         L.uvlm_batch_autotune.argtypes = [_vp, C.POINTER(C.c_int), _dp, C.c_int]
         L.uvlm_batch_steps.argtypes = [_vp, C.POINTER(_sz)]

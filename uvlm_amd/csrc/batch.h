@@ -21,6 +21,20 @@ struct BCase {
   double ff[3], origin[3], plug[3], flap[3];
 };
 
+// Ring-buffer rows of a case's wake at step s (after the shed of step s the
+// wake has s + 1 node rows and s circulation rows): node row i is physical
+// row b_node_row0 + i, circulation row i is physical row b_gam_row0 + i.
+// Both buffers hold cap = max_steps + 2 rows per case.
+__host__ __device__ inline long b_node_row0(const BCase& c, int s) { return (long)c.cap - (s + 1); }
+__host__ __device__ inline long b_gam_row0(const BCase& c, int s) { return (long)c.cap - s; }
+// Load history: (CD, CL) of case cs at step `step` at hist[b_hist_index].
+__host__ __device__ inline long b_hist_index(int step, int n_cases, int cs) {
+  return 2 * ((long)step * n_cases + cs);
+}
+// Chain elements of a lattice of `rows` ring rows (node rows, then node
+// columns; see k_b_pack_chains).
+__host__ __device__ inline long b_chain_elems(int rows, int C) { return 2L * (rows + 1) * (C + 1); }
+
 // Device-resident step counter and the work counters of the persistent
 // N-body launches.
 struct BState {
@@ -35,6 +49,22 @@ struct BatchCaseSpec {
   double chord, span, dt, alpha;
   double origin[3], ff[3], plug[3], flap[3];
 };
+
+// Offsets of every case in the concatenated arrays and the totals (host
+// only, no device work): the plan BatchEngine::init allocates.
+struct BatchPlan {
+  std::vector<BCase> cases;
+  long nodes = 0, rings = 0, edges = 0, A = 0, wake = 0, wgam = 0, cb = 0, cw = 0, te = 0;
+  long hist = 0;  // doubles of the load history
+  int maxN = 0;
+};
+int batch_plan(const BatchCaseSpec* specs, size_t n, size_t max_steps, BatchPlan* plan,
+               std::string* err);
+// Host replay of the store and load extents of every kernel of every step
+// 0..max_steps of a plan (the launches of BatchEngine::step_once, with the
+// index helpers above): 0 when all stay inside the allocations, else a
+// description of the first one that does not.
+int batch_check_extents(const BatchPlan& plan, size_t max_steps, std::string* err);
 
 class BatchEngine {
  public:
@@ -90,6 +120,7 @@ class BatchEngine {
   int* d_piv_ = nullptr;
   int* d_flag_ = nullptr;
   double* d_Asave_ = nullptr;
+  double* d_dblk_ = nullptr;  // factored diagonal block of the current panel, per case
   long tot_A_ = 0;
   bool force_pivoted_ = false;
   RingRec* d_rings_ = nullptr;

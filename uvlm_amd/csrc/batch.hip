@@ -144,12 +144,12 @@ __global__ void k_b_shed(const BCase* __restrict__ cases, const BState* __restri
   const BCase& c = cases[te_case[k]];
   const int j = te_j[k], s = st->step, C1 = c.C + 1;
   const double* src = wing + 3 * (c.node_off + (long)c.R * C1 + j);
-  double* dst = wake + 3 * (c.wake_off + (long)(c.cap - (s + 1)) * C1 + j);
+  double* dst = wake + 3 * (c.wake_off + b_node_row0(c, s) * C1 + j);
   dst[0] = src[0];
   dst[1] = src[1];
   dst[2] = src[2];
   if (s > 0 && j < c.C)
-    wgam[c.wgam_off + (long)(c.cap - s) * c.C + j] = gamma[c.ring_off + (long)(c.R - 1) * c.C + j];
+    wgam[c.wgam_off + b_gam_row0(c, s) * c.C + j] = gamma[c.ring_off + (long)(c.R - 1) * c.C + j];
 }
 
 __global__ void k_b_geometry(const BCase* __restrict__ cases, const int* __restrict__ ring_case,
@@ -253,9 +253,9 @@ __global__ void k_b_pack_chains(const BCase* __restrict__ cases, const BState* _
   long row0, gstride_off;
   const double* base;
   if (WAKE) {
-    row0 = compact ? 0 : (c.cap - (s + 1));
+    row0 = compact ? 0 : b_node_row0(c, s);
     base = nodes + 3 * (compact ? c.wake_off : c.wake_off);
-    gstride_off = c.wgam_off + (long)(c.cap - s) * c.C;
+    gstride_off = c.wgam_off + b_gam_row0(c, s) * c.C;
   } else {
     row0 = 0;
     base = nodes + 3 * c.node_off;
@@ -508,9 +508,14 @@ __global__ void k_b_laswp(const BCase* __restrict__ cases, int k0, double* __res
   }
 }
 
+// U12 solve per case.  dblk (nullable): the diagonal blocks k_b_panel_np
+// left (one kNB x kNB block per case, column-major); L11 is read from there
+// and workgroup 0 of each case stores its block in A.  Without dblk (the
+// pivoted path, factored in place by k_b_getf2) L11 is read from A.
 __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, int k0,
                                                double* __restrict__ Ab,
-                                               const int* __restrict__ only) {
+                                               const int* __restrict__ only,
+                                               const double* __restrict__ dblk) {
   if (only && !only[blockIdx.y]) return;
   const BCase& c = cases[blockIdx.y];
   const int n = c.N;
@@ -518,9 +523,13 @@ __global__ __launch_bounds__(64) void k_b_trsm(const BCase* __restrict__ cases, 
   const int kb = min(kNB, n - k0);
   double* A = Ab + c.A_off;
   __shared__ double L[kNB][kNB + 1];
+  const double* db = dblk ? dblk + (long)blockIdx.y * kNB * kNB : nullptr;
   for (int q = threadIdx.x; q < kNB * kNB; q += 64) {
     const int r = q % kNB, cc = q / kNB;
-    L[r][cc] = (r < kb && cc < kb) ? A[(long)(k0 + cc) * n + k0 + r] : 0.0;
+    const bool in = r < kb && cc < kb;
+    const double v = !in ? 0.0 : db ? db[cc * kNB + r] : A[(long)(k0 + cc) * n + k0 + r];
+    L[r][cc] = v;
+    if (db && in && blockIdx.x == 0) A[(long)(k0 + cc) * n + k0 + r] = v;
   }
   const int t = threadIdx.x;
   const int col = k0 + kb + blockIdx.x * 64 + t;
@@ -607,12 +616,16 @@ __global__ __launch_bounds__(256) void k_b_gemm(const BCase* __restrict__ cases,
 
 // Checked no-pivot panel (the single-case path's k_panel_np, per case):
 // grid (row blocks of 64, cases); every workgroup factors the case's top
-// block itself, workgroup 0 stores it; flag[case] <- 1 when a comparison of
-// partial pivoting would have chosen an interchange.
+// block itself; workgroup 0 writes it to the case's dblk slot (k_b_trsm
+// stores it in A: a workgroup of this launch dispatched after workgroup 0
+// finished would otherwise load the factored block), and into A directly
+// when it is the case's only workgroup; flag[case] <- 1 when a comparison
+// of partial pivoting would have chosen an interchange.
 __global__ __launch_bounds__(64) void k_b_panel_np(const BCase* __restrict__ cases, int k0,
                                                    double* __restrict__ Ab,
                                                    int* __restrict__ piv,
-                                                   int* __restrict__ flag) {
+                                                   int* __restrict__ flag,
+                                                   double* __restrict__ dblk) {
   const BCase& c = cases[blockIdx.y];
   const int n = c.N;
   if (k0 >= n) return;
@@ -653,9 +666,12 @@ __global__ __launch_bounds__(64) void k_b_panel_np(const BCase* __restrict__ cas
     }
   }
   if (blockIdx.x == 0) {
+    const bool alone = n - k0 - kb <= 64;  // the case's only active workgroup
+    double* db = dblk + (long)blockIdx.y * kNB * kNB;
     for (int q = t; q < kb * kb; q += 64) {
       const int r = q % kb, cc = q / kb;
-      A[(long)(k0 + cc) * n + k0 + r] = D[r][cc];
+      db[cc * kNB + r] = D[r][cc];
+      if (alone) A[(long)(k0 + cc) * n + k0 + r] = D[r][cc];
     }
     if (t < kb) piv[c.ring_off + k0 + t] = k0 + t;
   }
@@ -795,7 +811,7 @@ __global__ __launch_bounds__(256) void k_b_loads(
   if (threadIdx.x == 0) {
     const d3 F = add(d3{red[0][0], red[1][0], red[2][0]}, d3{red[3][0], red[4][0], red[5][0]});
     const double den = 0.5 * c.Q * c.Q * c.ref_area;
-    double* o = hist + 2 * ((long)st->step * n_cases + blockIdx.x);
+    double* o = hist + b_hist_index(st->step, n_cases, blockIdx.x);
     o[0] = F.x / den;
     o[1] = F.z / den;
   }
@@ -811,7 +827,7 @@ __global__ void k_b_gather(const BCase* __restrict__ cases, const BState* __rest
   const int C1 = c.C + 1, s = st->step;
   const long local = k - c.wake_off;
   if (local >= (long)(s + 1) * C1) return;
-  const double* src = wake + 3 * (c.wake_off + (long)(c.cap - (s + 1)) * C1 + local);
+  const double* src = wake + 3 * (c.wake_off + b_node_row0(c, s) * C1 + local);
   pos[3 * k + 0] = src[0];
   pos[3 * k + 1] = src[1];
   pos[3 * k + 2] = src[2];
@@ -853,7 +869,7 @@ __global__ void k_b_update(const BCase* __restrict__ cases, const BState* __rest
   } else {
     r = add(w, scl(a, c.dt));
   }
-  st3(wake + 3 * (c.wake_off + (long)(c.cap - (s + 1)) * C1 + local), r);
+  st3(wake + 3 * (c.wake_off + b_node_row0(c, s) * C1 + local), r);
 }
 
 // ---- launch-shape variants of the persistent N-body (autotuned) -----------
@@ -928,6 +944,114 @@ void BatchEngine::release() {
   allocs_.clear();
 }
 
+int batch_plan(const BatchCaseSpec* specs, size_t n, size_t max_steps, BatchPlan* plan,
+               std::string* err) {
+  const int cap = (int)max_steps + 2;
+  BatchPlan& P = *plan;
+  P = BatchPlan{};
+  P.cases.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const BatchCaseSpec& sp = specs[i];
+    if (sp.rows == 0 || sp.cols == 0 || (sp.cols & 1)) {
+      *err = "case " + std::to_string(i) + ": columns must be even and the lattice non-empty";
+      return UVLM_ERR_ARG;
+    }
+    if (sp.rows * sp.cols > 8192) {
+      *err = "case " + std::to_string(i) + ": lattice too large for the batched solve";
+      return UVLM_ERR_ARG;
+    }
+    BCase& c = P.cases[i];
+    std::memset(&c, 0, sizeof(c));
+    c.R = (int)sp.rows;
+    c.C = (int)sp.cols;
+    c.N = c.R * c.C;
+    c.E = c.R * (2 * c.C + 1);
+    c.cap = cap;
+    c.node_off = P.nodes;
+    c.ring_off = P.rings;
+    c.edge_off = P.edges;
+    c.A_off = P.A;
+    c.wake_off = P.wake;
+    c.wgam_off = P.wgam;
+    c.chb_off = P.cb;
+    c.chw_off = P.cw;
+    P.nodes += (long)(c.R + 1) * (c.C + 1);
+    P.rings += c.N;
+    P.edges += c.E;
+    P.A += (long)c.N * c.N;
+    P.wake += (long)cap * (c.C + 1);
+    P.wgam += (long)cap * c.C;
+    P.cb += b_chain_elems(c.R, c.C);
+    P.cw += b_chain_elems(cap - 1, c.C);
+    P.te += c.C + 1;
+    c.dt = sp.dt;
+    for (int d = 0; d < 3; ++d) {
+      c.ff[d] = sp.ff[d];
+      c.origin[d] = sp.origin[d];
+      c.plug[d] = sp.plug[d];
+      c.flap[d] = sp.flap[d];
+    }
+    c.alpha = sp.alpha;
+    c.Q = std::sqrt((sp.ff[0] * sp.ff[0] + sp.ff[1] * sp.ff[1]) + sp.ff[2] * sp.ff[2]);
+    c.ref_area = sp.chord * sp.span;
+    P.maxN = std::max(P.maxN, c.N);
+  }
+  P.hist = 2 * (long)n * (long)(max_steps + 2);
+  return UVLM_OK;
+}
+
+int batch_check_extents(const BatchPlan& P, size_t max_steps, std::string* err) {
+  const int n = (int)P.cases.size();
+  char buf[256];
+  auto bad = [&](const char* what, int cs, int s) {
+    snprintf(buf, sizeof(buf), "batch plan: %s out of bounds (case %d, step %d)", what, cs, s);
+    *err = buf;
+    return UVLM_ERR_CAPACITY;
+  };
+  // [lo, hi) must lie in [base, base + len) and in [0, total)
+  auto inside = [](long lo, long hi, long base, long len, long total) {
+    return lo >= base && hi <= base + len && lo >= 0 && hi <= total && lo <= hi;
+  };
+  for (int i = 0; i < n; ++i) {
+    const BCase& c = P.cases[i];
+    const int C1 = c.C + 1;
+    const long wlen = (long)c.cap * C1, glen = (long)c.cap * c.C;
+    for (int s = 0; s <= (int)max_steps; ++s) {
+      const long r0 = b_node_row0(c, s), g0 = b_gam_row0(c, s);
+      // k_b_shed: node row 0 (C + 1 nodes), circulation row 0 (s > 0)
+      if (!inside(c.wake_off + r0 * C1, c.wake_off + r0 * C1 + C1, c.wake_off, wlen, P.wake))
+        return bad("shed node row", i, s);
+      if (s > 0 && !inside(c.wgam_off + g0 * c.C, c.wgam_off + g0 * c.C + c.C, c.wgam_off, glen,
+                           P.wgam))
+        return bad("shed circulation row", i, s);
+      // k_b_gather / k_b_update / k_b_pack_chains<true> (ring buffer): node
+      // rows 0..s; circulation rows 0..s-1
+      if (!inside(c.wake_off + r0 * C1, c.wake_off + (r0 + s + 1) * C1, c.wake_off, wlen, P.wake))
+        return bad("wake node rows", i, s);
+      if (s > 0 && !inside(c.wgam_off + g0 * c.C, c.wgam_off + (g0 + s) * c.C, c.wgam_off, glen,
+                           P.wgam))
+        return bad("wake circulation rows", i, s);
+      // compact stage arrays (same offsets, rows 0..s) and the advection
+      // N-body's targets / outputs
+      if (!inside(c.wake_off, c.wake_off + (long)(s + 1) * C1, c.wake_off, wlen, P.wake))
+        return bad("stage rows", i, s);
+      // wake chain elements written and read per step
+      if (!inside(c.chw_off, c.chw_off + b_chain_elems(s, c.C), c.chw_off,
+                  b_chain_elems(c.cap - 1, c.C), P.cw))
+        return bad("wake chain elements", i, s);
+      // k_b_loads: (CD, CL) of this step
+      const long h = b_hist_index(s, n, i);
+      if (!inside(h, h + 2, 0, P.hist, P.hist)) return bad("load history", i, s);
+    }
+    if (!inside(c.chb_off, c.chb_off + b_chain_elems(c.R, c.C), c.chb_off,
+                b_chain_elems(c.R, c.C), P.cb))
+      return bad("bound chain elements", i, 0);
+    if (!inside(c.A_off, c.A_off + (long)c.N * c.N, c.A_off, (long)c.N * c.N, P.A))
+      return bad("influence matrix", i, 0);
+  }
+  return UVLM_OK;
+}
+
 int BatchEngine::init(hipStream_t s, const BatchCaseSpec* specs, const double* wing_init,
                       size_t n, size_t max_steps, int scheme, int ignore_steps,
                       std::string* err) {
@@ -946,63 +1070,21 @@ int BatchEngine::init(hipStream_t s, const BatchCaseSpec* specs, const double* w
     *err = "batched advection supports euler, RK2 and RK4";
     return UVLM_ERR_ARG;
   }
+  BatchPlan plan;
+  BTRY(batch_plan(specs, n, max_steps, &plan, err));
+  BTRY(batch_check_extents(plan, max_steps, err));
   const int cap = (int)max_steps + 2;
-  std::vector<BCase> hc(n);
-  long nodes = 0, rings = 0, edges = 0, Aoff = 0, woff = 0, goff = 0, cb = 0, cw = 0;
-  maxN_ = 0;
-  for (size_t i = 0; i < n; ++i) {
-    const BatchCaseSpec& sp = specs[i];
-    if (sp.rows == 0 || sp.cols == 0 || (sp.cols & 1)) {
-      *err = "case " + std::to_string(i) + ": columns must be even and the lattice non-empty";
-      return UVLM_ERR_ARG;
-    }
-    if (sp.rows * sp.cols > 8192) {
-      *err = "case " + std::to_string(i) + ": lattice too large for the batched solve";
-      return UVLM_ERR_ARG;
-    }
-    BCase& c = hc[i];
-    std::memset(&c, 0, sizeof(c));
-    c.R = (int)sp.rows;
-    c.C = (int)sp.cols;
-    c.N = c.R * c.C;
-    c.E = c.R * (2 * c.C + 1);
-    c.cap = cap;
-    c.node_off = nodes;
-    c.ring_off = rings;
-    c.edge_off = edges;
-    c.A_off = Aoff;
-    c.wake_off = woff;
-    c.wgam_off = goff;
-    c.chb_off = cb;
-    c.chw_off = cw;
-    nodes += (long)(c.R + 1) * (c.C + 1);
-    rings += c.N;
-    edges += c.E;
-    Aoff += (long)c.N * c.N;
-    woff += (long)cap * (c.C + 1);
-    goff += (long)cap * c.C;
-    cb += 2L * (c.R + 1) * (c.C + 1);
-    cw += 2L * cap * (c.C + 1);
-    c.dt = sp.dt;
-    for (int d = 0; d < 3; ++d) {
-      c.ff[d] = sp.ff[d];
-      c.origin[d] = sp.origin[d];
-      c.plug[d] = sp.plug[d];
-      c.flap[d] = sp.flap[d];
-    }
-    c.alpha = sp.alpha;
-    c.Q = std::sqrt((sp.ff[0] * sp.ff[0] + sp.ff[1] * sp.ff[1]) + sp.ff[2] * sp.ff[2]);
-    c.ref_area = sp.chord * sp.span;
-    maxN_ = std::max(maxN_, c.N);
-  }
+  const std::vector<BCase>& hc = plan.cases;
+  const long nodes = plan.nodes, rings = plan.rings, edges = plan.edges, Aoff = plan.A,
+             woff = plan.wake, goff = plan.wgam, cb = plan.cb, cw = plan.cw;
+  maxN_ = plan.maxN;
   tot_nodes_ = nodes;
   tot_rings_ = rings;
   tot_edges_ = edges;
   tot_wake_ = woff;
   tot_cb_ = cb;
   tot_cw_ = cw;
-  tot_te_ = 0;
-  for (const auto& c : hc) tot_te_ += c.C + 1;
+  tot_te_ = plan.te;
   cases_h_ = hc;
 
   std::vector<int> node_case(nodes), ring_case(rings), edge_case(edges), wn_case(woff), te_case,
@@ -1100,6 +1182,7 @@ int BatchEngine::init(hipStream_t s, const BatchCaseSpec* specs, const double* w
   BTRY(dev_alloc(allocs_, &d_A_, Aoff, err));
   BTRY(dev_alloc(allocs_, &d_Asave_, Aoff, err));
   BTRY(dev_alloc(allocs_, &d_flag_, n, err));
+  BTRY(dev_alloc(allocs_, &d_dblk_, (size_t)n * kNB * kNB, err));
   tot_A_ = Aoff;
   BTRY(dev_alloc(allocs_, &d_piv_, rings, err));
   BTRY(dev_alloc(allocs_, &d_wake_, 3 * woff, err));
@@ -1234,9 +1317,9 @@ int BatchEngine::step_once(std::string* err) {
   for (size_t p = 0; p + 1 < gemm_off_.size(); ++p) {
     const int k0 = (int)p * kNB;
     hipLaunchKernelGGL(k_b_panel_np, dim3(std::max(1, cdiv(maxN_ - k0, 64)), n_), dim3(64), 0, s,
-                       d_cases_, k0, d_A_, d_piv_, d_flag_);
+                       d_cases_, k0, d_A_, d_piv_, d_flag_, d_dblk_);
     hipLaunchKernelGGL(k_b_trsm, dim3(cdiv(maxN_, 64), n_), dim3(64), 0, s, d_cases_, k0, d_A_,
-                       (const int*)nullptr);
+                       (const int*)nullptr, (const double*)d_dblk_);
     const int ng = gemm_off_[p + 1] - gemm_off_[p];
     if (ng > 0)
       hipLaunchKernelGGL(k_b_gemm, dim3(ng), b256, 0, s, d_cases_, d_it_gemm_ + gemm_off_[p], k0,
@@ -1250,7 +1333,7 @@ int BatchEngine::step_once(std::string* err) {
     hipLaunchKernelGGL(k_b_laswp, dim3(cdiv(maxN_, 256), n_), b256, 0, s, d_cases_, k0, d_A_,
                        d_piv_, (const int*)d_flag_);
     hipLaunchKernelGGL(k_b_trsm, dim3(cdiv(maxN_, 64), n_), dim3(64), 0, s, d_cases_, k0, d_A_,
-                       (const int*)d_flag_);
+                       (const int*)d_flag_, (const double*)nullptr);
     const int ng = gemm_off_[p + 1] - gemm_off_[p];
     if (ng > 0)
       hipLaunchKernelGGL(k_b_gemm, dim3(ng), b256, 0, s, d_cases_, d_it_gemm_ + gemm_off_[p], k0,
@@ -1327,11 +1410,16 @@ int BatchEngine::run(size_t steps, bool use_graph, std::string* err) {
     return UVLM_ERR_CAPACITY;
   }
   if (use_graph) BTRY(capture(err));
+  // One step at a time: a step enqueues several hundred launches on the
+  // eager path (the batched LU alone is 7 per panel), and an unbounded
+  // backlog of them (4,382 packets were outstanding when a traced eager
+  // run lost a dispatch packet) is not something to rely on.
   for (size_t k = 0; k < steps; ++k) {
     if (use_graph)
       BHIP(hipGraphLaunch(graph_exec_, stream_));
     else
       BTRY(step_once(err));
+    BHIP(hipStreamSynchronize(stream_));
     ++steps_done_;
   }
   return UVLM_OK;
@@ -1405,10 +1493,10 @@ int BatchEngine::get_wake(int c, double* nodes, double* gamma, std::string* err)
   if (s < 0) return UVLM_OK;
   const int C1 = cs.C + 1;
   if (nodes)
-    BHIP(hipMemcpy(nodes, d_wake_ + 3 * (cs.wake_off + (long)(cs.cap - (s + 1)) * C1),
+    BHIP(hipMemcpy(nodes, d_wake_ + 3 * (cs.wake_off + b_node_row0(cs, s) * C1),
                    sizeof(double) * 3 * (s + 1) * C1, hipMemcpyDeviceToHost));
   if (gamma && s > 0)
-    BHIP(hipMemcpy(gamma, d_wgam_ + cs.wgam_off + (long)(cs.cap - s) * cs.C,
+    BHIP(hipMemcpy(gamma, d_wgam_ + cs.wgam_off + b_gam_row0(cs, s) * cs.C,
                    sizeof(double) * s * cs.C, hipMemcpyDeviceToHost));
   return UVLM_OK;
 }
@@ -1420,8 +1508,8 @@ int BatchEngine::get_loads(int c, double* out, size_t* count, std::string* err) 
   if (n)
     BHIP(hipMemcpy(h.data(), d_hist_, sizeof(double) * 2 * n_ * n, hipMemcpyDeviceToHost));
   for (size_t k = 0; k < n && out; ++k) {
-    out[2 * k + 0] = h[2 * (k * n_ + c) + 0];
-    out[2 * k + 1] = h[2 * (k * n_ + c) + 1];
+    out[2 * k + 0] = h[b_hist_index((int)k, n_, c) + 0];
+    out[2 * k + 1] = h[b_hist_index((int)k, n_, c) + 1];
   }
   *count = (size_t)steps_done_;
   return UVLM_OK;

@@ -82,6 +82,9 @@ struct uvlm_ctx {
   uvlm::InvSolver* inv = nullptr;
   uvlm::InvSolver* t_inv = nullptr;
   int inv_last_sweeps = 0;
+  // refinement sweeps of the mixed-precision solves in the simulation
+  long long ir_solves = 0, ir_sweeps = 0;
+  int ir_last = 0;
   // far-field correction network
   uvlm::MlpTrainer* mlp = nullptr;
   Buf mw, mx, mv, my, mg, mdev;
@@ -856,6 +859,9 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
                                      c->work.p, c->work.cap, &it);
     if (rc < 0) return fail(c, UVLM_ERR_ARG, "workspace too small");
     if (rc > 0) return fail(c, UVLM_ERR_NOCONV, "refinement did not converge");
+    c->ir_solves += 1;
+    c->ir_sweeps += it;
+    c->ir_last = it;
   } else if (solve_mode == UVLM_SOLVE_F64_INV) {
     // warm start from the previous step's circulations (zero after init)
     if (!c->inv && !(c->inv = new (std::nothrow) InvSolver()))
@@ -1200,6 +1206,23 @@ int uvlm_sim_get_loads(uvlm_ctx* c, double* out, size_t* count_inout) {
 int uvlm_profile_enable(uvlm_ctx* c, int enable) {
   if (!c) return UVLM_ERR_ARG;
   c->prof = enable != 0;
+  if (c->ff) c->ff->set_profile(c->prof);
+  return UVLM_OK;
+}
+
+int uvlm_profile_read_near(uvlm_ctx* c, size_t* launches, double* kernel_ms, double* pairs) {
+  if (!c) return UVLM_ERR_ARG;
+  size_t n = 0;
+  double ms = 0.0, pr = 0.0;
+  if (c->ff) {
+    TRY(set_dev(c));
+    std::string e;
+    const int rc = c->ff->profile_read(c->stream, &n, &ms, &pr, &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  }
+  if (launches) *launches = n;
+  if (kernel_ms) *kernel_ms = ms;
+  if (pairs) *pairs = pr;
   return UVLM_OK;
 }
 
@@ -1240,7 +1263,10 @@ int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const 
     return fail(c, UVLM_ERR_ARG, "problem too large");
   TRY(set_dev(c));
   if (nt == 0) return UVLM_OK;
-  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) {
+    c->ff = new (std::nothrow) FarField();
+    if (c->ff) c->ff->set_profile(c->prof);
+  }
   if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
   const size_t nb = 3 * num * (rows + 1) * (cols + 1), gb = num * rows * cols;
   TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
@@ -1294,7 +1320,10 @@ int mlp_upload(uvlm_ctx* c, const float* w, const float* X, const float* V, size
 int uvlm_farfield_set_mlp(uvlm_ctx* c, const float* weights, double ymax) {
   if (!c) return UVLM_ERR_ARG;
   TRY(set_dev(c));
-  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) {
+    c->ff = new (std::nothrow) FarField();
+    if (c->ff) c->ff->set_profile(c->prof);
+  }
   if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
   if (weights && !(ymax > 0.0)) return fail(c, UVLM_ERR_ARG, "ymax must be positive");
   std::string e;
@@ -1360,7 +1389,10 @@ int sim_samples(uvlm_ctx* c, double theta, size_t want, unsigned seed, std::vect
   f->clear();
   r->clear();
   if (c->wgrows == 0 || want == 0) return UVLM_OK;
-  if (!c->ff) c->ff = new (std::nothrow) FarField();
+  if (!c->ff) {
+    c->ff = new (std::nothrow) FarField();
+    if (c->ff) c->ff->set_profile(c->prof);
+  }
   if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
   const int C1 = c->C + 1, rows = c->wrows, V = c->num * rows * C1;
   const size_t vb = sizeof(double) * 3 * (size_t)V;
@@ -1432,7 +1464,10 @@ int uvlm_sim_farfield_learn(uvlm_ctx* c, double theta, size_t max_samples, size_
 int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol) {
   if (!c) return UVLM_ERR_ARG;
   if (theta > 0.0 && !(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
-  if (theta > 0.0 && !c->ff) c->ff = new (std::nothrow) FarField();
+  if (theta > 0.0 && !c->ff) {
+    c->ff = new (std::nothrow) FarField();
+    if (c->ff) c->ff->set_profile(c->prof);
+  }
   if (theta > 0.0 && !c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
   c->ff_theta = theta > 0.0 ? theta : 0.0;
   c->ff_audit_every = audit_every > 0 ? audit_every : 0;
@@ -1530,6 +1565,14 @@ int uvlm_mlp_train(uvlm_ctx* c, float* w, const float* feats, const float* targe
   return UVLM_OK;
 }
 
+int uvlm_sim_refine_stats(uvlm_ctx* c, long long* solves, long long* sweeps, int* last_sweeps) {
+  if (!c) return UVLM_ERR_ARG;
+  if (solves) *solves = c->ir_solves;
+  if (sweeps) *sweeps = c->ir_sweeps;
+  if (last_sweeps) *last_sweeps = c->ir_last;
+  return UVLM_OK;
+}
+
 int uvlm_sim_inverse_stats(uvlm_ctx* c, long long* sweeps, long long* refreshes,
                            long long* rebuilds, int* last_sweeps) {
   if (!c) return UVLM_ERR_ARG;
@@ -1562,6 +1605,19 @@ int uvlm_batch_init(uvlm_ctx* c, const uvlm_batch_case* cases, size_t n,
     return fail(c, rc, "%s", e.c_str());
   }
   return UVLM_OK;
+}
+
+int uvlm_batch_check_plan(const uvlm_batch_case* cases, size_t n, size_t max_steps, char* msg,
+                          size_t msg_len) {
+  if (!cases || n == 0 || n > (1u << 20) || max_steps > (1u << 20)) return UVLM_ERR_ARG;
+  BatchPlan plan;
+  std::string e;
+  int rc = batch_plan(reinterpret_cast<const BatchCaseSpec*>(cases), n, max_steps, &plan, &e);
+  if (rc == UVLM_OK) rc = batch_check_extents(plan, max_steps, &e);
+  if (msg && msg_len) {
+    std::snprintf(msg, msg_len, "%s", rc == UVLM_OK ? "" : e.c_str());
+  }
+  return rc;
 }
 
 namespace {

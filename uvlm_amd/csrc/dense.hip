@@ -51,8 +51,13 @@ __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) /
 constexpr int kTR = 128;  // rows per tile (= workgroup size)
 constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
 
-// Fast-path panel: every workgroup factors the top block itself (workgroup 0
-// stores it), then forms L21 = A21 U11^-1 for its 64 rows.  Rows live in
+// Fast-path panel: every workgroup factors the top block itself, then forms
+// L21 = A21 U11^-1 for its 64 rows.  Workgroup 0 writes the factored block
+// to dblk (kNB x kNB, column-major, ld kNB), not to A: a workgroup of the
+// same launch dispatched after workgroup 0 finished would otherwise load the
+// factored block instead of the original.  k_trsm reads L11 from dblk and
+// stores the block in A; a panel of one workgroup (no race) also stores it
+// in A itself, since no k_trsm follows the last panel.  Rows live in
 // registers (one lane per row, the block's columns unrolled); the factored
 // rows of the diagonal block are published through LDS one step at a time.
 // Columns and rows beyond kb are padded with the identity, which the
@@ -60,6 +65,7 @@ constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
 // info <- first zero pivot.
 template <typename T>
 __global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, int kb,
+                                                 T* __restrict__ dblk,
                                                  int* __restrict__ flag,
                                                  int* __restrict__ info) {
   __shared__ T D[kNB][kNB + 1];
@@ -98,7 +104,8 @@ __global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, i
   if (blockIdx.x == 0) {
     for (int q = t; q < kb * kb; q += 64) {
       const int r = q % kb, c = q / kb;
-      A[(size_t)(k0 + c) * lda + k0 + r] = D[r][c];
+      dblk[c * kNB + r] = D[r][c];
+      if (gridDim.x == 1) A[(size_t)(k0 + c) * lda + k0 + r] = D[r][c];
     }
     if (t == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
   }
@@ -392,14 +399,21 @@ __global__ __launch_bounds__(256) void k_apply_moves(T* A, int lda, int n, int k
 }
 
 // U12 = L11^-1 A12 for the columns [k0+kb, n): one lane per column held in
-// registers, L11 in LDS (zero beyond kb).
+// registers, L11 in LDS (zero beyond kb).  dblk (nullable): the factored
+// diagonal block as k_panel_np left it (column-major, ld kNB); L11 is read
+// from there and workgroup 0 stores the block in A.  Without dblk L11 is
+// read from A.
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb) {
+__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb,
+                                             const T* __restrict__ dblk) {
   __shared__ T L[kNB][kNB + 1];
   const int t = threadIdx.x;
   for (int q = t; q < kNB * kNB; q += 64) {
     const int r = q % kNB, c = q / kNB;
-    L[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
+    const bool in = r < kb && c < kb;
+    const T v = !in ? T(0) : dblk ? dblk[c * kNB + r] : A[(size_t)(k0 + c) * lda + k0 + r];
+    L[r][c] = v;
+    if (dblk && in && blockIdx.x == 0) A[(size_t)(k0 + c) * lda + k0 + r] = v;
   }
   const int col = k0 + kb + blockIdx.x * 64 + t;
   T* a = A + (size_t)col * lda + k0;
@@ -700,7 +714,9 @@ void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* b
 // tree nominations and arrival counters of the panel kernel.
 size_t lu_work_ints(int n) {
   const int tiles = cdiv(n, kTR);
-  return 4 * kNB + 64 + (size_t)2 * tiles * kNB + 2 * (size_t)tiles + 64;
+  // + the panel's diagonal block (kNB x kNB doubles, 256-byte aligned)
+  return 4 * kNB + 64 + (size_t)2 * tiles * kNB + 2 * (size_t)tiles + 64 +
+         2 * kNB * kNB + 64;
 }
 
 // Blocked right-looking LU.  pivoted = false: checked no-pivot panels
@@ -713,6 +729,8 @@ void lu_factor_t(hipStream_t s, T* A, int n, int* perm, int* info, int* work, bo
   int* cand = work + 4 * kNB + 64;
   int* ctr = cand + (size_t)2 * tiles * kNB;
   const int nctr = 2 * tiles;
+  T* dblk = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(ctr + nctr + 64) + 255) &
+                                 ~(uintptr_t)255);
   hipLaunchKernelGGL(k_lu_init, dim3(cdiv(std::max(n, nctr), 256)), dim3(256), 0, s, info,
                      ctr, nctr, perm, n);
   for (int k0 = 0; k0 < n; k0 += kNB) {
@@ -728,11 +746,11 @@ void lu_factor_t(hipStream_t s, T* A, int n, int* perm, int* info, int* work, bo
                          n, n, k0, kb, (const int*)mv, perm);
     } else {
       hipLaunchKernelGGL((k_panel_np<T>), dim3(std::max(1, cdiv(rest, 64))), dim3(64), 0, s,
-                         A, n, n, k0, kb, info + 1, info);
+                         A, n, n, k0, kb, dblk, info + 1, info);
     }
     if (rest > 0) {
       hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A,
-                         n, n, k0, kb);
+                         n, n, k0, kb, pivoted ? (const T*)nullptr : (const T*)dblk);
       dim3 g(cdiv(rest, 64), cdiv(rest, 64));
       if constexpr (GEMM == 0) {
         hipLaunchKernelGGL(k_gemm_f64, g, dim3(256), 0, s, (double*)A, n, n, k0, kb);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace uvlm {
@@ -122,6 +123,12 @@ class FarField {
   // enable = false switches the correction off.
   int set_mlp(hipStream_t s, const float* w, bool enable, float ymax, std::string* err);
   bool mlp_enabled() const { return mlp_on_; }
+  // Measurement of the near sum (k_ff_near): with profiling on, every launch
+  // is bracketed by HIP events on its stream and counts its (target,
+  // segment) pairs on the device; profile_read returns the launches, their
+  // summed duration and pairs since the last read.
+  void set_profile(bool on) { prof_ = on; }
+  int profile_read(hipStream_t s, size_t* launches, double* ms, double* pairs, std::string* err);
   // Near blocks in FP32 with FP64 compensation and near-core re-evaluation
   // (the F32C precision) instead of FP64.
   void set_f32c(bool on) { f32c_ = on; }
@@ -155,6 +162,11 @@ class FarField {
          c_sort_tmp_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
          c_near_ = 0, c_far_ = 0, c_cnt_ = 0, c_front_ = 0, c_part_ = 0, c_w_ = 0, c_wf_ = 0,
          c_mstat_ = 0, c_samp_ = 0;
+  // near-sum measurement
+  bool prof_ = false, np_live_ = false;
+  void* d_np_ = nullptr;
+  size_t c_np_ = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used_, ev_free_;
   // state of the last velocity call (for sampling and statistics)
   int last_ntiles_ = 0, last_nt_ = 0;
 };

@@ -658,7 +658,8 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
                                                 const int* __restrict__ near,
                                                 const int2* __restrict__ cnt,
                                                 const int* __restrict__ order,
-                                                double* __restrict__ part) {
+                                                double* __restrict__ part,
+                                                unsigned long long* __restrict__ npairs) {
   constexpr int kLen = F32 ? kRec : kRecSeg;  // the FP64 walk needs no FP32 vectors
   constexpr int kPer = (kLen + 63) / 64;
   __shared__ double sr[kPer * 64];
@@ -679,9 +680,11 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
     }
   };
   if (g < count) load(id);
+  unsigned segs = 0;  // segments walked by this group (for the pair count)
   for (int k = g; k < count; k += kNearGroups) {
     const FFCluster c = cl[id];
     const int nr = c.r1 - c.r0, nc = c.c1 - c.c0;
+    segs += (unsigned)((nr + 1) * nc + (nc + 1) * nr);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kPer; ++u) sr[lane + 64 * u] = pre[u];
@@ -701,6 +704,10 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
     part[((long)g * 3 + 0) * nt + tg] = ux;
     part[((long)g * 3 + 1) * nt + tg] = uy;
     part[((long)g * 3 + 2) * nt + tg] = uz;
+  }
+  if (npairs) {  // (target, segment) pairs of the tile's valid targets
+    const unsigned long long nv = (unsigned long long)__popcll(__ballot(tg >= 0));
+    if (lane == 0 && segs) atomicAdd(npairs, nv * segs);
   }
 }
 
@@ -1061,9 +1068,48 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 FarField::~FarField() {
   void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_sort_tmp_,
                   d_box_,  d_cl_,    d_mom_,  d_map_,  d_tiles_, d_near_, d_far_,
-                  d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_};
+                  d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_, d_np_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  for (auto& e : ev_used_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : ev_free_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+int FarField::profile_read(hipStream_t s, size_t* launches, double* ms, double* pairs,
+                           std::string* err) {
+  *launches = 0;
+  *ms = *pairs = 0.0;
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    *err = "far-field profile: stream synchronisation failed";
+    return UVLM_ERR_HIP;
+  }
+  for (auto& e : ev_used_) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e.first, e.second) != hipSuccess) {
+      *err = "far-field profile: event time failed";
+      return UVLM_ERR_HIP;
+    }
+    *ms += t;
+    ev_free_.push_back(e);
+  }
+  *launches = ev_used_.size();
+  ev_used_.clear();
+  if (np_live_) {
+    unsigned long long h = 0;
+    if (hipMemcpy(&h, d_np_, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
+      *err = "far-field profile: pair count read failed";
+      return UVLM_ERR_HIP;
+    }
+    *pairs = (double)h;
+    np_live_ = false;  // the next profiled launch clears the count
+  }
+  return UVLM_OK;
 }
 
 int FarField::ensure(void** p, size_t* cap, size_t bytes, std::string* err) {
@@ -1204,16 +1250,39 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (const int2*)d_cnt_, ntiles, (unsigned long long*)d_key_, (int*)d_idx_);
   if ((rc = sort_pairs(s, ntiles, err)) != UVLM_OK) return rc;
   const int* order = (const int*)d_idx2_;
+  unsigned long long* npairs = nullptr;
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (prof_) {
+    if ((rc = ensure(&d_np_, &c_np_, 2 * sizeof(unsigned long long), err)) != UVLM_OK) return rc;
+    if (!np_live_) {
+      hipLaunchKernelGGL(k_ff_zero2, dim3(1), dim3(1), 0, s, (unsigned long long*)d_np_);
+      np_live_ = true;
+    }
+    npairs = (unsigned long long*)d_np_;
+    if (!ev_free_.empty()) {
+      ev = ev_free_.back();
+      ev_free_.pop_back();
+    } else if (hipEventCreate(&ev.first) != hipSuccess ||
+               hipEventCreate(&ev.second) != hipSuccess) {
+      *err = "hipEventCreate failed";
+      return UVLM_ERR_HIP;
+    }
+    (void)hipEventRecord(ev.first, s);
+  }
   if (f32c_)
     hipLaunchKernelGGL(k_ff_near<true>, dim3(ntiles, kNearGroups), dim3(64), 0, s,
                        (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
                        (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_, order,
-                       (double*)d_part_);
+                       (double*)d_part_, npairs);
   else
     hipLaunchKernelGGL(k_ff_near<false>, dim3(ntiles, kNearGroups), dim3(64), 0, s,
                        (const FFCluster*)d_cl_, (const double*)d_rec_, targets, nt,
                        (const int*)map, n0, (const int*)d_near_, (const int2*)d_cnt_, order,
-                       (double*)d_part_);
+                       (double*)d_part_, npairs);
+  if (ev.first) {
+    (void)hipEventRecord(ev.second, s);
+    ev_used_.push_back(ev);
+  }
   if (mlp_on_) {
     hipLaunchKernelGGL(k_ff_far<true>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
                        (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,

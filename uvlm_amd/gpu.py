@@ -413,6 +413,15 @@ class Simulation:
                     "profile_read")
         return n.value, ms.value, pr.value
 
+    def profile_read_near(self):
+        """(launches, kernel milliseconds, pairs) of the far-field split's
+        near-block sums since the last read."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        n, ms, pr = C.c_size_t(), C.c_double(), C.c_double()
+        self._check(N.gpu_lib().uvlm_profile_read_near(ctx, C.byref(n), C.byref(ms),
+                                                       C.byref(pr)), "profile_read_near")
+        return n.value, ms.value, pr.value
+
     def pairs(self):
         e, r = C.c_double(), C.c_double()
         self._check(self._L.uvlm_host_sim_pairs(self._h, C.byref(e), C.byref(r)), "pairs")
@@ -486,6 +495,48 @@ class Simulation:
                     "inverse_stats")
         return sw.value, rf.value, rb.value, last.value
 
+    def refine_stats(self):
+        """Mixed-precision solve counters: (solves, refinement sweeps, sweeps
+        of the last solve)."""
+        ctx = self._L.uvlm_host_sim_context(self._h)
+        n, sw, last = C.c_longlong(), C.c_longlong(), C.c_int()
+        self._check(N.gpu_lib().uvlm_sim_refine_stats(ctx, C.byref(n), C.byref(sw),
+                                                      C.byref(last)), "refine_stats")
+        return n.value, sw.value, last.value
+
+
+def batch_case_specs(cases):
+    """uvlm_batch_case records of workload dicts (one wing each, no twist or
+    bend)."""
+    n = len(cases)
+    specs = (N.BatchCase * n)()
+    for k, case in enumerate(cases):
+        if len(case["wings"]) != 1:
+            raise ValueError("batched cases have exactly one wing")
+        w = case["wings"][0]
+        for name in ("twist_amp", "bend_amp"):
+            if np.any(np.asarray(w.get(name, 0.0)) != 0):
+                raise ValueError("batched cases have no twist or bend")
+        sp = specs[k]
+        sp.rows, sp.cols = int(case["rows"]), int(case["cols"])
+        sp.chord, sp.span = float(w["chord"]), float(w["span"])
+        sp.dt, sp.alpha = float(case["dt"]), float(w.get("alpha", 0.0))
+        for d in range(3):
+            sp.origin[d] = float(w.get("origin", (0, 0, 0))[d])
+            sp.ff[d] = float(case["forward_flight"][d])
+            sp.plug[d] = float(w.get("plug", (0, 0, 0))[d])
+            sp.flap[d] = float(w.get("flap", (0, 0, 0))[d])
+    return specs
+
+
+def batch_check_plan(cases, max_steps: int):
+    """Host-only replay of a batch's access extents (uvlm_batch_check_plan):
+    (return code, message)."""
+    specs = batch_case_specs(cases)
+    msg = C.create_string_buffer(256)
+    rc = N.gpu_lib().uvlm_batch_check_plan(specs, len(cases), max_steps, msg, 256)
+    return rc, msg.value.decode()
+
 
 class Batch:
     """Many independent single-wing cases advanced in lockstep on one GPU
@@ -498,26 +549,11 @@ class Batch:
         self._L = self.ctx._L
         self.cases = list(cases)
         n = len(self.cases)
-        specs = (N.BatchCase * n)()
+        specs = batch_case_specs(self.cases)
         walls = []
-        for k, case in enumerate(self.cases):
-            if len(case["wings"]) != 1:
-                raise ValueError("batched cases have exactly one wing")
-            w = case["wings"][0]
-            for name in ("twist_amp", "bend_amp"):
-                if np.any(np.asarray(w.get(name, 0.0)) != 0):
-                    raise ValueError("batched cases have no twist or bend")
+        for case in self.cases:
             R, Cc = int(case["rows"]), int(case["cols"])
-            sp = specs[k]
-            sp.rows, sp.cols = R, Cc
-            sp.chord, sp.span = float(w["chord"]), float(w["span"])
-            sp.dt, sp.alpha = float(case["dt"]), float(w.get("alpha", 0.0))
-            for d in range(3):
-                sp.origin[d] = float(w.get("origin", (0, 0, 0))[d])
-                sp.ff[d] = float(case["forward_flight"][d])
-                sp.plug[d] = float(w.get("plug", (0, 0, 0))[d])
-                sp.flap[d] = float(w.get("flap", (0, 0, 0))[d])
-            walls.append(build_wing(w, R, Cc).reshape(-1))
+            walls.append(build_wing(case["wings"][0], R, Cc).reshape(-1))
         nodes = _f64(np.concatenate(walls))
         self.ctx._check(self._L.uvlm_batch_init(self.ctx._h, specs, n, _p(nodes), max_steps,
                                                 N.SCHEMES[advect], ignore_steps),
