@@ -104,6 +104,25 @@ int uvlm_influence_matrix(uvlm_ctx* ctx, const double* nodes, size_t num,
                           size_t rows, size_t cols, const double* cpos,
                           const double* normals, unsigned flags, double* A);
 
+/* Allocation-free variant of uvlm_induced_velocity, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  targets [nt][3], nodes [num][rows+1][cols+1][3], gamma
+ * [num][rows][cols] and out [nt][3] are DEVICE pointers; the launches go on
+ * `stream` (a hipStream_t; NULL = the context's stream) and the call returns
+ * without waiting for them.  Source records and partial sums live in a
+ * fixed 64 MiB device array per GPU (16 MiB of records: at most 262,144
+ * chain elements, or 131,072 rings with UVLM_PREC_F64_EXACT; 48 MiB of
+ * partial sums: 3 x chunks x nt doubles); larger calls return
+ * UVLM_ERR_CAPACITY before launching anything.  Results are bitwise those
+ * of uvlm_induced_velocity with the same flags.  Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (two streams,
+ * two threads) race. */
+int uvlm_induced_velocity_noalloc(uvlm_ctx* ctx, const double* targets, size_t nt,
+                                  const double* nodes, const double* gamma, size_t num,
+                                  size_t rows, size_t cols, unsigned flags, double* out,
+                                  void* stream);
+
 /* Replaces the legacy internal::CalcMatrix (linear.inl:14-32):
  * A[i][j] = ring_j.BiotSavartLaw(targets_i, Gamma=1) . normals_i for 4-node
  * rings [n][4][3] (segments node[(k+1)%4] -> node[k]), row-major [n][n]. */

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -313,6 +314,40 @@ int ff_current(uvlm_ctx* c) {
 // ===========================================================================
 extern "C" {
 
+// ---------------------------------------------------------------------------
+// Scratch of uvlm_induced_velocity_noalloc: a file-scope device array (each
+// device holds its own copy until the process ends).  Host code never uses
+// its name as an address: the device address of each device's copy is looked
+// up once per device (hipGetSymbolAddress, with that device current) and
+// kept in g_scratch_addr.
+constexpr size_t kScratchRecBytes = size_t(16) << 20;   // source records
+constexpr size_t kScratchPartBytes = size_t(48) << 20;  // chunk partial sums
+constexpr size_t kScratchBytes = kScratchRecBytes + kScratchPartBytes;
+static __device__ __align__(256) unsigned char g_scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchDevices] = {};
+
+// Device address of this device's scratch (device `dev` current); looked up
+// on the first call per device.
+int scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kScratchDevices) return UVLM_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return UVLM_ERR_HIP;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return UVLM_OK;
+}
+}  // namespace
+
 int uvlm_create(int device, uvlm_ctx** out) {
   if (!out) return UVLM_ERR_ARG;
   *out = nullptr;
@@ -331,6 +366,12 @@ int uvlm_create(int device, uvlm_ctx** out) {
     return UVLM_ERR_HIP;
   }
   c->td = cutoff_dsq_threshold();
+  void* scratch = nullptr;  // set-up of uvlm_induced_velocity_noalloc's scratch
+  if (scratch_address(device, &scratch) != UVLM_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return UVLM_ERR_HIP;
+  }
   *out = c;
   return UVLM_OK;
 }
@@ -464,6 +505,63 @@ int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return UVLM_OK;
+}
+
+int uvlm_induced_velocity_noalloc(uvlm_ctx* c, const double* targets, size_t nt,
+                                  const double* nodes, const double* gamma, size_t num,
+                                  size_t rows, size_t cols, unsigned flags, double* out,
+                                  void* stream) {
+  if (!c) return UVLM_ERR_ARG;
+  if (nt && (!targets || !out)) return fail(c, UVLM_ERR_ARG, "null target/out");
+  TRY(check_prec(c, flags));
+  if (nt == 0) return UVLM_OK;
+  const size_t nr = num * rows * cols;
+  if (nr && (!nodes || !gamma)) return fail(c, UVLM_ERR_ARG, "null nodes/gamma");
+  const int prec = prec_of(flags);
+  const bool seq = (flags & UVLM_SEQUENTIAL) != 0;
+  // sizes first: nothing is launched for a call that does not fit
+  const size_t rec_bytes = prec == kF64Exact
+                               ? sizeof(RingRec) * nr
+                               : sizeof(ChainRec) * (size_t)2 * num * (rows + 1) * (cols + 1);
+  if (nt > (size_t)INT32_MAX / 4 || nr > (size_t)INT32_MAX / 4 || rec_bytes > kScratchRecBytes)
+    return fail(c, UVLM_ERR_CAPACITY,
+                "noalloc: %zu source records (%zu bytes) exceed the %zu-byte scratch", nr,
+                rec_bytes, kScratchRecBytes);
+  ChunkPlan plan{};
+  int ne = 0;
+  if (nr) {
+    ne = prec == kF64Exact ? (int)nr : chain_elems((int)num, (int)rows, (int)cols);
+    plan = plan_chunks((int)nt, 0, ne, seq);
+  }
+  const size_t part_bytes = sizeof(double) * 3 * (size_t)std::max(1, plan.total()) * nt;
+  if (part_bytes > kScratchPartBytes)
+    return fail(c, UVLM_ERR_CAPACITY,
+                "noalloc: %zu targets x %d chunks need %zu bytes of partial sums (at most %zu)",
+                nt, plan.total(), part_bytes, kScratchPartBytes);
+  TRY(set_dev(c));
+  void* scratch = nullptr;
+  if (scratch_address(c->device, &scratch) != UVLM_OK)
+    return fail(c, UVLM_ERR_HIP, "noalloc: scratch address lookup failed");
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
+  void* recs = scratch;
+  double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(scratch) +
+                                           kScratchRecBytes);
+  if (nr) {
+    if (prec == kF64Exact) {
+      launch_pack_lattice(s, nodes, (int)rows + 1, 0, gamma, (int)rows, 0, (int)num,
+                          (int)rows, (int)cols, static_cast<RingRec*>(recs));
+      launch_nbody(s, prec, kSegVoring, targets, (int)nt, static_cast<const RingRec*>(recs),
+                   plan, part, (int)nt, c->td);
+    } else {
+      launch_pack_chains(s, nodes, (int)rows + 1, 0, gamma, (int)rows, 0, (int)num, (int)rows,
+                         (int)cols, static_cast<ChainRec*>(recs), prec == kF32C);
+      launch_nbody_chain(s, prec, targets, (int)nt, static_cast<const ChainRec*>(recs), plan,
+                         part, (int)nt);
+    }
+  }
+  // no sources: zero chunks sum to zero (a kernel, not a memset)
+  launch_sum_chunks(s, part, (int)nt, (int)nt, nr ? plan.total() : 0, out);
+  return check_launch(c);
 }
 
 int uvlm_ring_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
