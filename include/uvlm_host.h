@@ -99,6 +99,75 @@ int uvlm_host_legacy_run(const uvlm_case_spec* spec, size_t steps, double* bound
                          double* bound_gamma, double* wake_nodes, double* wake_gamma,
                          size_t* nw_inout);
 
+/* --- outputs ------------------------------------------------------------- */
+/* SimpleSimulator::set_result_path / set_sheet_path (simple_simulator.cpp
+ * :52-66) of a created simulation; either path may be NULL (unchanged).
+ * Every later step appends a Snapshot2 record (OutputPanels, :315-321) or an
+ * AllVortexSheets record (OutputSheet, :323-332).  Records: the message's
+ * byte length as a base-128 varint, then the message in protobuf wire
+ * format (the reference's recordio framing is not in its tree). */
+int uvlm_host_sim_set_outputs(uvlm_host_sim* s, const char* result_path,
+                              const char* sheet_path);
+/* The messages OutputPanels and OutputSheet write for a state given
+ * explicitly (MultipleSheet layout: wing nodes [num][rows+1][cols+1][3],
+ * circulations [num][rows][cols], wake likewise with wake_rows ring rows):
+ * a Snapshot2 of the wing then wake rings (output.cpp:11-32) and an
+ * AllVortexSheets at time t (proto_adaptor.h:115-137), serialised.
+ * len_inout: capacity of out in, message length out (out may be NULL to
+ * query the length). */
+int uvlm_host_encode_panels(const double* wing_nodes, const double* wing_gamma, size_t num,
+                            size_t rows, size_t cols, const double* wake_nodes,
+                            const double* wake_gamma, size_t wake_rows, unsigned char* out,
+                            size_t* len_inout);
+int uvlm_host_encode_sheets(const double* wing_nodes, const double* wing_gamma, size_t num,
+                            size_t rows, size_t cols, const double* wake_nodes,
+                            const double* wake_gamma, size_t wake_rows, double t,
+                            unsigned char* out, size_t* len_inout);
+/* Options of the following uvlm_host_legacy_run calls: simulator::
+ * SetOutputPath (a Snapshot2 file output_path/%08lu per step,
+ * simulator.cpp:118-159), SetOutputLoadPath (one "t CX CY CZ Pin Pout ..."
+ * row per step, :162-198) and --use_joukowski (the Joukowski load instead of
+ * Katz-Plotkin's).  NULL or empty paths disable the output. */
+int uvlm_host_legacy_set_outputs(const char* output_path, const char* load_path,
+                                 int use_joukowski);
+
+/* --- post-processing tools ---------------------------------------------
+ * A snapshot argument names a file holding one Snapshot2 message
+ * (recordio = 0, as the legacy loop writes them) or a record stream of them
+ * (recordio != 0, as uvlm_host_sim_set_outputs writes them; index selects
+ * the record). */
+/* tools/velocity_at_main.cpp:54-97: TSV "step t x y z" of the velocity at
+ * pos induced by sheet `id` of the wing (wing != 0) and/or wake (wake != 0)
+ * sheets of every AllVortexSheets record of `input`. */
+int uvlm_host_velocity_at(const char* input, const double pos[3], int id, int wing, int wake,
+                          const char* output, size_t* records);
+/* Velocity at points [n][3] of every ring of a snapshot (the kernel of the
+ * plane and grid tools; InducedVelocity over the rings, shed.inl:22-33). */
+int uvlm_host_snapshot_velocity(const char* snapshot, int recordio, size_t index,
+                                const double* points, size_t n, double* out);
+/* tools/2d_velocity_main.cpp:59-157: plane "xz" etc., range = min0, max0,
+ * min1, max1, the third coordinate `at`, resolution^2 points, filetype
+ * "csv" or "tsv". */
+int uvlm_host_plane_velocity(const char* snapshot, int recordio, size_t index,
+                             const char* plane, const double range[4], double at,
+                             size_t resolution, const char* output, const char* filetype);
+/* tools/3d_velocity_main.cpp:78-132: velocity + (1, 0, 0) on a
+ * resolution^3 grid spanning [lo, hi], legacy-VTK structured points. */
+int uvlm_host_grid_velocity(const char* snapshot, int recordio, size_t index,
+                            const double lo[3], const double hi[3], size_t resolution,
+                            const char* output, size_t* points);
+/* calc_impulse_main.cpp:55-85 for one Snapshot2 file of the legacy loop:
+ * t, the impulse of the first container's rings, the sum of their Lamb
+ * vectors and the impulse of the wake rings. */
+int uvlm_host_calc_impulse(const char* snapshot, double* t, double impulse[3],
+                           double other_term[3], double wake_impulse[3]);
+/* calc_impulse.h:14-30 CalcLambVectorOnPanel for one ring [4][3] with
+ * circulation gamma: per-segment surface velocities v_nodes [4][3], the
+ * freestream, and n_rings source rings [n][4][3] with their circulations. */
+int uvlm_host_lamb_vector(const double* ring, double gamma, const double* v_nodes,
+                          const double freestream[3], const double* rings,
+                          const double* ring_gamma, size_t n_rings, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

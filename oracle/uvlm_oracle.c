@@ -830,6 +830,142 @@ static void advect(orc_sim* s, double dt) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Legacy loads and the Lamb vector (rings as [4][3] nodes, legacy order). */
+
+static v3 rings_velocity(v3 x, const double* rings, const double* gamma, size_t n,
+                         int chordwise) {
+  v3 res = mk(0, 0, 0);
+  for (size_t r = 0; r < n; r++)
+    res = add(res, ring_bs(rings + 12 * r, gamma[r], x, chordwise));
+  return res;
+}
+
+/* calc_impulse.h:14-30: sum over the segments node[(k+1)%4] -> node[k] of
+ * (v_node[k] + u(mid) - U) x dl * gamma, u from the source rings. */
+void orc_lamb_vector(const double ring[12], double gamma, const double v_nodes[12],
+                     const double U[3], const double* rings, const double* ring_gamma,
+                     size_t n, double out[3]) {
+  v3 res = mk(0, 0, 0);
+  for (int k = 0; k < 4; k++) {
+    v3 start = ld(ring + 3 * ((k + 1) % 4)), end = ld(ring + 3 * k);
+    v3 dl = sub(end, start);
+    v3 pos = dvs(add(start, end), 2);
+    v3 u = rings_velocity(pos, rings, ring_gamma, n, 0);
+    v3 ue = sub(add(ld(v_nodes + 3 * k), u), ld(U));
+    res = add(res, scl(crs(ue, dl), gamma));
+  }
+  st(out, res);
+}
+
+static double ring_b(const double* r) { return nrm(sub(ld(r), ld(r + 9))); } /* CalcB */
+static double ring_c(const double* r) { return nrm(sub(ld(r), ld(r + 3))); } /* CalcC */
+
+/* katz_plotkin.cpp:16-68 on one container (R x C rings, row-major) whose
+ * wake and chordwise-bound velocities are zero (the loop's ring set is
+ * never filled, simulator.cpp:37); F starts from zero.  out = F, Pin, Pout. */
+void orc_katz_plotkin_load(const double* rings, const double* gamma,
+                           const double* gamma_prev, size_t R, size_t C,
+                           const orc_wing_spec* w, const double U[3], double rho,
+                           double t, double dt, double out[5]) {
+  v3 F = mk(0, 0, 0);
+  double Pin = 0;
+  for (size_t i = 0; i < R; i++)
+    for (size_t j = 0; j < C; j++) {
+      const size_t K = i * C + j;
+      const double* r = rings + 12 * K;
+      double cc[3], nn[3], tt[3];
+      orc_ring_centroid(r, cc);
+      orc_ring_normal(r, nn);
+      orc_ring_tangent(r, tt);
+      const v3 centroid = ld(cc), n = ld(nn), tg = ld(tt);
+      const double db = ring_b(r), dc = ring_c(r);
+      const v3 Um = sub(ld(U), morph_velocity(w, centroid, t)); /* CalcUm */
+      const v3 u = normalized(Um);                            /* projection */
+      const double P[3][3] = {{1 - u.x * u.x, 0 - u.x * u.y, 0 - u.x * u.z},
+                              {0 - u.y * u.x, 1 - u.y * u.y, 0 - u.y * u.z},
+                              {0 - u.z * u.x, 0 - u.z * u.y, 1 - u.z * u.z}};
+      const double alpha = atan2(dot(Um, n), dot(Um, tg));
+      const double dg = gamma[K] - (i == 0 ? 0 : gamma[K - C]); /* DeltaGamma */
+      const v3 grad = dvs(scl(tg, dg), dc);                      /* Grad */
+      const double Lst = rho * dot(Um, grad) * db * dc * cos(alpha);
+      const v3 Pn = mk((P[0][0] * n.x + P[0][1] * n.y) + P[0][2] * n.z,
+                       (P[1][0] * n.x + P[1][1] * n.y) + P[1][2] * n.z,
+                       (P[2][0] * n.x + P[2][1] * n.y) + P[2][2] * n.z);
+      const double Dst = rho * (-dot(Um, Pn) * dg * db);
+      double dg_dt = (gamma[K] - gamma_prev[K]) / dt;
+      if (i > 0) dg_dt += (gamma[K - C] - gamma_prev[K - C]) / dt;
+      dg_dt /= 2;
+      const double Lunst = rho * dg_dt * db * dc * cos(alpha);
+      const double Dunst = rho * dg_dt * db * dc * sin(alpha);
+      const v3 e_drag = dvs(Um, nrm(Um));
+      const v3 e_lift = Pn;
+      const v3 dF = add(scl(e_drag, Dst + Dunst), scl(e_lift, Lst + Lunst));
+      F = add(F, dF);
+      Pin += (-1) * dot(Um, n) * dot(dF, n);
+    }
+  out[0] = F.x;
+  out[1] = F.y;
+  out[2] = F.z;
+  out[3] = Pin;
+  out[4] = 0;
+}
+
+/* joukowski.h:63-99 on one container: per ring, U x (end - start) rho gamma
+ * over the segments node[(k+1)%4] -> node[k] (k = 1 of the last row
+ * skipped), U = velocity of the source rings at the midpoint + freestream -
+ * surface velocity at the reference midpoint; plus normal rho dG/dt CalcB
+ * CalcC; Pin = sum dF . (surface velocity at the reference centroid),
+ * Pout = -F.x |U|.  rings0: reference nodes.  out = F, Pin, Pout. */
+void orc_joukowski_container_load(const double* rings, const double* rings0,
+                                  const double* gamma, const double* gamma_prev,
+                                  size_t R, size_t C, const double* src,
+                                  const double* src_gamma, size_t n_src,
+                                  const orc_wing_spec* w, const double U[3], double rho,
+                                  double t, double dt, double out[5]) {
+  const size_t N = R * C;
+  double* dF = (double*)xcalloc(3 * N, sizeof(double));
+  double Fx = 0, Fy = 0, Fz = 0, Pin = 0;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (size_t K = 0; K < N; K++) {
+    const size_t i = K / C;
+    const double* r = rings + 12 * K;
+    const double* r0 = rings0 + 12 * K;
+    v3 st_ = mk(0, 0, 0);
+    for (int k = 0; k < 4; k++) {
+      if (i + 1 == R && k == 1) continue;
+      v3 start = ld(r + 3 * ((k + 1) % 4)), end = ld(r + 3 * k);
+      v3 start0 = ld(r0 + 3 * ((k + 1) % 4)), end0 = ld(r0 + 3 * k);
+      v3 mid = dvs(add(start, end), 2), mid0 = dvs(add(start0, end0), 2);
+      v3 Uind = rings_velocity(mid, src, src_gamma, n_src, 0);
+      v3 Um = sub(ld(U), morph_velocity(w, mid0, t));
+      v3 Uv = add(Uind, Um);
+      st_ = add(st_, scl(scl(crs(Uv, sub(end, start)), rho), gamma[K]));
+    }
+    double nn[3];
+    orc_ring_normal(r, nn);
+    const double dGdt = (gamma[K] - gamma_prev[K]) / dt;
+    v3 un = scl(scl(scl(scl(ld(nn), rho), dGdt), ring_b(r)), ring_c(r));
+    st(dF + 3 * K, add(st_, un));
+  }
+  for (size_t K = 0; K < N; K++) {
+    const double* r0 = rings0 + 12 * K;
+    double c0[3];
+    orc_ring_centroid(r0, c0);
+    v3 d = ld(dF + 3 * K);
+    Fx += d.x;
+    Fy += d.y;
+    Fz += d.z;
+    Pin += dot(d, morph_velocity(w, ld(c0), t));
+  }
+  free(dF);
+  out[0] = Fx;
+  out[1] = Fy;
+  out[2] = Fz;
+  out[3] = Pin;
+  out[4] = Fx * nrm(ld(U)) * (-1);
+}
+
 /* One Advect call alone (advect.cpp:11-216) on the current wake, the
  * current wing and its circulations as sources. */
 void orc_sim_advect(orc_sim* s, double dt) { advect(s, dt); }

@@ -192,6 +192,29 @@ void orc_sim_advect(orc_sim* s, double dt);
 int orc_sim_replace_wake(orc_sim* s, size_t node_rows, size_t gamma_rows,
                          const double* wake_nodes, const double* wake_gamma);
 
+/* calc_impulse.h:14-30 CalcLambVectorOnPanel: ring [4][3] (segments
+ * node[(k+1)%4] -> node[k]), per-segment surface velocities v_nodes [4][3],
+ * freestream U, n source rings [n][4][3] with circulations (sum then
+ * scale, vortex.cpp:41-50). */
+void orc_lamb_vector(const double ring[12], double gamma, const double v_nodes[12],
+                     const double U[3], const double* rings, const double* ring_gamma,
+                     size_t n, double out[3]);
+/* katz_plotkin.cpp:16-68: one container of R x C rings [R*C][4][3] with
+ * circulations now and one step before; out = F[3], Pin, Pout. */
+void orc_katz_plotkin_load(const double* rings, const double* gamma,
+                           const double* gamma_prev, size_t R, size_t C,
+                           const orc_wing_spec* w, const double U[3], double rho,
+                           double t, double dt, double out[5]);
+/* joukowski.h:63-99 (container form): rings and their reference nodes
+ * rings0, circulations, and every ring of the shared vector as the source;
+ * out = F[3], Pin, Pout. */
+void orc_joukowski_container_load(const double* rings, const double* rings0,
+                                  const double* gamma, const double* gamma_prev,
+                                  size_t R, size_t C, const double* src,
+                                  const double* src_gamma, size_t n_src,
+                                  const orc_wing_spec* w, const double U[3], double rho,
+                                  double t, double dt, double out[5]);
+
 /* Core cut-off of the segment law (vortex.cpp:31; default 1e-10): a pair
  * returns zero when |r1 x r2|^2 < eps, |r1| < eps or |r2| < eps. */
 void orc_set_cutoff(double eps);

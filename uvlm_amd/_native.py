@@ -54,6 +54,10 @@ HOST_SYMBOLS = [
     "uvlm_host_sim_gamma", "uvlm_host_sim_wake_shape", "uvlm_host_sim_wake",
     "uvlm_host_sim_set_wake", "uvlm_host_sim_loads", "uvlm_host_sim_pairs",
     "uvlm_host_sim_context", "uvlm_host_build_wing", "uvlm_host_legacy_run",
+    "uvlm_host_sim_set_outputs", "uvlm_host_legacy_set_outputs", "uvlm_host_velocity_at",
+    "uvlm_host_snapshot_velocity", "uvlm_host_plane_velocity", "uvlm_host_grid_velocity",
+    "uvlm_host_calc_impulse", "uvlm_host_lamb_vector", "uvlm_host_encode_panels",
+    "uvlm_host_encode_sheets",
 ]
 
 
@@ -207,5 +211,21 @@ def host_lib():
         L.uvlm_host_legacy_run.argtypes = [C.POINTER(CaseSpec), _sz, _dp, _dp, _dp, _dp,
                                            C.POINTER(_sz)]
         L.uvlm_host_sim_context.restype = _vp
+        _cp = C.c_char_p
+        L.uvlm_host_sim_set_outputs.argtypes = [_vp, _cp, _cp]
+        L.uvlm_host_legacy_set_outputs.argtypes = [_cp, _cp, C.c_int]
+        L.uvlm_host_velocity_at.argtypes = [_cp, _dp, C.c_int, C.c_int, C.c_int, _cp,
+                                            C.POINTER(_sz)]
+        L.uvlm_host_snapshot_velocity.argtypes = [_cp, C.c_int, _sz, _dp, _sz, _dp]
+        L.uvlm_host_plane_velocity.argtypes = [_cp, C.c_int, _sz, _cp, _dp, C.c_double, _sz,
+                                               _cp, _cp]
+        L.uvlm_host_grid_velocity.argtypes = [_cp, C.c_int, _sz, _dp, _dp, _sz, _cp,
+                                              C.POINTER(_sz)]
+        L.uvlm_host_calc_impulse.argtypes = [_cp, _dp, _dp, _dp, _dp]
+        L.uvlm_host_lamb_vector.argtypes = [_dp, C.c_double, _dp, _dp, _dp, _dp, _sz, _dp]
+        L.uvlm_host_encode_panels.argtypes = [_dp, _dp, _sz, _sz, _sz, _dp, _dp, _sz, _vp,
+                                              C.POINTER(_sz)]
+        L.uvlm_host_encode_sheets.argtypes = [_dp, _dp, _sz, _sz, _sz, _dp, _dp, _sz,
+                                              C.c_double, _vp, C.POINTER(_sz)]
         _host = L
     return _host

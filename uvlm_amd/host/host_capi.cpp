@@ -2,12 +2,15 @@
 // UVLM::simulator::SimpleSimulator.
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 
 #include "../../include/uvlm_host.h"
 #include "legacy_simulator.h"
+#include "output.h"
 #include "simple_simulator.h"
+#include "tools.h"
 
 using UVLM::Morphing;
 using UVLM::Vector3d;
@@ -20,6 +23,21 @@ struct uvlm_host_sim {
 
 namespace {
 thread_local std::string g_err;
+// output options of the next uvlm_host_legacy_run
+struct LegacyOutputs {
+  std::string output_path, load_path;
+  bool use_joukowski = false;
+};
+LegacyOutputs g_legacy;
+
+bool readable(const char* path) {
+  std::ifstream f(path, std::ios::binary);
+  return static_cast<bool>(f);
+}
+bool writable(const char* path) {
+  std::ofstream f(path, std::ios::app);
+  return static_cast<bool>(f);
+}
 
 int err(const char* msg) {
   g_err = msg;
@@ -206,6 +224,9 @@ int uvlm_host_legacy_run(const uvlm_case_spec* sp, size_t steps, double* bound_n
   if (sp->rows == 0 || sp->cols == 0) return err("empty lattice");
   if ((wake_nodes || wake_gamma) && !nw_inout) return err("wake capacity missing");
   sim::ResetSimulator();
+  sim::SetOutputPath(g_legacy.output_path);
+  sim::SetOutputLoadPath(g_legacy.load_path);
+  sim::SetUseJoukowski(g_legacy.use_joukowski);
   for (int n = 0; n < sp->num_wings; ++n) {
     const uvlm_wing_spec& w = sp->wings[n];
     std::unique_ptr<UVLM::wing::WingGenerator> gen;
@@ -243,6 +264,202 @@ int uvlm_host_legacy_run(const uvlm_case_spec* sp, size_t steps, double* bound_n
   };
   for (std::size_t k = 0; k < nb; ++k) put(v[k], bound_nodes, bound_gamma, k);
   for (std::size_t k = 0; k < nw; ++k) put(v[nb + k], wake_nodes, wake_gamma, k);
+  return UVLM_OK;
+}
+
+int uvlm_host_sim_set_outputs(uvlm_host_sim* s, const char* result_path,
+                              const char* sheet_path) {
+  if (!s) return err("null simulation");
+  if (result_path) {
+    if (!*result_path || !writable(result_path)) return err("cannot open the result path");
+    s->sim.set_result_path(result_path);
+  }
+  if (sheet_path) {
+    if (!*sheet_path || !writable(sheet_path)) return err("cannot open the sheet path");
+    s->sim.set_sheet_path(sheet_path);
+  }
+  return UVLM_OK;
+}
+
+int uvlm_host_legacy_set_outputs(const char* output_path, const char* load_path,
+                                 int use_joukowski) {
+  g_legacy.output_path = output_path ? output_path : "";
+  g_legacy.load_path = load_path ? load_path : "";
+  g_legacy.use_joukowski = use_joukowski != 0;
+  return UVLM_OK;
+}
+
+namespace {
+// Sheets of an explicit state in the MultipleSheet layout.
+void sheets_of(const double* nodes, const double* gamma, size_t num, size_t rows, size_t cols,
+               UVLM::MultipleSheet<Vector3d>* pos, UVLM::MultipleSheet<double>* g) {
+  pos->resize(num, rows + 1, cols + 1);
+  g->resize(num, rows, cols);
+  for (size_t k = 0; k < pos->size(); ++k)
+    (*pos)[k] = Vector3d(nodes[3 * k], nodes[3 * k + 1], nodes[3 * k + 2]);
+  for (size_t k = 0; k < g->size(); ++k) (*g)[k] = gamma[k];
+}
+int put_message(const std::string& m, unsigned char* out, size_t* len_inout) {
+  if (!len_inout) return err("null length");
+  const size_t cap = *len_inout;
+  *len_inout = m.size();
+  if (!out) return UVLM_OK;
+  if (cap < m.size()) return err("output buffer too small");
+  std::memcpy(out, m.data(), m.size());
+  return UVLM_OK;
+}
+}  // namespace
+
+int uvlm_host_encode_panels(const double* wing_nodes, const double* wing_gamma, size_t num,
+                            size_t rows, size_t cols, const double* wake_nodes,
+                            const double* wake_gamma, size_t wake_rows, unsigned char* out,
+                            size_t* len_inout) {
+  if (!wing_nodes || !wing_gamma || (wake_rows && (!wake_nodes || !wake_gamma)))
+    return err("null argument");
+  UVLM::MultipleSheet<Vector3d> wp, kp;
+  UVLM::MultipleSheet<double> wg, kg;
+  sheets_of(wing_nodes, wing_gamma, num, rows, cols, &wp, &wg);
+  UVLM::proto::Snapshot2 snapshot;
+  UVLM::output::SheetToSnapshot(&snapshot, wp, wg);
+  if (wake_rows) {
+    sheets_of(wake_nodes, wake_gamma, num, wake_rows, cols, &kp, &kg);
+    UVLM::output::SheetToSnapshot(&snapshot, kp, kg);
+  }
+  return put_message(UVLM::proto::Serialize(snapshot), out, len_inout);
+}
+
+int uvlm_host_encode_sheets(const double* wing_nodes, const double* wing_gamma, size_t num,
+                            size_t rows, size_t cols, const double* wake_nodes,
+                            const double* wake_gamma, size_t wake_rows, double t,
+                            unsigned char* out, size_t* len_inout) {
+  if (!wing_nodes || !wing_gamma || (wake_rows && (!wake_nodes || !wake_gamma)))
+    return err("null argument");
+  UVLM::MultipleSheet<Vector3d> wp, kp;
+  UVLM::MultipleSheet<double> wg, kg;
+  sheets_of(wing_nodes, wing_gamma, num, rows, cols, &wp, &wg);
+  sheets_of(wake_rows ? wake_nodes : wing_nodes, wake_rows ? wake_gamma : wing_gamma, num,
+            wake_rows, cols, &kp, &kg);
+  UVLM::proto::AllVortexSheets sheets;
+  sheets.has_t = sheets.has_wing = sheets.has_wake = true;
+  sheets.t = t;
+  sheets.wing = UVLM::proto_adaptor::ToVortexSheet(wp, wg);
+  sheets.wake = UVLM::proto_adaptor::ToVortexSheet(kp, kg);
+  return put_message(UVLM::proto::Serialize(sheets), out, len_inout);
+}
+
+int uvlm_host_velocity_at(const char* input, const double pos[3], int id, int wing, int wake,
+                          const char* output, size_t* records) {
+  if (!input || !pos || !output) return err("null argument");
+  if (!(wing || wake)) return err("At least one of --wing or --wake must be true.");
+  if (id < 0) return err("id must be >= 0");
+  if (!readable(input)) return err("cannot open the input");
+  if (!writable(output)) return err("cannot open the output");
+  const std::size_t n = UVLM::tools::VelocityAt(input, Vector3d(pos[0], pos[1], pos[2]), id,
+                                                wing != 0, wake != 0, output);
+  if (records) *records = n;
+  return UVLM_OK;
+}
+
+namespace {
+// One Snapshot2: a file holding one message (recordio = 0) or record `index`
+// of a record stream.
+int load_snapshot(const char* path, int recordio, size_t index, UVLM::proto::Snapshot2* s) {
+  if (!path) return err("null argument");
+  if (!readable(path)) return err("cannot open the snapshot");
+  if (!recordio) {
+    *s = UVLM::tools::ReadSnapshot2(path);
+    return UVLM_OK;
+  }
+  std::ifstream ifs(path, std::ios::binary);
+  UVLM::RecordReader reader(&ifs);
+  for (size_t k = 0;; ++k) {
+    if (!reader.ReadProtocolMessage(s)) return err("no such record in the snapshot stream");
+    if (k == index) return UVLM_OK;
+  }
+}
+}  // namespace
+
+int uvlm_host_snapshot_velocity(const char* snapshot, int recordio, size_t index,
+                                const double* points, size_t n, double* out) {
+  if (n && (!points || !out)) return err("null argument");
+  UVLM::proto::Snapshot2 s;
+  if (const int rc = load_snapshot(snapshot, recordio, index, &s)) return rc;
+  std::vector<Vector3d> p(n);
+  for (size_t k = 0; k < n; ++k) p[k] = Vector3d(points[3 * k], points[3 * k + 1], points[3 * k + 2]);
+  const auto v = UVLM::tools::SnapshotVelocity(s, p);
+  for (size_t k = 0; k < n; ++k)
+    for (int d = 0; d < 3; ++d) out[3 * k + d] = v[k][d];
+  return UVLM_OK;
+}
+
+int uvlm_host_plane_velocity(const char* snapshot, int recordio, size_t index,
+                             const char* plane, const double range[4], double at,
+                             size_t resolution, const char* output, const char* filetype) {
+  if (!plane || !range || !output || !filetype) return err("null argument");
+  const std::string pl(plane), ft(filetype);
+  if (pl.size() != 2 || pl[0] == pl[1] || std::string("xyz").find(pl[0]) == std::string::npos ||
+      std::string("xyz").find(pl[1]) == std::string::npos)
+    return err("plane must be two different letters of 'xyz'");
+  if (ft != "csv" && ft != "tsv") return err("filetype must be csv or tsv");
+  if (resolution < 2) return err("resolution must be at least 2");
+  if (!writable(output)) return err("cannot open the output");
+  UVLM::proto::Snapshot2 s;
+  if (const int rc = load_snapshot(snapshot, recordio, index, &s)) return rc;
+  UVLM::tools::WritePlane(s, pl, range[0], range[1], range[2], range[3], at, resolution, output,
+                          ft);
+  return UVLM_OK;
+}
+
+int uvlm_host_grid_velocity(const char* snapshot, int recordio, size_t index,
+                            const double lo[3], const double hi[3], size_t resolution,
+                            const char* output, size_t* points) {
+  if (!lo || !hi || !output) return err("null argument");
+  if (resolution < 2) return err("resolution must be at least 2");
+  if (!writable(output)) return err("cannot open the output");
+  UVLM::proto::Snapshot2 s;
+  if (const int rc = load_snapshot(snapshot, recordio, index, &s)) return rc;
+  const size_t n = UVLM::tools::WriteGrid(s, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2],
+                                          resolution, output);
+  if (points) *points = n;
+  return UVLM_OK;
+}
+
+int uvlm_host_calc_impulse(const char* snapshot, double* t, double impulse[3],
+                           double other_term[3], double wake_impulse[3]) {
+  UVLM::proto::Snapshot2 s;
+  if (const int rc = load_snapshot(snapshot, 0, 0, &s)) return rc;
+  if (s.container_shapes.empty()) return err("snapshot without containers");
+  size_t first = (size_t)s.container_shapes[0].rows * s.container_shapes[0].cols;
+  if (s.morphing_velocities.size() < first) return err("snapshot without morphing velocities");
+  const auto r = UVLM::tools::CalcImpulse(s);
+  if (t) *t = r.t;
+  for (int d = 0; d < 3; ++d) {
+    if (impulse) impulse[d] = r.impulse[d];
+    if (other_term) other_term[d] = r.other_term[d];
+    if (wake_impulse) wake_impulse[d] = r.wake_impulse[d];
+  }
+  return UVLM_OK;
+}
+
+int uvlm_host_lamb_vector(const double* ring, double gamma, const double* v_nodes,
+                          const double freestream[3], const double* rings,
+                          const double* ring_gamma, size_t n_rings, double out[3]) {
+  if (!ring || !v_nodes || !freestream || !out || (n_rings && (!rings || !ring_gamma)))
+    return err("null argument");
+  UVLM::VortexRing v;
+  for (int k = 0; k < 4; ++k) v.PushNode(ring[3 * k], ring[3 * k + 1], ring[3 * k + 2]);
+  v.set_gamma(gamma);
+  std::vector<Vector3d> vn(4);
+  for (int k = 0; k < 4; ++k) vn[k] = Vector3d(v_nodes[3 * k], v_nodes[3 * k + 1], v_nodes[3 * k + 2]);
+  std::vector<UVLM::VortexRing> src(n_rings);
+  for (size_t q = 0; q < n_rings; ++q) {
+    for (int k = 0; k < 4; ++k)
+      src[q].PushNode(rings[12 * q + 3 * k], rings[12 * q + 3 * k + 1], rings[12 * q + 3 * k + 2]);
+    src[q].set_gamma(ring_gamma[q]);
+  }
+  const Vector3d r = UVLM::calc_impulse::CalcLambVectorOnPanel(
+      v, vn, Vector3d(freestream[0], freestream[1], freestream[2]), src);
+  for (int d = 0; d < 3; ++d) out[d] = r[d];
   return UVLM_OK;
 }
 

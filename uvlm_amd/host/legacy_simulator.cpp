@@ -6,6 +6,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <sstream>
+
+#include "calc_load.h"
+#include "output.h"
 
 namespace UVLM {
 namespace internal {
@@ -91,6 +95,10 @@ struct State {
   Vector3d inlet = Vector3d(1, 0, 0);
   std::string output_path, output_load_path;
   bool disable_wake = false;
+  bool use_joukowski = false;  // --use_joukowski (simulator.cpp:27)
+  // the global ring set the Katz-Plotkin load reads; the reference never
+  // fills it (simulator.cpp:37)
+  UVLMVortexRing rings;
   std::size_t wake_offset = 0;  // simulator.cpp:49-54 (computed once)
 };
 State& state() {
@@ -161,6 +169,81 @@ void SolveLinearProblemStep(const double t) {
   for (std::size_t i = 0; i < off; i++) s.vortices->at(i).set_gamma(gamma[i]);
 }
 
+// simulator.cpp:118-159: Snapshot2 of the step (time, container shapes,
+// every ring with its reference nodes, surface velocities per bound ring)
+// to output_path/%08lu.  Written when an output path is set (the reference
+// writes unconditionally and aborts when "/%08lu" cannot be opened).
+void OutputSnapshot2(const std::size_t step, const double t) {
+  State& s = state();
+  if (s.output_path.empty()) return;
+  proto::Snapshot2 snapshot;
+  snapshot.has_t = true;
+  snapshot.t = t;
+  for (const auto& c : s.containers) {
+    proto::ContainerShape shape;
+    shape.rows = static_cast<uint32_t>(c.rows());
+    shape.cols = static_cast<uint32_t>(c.cols());
+    shape.has_id = shape.has_chord = shape.has_span = true;
+    shape.id = static_cast<uint32_t>(c.id());
+    shape.chord = c.chord();
+    shape.span = c.span();
+    snapshot.container_shapes.push_back(shape);
+  }
+  for (const auto& v : *s.vortices) snapshot.vortices.push_back(VortexRingToProto(v));
+  for (std::size_t i = 0; i < s.containers.size(); i++) {
+    const Morphing& m = s.morphings[i];
+    for (const auto& v : s.containers[i]) {
+      proto::MorphingVelocity mv;
+      mv.has_freestream = mv.has_center = true;
+      mv.freestream = Vector3dToPoint(s.inlet);
+      Vector3d vc;
+      m.Velocity(&vc, v.ReferenceCentroid(), t);
+      mv.center = Vector3dToPoint(vc);
+      v.ForEachSegment([&](const Vector3d&, const Vector3d&, const Vector3d& start0,
+                           const Vector3d& end0) {
+        Vector3d vel;
+        m.Velocity(&vel, (start0 + end0) / 2, t);
+        mv.nodes.push_back(Vector3dToPoint(vel));
+      });
+      snapshot.morphing_velocities.push_back(mv);
+    }
+  }
+  char filename[4096];
+  std::snprintf(filename, sizeof(filename), "%s/%08lu", s.output_path.c_str(),
+                static_cast<unsigned long>(step));
+  std::ofstream ofs(filename, std::ios::binary);
+  if (!ofs) fatal("Unable to open the snapshot file");
+  const std::string bytes = proto::Serialize(snapshot);
+  ofs.write(bytes.data(), static_cast<std::streamsize>(bytes.size()));
+}
+
+// simulator.cpp:162-198: per wing (CX, CY, CZ, Pin, Pout) of the
+// Katz-Plotkin load (default) or the Joukowski load, appended as one row
+// "t ..." to the load file.
+void CalcLoadProcess(const double t, const double dt) {
+  State& s = state();
+  std::ostringstream row;
+  row << t;
+  for (std::size_t i = 0; i < s.containers.size(); i++) {
+    const auto& c = s.containers[i];
+    const auto& c_prev = s.containers_prev[i];
+    const auto& m = s.morphings[i];
+    const double rho = 1;
+    const double S = c.chord() * c.span();
+    const calc_load::AerodynamicLoad load =
+        s.use_joukowski
+            ? calc_load::CalcLoadJoukowski(c, c_prev, m, s.inlet, rho, t, dt)
+            : calc_load::CalcLoadKatzPlotkin(c, c_prev, m, s.rings, s.inlet, rho, t, dt);
+    const double U = s.inlet.norm();
+    const Vector3d coeff = load.F / (0.5 * rho * U * U * S);
+    row << "\t" << coeff.x() << "\t" << coeff.y() << "\t" << coeff.z() << "\t" << load.Pin
+        << "\t" << load.Pout;
+  }
+  std::ofstream ofs(s.output_load_path, std::ios::app);
+  if (!ofs) fatal("Unable to open the load file");
+  ofs << row.str() << std::endl;
+}
+
 }  // namespace
 
 void InitSimulator() {}
@@ -178,8 +261,9 @@ void SetInlet(double x, double y, double z) { state().inlet = Vector3d(x, y, z);
 void SetOutputPath(const std::string& path) { state().output_path = path; }
 void SetOutputLoadPath(const std::string& path) { state().output_load_path = path; }
 void SetDisableWake(bool disable) { state().disable_wake = disable; }
+void SetUseJoukowski(bool on) { state().use_joukowski = on; }
 
-// simulator.cpp:220-269 (without the snapshot and load outputs)
+// simulator.cpp:220-269
 void Start(const std::size_t steps, const double dt) {
   State& s = state();
   CheckReady();
@@ -192,6 +276,8 @@ void Start(const std::size_t steps, const double dt) {
     CopyContainers(s.containers.cbegin(), s.containers.cend(), s.containers_prev.begin());
     MorphingProcess(t);
     SolveLinearProblemStep(t);
+    OutputSnapshot2(step, t);
+    if (!s.output_load_path.empty()) CalcLoadProcess(t, dt);
     if (step == steps) break;
     if (!s.disable_wake) {
       // wing 0's trailing-edge rings become the newest wake rings; all wake

@@ -4,9 +4,10 @@
 // Rings are kept as UVLM::VortexRing objects in one shared vector (bound
 // rings of every wing, then the wake), windows of it as VortexContainer, as
 // in the reference.  The ring sums, the legacy matrix and the advection run
-// on the GPU through the functions of legacy.h.  Not written: the Snapshot2
-// protobuf output and the load file (the Katz-Plotkin and Joukowski load
-// module, calc_load/, is not part of this library); the paths are recorded.
+// on the GPU through the functions of legacy.h.  Each step writes the
+// Snapshot2 message to output_path/%08lu (when a path is set) and appends
+// the loads (Katz-Plotkin, or Joukowski with SetUseJoukowski) to the load
+// file (when a load path is set).
 #pragma once
 
 #include <memory>
@@ -70,6 +71,9 @@ void SetOutputLoadPath(const std::string& path);
 void Start(std::size_t steps, const double dt);
 // disable_wake of the reference's --disable_wake flag.
 void SetDisableWake(bool disable);
+// use_joukowski of the reference's --use_joukowski flag (simulator.cpp:27):
+// the load file gets the Joukowski load instead of Katz-Plotkin's.
+void SetUseJoukowski(bool on);
 
 // State after Start, and a reset of the global state (not in the
 // reference, whose state lives for the process).

@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "device.h"
+#include "output.h"
 
 namespace UVLM {
 namespace simulator {
@@ -47,6 +48,47 @@ SimpleSimulator::SimpleSimulator() : forward_flight_(0, 0, 0) {}
 
 SimpleSimulator::~SimpleSimulator() {
   if (ctx_) uvlm_destroy(ctx_);
+}
+
+void SimpleSimulator::set_result_path(const std::string& path) {
+  if (path.empty()) die("Invalid path");
+  ofs_result_.reset(new std::ofstream(path, std::ios::binary));
+  if (!*ofs_result_) die("cannot open the result path");
+  result_path_ = path;
+  writer_.reset(new RecordWriter(ofs_result_.get()));
+}
+
+void SimpleSimulator::set_sheet_path(const std::string& path) {
+  if (path.empty()) die("Invalid path");
+  ofs_sheet_.reset(new std::ofstream(path, std::ios::binary));
+  if (!*ofs_sheet_) die("cannot open the sheet path");
+  sheet_path_ = path;
+  sheet_writer_.reset(new RecordWriter(ofs_sheet_.get()));
+}
+
+// simple_simulator.cpp:315-321: the rings of the wing, then of the wake (no
+// time is set on this message in the reference).
+void SimpleSimulator::OutputPanels(std::size_t step, double dt) {
+  proto::Snapshot2 snapshot;
+  MultipleSheet<Vector3d> wpos;
+  MultipleSheet<double> wgam;
+  wake(&wpos, &wgam);
+  output::SheetToSnapshot(&snapshot, wing_pos_, wing_gamma());
+  output::SheetToSnapshot(&snapshot, wpos, wgam);
+  if (!writer_->WriteProtocolMessage(snapshot)) die("result write failed");
+}
+
+// simple_simulator.cpp:323-332
+void SimpleSimulator::OutputSheet(std::size_t step, double dt) {
+  proto::AllVortexSheets sheets;
+  sheets.has_t = sheets.has_wing = sheets.has_wake = true;
+  sheets.t = step * dt;
+  MultipleSheet<Vector3d> wpos;
+  MultipleSheet<double> wgam;
+  wake(&wpos, &wgam);
+  sheets.wing = proto_adaptor::ToVortexSheet(wing_pos_, wing_gamma());
+  sheets.wake = proto_adaptor::ToVortexSheet(wpos, wgam);
+  if (!sheet_writer_->WriteProtocolMessage(sheets)) die("sheet write failed");
 }
 
 void SimpleSimulator::AddWing(wing::WingGenerator* wing_generator,
@@ -149,6 +191,9 @@ void SimpleSimulator::MainLoop(const std::size_t step, const double dt) {
   gpu::Check(uvlm_sim_solve(ctx_, vm_cp_[0].data(), forward_flight_.data(), solve_mode_,
                             nullptr),
              "uvlm_sim_solve", ctx_);
+  // output (:271-278)
+  if (writer_) OutputPanels(step, dt);
+  if (sheet_writer_) OutputSheet(step, dt);
   // CalcLoad (:279-282)
   if (loads) {
     gpu::Check(uvlm_sim_loads(ctx_, vm_edges_[0].data(), forward_flight_.data(), dt,

@@ -8,6 +8,7 @@
 // loads and advection (simple_simulator.cpp:240-286).
 #pragma once
 
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -16,6 +17,7 @@
 #include "advect.h"
 #include "morphing.h"
 #include "multiple_sheet.h"
+#include "proto_io.h"
 #include "vector3d.h"
 #include "wing.h"
 
@@ -49,10 +51,13 @@ class SimpleSimulator {
   void Run(const std::size_t steps, const double dt);
 
   void set_forward_flight(const Vector3d& v) { forward_flight_ = v; }
-  // Snapshot2 / AllVortexSheets recordio output (protobuf) is not written
-  // by this library: the paths are recorded and nothing is written.
-  void set_result_path(const std::string& path) { result_path_ = path; }
-  void set_sheet_path(const std::string& path) { sheet_path_ = path; }
+  // simple_simulator.cpp:52-66: every step, after the solve, a Snapshot2
+  // record of the wing and wake rings (OutputPanels, :315-321) to the
+  // result path and an AllVortexSheets record (OutputSheet, :323-332) to
+  // the sheet path; the record framing is the one of proto_io.h.  Aborts on
+  // an empty or unwritable path, as the reference does.
+  void set_result_path(const std::string& path);
+  void set_sheet_path(const std::string& path);
   // Load TSV "t CD0 CL0 CD1 CL1 ..." (simple_simulator.cpp:229-232,334-345).
   void set_load_path(const std::string& loadpath) { load_path_ = loadpath; }
   void set_advection(advect::Advection* advection) { advection_.reset(advection); }
@@ -89,6 +94,8 @@ class SimpleSimulator {
   void BuildWing();
   void MainLoop(const std::size_t step, const double dt);
   void WriteLoads() const;
+  void OutputPanels(std::size_t step, double dt);
+  void OutputSheet(std::size_t step, double dt);
 
   uvlm_ctx* ctx_ = nullptr;
   std::unique_ptr<advect::Advection> advection_;
@@ -101,6 +108,8 @@ class SimpleSimulator {
   std::vector<Vector3d> vm_cp_, vm_edges_;
   Vector3d forward_flight_;
   std::string result_path_, sheet_path_, load_path_;
+  std::unique_ptr<std::ofstream> ofs_result_, ofs_sheet_;
+  std::unique_ptr<RecordWriter> writer_, sheet_writer_;
   std::size_t rows_ = 0, cols_ = 0;
   unsigned precision_ = UVLM_PREC_F64_EXACT;
   int solve_mode_ = UVLM_SOLVE_F64;

@@ -45,6 +45,7 @@ void ring_sum(const std::vector<Vector3d>& targets, const double* rings,
               const double* gamma, std::size_t n, unsigned flags,
               std::vector<Vector3d>* out) {
   out->assign(targets.size(), Vector3d());
+  if (n == 0 || targets.empty()) return;  // nothing to sum: no device work
   uvlm_ctx* c = gpu::Context();
   gpu::Check(uvlm_ring_induced_velocity(c, targets.empty() ? nullptr : targets[0].data(),
                                         targets.size(), rings, gamma, n,
