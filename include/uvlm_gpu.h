@@ -71,6 +71,15 @@ typedef struct uvlm_ctx uvlm_ctx;
 int uvlm_create(int device, uvlm_ctx** out);
 int uvlm_destroy(uvlm_ctx* ctx);
 const char* uvlm_last_error(const uvlm_ctx* ctx);
+/* Core cut-off of the context's Biot-Savart sums (vortex.cpp:31 hard-codes
+ * eps = 1e-10; a configured core such as 1e-6 keeps the rule's shape: a
+ * segment contributes zero when |r1 x r2|^2 < eps, |r1| < eps or
+ * |r2| < eps).  Applies to later calls with UVLM_PREC_F64_EXACT or
+ * UVLM_PREC_F64 (ring sums, segment chains, influence matrices); the F32C
+ * sums, the far-field split and batches keep the reference's 1e-10 and
+ * refuse other values (UVLM_ERR_ARG).  eps in (0, 1e-2). */
+int uvlm_set_cutoff(uvlm_ctx* ctx, double eps);
+
 /* Wait for all work queued on the context's stream. */
 int uvlm_sync(uvlm_ctx* ctx);
 

@@ -59,6 +59,7 @@ typedef struct {
   int loads;             /* compute CD/CL every step */
   unsigned precision;    /* UVLM_PREC_* */
   int solve_mode;        /* UVLM_SOLVE_* */
+  double cutoff;         /* core cut-off of the sums; 0: the reference's 1e-10 */
 } uvlm_case_spec;
 
 typedef struct uvlm_host_sim uvlm_host_sim;

@@ -304,3 +304,59 @@ def test_full_size_precision_path(solve):
     for k, e in errs.items():
         assert e < 1e-6, (k, e)
     assert errs["gamma"] > 0.0
+
+
+def test_core_cutoff_variant(orc):
+    """The plate workload with the configured core cut-off 1e-6 (the rule of
+    vortex.cpp:31 with eps = 1e-6).  The parity configuration (reference-order
+    sums, FullPivLU order) is bitwise the oracle run with the same cut-off;
+    the default fast mode agrees to 1e-10; and the run differs from the
+    1e-10 run (the variant changes the result)."""
+    steps = 40
+    case = W.plate_c1_core(steps=steps)
+    parity = dict(case, precision=N.PREC_F64_EXACT | N.SEQUENTIAL,
+                  solve_mode=N.SOLVE_F64_FULLPIV)
+    fast = dict(case, precision=N.PREC_F64, solve_mode=N.SOLVE_F64_INV)
+    default = dict(fast, cutoff=0.0)
+    sims = {k: Simulation(v, steps) for k, v in
+            (("parity", parity), ("fast", fast), ("default", default))}
+    orc.set_cutoff(1e-6)
+    try:
+        o = orc.Sim(case)
+        loads = []
+        for s in range(steps + 1):
+            for g in sims.values():
+                g.step(s)
+            o.step(s)
+            loads.append(o.loads().copy())
+    finally:
+        orc.set_cutoff(1e-10)
+    for g in sims.values():
+        g.sync()
+    p = sims["parity"]
+    np.testing.assert_array_equal(p.gamma(), o.gamma())
+    np.testing.assert_array_equal(p.wake()[0], o.wake()[0])
+    assert rel_err(p.loads(), np.array(loads)) < 1e-13
+    f = sims["fast"]
+    assert rel_err(f.gamma(), o.gamma()) < TOL
+    assert rel_err(f.wake()[0], o.wake()[0]) < TOL
+    assert rel_err(f.loads(), np.array(loads)) < TOL
+    assert rel_err(sims["default"].loads(), np.array(loads)) > 1e-6
+
+
+def test_core_cutoff_refusals(ctx):
+    """F32C sums and the far-field split keep the reference's cut-off."""
+    from uvlm_amd import UVLMError
+    rng = np.random.default_rng(0)
+    nodes = rng.normal(size=(1, 3, 5, 3))
+    gamma = rng.normal(size=(1, 2, 4))
+    x = rng.normal(size=(7, 3))
+    assert ctx._L.uvlm_set_cutoff(ctx._h, 1e-6) == N.OK
+    try:
+        ctx.induced_velocity(x, nodes, gamma, N.PREC_F64)  # accepted
+        with pytest.raises(UVLMError) as e:
+            ctx.induced_velocity(x, nodes, gamma, N.PREC_F32C)
+        assert e.value.code == N.ERR_ARG
+        assert ctx._L.uvlm_set_cutoff(ctx._h, -1.0) == N.ERR_ARG
+    finally:
+        assert ctx._L.uvlm_set_cutoff(ctx._h, 1e-10) == N.OK

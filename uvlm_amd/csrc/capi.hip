@@ -44,7 +44,7 @@ struct uvlm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  double td = 0.0;
+  CutOff cut{};  // core cut-off of the segment law (default: the reference's)
   // stateless scratch
   Buf t_a, t_b, t_c, t_out, t_recs, t_partial, t_A, t_x, t_ipiv, t_info, t_work;
   // simulation
@@ -190,10 +190,14 @@ int set_dev(uvlm_ctx* c) {
   return UVLM_OK;
 }
 
+bool default_cut(const uvlm_ctx* c) { return c->cut.eps == kCutoff; }
+
 int check_prec(uvlm_ctx* c, unsigned flags) {
   const int p = prec_of(flags);
   if (p != kF64Exact && p != kF64Fast && p != kF32C)
     return fail(c, UVLM_ERR_ARG, "unsupported precision flag %u", flags & 0x0fu);
+  if (p == kF32C && !default_cut(c))
+    return fail(c, UVLM_ERR_ARG, "UVLM_PREC_F32C runs with the default cut-off only");
   return UVLM_OK;
 }
 
@@ -217,7 +221,7 @@ int nbody(uvlm_ctx* c, Buf& partial, const double* targets, int nt,
     HIPCHK(c, hipEventRecord(ev.first, c->stream));
   }
   launch_nbody(c->stream, prec_of(flags), seg, targets, nt, recs, plan,
-               P<double>(partial), nt, c->td);
+               P<double>(partial), nt, c->cut);
   if (ev.first) {
     HIPCHK(c, hipEventRecord(ev.second, c->stream));
     c->ev_used.push_back(ev);
@@ -264,7 +268,7 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
   }
   launch_nbody_chain(c->stream, prec, targets, nt,
                      P<ChainRec>(c->chains) + (bound ? 0 : eb), plan,
-                     P<double>(c->partial), nt);
+                     P<double>(c->partial), nt, c->cut);
   if (ev.first) {
     HIPCHK(c, hipEventRecord(ev.second, c->stream));
     c->ev_used.push_back(ev);
@@ -365,7 +369,7 @@ int uvlm_create(int device, uvlm_ctx** out) {
     delete c;
     return UVLM_ERR_HIP;
   }
-  c->td = cutoff_dsq_threshold();
+  c->cut = make_cutoff(kCutoff);
   void* scratch = nullptr;  // set-up of uvlm_induced_velocity_noalloc's scratch
   if (scratch_address(device, &scratch) != UVLM_OK) {
     (void)hipStreamDestroy(c->stream);
@@ -420,6 +424,22 @@ const char* uvlm_last_error(const uvlm_ctx* c) {
   return c ? c->err.c_str() : "no context";
 }
 
+int uvlm_set_cutoff(uvlm_ctx* c, double eps) {
+  if (!c) return UVLM_ERR_ARG;
+  if (!(eps > 0.0) || !(eps < 1e-2))
+    return fail(c, UVLM_ERR_ARG, "cut-off %g outside (0, 1e-2)", eps);
+  if (eps != kCutoff) {
+    if (c->sim && prec_of(c->flags) == kF32C)
+      return fail(c, UVLM_ERR_ARG, "UVLM_PREC_F32C runs with the default cut-off only");
+    if (c->ff && c->ff_theta > 0.0)
+      return fail(c, UVLM_ERR_ARG, "the far-field split runs with the default cut-off only");
+    if (c->batch)
+      return fail(c, UVLM_ERR_ARG, "batches run with the default cut-off only");
+  }
+  c->cut = make_cutoff(eps);
+  return UVLM_OK;
+}
+
 int uvlm_sync(uvlm_ctx* c) {
   if (!c) return UVLM_ERR_ARG;
   TRY(set_dev(c));
@@ -444,7 +464,7 @@ int uvlm_biot_savart(uvlm_ctx* c, const double* start, const double* end,
   HIPCHK(c, hipMemcpyAsync(c->t_c.p, pos, bytes, hipMemcpyHostToDevice, c->stream));
   launch_segments(c->stream, prec_of(flags) == kF64Exact ? kF64Exact : kF64Fast,
                   P<double>(c->t_a), P<double>(c->t_b), P<double>(c->t_c), (int)n,
-                  c->td, P<double>(c->t_out));
+                  c->cut, P<double>(c->t_out));
   TRY(check_launch(c));
   HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -495,7 +515,8 @@ int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
     TRY(ensure(c, c->t_partial,
                sizeof(double) * 3 * (size_t)std::max(1, plan.total()) * nt));
     launch_nbody_chain(c->stream, prec_of(flags), P<double>(c->t_a), (int)nt,
-                       P<ChainRec>(c->t_recs), plan, P<double>(c->t_partial), (int)nt);
+                       P<ChainRec>(c->t_recs), plan, P<double>(c->t_partial), (int)nt,
+                       c->cut);
     TRY(check_launch(c));
   }
   launch_sum_chunks(c->stream, P<double>(c->t_partial), (int)nt, (int)nt,
@@ -551,12 +572,12 @@ int uvlm_induced_velocity_noalloc(uvlm_ctx* c, const double* targets, size_t nt,
       launch_pack_lattice(s, nodes, (int)rows + 1, 0, gamma, (int)rows, 0, (int)num,
                           (int)rows, (int)cols, static_cast<RingRec*>(recs));
       launch_nbody(s, prec, kSegVoring, targets, (int)nt, static_cast<const RingRec*>(recs),
-                   plan, part, (int)nt, c->td);
+                   plan, part, (int)nt, c->cut);
     } else {
       launch_pack_chains(s, nodes, (int)rows + 1, 0, gamma, (int)rows, 0, (int)num, (int)rows,
                          (int)cols, static_cast<ChainRec*>(recs), prec == kF32C);
       launch_nbody_chain(s, prec, targets, (int)nt, static_cast<const ChainRec*>(recs), plan,
-                         part, (int)nt);
+                         part, (int)nt, c->cut);
     }
   }
   // no sources: zero chunks sum to zero (a kernel, not a memset)
@@ -631,7 +652,7 @@ int uvlm_influence_matrix(uvlm_ctx* c, const double* nodes, size_t num,
   launch_pack_lattice(c->stream, P<double>(c->t_a), (int)rows + 1, 0, nullptr, 0,
                       0, (int)num, (int)rows, (int)cols, P<RingRec>(c->t_recs));
   launch_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b),
-                P<double>(c->t_c), (int)N, c->td, P<double>(c->t_A));
+                P<double>(c->t_c), (int)N, c->cut, P<double>(c->t_A));
   TRY(check_launch(c));
   // device matrix is column-major: A^T row-major == A column-major
   double* tmp = new (std::nothrow) double[N * N];
@@ -675,7 +696,7 @@ int uvlm_ring_influence_matrix(uvlm_ctx* c, const double* rings, size_t n,
   launch_pack_rings(c->stream, P<double>(c->t_a), P<double>(c->t_out), (int)n,
                     P<RingRec>(c->t_recs));
   launch_ring_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b), P<double>(c->t_c),
-                     (int)n, c->td, P<double>(c->t_A));
+                     (int)n, c->cut, P<double>(c->t_A));
   TRY(check_launch(c));
   std::vector<double> tmp(n * n);
   HIPCHK(c, hipMemcpyAsync(tmp.data(), c->t_A.p, sizeof(double) * n * n,
@@ -936,7 +957,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   launch_rhs(c->stream, P<double>(c->uw), P<double>(c->vm), P<double>(c->nrm), N, ff,
              P<double>(c->rhs));
   launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
-                c->td, P<double>(c->A));
+                c->cut, P<double>(c->A));
   TRY(check_launch(c));
   c->pairs_eval += 4.0 * N * (double)N;
   c->pairs_ref += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
@@ -1357,6 +1378,8 @@ int uvlm_farfield_velocity(uvlm_ctx* c, const double* targets, size_t nt, const 
                            double theta, double* out) {
   if (!c || (nt && (!targets || !out)) || !nodes || !gamma) return UVLM_ERR_ARG;
   if (!(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (!default_cut(c))
+    return fail(c, UVLM_ERR_ARG, "the far-field split runs with the default cut-off only");
   if (nt > (1u << 26) || num * (rows + 1) * (cols + 1) > (1u << 26))
     return fail(c, UVLM_ERR_ARG, "problem too large");
   TRY(set_dev(c));
@@ -1562,6 +1585,8 @@ int uvlm_sim_farfield_learn(uvlm_ctx* c, double theta, size_t max_samples, size_
 int uvlm_sim_set_farfield(uvlm_ctx* c, double theta, int audit_every, double tol) {
   if (!c) return UVLM_ERR_ARG;
   if (theta > 0.0 && !(theta > 1.0)) return fail(c, UVLM_ERR_ARG, "theta must exceed 1");
+  if (theta > 0.0 && !default_cut(c))
+    return fail(c, UVLM_ERR_ARG, "the far-field split runs with the default cut-off only");
   if (theta > 0.0 && !c->ff) {
     c->ff = new (std::nothrow) FarField();
     if (c->ff) c->ff->set_profile(c->prof);
@@ -1688,6 +1713,7 @@ int uvlm_batch_init(uvlm_ctx* c, const uvlm_batch_case* cases, size_t n,
   if (!c || !cases || !wing_nodes || n == 0) return UVLM_ERR_ARG;
   if (n > (1u << 20) || max_steps > (1u << 20) || ignore_steps > (1u << 20))
     return fail(c, UVLM_ERR_ARG, "batch size or step count out of range");
+  if (!default_cut(c)) return fail(c, UVLM_ERR_ARG, "batches run with the default cut-off only");
   TRY(set_dev(c));
   delete c->batch;
   c->batch = new (std::nothrow) BatchEngine();

@@ -43,7 +43,7 @@ __device__ __forceinline__ void chunk_range(const ChunkPlan& p, int c, int& b, i
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_chain_f64(
     const double* __restrict__ targets, int nt, const ChainRec* __restrict__ recs,
-    ChunkPlan plan, double* __restrict__ partial, int ld) {
+    ChunkPlan plan, double* __restrict__ partial, int ld, CutOff cut) {
   __shared__ double2 tile[kTile * 4];
   int rb, re;
   chunk_range(plan, blockIdx.y, rb, re);
@@ -74,16 +74,18 @@ __global__ __launch_bounds__(kBlock) void k_chain_f64(
     for (int q = threadIdx.x; q < cnt * 4; q += kBlock) {
       const double2 v = src[(size_t)base * 4 + q];
       tile[q] = v;
-      lng |= (q & 3) == 3 && !(v.y < dev::kShortSeg2);
+      lng |= (q & 3) == 3 && !(v.y < cut.short2);
     }
     const bool any_long = __syncthreads_or(lng);
     const double* tl = reinterpret_cast<const double*>(tile);
     if (any_long) {
 #pragma unroll 1
-      for (int k = 0; k < cnt; ++k) dev::chain_step64<true>(tl + 8 * k, x, y, z, cr, ux, uy, uz);
+      for (int k = 0; k < cnt; ++k)
+        dev::chain_step64<true>(tl + 8 * k, x, y, z, cr, ux, uy, uz, cut.eps, cut.eps2);
     } else {
 #pragma unroll 2
-      for (int k = 0; k < cnt; ++k) dev::chain_step64<false>(tl + 8 * k, x, y, z, cr, ux, uy, uz);
+      for (int k = 0; k < cnt; ++k)
+        dev::chain_step64<false>(tl + 8 * k, x, y, z, cr, ux, uy, uz, cut.eps, cut.eps2);
     }
   }
   if (t < nt) {
@@ -294,7 +296,7 @@ int chain_slots(int prec) {
 
 void launch_nbody_chain(hipStream_t s, int prec, const double* targets, int nt,
                         const ChainRec* recs, const ChunkPlan& plan, double* partial,
-                        int ld) {
+                        int ld, const CutOff& cut) {
   if (nt <= 0 || plan.total() <= 0) return;
   dim3 grid(cdiv(nt, kBlock), plan.total());
   if (prec == kF32C)
@@ -302,7 +304,7 @@ void launch_nbody_chain(hipStream_t s, int prec, const double* targets, int nt,
                        partial, ld);
   else
     hipLaunchKernelGGL(k_chain_f64, grid, dim3(kBlock), 0, s, targets, nt, recs, plan,
-                       partial, ld);
+                       partial, ld, cut);
 }
 
 }  // namespace uvlm

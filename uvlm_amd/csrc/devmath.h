@@ -63,7 +63,8 @@ constexpr double kShortSeg2 = 1e9;
 template <bool Ends = true>
 __device__ __forceinline__ void chain_step64(const double* e, double x, double y, double z,
                                              Carry64& c, double& ux, double& uy,
-                                             double& uz) {
+                                             double& uz, double eps = kCutoff,
+                                             double eps2 = kCutoff * kCutoff) {
   const double rbx = x - e[0], rby = y - e[1], rbz = z - e[2];
   const double rbs = __builtin_fma(rbz, rbz, __builtin_fma(rby, rby, rbx * rbx));
   const double ib = rsq64_1(rbs);
@@ -75,8 +76,8 @@ __device__ __forceinline__ void chain_step64(const double* e, double x, double y
   const double oa = __builtin_fma(r0z, c.rz, __builtin_fma(r0y, c.ry, r0x * c.rx));
   const double ob = oa - e[7];
   const double tt = __builtin_fma(oa, c.inv, -ob * ib);
-  bool cut = dsq < kCutoff;
-  if (Ends) cut = cut | (c.rs < kCutoff * kCutoff) | (rbs < kCutoff * kCutoff);
+  bool cut = dsq < eps;
+  if (Ends) cut = cut | (c.rs < eps2) | (rbs < eps2);
   double K = tt * e[3] * rcp64_1(dsq);
   K = cut ? 0.0 : K;
   ux = __builtin_fma(dx, K, ux);

@@ -21,6 +21,17 @@ namespace uvlm {
 
 // Cut-off length of the reference's BiotSavartLaw (vortex.cpp:31).
 constexpr double kCutoff = 1e-10;
+// A run's core cut-off (the reference hard-codes 1e-10; another value keeps
+// the rule's shape): a pair is zero when |r1 x r2|^2 < eps, |r1| < eps or
+// |r2| < eps.  td: the smallest x with fl(fl(sqrt(x))^2) >= eps, so that the
+// exact form tests |d|^2 < td instead of squaring the rounded norm; eps2 =
+// eps^2 for the forms that test squared lengths; short2: segments with
+// |r0|^2 below it imply the end-point tests through the |d|^2 test
+// (|d| <= |r1| (|r1| + |r0|) < eps when |r0|^2 < 0.1 / eps).
+struct CutOff {
+  double eps, td, eps2, short2;
+};
+CutOff make_cutoff(double eps);
 // 1. / 4. / M_PI as the reference folds it (vortex.cpp:36).
 constexpr double kInv4Pi = 1. / 4. / 3.14159265358979323846;
 
@@ -98,8 +109,8 @@ ChunkPlan plan_chunks(int n_targets, int nb, int nw, bool sequential, int slots 
 int chain_slots(int prec);
 
 // ---- launchers (stream-ordered, asynchronous) ----------------------------
-// Threshold T with  fl(fl(sqrt(x))^2) < kCutoff  <=>  x < T  (x >= 0).
-double cutoff_dsq_threshold();
+// Threshold T with  fl(fl(sqrt(x))^2) < eps  <=>  x < T  (x >= 0).
+double cutoff_dsq_threshold(double eps = kCutoff);
 
 void launch_pack_lattice(hipStream_t s, const double* nodes, int sheet_rows,
                          int row_off, const double* gamma, int g_sheet_rows,
@@ -118,16 +129,17 @@ void launch_pack_chains(hipStream_t s, const double* nodes, int sheet_rows,
 // plan semantics as launch_nbody, in units of chain elements.
 void launch_nbody_chain(hipStream_t s, int prec, const double* targets,
                         int n_targets, const ChainRec* recs,
-                        const ChunkPlan& plan, double* partial, int ld);
+                        const ChunkPlan& plan, double* partial, int ld,
+                        const CutOff& cut);
 
 // Partial sums partial[(c*3 + d)*ld + t] of target t over chunk c.
 void launch_nbody(hipStream_t s, int prec, int seg, const double* targets,
                   int n_targets, const RingRec* recs, const ChunkPlan& plan,
-                  double* partial, int ld, double td);
+                  double* partial, int ld, const CutOff& cut);
 
 // Independent segment evaluations out[k] = BiotSavartLaw(a[k], b[k], x[k]).
 void launch_segments(hipStream_t s, int prec, const double* a, const double* b,
-                     const double* x, int n, double td, double* out);
+                     const double* x, int n, const CutOff& cut, double* out);
 
 // out[t] = sum of all chunks (in chunk order), AoS.
 void launch_sum_chunks(hipStream_t s, const double* partial, int ld,

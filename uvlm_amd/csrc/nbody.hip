@@ -31,7 +31,7 @@ constexpr int kTile = 64;
 __device__ __forceinline__ void seg_exact(double ax, double ay, double az,
                                           double bx, double by, double bz,
                                           double x, double y, double z,
-                                          double td, double& ox, double& oy,
+                                          const CutOff& k, double& ox, double& oy,
                                           double& oz) {
   const double r0x = bx - ax, r0y = by - ay, r0z = bz - az;
   const double r1x = x - ax, r1y = y - ay, r1z = z - az;
@@ -42,7 +42,7 @@ __device__ __forceinline__ void seg_exact(double ax, double ay, double az,
   const double dsq = (dx * dx + dy * dy) + dz * dz;
   const double r1n = sqrt((r1x * r1x + r1y * r1y) + r1z * r1z);
   const double r2n = sqrt((r2x * r2x + r2y * r2y) + r2z * r2z);
-  const bool cut = (dsq < td) | (r1n < kCutoff) | (r2n < kCutoff);
+  const bool cut = (dsq < k.td) | (r1n < k.eps) | (r2n < k.eps);
   const double wx = r1x / r1n - r2x / r2n;
   const double wy = r1y / r1n - r2y / r2n;
   const double wz = r1z / r1n - r2z / r2n;
@@ -75,8 +75,8 @@ __device__ __forceinline__ double rcp64(double a) {
 __device__ __forceinline__ void seg_fast(double ax, double ay, double az,
                                          double bx, double by, double bz,
                                          double x, double y, double z,
-                                         double g, double& ux, double& uy,
-                                         double& uz) {
+                                         double g, const CutOff& k, double& ux,
+                                         double& uy, double& uz) {
   const double r0x = bx - ax, r0y = by - ay, r0z = bz - az;
   const double r1x = x - ax, r1y = y - ay, r1z = z - az;
   const double r2x = x - bx, r2y = y - by, r2z = z - bz;
@@ -88,8 +88,7 @@ __device__ __forceinline__ void seg_fast(double ax, double ay, double az,
   const double r2s = __builtin_fma(r2z, r2z, __builtin_fma(r2y, r2y, r2x * r2x));
   const double o1 = __builtin_fma(r0z, r1z, __builtin_fma(r0y, r1y, r0x * r1x));
   const double o2 = __builtin_fma(r0z, r2z, __builtin_fma(r0y, r2y, r0x * r2x));
-  const bool cut = (dsq < kCutoff) | (r1s < kCutoff * kCutoff) |
-                   (r2s < kCutoff * kCutoff);
+  const bool cut = (dsq < k.eps) | (r1s < k.eps2) | (r2s < k.eps2);
   const double t = __builtin_fma(o1, rsq64(r1s), -o2 * rsq64(r2s));
   double K = (kInv4Pi * g) * t * rcp64(dsq);
   K = cut ? 0.0 : K;
@@ -102,7 +101,7 @@ __device__ __forceinline__ void seg_fast(double ax, double ay, double az,
 // Ring sums in the reference's order.
 template <int SEG>
 __device__ __forceinline__ void ring_exact(const double* r, double x, double y,
-                                           double z, double td, double& ax,
+                                           double z, const CutOff& cut, double& ax,
                                            double& ay, double& az) {
   const double g = r[12];
   double sx, sy, sz;
@@ -112,7 +111,7 @@ __device__ __forceinline__ void ring_exact(const double* r, double x, double y,
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int a = 3 * k, b = 3 * ((k + 1) & 3);
-      seg_exact(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, td,
+      seg_exact(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, cut,
                 sx, sy, sz);
       ux = ux + sx * g;
       uy = uy + sy * g;
@@ -124,7 +123,7 @@ __device__ __forceinline__ void ring_exact(const double* r, double x, double y,
     for (int k = 0; k < 4; ++k) {
       if (SEG == kSegRingChord && (k & 1)) continue;
       const int a = 3 * ((k + 1) & 3), b = 3 * k;
-      seg_exact(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, td,
+      seg_exact(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, cut,
                 sx, sy, sz);
       if (SEG == kSegRingPerSeg) {
         ux = ux + sx * g;
@@ -149,8 +148,8 @@ __device__ __forceinline__ void ring_exact(const double* r, double x, double y,
 
 template <int SEG>
 __device__ __forceinline__ void ring_fast(const double* r, double x, double y,
-                                          double z, double& ax, double& ay,
-                                          double& az) {
+                                          double z, const CutOff& cut, double& ax,
+                                          double& ay, double& az) {
   const double g = r[12];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -163,7 +162,7 @@ __device__ __forceinline__ void ring_fast(const double* r, double x, double y,
       a = 3 * ((k + 1) & 3);
       b = 3 * k;
     }
-    seg_fast(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, g, ax,
+    seg_fast(r[a], r[a + 1], r[a + 2], r[b], r[b + 1], r[b + 2], x, y, z, g, cut, ax,
              ay, az);
   }
 }
@@ -182,7 +181,7 @@ __device__ __forceinline__ void chunk_range(const ChunkPlan& p, int c, int& b,
 template <int PREC, int SEG>
 __global__ __launch_bounds__(kBlock) void k_nbody(
     const double* __restrict__ targets, int nt, const RingRec* __restrict__ recs,
-    ChunkPlan plan, double* __restrict__ partial, int ld, double td) {
+    ChunkPlan plan, double* __restrict__ partial, int ld, CutOff cut) {
   __shared__ double2 tile[kTile * 8];
   int rb, re;
   chunk_range(plan, blockIdx.y, rb, re);
@@ -203,9 +202,9 @@ __global__ __launch_bounds__(kBlock) void k_nbody(
     const double* tl = reinterpret_cast<const double*>(tile);
     for (int k = 0; k < cnt; ++k) {
       if (PREC == kF64Exact)
-        ring_exact<SEG>(tl + 16 * k, x, y, z, td, ax, ay, az);
+        ring_exact<SEG>(tl + 16 * k, x, y, z, cut, ax, ay, az);
       else
-        ring_fast<SEG>(tl + 16 * k, x, y, z, ax, ay, az);
+        ring_fast<SEG>(tl + 16 * k, x, y, z, cut, ax, ay, az);
     }
   }
   if (t < nt) {
@@ -358,7 +357,7 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 template <int PREC>
 __global__ void k_segments(const double* __restrict__ a, const double* __restrict__ b,
-                           const double* __restrict__ x, int n, double td,
+                           const double* __restrict__ x, int n, CutOff cut,
                            double* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -367,9 +366,9 @@ __global__ void k_segments(const double* __restrict__ a, const double* __restric
   const double* X = x + 3 * (size_t)k;
   double ox = 0.0, oy = 0.0, oz = 0.0;
   if (PREC == kF64Exact)
-    seg_exact(A[0], A[1], A[2], B[0], B[1], B[2], X[0], X[1], X[2], td, ox, oy, oz);
+    seg_exact(A[0], A[1], A[2], B[0], B[1], B[2], X[0], X[1], X[2], cut, ox, oy, oz);
   else
-    seg_fast(A[0], A[1], A[2], B[0], B[1], B[2], X[0], X[1], X[2], 1.0, ox, oy, oz);
+    seg_fast(A[0], A[1], A[2], B[0], B[1], B[2], X[0], X[1], X[2], 1.0, cut, ox, oy, oz);
   out[3 * (size_t)k + 0] = ox;
   out[3 * (size_t)k + 1] = oy;
   out[3 * (size_t)k + 2] = oz;
@@ -378,16 +377,16 @@ __global__ void k_segments(const double* __restrict__ a, const double* __restric
 }  // namespace
 
 // ---------------------------------------------------------------------------
-double cutoff_dsq_threshold() {
-  // f(x) = fl(fl(sqrt(x))^2) < kCutoff is monotone in x >= 0; find the
+double cutoff_dsq_threshold(double eps) {
+  // f(x) = fl(fl(sqrt(x))^2) < eps is monotone in x >= 0; find the
   // smallest x with f(x) false by bisection on the IEEE bit patterns.
-  auto f = [](double x) {
+  auto f = [eps](double x) {
     volatile double s = std::sqrt(x);
     volatile double p = s * s;
-    return p < kCutoff;
+    return p < eps;
   };
   uint64_t lo, hi;
-  double dlo = 0.0, dhi = 1e-9;
+  double dlo = 0.0, dhi = 2.0 * eps;
   std::memcpy(&lo, &dlo, 8);
   std::memcpy(&hi, &dhi, 8);
   while (hi - lo > 1) {
@@ -402,6 +401,15 @@ double cutoff_dsq_threshold() {
   double T;
   std::memcpy(&T, &hi, 8);
   return T;
+}
+
+CutOff make_cutoff(double eps) {
+  CutOff c;
+  c.eps = eps;
+  c.td = cutoff_dsq_threshold(eps);
+  c.eps2 = eps * eps;
+  c.short2 = 0.1 / eps;
+  return c;
 }
 
 ChunkPlan plan_chunks(int nt, int nb, int nw, bool sequential, int slots) {
@@ -465,12 +473,12 @@ void launch_set_gamma(hipStream_t s, const double* gamma, int n, RingRec* out) {
 
 void launch_nbody(hipStream_t s, int prec, int seg, const double* targets,
                   int nt, const RingRec* recs, const ChunkPlan& plan,
-                  double* partial, int ld, double td) {
+                  double* partial, int ld, const CutOff& cut) {
   if (nt <= 0 || plan.total() <= 0) return;
   dim3 grid(cdiv(nt, kBlock), plan.total());
 #define UVLM_NB(P, S)                                                          \
   hipLaunchKernelGGL((k_nbody<P, S>), grid, dim3(kBlock), 0, s, targets, nt, \
-                     recs, plan, partial, ld, td)
+                     recs, plan, partial, ld, cut)
   if (prec == kF64Exact) {
     switch (seg) {
       case kSegVoring: UVLM_NB(kF64Exact, kSegVoring); break;
@@ -490,14 +498,14 @@ void launch_nbody(hipStream_t s, int prec, int seg, const double* targets,
 }
 
 void launch_segments(hipStream_t s, int prec, const double* a, const double* b,
-                     const double* x, int n, double td, double* out) {
+                     const double* x, int n, const CutOff& cut, double* out) {
   if (n <= 0) return;
   if (prec == kF64Exact)
     hipLaunchKernelGGL((k_segments<kF64Exact>), dim3(cdiv(n, 256)), dim3(256), 0,
-                       s, a, b, x, n, td, out);
+                       s, a, b, x, n, cut, out);
   else
     hipLaunchKernelGGL((k_segments<kF64Fast>), dim3(cdiv(n, 256)), dim3(256), 0,
-                       s, a, b, x, n, td, out);
+                       s, a, b, x, n, cut, out);
 }
 
 void launch_sum_chunks(hipStream_t s, const double* partial, int ld, int nt,

@@ -8,10 +8,10 @@ namespace uvlm {
 void launch_geometry(hipStream_t s, const double* nodes, int num, int R, int C,
                      double* cp, double* nrm, double* area);
 void launch_matrix(hipStream_t s, const RingRec* recs, const double* cp,
-                   const double* nrm, int N, double td, double* A);
+                   const double* nrm, int N, const CutOff& cut, double* A);
 // Legacy ring order (A[L*N + K] = ring_L.BiotSavartLaw(cp_K, 1) . n_K).
 void launch_ring_matrix(hipStream_t s, const RingRec* recs, const double* cp,
-                        const double* nrm, int N, double td, double* A);
+                        const double* nrm, int N, const CutOff& cut, double* A);
 void launch_rhs(hipStream_t s, const double* uw, const double* vm,
                 const double* nrm, int N, const double* ff, double* rhs);
 void launch_shed(hipStream_t s, const double* wing, const double* gamma, int num,

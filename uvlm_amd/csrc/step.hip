@@ -62,13 +62,13 @@ __global__ void k_geometry(const double* __restrict__ nodes, int num, int R,
 }
 
 // BiotSavartLaw (vortex.cpp:21-39), same arithmetic as the N-body kernel.
-__device__ __forceinline__ d3 seg_exact(d3 a, d3 b, d3 x, double td) {
+__device__ __forceinline__ d3 seg_exact(d3 a, d3 b, d3 x, const CutOff& k) {
   const d3 r0 = sub(b, a), r1 = sub(x, a), r2 = sub(x, b);
   const d3 d = crs(r1, r2);
   const double dsq = sqn(d);
   const double r1n = sqrt(sqn(r1));
   const double r2n = sqrt(sqn(r2));
-  const bool cut = (dsq < td) | (r1n < kCutoff) | (r2n < kCutoff);
+  const bool cut = (dsq < k.td) | (r1n < k.eps) | (r2n < k.eps);
   const d3 w = sub(dvs(r1, r1n), dvs(r2, r2n));
   const double K = kInv4Pi / dsq * dot(r0, w);
   return cut ? d3{0.0, 0.0, 0.0} : scl(d, K);
@@ -83,7 +83,7 @@ template <bool LEGACY>
 __global__ __launch_bounds__(256) void k_matrix(const RingRec* __restrict__ recs,
                                                 const double* __restrict__ cp,
                                                 const double* __restrict__ nrm,
-                                                int N, double td,
+                                                int N, CutOff cut,
                                                 double* __restrict__ A) {
   __shared__ double2 tile[kMatRings * 8];
   const int l0 = blockIdx.y * kMatRings;
@@ -103,9 +103,9 @@ __global__ __launch_bounds__(256) void k_matrix(const RingRec* __restrict__ recs
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (LEGACY)
-        u = add(u, seg_exact(p[(k + 1) & 3], p[k], x, td));
+        u = add(u, seg_exact(p[(k + 1) & 3], p[k], x, cut));
       else
-        u = add(u, seg_exact(p[k], p[(k + 1) & 3], x, td));
+        u = add(u, seg_exact(p[k], p[(k + 1) & 3], x, cut));
     }
     A[(size_t)(l0 + l) * N + K] = dot(u, nl);
   }
@@ -309,15 +309,15 @@ void launch_geometry(hipStream_t s, const double* nodes, int num, int R, int C,
 }
 
 void launch_matrix(hipStream_t s, const RingRec* recs, const double* cp,
-                   const double* nrm, int N, double td, double* A) {
+                   const double* nrm, int N, const CutOff& cut, double* A) {
   dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
-  hipLaunchKernelGGL(k_matrix<false>, g, dim3(256), 0, s, recs, cp, nrm, N, td, A);
+  hipLaunchKernelGGL(k_matrix<false>, g, dim3(256), 0, s, recs, cp, nrm, N, cut, A);
 }
 
 void launch_ring_matrix(hipStream_t s, const RingRec* recs, const double* cp,
-                        const double* nrm, int N, double td, double* A) {
+                        const double* nrm, int N, const CutOff& cut, double* A) {
   dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
-  hipLaunchKernelGGL(k_matrix<true>, g, dim3(256), 0, s, recs, cp, nrm, N, td, A);
+  hipLaunchKernelGGL(k_matrix<true>, g, dim3(256), 0, s, recs, cp, nrm, N, cut, A);
 }
 
 void launch_rhs(hipStream_t s, const double* uw, const double* vm,

@@ -264,6 +264,7 @@ def case_spec(case: dict) -> N.CaseSpec:
     cs.loads = int(bool(case.get("loads", True)))
     cs.precision = int(case.get("precision", N.PREC_F64_EXACT))
     cs.solve_mode = int(case.get("solve_mode", N.SOLVE_F64))
+    cs.cutoff = float(case.get("cutoff", 0.0))
     return cs
 
 

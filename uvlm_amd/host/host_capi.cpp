@@ -146,6 +146,7 @@ int uvlm_host_sim_create(const uvlm_case_spec* sp, size_t max_steps, uvlm_host_s
   h->sim.set_advection(adv);
   h->sim.set_precision(sp->precision);
   h->sim.set_solve_mode(sp->solve_mode);
+  if (sp->cutoff > 0.0) h->sim.set_cutoff(sp->cutoff);
   h->sim.set_erase_wake_after(sp->erase_wake_after);
   h->sim.set_compute_loads(sp->loads != 0);
   h->sim.Prepare(max_steps, sp->dt);

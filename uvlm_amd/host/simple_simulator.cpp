@@ -156,6 +156,7 @@ void SimpleSimulator::Prepare(std::size_t max_steps, double dt) {
       std::abort();
     }
   }
+  if (cutoff_ > 0.0) gpu::Check(uvlm_set_cutoff(ctx_, cutoff_), "uvlm_set_cutoff", ctx_);
   gpu::Check(uvlm_sim_init(ctx_, num(), rows_, cols_, max_steps, precision_),
              "uvlm_sim_init", ctx_);
   dt_ = dt;

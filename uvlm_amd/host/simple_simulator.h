@@ -66,6 +66,9 @@ class SimpleSimulator {
   // UVLM_PREC_* of the Biot-Savart sums and UVLM_SOLVE_* of the solve.
   void set_precision(unsigned flags) { precision_ = flags; }
   void set_solve_mode(int mode) { solve_mode_ = mode; }
+  // Core cut-off of the Biot-Savart sums (0: the reference's 1e-10;
+  // uvlm_set_cutoff).
+  void set_cutoff(double eps) { cutoff_ = eps; }
   // --erase_wake_after (simple_simulator.cpp:14,307-311); 0 disables.
   void set_erase_wake_after(std::size_t n) { erase_wake_after_ = n; }
   // Compute loads every step even without a load path.
@@ -114,6 +117,7 @@ class SimpleSimulator {
   unsigned precision_ = UVLM_PREC_F64_EXACT;
   int solve_mode_ = UVLM_SOLVE_F64;
   std::size_t erase_wake_after_ = 0;
+  double cutoff_ = 0.0;
   bool compute_loads_ = false;
   bool prepared_ = false;
   double dt_ = 0;

@@ -56,6 +56,15 @@ def plate_c1(steps: int = 200, rows: int = 8, cols: int = 16) -> dict:
             "dt": 1.0 / 8.0, "advect": "euler", "steps": steps}
 
 
+def plate_c1_core(steps: int = 200, rows: int = 8, cols: int = 16) -> dict:
+    """The plate workload with the configured core cut-off 1e-6 (the
+    reference's rule with eps = 1e-6 instead of its hard-coded 1e-10)."""
+    c = plate_c1(steps=steps, rows=rows, cols=cols)
+    c["name"] = "plate_8x16_euler_core1e-6"
+    c["cutoff"] = 1e-6
+    return c
+
+
 def naca4412_c2(steps: int = 1000, rows: int = 20, cols: int = 80) -> dict:
     """NACA4412 camber, AR 6, 20x80 rings, root-hinged flap +/-30 deg at
     k=0.3, 5 deg pitch, 40 steps per cycle, reference "RK4"."""
