@@ -80,6 +80,15 @@ const char* uvlm_last_error(const uvlm_ctx* ctx);
  * refuse other values (UVLM_ERR_ARG).  eps in (0, 1e-2). */
 int uvlm_set_cutoff(uvlm_ctx* ctx, double eps);
 
+/* The stateless sums (uvlm_induced_velocity, uvlm_ring_induced_velocity)
+ * keep the last source set packed on the device, with a host copy of its
+ * input: a call whose sources equal it byte for byte (same shape and record
+ * layout) skips the upload and the packing, so querying one lattice target
+ * by target costs the targets' transfers only.  Counts of source uploads
+ * and of such reuses since the context was created (no reference
+ * counterpart). */
+int uvlm_source_cache_stats(uvlm_ctx* ctx, long long* uploads, long long* hits);
+
 /* Wait for all work queued on the context's stream. */
 int uvlm_sync(uvlm_ctx* ctx);
 

@@ -286,3 +286,36 @@ def test_solve_f64_large_global_substitution(ctx):
     b = rng.normal(size=n)
     x, _ = ctx.solve(A, b)
     np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-13)
+
+
+def test_source_cache_target_by_target(ctx):
+    """A lattice queried one target at a time (the reference's per-target
+    InducedVelocity calls, velocity.h:14-24) is uploaded and packed once:
+    every later call with the same bytes reuses the device records, and the
+    results are bitwise those of one batched call.  Any changed byte, shape
+    or record layout uploads again."""
+    import ctypes as C
+    rng = np.random.default_rng(9)
+    nodes = rng.normal(size=(2, 4, 7, 3))
+    gamma = rng.normal(size=(2, 3, 6))
+    x = rng.normal(size=(40, 3))
+
+    def stats():
+        u, h = C.c_longlong(), C.c_longlong()
+        assert ctx._L.uvlm_source_cache_stats(ctx._h, C.byref(u), C.byref(h)) == N.OK
+        return u.value, h.value
+
+    for flags in (N.PREC_F64_EXACT, N.PREC_F64):
+        want = ctx.induced_velocity(x, nodes, gamma, flags)
+        u0, h0 = stats()
+        one = np.stack([ctx.induced_velocity(x[k], nodes, gamma, flags)[0]
+                        for k in range(len(x))])
+        np.testing.assert_array_equal(one, want)
+        u1, h1 = stats()
+        assert (u1 - u0, h1 - h0) == (0, len(x))
+    g2 = gamma.copy()
+    g2[1, 2, 5] += 1e-12
+    u0, _ = stats()
+    changed = ctx.induced_velocity(x, nodes, g2, N.PREC_F64)
+    assert stats()[0] == u0 + 1 and np.any(changed != want)
+    np.testing.assert_array_equal(ctx.induced_velocity(x, nodes, gamma, N.PREC_F64), want)

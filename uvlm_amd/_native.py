@@ -29,7 +29,7 @@ MAX_WINGS = 16
 
 # Every exported symbol of the two headers (checked by the CPU test suite).
 GPU_SYMBOLS = [
-    "uvlm_create", "uvlm_destroy", "uvlm_last_error", "uvlm_sync", "uvlm_set_cutoff",
+    "uvlm_create", "uvlm_destroy", "uvlm_last_error", "uvlm_sync", "uvlm_set_cutoff", "uvlm_source_cache_stats",
     "uvlm_biot_savart", "uvlm_induced_velocity", "uvlm_induced_velocity_noalloc", "uvlm_ring_induced_velocity",
     "uvlm_influence_matrix", "uvlm_ring_influence_matrix", "uvlm_solve",
     "uvlm_sim_init", "uvlm_sim_set_wing",
@@ -127,6 +127,8 @@ def gpu_lib():
         L.uvlm_last_error.restype = C.c_char_p
         L.uvlm_sync.argtypes = [_vp]
         L.uvlm_set_cutoff.argtypes = [_vp, C.c_double]
+        L.uvlm_source_cache_stats.argtypes = [_vp, C.POINTER(C.c_longlong),
+                                              C.POINTER(C.c_longlong)]
         L.uvlm_induced_velocity_noalloc.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz,
                                                     C.c_uint, _vp, _vp]
         L.uvlm_biot_savart.argtypes = [_vp, _dp, _dp, _dp, _sz, C.c_uint, _dp]

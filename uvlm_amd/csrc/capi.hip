@@ -86,6 +86,18 @@ struct uvlm_ctx {
   // refinement sweeps of the mixed-precision solves in the simulation
   long long ir_solves = 0, ir_sweeps = 0;
   int ir_last = 0;
+  // Sources of the last stateless sum, packed on the device.  The host copy
+  // detects an identical next call (same kind, record layout, shape and
+  // bytes), which then skips the upload and the packing: a caller that
+  // queries one lattice target by target pays for its targets only.
+  struct SourceCache {
+    int kind = -1;  // 0 lattice rings, 1 lattice chains, 2 free rings
+    int layout = 0;
+    size_t a = 0, b = 0, cc = 0;
+    std::vector<double> nodes, gamma;
+  } src;
+  Buf src_recs;
+  long long src_hits = 0, src_uploads = 0;
   // far-field correction network
   uvlm::MlpTrainer* mlp = nullptr;
   Buf mw, mx, mv, my, mg, mdev;
@@ -191,6 +203,26 @@ int set_dev(uvlm_ctx* c) {
 }
 
 bool default_cut(const uvlm_ctx* c) { return c->cut.eps == kCutoff; }
+
+bool src_same(const uvlm_ctx* c, int kind, int layout, size_t a, size_t b, size_t cc,
+              const double* nodes, size_t nn, const double* gamma, size_t ng) {
+  const auto& s = c->src;
+  return s.kind == kind && s.layout == layout && s.a == a && s.b == b && s.cc == cc &&
+         s.nodes.size() == nn && s.gamma.size() == ng &&
+         std::memcmp(s.nodes.data(), nodes, sizeof(double) * nn) == 0 &&
+         std::memcmp(s.gamma.data(), gamma, sizeof(double) * ng) == 0;
+}
+void src_store(uvlm_ctx* c, int kind, int layout, size_t a, size_t b, size_t cc,
+               const double* nodes, size_t nn, const double* gamma, size_t ng) {
+  auto& s = c->src;
+  s.kind = kind;
+  s.layout = layout;
+  s.a = a;
+  s.b = b;
+  s.cc = cc;
+  s.nodes.assign(nodes, nodes + nn);
+  s.gamma.assign(gamma, gamma + ng);
+}
 
 int check_prec(uvlm_ctx* c, unsigned flags) {
   const int p = prec_of(flags);
@@ -440,6 +472,13 @@ int uvlm_set_cutoff(uvlm_ctx* c, double eps) {
   return UVLM_OK;
 }
 
+int uvlm_source_cache_stats(uvlm_ctx* c, long long* uploads, long long* hits) {
+  if (!c) return UVLM_ERR_ARG;
+  if (uploads) *uploads = c->src_uploads;
+  if (hits) *hits = c->src_hits;
+  return UVLM_OK;
+}
+
 int uvlm_sync(uvlm_ctx* c) {
   if (!c) return UVLM_ERR_ARG;
   TRY(set_dev(c));
@@ -487,35 +526,46 @@ int uvlm_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
   }
   if (!nodes || !gamma) return fail(c, UVLM_ERR_ARG, "null nodes/gamma");
   const size_t nn = num * (rows + 1) * (cols + 1);
+  const int prec = prec_of(flags);
+  const int kind = prec == kF64Exact ? 0 : 1;
+  const int ne = kind ? chain_elems((int)num, (int)rows, (int)cols) : 0;
   TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
-  TRY(ensure(c, c->t_b, sizeof(double) * 3 * nn));
-  TRY(ensure(c, c->t_c, sizeof(double) * nr));
-  TRY(ensure(c, c->t_recs, sizeof(RingRec) * nr));
   TRY(ensure(c, c->t_out, sizeof(double) * 3 * nt));
   HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt,
                            hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->t_b.p, nodes, sizeof(double) * 3 * nn,
-                           hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * nr,
-                           hipMemcpyHostToDevice, c->stream));
+  if (src_same(c, kind, prec, num, rows, cols, nodes, 3 * nn, gamma, nr)) {
+    c->src_hits += 1;
+  } else {
+    c->src.kind = -1;  // invalid until the new set is packed
+    TRY(ensure(c, c->t_b, sizeof(double) * 3 * nn));
+    TRY(ensure(c, c->t_c, sizeof(double) * nr));
+    TRY(ensure(c, c->src_recs, kind ? sizeof(ChainRec) * ne : sizeof(RingRec) * nr));
+    HIPCHK(c, hipMemcpyAsync(c->t_b.p, nodes, sizeof(double) * 3 * nn,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * nr,
+                             hipMemcpyHostToDevice, c->stream));
+    if (kind == 0)
+      launch_pack_lattice(c->stream, P<double>(c->t_b), (int)rows + 1, 0,
+                          P<double>(c->t_c), (int)rows, 0, (int)num, (int)rows,
+                          (int)cols, P<RingRec>(c->src_recs));
+    else
+      launch_pack_chains(c->stream, P<double>(c->t_b), (int)rows + 1, 0, P<double>(c->t_c),
+                         (int)rows, 0, (int)num, (int)rows, (int)cols,
+                         P<ChainRec>(c->src_recs), prec == kF32C);
+    TRY(check_launch(c));
+    src_store(c, kind, prec, num, rows, cols, nodes, 3 * nn, gamma, nr);
+    c->src_uploads += 1;
+  }
   ChunkPlan plan;
-  if (prec_of(flags) == kF64Exact) {
-    launch_pack_lattice(c->stream, P<double>(c->t_b), (int)rows + 1, 0,
-                        P<double>(c->t_c), (int)rows, 0, (int)num, (int)rows,
-                        (int)cols, P<RingRec>(c->t_recs));
-    TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
+  if (kind == 0) {
+    TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->src_recs), 0,
               (int)nr, flags, kSegVoring, &plan));
   } else {
-    const int ne = chain_elems((int)num, (int)rows, (int)cols);
-    TRY(ensure(c, c->t_recs, sizeof(ChainRec) * ne));
-    launch_pack_chains(c->stream, P<double>(c->t_b), (int)rows + 1, 0, P<double>(c->t_c),
-                       (int)rows, 0, (int)num, (int)rows, (int)cols,
-                       P<ChainRec>(c->t_recs), prec_of(flags) == kF32C);
     plan = plan_chunks((int)nt, 0, ne, (flags & UVLM_SEQUENTIAL) != 0);
     TRY(ensure(c, c->t_partial,
                sizeof(double) * 3 * (size_t)std::max(1, plan.total()) * nt));
-    launch_nbody_chain(c->stream, prec_of(flags), P<double>(c->t_a), (int)nt,
-                       P<ChainRec>(c->t_recs), plan, P<double>(c->t_partial), (int)nt,
+    launch_nbody_chain(c->stream, prec, P<double>(c->t_a), (int)nt,
+                       P<ChainRec>(c->src_recs), plan, P<double>(c->t_partial), (int)nt,
                        c->cut);
     TRY(check_launch(c));
   }
@@ -604,20 +654,28 @@ int uvlm_ring_induced_velocity(uvlm_ctx* c, const double* targets, size_t nt,
   else if (flags & UVLM_SCALE_PER_RING)
     seg = kSegRingPerRing;
   TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
-  TRY(ensure(c, c->t_b, sizeof(double) * 12 * n_rings));
-  TRY(ensure(c, c->t_c, sizeof(double) * n_rings));
-  TRY(ensure(c, c->t_recs, sizeof(RingRec) * n_rings));
   TRY(ensure(c, c->t_out, sizeof(double) * 3 * nt));
   HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt,
                            hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->t_b.p, rings, sizeof(double) * 12 * n_rings,
-                           hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * n_rings,
-                           hipMemcpyHostToDevice, c->stream));
-  launch_pack_rings(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)n_rings,
-                    P<RingRec>(c->t_recs));
+  if (src_same(c, 2, 0, n_rings, 0, 0, rings, 12 * n_rings, gamma, n_rings)) {
+    c->src_hits += 1;
+  } else {
+    c->src.kind = -1;
+    TRY(ensure(c, c->t_b, sizeof(double) * 12 * n_rings));
+    TRY(ensure(c, c->t_c, sizeof(double) * n_rings));
+    TRY(ensure(c, c->src_recs, sizeof(RingRec) * n_rings));
+    HIPCHK(c, hipMemcpyAsync(c->t_b.p, rings, sizeof(double) * 12 * n_rings,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->t_c.p, gamma, sizeof(double) * n_rings,
+                             hipMemcpyHostToDevice, c->stream));
+    launch_pack_rings(c->stream, P<double>(c->t_b), P<double>(c->t_c), (int)n_rings,
+                      P<RingRec>(c->src_recs));
+    TRY(check_launch(c));
+    src_store(c, 2, 0, n_rings, 0, 0, rings, 12 * n_rings, gamma, n_rings);
+    c->src_uploads += 1;
+  }
   ChunkPlan plan;
-  TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->t_recs), 0,
+  TRY(nbody(c, c->t_partial, P<double>(c->t_a), (int)nt, P<RingRec>(c->src_recs), 0,
             (int)n_rings, flags, seg, &plan));
   launch_sum_chunks(c->stream, P<double>(c->t_partial), (int)nt, (int)nt,
                     plan.total(), P<double>(c->t_out));
