@@ -37,7 +37,10 @@ state of the plain run, on the host's cores.
 Other modes: --workload c1|c2|c3|c5 runs one configuration alone (same
 window rules, `--precision`, `--solve`, `--farfield` choose the method),
 --workload c4 the 256 batched cases; --no-plain, --no-c2, --no-cpu skip
-legs; --calibrate-copy runs a plain device copy of known size for the PMC
+legs; --spread adds a second FP64 run of the tandem workload that differs
+from the plain one only by rounding (kept-inverse solve instead of the LU)
+and reports, per 100-step window, the fast run's CL/CT differences beside
+the two FP64 runs' spread; --calibrate-copy runs a plain device copy of known size for the PMC
 traffic calibration.
 """
 from __future__ import annotations
@@ -171,6 +174,39 @@ def load_differences(loads, ref, early=30):
     return out
 
 
+def windowed_max(loads, ref, width=100):
+    """Largest CL/CT difference relative to each wing's largest |CL|, |CT|
+    of `ref`, per window of `width` steps: [quantity][window][wing]."""
+    n = min(len(loads), len(ref))
+    out = {}
+    for q, name in ((1, "CL"), (0, "CT")):
+        a, b = loads[:n, :, q], ref[:n, :, q]
+        scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+        d = np.abs(a - b) / scale
+        out[name] = [d[k:k + width].max(axis=0).tolist() for k in range(0, n, width)]
+    return out
+
+
+def spread_report(fast, plain, plain2, width=100):
+    """Pointwise differences of the fast run from the plain run, beside the
+    spread of two FP64 runs that differ only by rounding (the plain method
+    with the kept-inverse solve in place of the LU), over the same steps."""
+    f, p, q = fast["loads"], plain["loads"], plain2["loads"]
+    out = {"second_fp64_run": "FP64 segment chains, kept-inverse FP64 solve",
+           "window_steps": width,
+           "fast_vs_plain": windowed_max(f, p, width),
+           "fp64_vs_fp64": windowed_max(q, p, width),
+           "fast_vs_second_fp64": windowed_max(f, q, width),
+           "fp64_vs_fp64_all_steps": load_differences(q, p)}
+    ratio = {}
+    for name in ("CL", "CT"):
+        a = np.array(out["fast_vs_plain"][name])
+        b = np.array(out["fp64_vs_fp64"][name])
+        ratio[name] = (a / np.maximum(b, 1e-300)).max(axis=0).tolist()
+    out["largest_window_ratio_fast_over_fp64_spread"] = ratio
+    return out
+
+
 def cpu_baseline(sim, case, seconds):
     """Oracle flow velocity of every bound and wake ring (advect.h:16-24) at
     a sample of the wake nodes of the final state, timed on the host cores
@@ -272,6 +308,11 @@ def bench_tandem(args):
             **load_differences(fast["loads"], plain["loads"])}
         out["vs_plain_fp64"]["plain_chain_roofline"] = roofline(
             "k_chain_f64 (plain run)", plain["chain"], plain["t_run"], PEAK_FP64_TFLOPS)
+        if args.spread:
+            log("second FP64 run (rounding spread)")
+            r2 = run_case(case, dict(PLAIN, solve="inv"), args.steps, args.warmup, "fp64-b")
+            r2.pop("sim").close()
+            out["rounding_spread"] = spread_report(fast, plain, r2)
     if not args.no_c2:
         log("NACA4412 block")
         c2 = W.naca4412_c2()
@@ -391,6 +432,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--spread", action="store_true",
+                    help="c5: also run a second FP64 run that differs from the plain one "
+                         "only by rounding, and report both runs' differences per window")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
 

@@ -16,6 +16,26 @@
  *   3-term sums of vectors are evaluated left to right.
  * The library is compiled with -O2 -ffp-contract=off (no FMA contraction),
  * matching the reference's plain -O2 x86-64 build.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   biot_savart, voring, induced/ring velocity: the reference's known
+ *     answers (vortex_test.cpp, shed_test.cpp), long-double quadrature,
+ *     the dipole limit, the cut-off threshold; set_cutoff: the same at
+ *     eps = 1e-6;
+ *   ring geometry, impulse: vortex_test.cpp, calc_impulse_test.cpp;
+ *   fullpiv / partialpiv solves: numpy, rank deficiency, the legacy
+ *     closed form (linear_test.cpp);
+ *   wing generators, morphing: wing_test.cpp, naca4digit_test.cpp with the
+ *     reference's NACA62XX.dat, closed forms, finite differences;
+ *   sim (shedding, advection schemes, loads): invariants of the loop,
+ *     local-error orders of every scheme, the steady Kutta-Joukowski lift
+ *     and a closed form of the unsteady term;
+ *   lamb_vector: calc_impulse_test.cpp; katz_plotkin_load and
+ *     joukowski_container_load: flat-plate closed forms.
+ * Parity unpinned: Eigen's internal reduction orders inside
+ * FullPivLU::solve and its 3-term sums (fixed here as stated above);
+ * katz_plotkin_load starts F from zero where the reference leaves its
+ * accumulator uninitialised (katz_plotkin.cpp:22).
  */
 #ifndef UVLM_ORACLE_H
 #define UVLM_ORACLE_H
