@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 GPU_LIB = os.path.join(LIBDIR, "libuvlm_gpu.so")
 HOST_LIB = os.path.join(LIBDIR, "libuvlm.so")
 SOURCES = ["capi.hip", "nbody.hip", "chain.hip", "dense.hip", "step.hip", "batch.hip",
-           "farfield.hip", "inv_solve.hip", "mlp.hip"]
+           "farfield.hip", "inv_solve.hip", "mlp.hip", "gemm.hip"]
 ARCH = os.environ.get("UVLM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -56,7 +56,7 @@ def build_gpu(verbose: bool = False) -> str:
                 print(out)
     tmp = GPU_LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp,
-          "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

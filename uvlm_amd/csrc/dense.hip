@@ -6,23 +6,31 @@
 // pivoting to ~cond(A) * eps.  UVLM_SOLVE_F64_FULLPIV reproduces FullPivLU's
 // elimination order exactly.
 //
-// Right-looking blocked LU, panel width kNB = 32, column-major:
+// Right-looking blocked LU, column-major.  Fast path (no row interchanges,
+// every choice of partial pivoting checked), 64-column steps:
+//   k_diag    the 64 x 64 diagonal block factored in registers, and the
+//             inverses of its unit-lower and upper factors
+//   L21 = A21 U11^-1 and U12 = L11^-1 A12 (one GEMM launch, both in place)
+//   A22 -= L21 U12
+// on the matrix cores (gemm.hip).  Fallback (a check failed): tournament
+// pivoting with 32-column panels,
 //   k_tslu    panel factorisation by tournament pivoting (LDS tiles + tree)
 //   k_tslu_l21 L21 = A21 U11^-1 below the panel's pivot block
 //   k_apply_moves  the panel's row permutation applied to the other columns
 //   k_trsm    U12 = L11^-1 A12 (one lane per column in registers, L11 in LDS)
-//   k_gemm    A22 -= L21 U12 on the FP64 matrix cores
-//             (v_mfma_f64_16x16x4f64, 64x64 tile per workgroup)
-// followed by the permutation and blocked unit-lower and upper substitution
-// (k_trsv_lower, k_trsv_upper, one launch per 256-row block column).  The mixed-precision modes factor a copy in FP32 (or with
-// BF16-rounded MFMA inputs) and refine the FP64 solution with FP64
+//   A22 -= L21 U12 (gemm.hip),
+// then the inverses of the factors' 64 x 64 diagonal blocks.  The solve is
+// the permutation and blocked substitution through those inverses
+// (k_fwd, k_bwd: one launch per 64-row block).  The mixed-precision modes
+// factor a copy in FP32 (the trailing updates in FP32, or with BF16
+// operands and FP32 accumulation) and refine the FP64 solution with FP64
 // residuals r = b - A x.
 #include <algorithm>
 #include <climits>
 #include <cmath>
 
+#include "gemm.h"
 #include "internal.h"
-#include "lu_dev.h"
 
 namespace uvlm {
 namespace {
@@ -35,104 +43,204 @@ __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) /
 
 
 // ---- panel factorisation ----------------------------------------------------
-// Fast path (k_panel_np): the panel is factored without row interchanges and
-// every comparison partial pivoting would make is checked on the way: the
-// pivot |a_jj| against each |a_ij| below it, on the updated values.  When no
-// check fails, partial pivoting would have chosen no interchange at all and
-// the factors are exactly GEPP's.  The influence matrix of a vortex lattice
-// takes this path (a ring's self-influence dominates its column).  When a
-// check fails the factorisation is redone from a copy of A with tournament
-// pivoting (k_tslu): the panel rows are cut into tiles of kTR rows, each
-// tile factored with partial pivoting in LDS nominates its kb pivot rows,
-// and the nominations are merged kFan tiles at a time up a tree inside the
-// same launch (the last workgroup to finish a group merges it, so no
+// Fallback panel (k_tslu): the panel rows are cut into tiles of kTR rows,
+// each tile factored with partial pivoting in LDS nominates its kb pivot
+// rows, and the nominations are merged kFan tiles at a time up a tree inside
+// the same launch (the last workgroup to finish a group merges it, so no
 // workgroup waits).  This is the pivoting of communication-avoiding LU
 // (Grigori, Demmel, Xiang 2011).
 constexpr int kTR = 128;  // rows per tile (= workgroup size)
 constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
 
-// Fast-path panel: every workgroup factors the top block itself, then forms
-// L21 = A21 U11^-1 for its 64 rows.  Workgroup 0 writes the factored block
-// to dblk (kNB x kNB, column-major, ld kNB), not to A: a workgroup of the
-// same launch dispatched after workgroup 0 finished would otherwise load the
-// factored block instead of the original.  k_trsm reads L11 from dblk and
-// stores the block in A; a panel of one workgroup (no race) also stores it
-// in A itself, since no k_trsm follows the last panel.  Rows live in
-// registers (one lane per row, the block's columns unrolled); the factored
-// rows of the diagonal block are published through LDS one step at a time.
-// Columns and rows beyond kb are padded with the identity, which the
-// elimination leaves unchanged.  flag <- 1 when a pivoting check fails;
-// info <- first zero pivot.
+// ---- 64-column steps of the fast path --------------------------------------
+// The fast path factors A without row interchanges and checks every
+// comparison partial pivoting would make: the pivot |a_jj| against each
+// |a_ij| below it, on the updated values, i.e. |l_ij| <= 1 for every entry of
+// L.  When no check fails, partial pivoting would have chosen no interchange
+// and the factors are GEPP's (up to rounding in exact ties).  The influence
+// matrix of a vortex lattice takes this path (a ring's self-influence
+// dominates its column).  A failed check sets flag; the caller refactors a
+// saved copy with tournament pivoting.
+//
+// An FP64 operation here has a latency of ~40 cycles and a division ~100,
+// so the kernels keep their serial chains in registers of one wavefront:
+// no barrier inside a chain, loads staged into registers before the
+// arithmetic, pivot rows broadcast with readlane.
+constexpr int kDB = 64;
+
+// v from lane `l` (a constant after unrolling) to every lane
+__device__ __forceinline__ double bcast(double v, int l) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The kb x kb diagonal block at k0 (identity beyond kb) factored in place by
+// one wavefront, lane i holding row i: at step j every lane reads the pivot
+// row from lane j and lanes i > j eliminate.  |l| <= 1 checked (flag), the
+// first zero pivot reported (info).
 template <typename T>
-__global__ __launch_bounds__(64) void k_panel_np(T* A, int lda, int n, int k0, int kb,
-                                                 T* __restrict__ dblk,
-                                                 int* __restrict__ flag,
-                                                 int* __restrict__ info) {
-  __shared__ T D[kNB][kNB + 1];
-  __shared__ int s_flag;
-  const int t = threadIdx.x;
-  if (t == 0) s_flag = 0;
-  const int row = k0 + kb + blockIdx.x * 64 + t;
-  T x[kNB];  // this lane's row below the panel's diagonal block
+__global__ __launch_bounds__(64) void k_lu64(T* A, int lda, int n, int k0, int* flag,
+                                             int* info) {
+  const int i = threadIdx.x;
+  const int kb = min(kDB, n - k0);
+  const int ic = min(i, kb - 1);
+  T x[kDB];
 #pragma unroll
-  for (int c = 0; c < kNB; ++c)
-    x[c] = (row < n && c < kb) ? A[(size_t)(k0 + c) * lda + row] : T(0);
-  T d[kNB];  // row t of the diagonal block (lanes t < kNB)
+  for (int c = 0; c < kDB; ++c) x[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + ic];
 #pragma unroll
-  for (int c = 0; c < kNB; ++c)
-    d[c] = (t < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + t] : (c == t ? T(1) : T(0));
-  int zero = -1;
+  for (int c = 0; c < kDB; ++c)
+    if (!(i < kb && c < kb)) x[c] = c == i ? T(1) : T(0);
   bool bad = false;
+  int zero = -1;
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (t == j) {  // row j is final: publish it
+  for (int j = 0; j < kDB; ++j) {
+    const T piv = bcast(x[j], j);
+    if (piv == T(0) && zero < 0 && j < kb) zero = j;
+    const bool below = i > j;
+    T l = x[j];
+    bad |= below && fabs(l) > fabs(piv);
+    if (piv != T(0)) l = l / piv;
+    if (below) x[j] = l;
+    const T lm = below ? l : T(0);
 #pragma unroll
-      for (int c = 0; c < kNB; ++c) D[j][c] = d[c];
+    for (int c = j + 1; c < kDB; ++c) x[c] = __builtin_fma(-lm, bcast(x[c], j), x[c]);
+  }
+  if (i < kb) {
+#pragma unroll
+    for (int c = 0; c < kDB; ++c)
+      if (c < kb) A[(size_t)(k0 + c) * lda + k0 + i] = x[c];
+  }
+  if (__any(bad) && i == 0) atomicOr(flag, 1);
+  if (i == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
+}
+
+// L21 = A21 U11^-1 (workgroups [0, nrb): 64 rows each, lane = row) and U12 =
+// L11^-1 A12 (workgroups from nrb: 64 columns each, lane = column, the tile
+// staged through LDS so that global loads and stores stay contiguous), by
+// substitution against the factored diagonal block D in LDS; U's diagonal is
+// applied through reciprocals.  |l| <= 1 checked on L21 (flag).
+template <typename T>
+__global__ __launch_bounds__(64) void k_panel64(T* A, int lda, int n, int k0, int nrb,
+                                                int* flag) {
+  __shared__ T D[kDB][kDB + 1];
+  __shared__ T W[kDB][kDB + 1];
+  __shared__ T rcp[kDB];
+  const int t = threadIdx.x;
+  const int kb = min(kDB, n - k0), s0 = k0 + kb;
+  {
+    T v[kDB];
+#pragma unroll
+    for (int c = 0; c < kDB; ++c)
+      v[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(t, kb - 1)];
+#pragma unroll
+    for (int c = 0; c < kDB; ++c) D[t][c] = (t < kb && c < kb) ? v[c] : (t == c ? T(1) : T(0));
+  }
+  __syncthreads();
+  rcp[t] = T(1) / D[t][t];
+  __syncthreads();
+  if ((int)blockIdx.x < nrb) {
+    const int row = s0 + kDB * blockIdx.x + t, rc = min(row, n - 1);
+    T x[kDB];
+#pragma unroll
+    for (int c = 0; c < kDB; ++c) x[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + rc];
+#pragma unroll
+    for (int c = 0; c < kDB; ++c)
+      if (c >= kb) x[c] = T(0);
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kDB; ++k) {
+      const T xk = x[k] * rcp[k];
+      x[k] = xk;
+      bad |= fabs(xk) > T(1);
+#pragma unroll
+      for (int c = k + 1; c < kDB; ++c) x[c] = __builtin_fma(-xk, D[k][c], x[c]);
+    }
+    if (row < n) {
+#pragma unroll
+      for (int c = 0; c < kDB; ++c)
+        if (c < kb) A[(size_t)(k0 + c) * lda + row] = x[c];
+    }
+    if (__any(bad && row < n) && t == 0) atomicOr(flag, 1);
+  } else {
+    const int cb = s0 + kDB * (blockIdx.x - nrb);  // first column of the tile
+    {
+      T v[kDB];
+#pragma unroll
+      for (int c = 0; c < kDB; ++c)
+        v[c] = A[(size_t)min(cb + c, n - 1) * lda + k0 + min(t, kb - 1)];
+#pragma unroll
+      for (int c = 0; c < kDB; ++c) W[c][t] = t < kb ? v[c] : T(0);  // W[column][row]
     }
     __syncthreads();
-    const T piv = D[j][j];
-    if (piv == T(0) && zero < 0 && j < kb) zero = j;
-    if (t > j && t < kNB) {
-      T l = d[j];
-      bad |= fabs(l) > fabs(piv);
-      if (piv != T(0)) l = l / piv;
-      d[j] = l;
+    T y[kDB];
 #pragma unroll
-      for (int c = j + 1; c < kNB; ++c) d[c] = d[c] - l * D[j][c];
+    for (int r = 0; r < kDB; ++r) y[r] = W[t][r];
+#pragma unroll
+    for (int k = 0; k < kDB; ++k) {
+      const T yk = y[k];
+#pragma unroll
+      for (int r = k + 1; r < kDB; ++r) y[r] = __builtin_fma(-D[r][k], yk, y[r]);
     }
-  }
-  if (blockIdx.x == 0) {
-    for (int q = t; q < kb * kb; q += 64) {
-      const int r = q % kb, c = q / kb;
-      dblk[c * kNB + r] = D[r][c];
-      if (gridDim.x == 1) A[(size_t)(k0 + c) * lda + k0 + r] = D[r][c];
-    }
-    if (t == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
-  }
-  const T* Db = &D[0][0];
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    // row offset passed through an empty asm that also takes x[j - 1]: the
-    // reads of row j are issued once step j - 1 starts, not all at the top
-    // (which would hold the whole block in registers)
-    int o = j * (kNB + 1);
-    asm volatile("" : "+s"(o) : "v"(x[j > 0 ? j - 1 : 0]));
-    const T piv = Db[o + j];
-    T xj = x[j];
-    bad |= fabs(xj) > fabs(piv);
-    if (piv != T(0)) xj = xj / piv;
-    x[j] = xj;
+    for (int r = 0; r < kDB; ++r) W[t][r] = y[r];
+    __syncthreads();
 #pragma unroll
-    for (int c = j + 1; c < kNB; ++c) x[c] = x[c] - xj * Db[o + c];
+    for (int c = 0; c < kDB; ++c)
+      if (t < kb && cb + c < n) A[(size_t)(cb + c) * lda + k0 + t] = W[c][t];
   }
-  if (row < n) {
+}
+
+// Inverses of the factored 64 x 64 diagonal blocks (one workgroup per block,
+// identity beyond the matrix), column-major with ld 64 at invL, invU + 4096
+// b: wavefront 0 forms L^-1, wavefront 1 U^-1, one lane per column, by
+// forward and backward elimination on the identity in registers.
+template <typename T>
+__global__ __launch_bounds__(128) void k_diag_inv(const T* A, int lda, int n, T* invL,
+                                                  T* invU) {
+  __shared__ T M[kDB][kDB + 1];
+  __shared__ T rcp[kDB];
+  const int k0 = kDB * blockIdx.x, kb = min(kDB, n - k0);
+  const int t = threadIdx.x, lane = t & 63;
+  invL += (size_t)kDB * kDB * blockIdx.x;
+  invU += (size_t)kDB * kDB * blockIdx.x;
+  if (t < 64) {
+    T v[kDB];
 #pragma unroll
-    for (int c = 0; c < kNB; ++c)
-      if (c < kb) A[(size_t)(k0 + c) * lda + row] = x[c];
+    for (int c = 0; c < kDB; ++c)
+      v[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(t, kb - 1)];
+#pragma unroll
+    for (int c = 0; c < kDB; ++c) M[t][c] = (t < kb && c < kb) ? v[c] : (t == c ? T(1) : T(0));
   }
-  if (bad) s_flag = 1;
   __syncthreads();
-  if (t == 0 && s_flag) atomicOr(flag, 1);
+  if (t < 64) rcp[t] = T(1) / M[t][t];
+  __syncthreads();
+  T y[kDB];
+#pragma unroll
+  for (int r = 0; r < kDB; ++r) y[r] = r == lane ? T(1) : T(0);
+  if (t < 64) {  // L y = e_lane, unit lower
+#pragma unroll
+    for (int k = 0; k < kDB; ++k) {
+      const T yk = y[k];
+#pragma unroll
+      for (int r = k + 1; r < kDB; ++r) y[r] = __builtin_fma(-M[r][k], yk, y[r]);
+    }
+  } else {  // U y = e_lane
+#pragma unroll
+    for (int k = kDB - 1; k >= 0; --k) {
+      const T yk = y[k] * rcp[k];
+      y[k] = yk;
+#pragma unroll
+      for (int r = 0; r < k; ++r) y[r] = __builtin_fma(-M[r][k], yk, y[r]);
+    }
+  }
+  T* out = (t < 64 ? invL : invU) + lane * kDB;
+#pragma unroll
+  for (int r = 0; r < kDB; ++r) out[r] = y[r];
 }
 
 // Partial-pivoting LU of tile rows [0, rows) x columns [0, kb) in LDS; the
@@ -399,21 +507,14 @@ __global__ __launch_bounds__(256) void k_apply_moves(T* A, int lda, int n, int k
 }
 
 // U12 = L11^-1 A12 for the columns [k0+kb, n): one lane per column held in
-// registers, L11 in LDS (zero beyond kb).  dblk (nullable): the factored
-// diagonal block as k_panel_np left it (column-major, ld kNB); L11 is read
-// from there and workgroup 0 stores the block in A.  Without dblk L11 is
-// read from A.
+// registers, L11 in LDS (zero beyond kb).
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb,
-                                             const T* __restrict__ dblk) {
+__global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int kb) {
   __shared__ T L[kNB][kNB + 1];
   const int t = threadIdx.x;
   for (int q = t; q < kNB * kNB; q += 64) {
     const int r = q % kNB, c = q / kNB;
-    const bool in = r < kb && c < kb;
-    const T v = !in ? T(0) : dblk ? dblk[c * kNB + r] : A[(size_t)(k0 + c) * lda + k0 + r];
-    L[r][c] = v;
-    if (dblk && in && blockIdx.x == 0) A[(size_t)(k0 + c) * lda + k0 + r] = v;
+    L[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : T(0);
   }
   const int col = k0 + kb + blockIdx.x * 64 + t;
   T* a = A + (size_t)col * lda + k0;
@@ -438,148 +539,20 @@ __global__ __launch_bounds__(64) void k_trsm(T* A, int lda, int n, int k0, int k
     if (i < kb) a[i] = x[i];
 }
 
-// A22 -= L21 * U12 with the FP64 matrix cores.  The product is formed
-// transposed (D^T = U12^T L21^T) so that each group of 16 lanes holds 16
-// consecutive rows of one column: stores are contiguous in column-major A.
-// v_mfma_f64_16x16x4f64 operand layout: a[lane] = X[lane%16][lane/16],
-// b[lane] = Y[lane/16][lane%16], d[r] = D[4*r+lane/16][lane%16] (the
-// result rows interleave across the four lane groups; the f32 variant
-// below keeps them blocked: d[r] = D[4*(lane/16)+r][lane%16]).
-__global__ __launch_bounds__(256) void k_gemm_f64(double* A, int lda, int n,
-                                                  int k0, int kb) {
-  __shared__ double sA[64][kNB + 1];  // L21 tile: rows x k
-  __shared__ double sB[kNB][64 + 1];  // U12 tile: k x cols
-  const int s0 = k0 + kb;
-  const int m0 = s0 + blockIdx.y * 64;
-  const int c0 = s0 + blockIdx.x * 64;
-  for (int q = threadIdx.x; q < 64 * kNB; q += 256) {
-    const int r = q % 64, k = q / 64;
-    sA[r][k] = (m0 + r < n && k < kb) ? A[(size_t)(k0 + k) * lda + m0 + r] : 0.0;
-  }
-  for (int q = threadIdx.x; q < kNB * 64; q += 256) {
-    const int k = q % kNB, c = q / kNB;
-    sB[k][c] = (c0 + c < n && k < kb) ? A[(size_t)(c0 + c) * lda + k0 + k] : 0.0;
-  }
-  __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1;
-  const int l16 = lane & 15, lq = lane >> 4;
-  v4d acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int k = 0; k < kNB; k += 4) {
-    double ua[2], lb[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) ua[j] = sB[k + lq][wc * 32 + j * 16 + l16];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) lb[i] = sA[wr * 32 + i * 16 + l16][k + lq];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[j], lb[i], acc[i][j],
-                                                          0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr * 32 + i * 16 + l16;
-        const int col = c0 + wc * 32 + j * 16 + 4 * r + lq;
-        if (row < n && col < n) {
-          double* p = A + (size_t)col * lda + row;
-          *p = *p - acc[i][j][r];
-        }
-      }
-}
-
-// FP32 trailing update with the FP32 matrix cores (v_mfma_f32_16x16x4f32,
-// same operand layout as the f64 variant for one 16x16 block), or with
-// BF16-rounded operands when BF16 is set (inputs rounded to bfloat16,
-// products accumulated in FP32: the precision of v_mfma_f32_*_bf16).
-__device__ __forceinline__ float round_bf16(float x) {
-  unsigned u = __float_as_uint(x);
-  const unsigned lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  u &= 0xffff0000u;
-  return __uint_as_float(u);
-}
-
-template <bool BF16>
-__global__ __launch_bounds__(256) void k_gemm_f32(float* A, int lda, int n,
-                                                  int k0, int kb) {
-  __shared__ float sA[64][kNB + 1];
-  __shared__ float sB[kNB][64 + 1];
-  const int s0 = k0 + kb;
-  const int m0 = s0 + blockIdx.y * 64;
-  const int c0 = s0 + blockIdx.x * 64;
-  for (int q = threadIdx.x; q < 64 * kNB; q += 256) {
-    const int r = q % 64, k = q / 64;
-    float v = (m0 + r < n && k < kb) ? A[(size_t)(k0 + k) * lda + m0 + r] : 0.f;
-    sA[r][k] = BF16 ? round_bf16(v) : v;
-  }
-  for (int q = threadIdx.x; q < kNB * 64; q += 256) {
-    const int k = q % kNB, c = q / kNB;
-    float v = (c0 + c < n && k < kb) ? A[(size_t)(c0 + c) * lda + k0 + k] : 0.f;
-    sB[k][c] = BF16 ? round_bf16(v) : v;
-  }
-  __syncthreads();
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1;
-  const int l16 = lane & 15, lq = lane >> 4;
-  v4f acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int k = 0; k < kNB; k += 4) {
-    float ua[2], lb[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) ua[j] = sB[k + lq][wc * 32 + j * 16 + l16];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) lb[i] = sA[wr * 32 + i * 16 + l16][k + lq];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ua[j], lb[i], acc[i][j],
-                                                          0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr * 32 + i * 16 + l16;
-        const int col = c0 + wc * 32 + j * 16 + 4 * lq + r;
-        if (row < n && col < n) {
-          float* p = A + (size_t)col * lda + row;
-          *p = *p - acc[i][j][r];
-        }
-      }
-}
-
-// x = U^-1 L^-1 (P b), blocked over launches: launch j handles block column
-// j of kTB rows.  Each workgroup solves the diagonal block itself (64-row
-// sub-blocks solved by wavefront 0 with lane shuffles, no barrier inside a
-// sub-block, then applied to the block's later rows), workgroup 0 stores
-// the block's solution, and every workgroup applies it to its own kTB rows
-// of the rest.  A launch reads the block's right-hand side from the work
-// vector w and stores the solution in another vector y, so no workgroup can
-// read a solution in place of its right-hand side (forward: w = tmp, y =
-// out; backward: w = out, y = tmp, then copied to out).  The per-row operation sequence is that of one sweep over
-// 64-row blocks (forward from row 0, backward from row n), so the result
-// does not depend on kTB.  The forward grid of 64-row blocks starts at row
-// 0 and the backward one ends at row n; kTB-row blocks follow the same
-// alignment.
-constexpr int kTB = 256;
+// x = U^-1 L^-1 (P b) through the inverses of the 64 x 64 diagonal blocks,
+// one launch per group of 256 rows.  Forward launch for the group [j0, j0 +
+// 256): every workgroup solves the group itself in LDS (x_s = L_ss^-1 w_s
+// for its four blocks in turn, each applied to the group's later rows),
+// workgroup 0 stores the group's solution in y, and each workgroup subtracts
+// L_(rows, group) x_group from w at its 64 rows below the group (four
+// partial sums per row over 64 columns each, combined in a fixed order).
+// Backward launches go up from the last group with U.  A launch updates
+// only rows outside the group it reads and writes the group's solution to
+// another vector, so no workgroup reads what another writes in the same
+// launch (forward: w = tmp, y = out; backward: w = out, y = tmp, then copied
+// to out).  Loads use clamped in-bounds addresses and zero weights instead
+// of branches, so they are all in flight together.
+constexpr int kGB = 256;
 
 template <typename T>
 __global__ void k_permute(const T* __restrict__ b, const int* __restrict__ perm, int n,
@@ -588,199 +561,224 @@ __global__ void k_permute(const T* __restrict__ b, const int* __restrict__ perm,
   if (i < n) x[i] = b[perm[i]];
 }
 
-// Forward (unit lower) step for rows [j0, j0 + jb).
-template <typename T>
-__global__ __launch_bounds__(kTB) void k_trsv_lower(const T* __restrict__ LU, int n,
-                                                    T* __restrict__ x, T* __restrict__ y,
-                                                    int j0, int jb) {
-  __shared__ T D[64][65];
-  __shared__ T xs[kTB];
-  const int tid = threadIdx.x, lane = tid & 63;
-  if (tid < jb) xs[tid] = x[j0 + tid];
-  for (int s0 = 0; s0 < jb; s0 += 64) {
-    const int sb = min(64, jb - s0);
-    __syncthreads();
-    for (int q = tid; q < 64 * 64; q += kTB) {
-      const int r = q % 64, c = q / 64;
-      D[r][c] = (r < sb && c < sb) ? LU[(size_t)(j0 + s0 + c) * n + j0 + s0 + r] : T(0);
-    }
-    __syncthreads();
-    if (tid < 64) {
-      T v = lane < sb ? xs[s0 + lane] : T(0);
-      for (int i = 0; i < sb; ++i) {
-        const T xi = __shfl(v, i);
-        if (lane > i && lane < sb) v = v - D[lane][i] * xi;
-      }
-      if (lane < sb) xs[s0 + lane] = v;
-    }
-    __syncthreads();
-    const int r = s0 + sb + tid;  // later rows of the diagonal block
-    if (r < jb) {
-      T acc = xs[r];
-      const T* row = LU + (size_t)(j0 + s0) * n + j0 + r;
-      for (int i = 0; i < sb; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
-      xs[r] = acc;
-    }
-  }
-  __syncthreads();
-  if (blockIdx.x == 0 && tid < jb) y[j0 + tid] = xs[tid];
-  const int r = j0 + jb + blockIdx.x * kTB + tid;
-  if (r < n) {
-    T acc = x[r];
-    const T* row = LU + (size_t)j0 * n + r;
-#pragma unroll 16
-    for (int i = 0; i < jb; ++i) acc = acc - row[(size_t)i * n] * xs[i];
-    x[r] = acc;
-  }
-}
-
-// Backward (upper) step for rows [j0, j0 + jb); rows above are updated.
-template <typename T>
-__global__ __launch_bounds__(kTB) void k_trsv_upper(const T* __restrict__ LU, int n,
-                                                    T* __restrict__ x, T* __restrict__ y,
-                                                    int j0, int jb) {
-  __shared__ T D[64][65];
-  __shared__ T xs[kTB];
-  const int tid = threadIdx.x, lane = tid & 63;
-  if (tid < jb) xs[tid] = x[j0 + tid];
-  for (int se = jb; se > 0; se -= 64) {
-    const int s0 = max(0, se - 64), sb = se - s0;
-    __syncthreads();
-    for (int q = tid; q < 64 * 64; q += kTB) {
-      const int r = q % 64, c = q / 64;
-      D[r][c] = (r < sb && c < sb) ? LU[(size_t)(j0 + s0 + c) * n + j0 + s0 + r] : T(0);
-    }
-    __syncthreads();
-    if (tid < 64) {
-      T v = lane < sb ? xs[s0 + lane] : T(0);
-      const T dl = lane < sb ? D[lane][lane] : T(1);
-      for (int i = sb - 1; i >= 0; --i) {
-        if (lane == i) v = v / dl;
-        const T xi = __shfl(v, i);
-        if (lane < i) v = v - D[lane][i] * xi;
-      }
-      if (lane < sb) xs[s0 + lane] = v;
-    }
-    __syncthreads();
-    if (tid < s0) {  // earlier rows of the diagonal block
-      T acc = xs[tid];
-      const T* row = LU + (size_t)(j0 + s0) * n + j0 + tid;
-      for (int i = 0; i < sb; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
-      xs[tid] = acc;
-    }
-  }
-  __syncthreads();
-  if (blockIdx.x == 0 && tid < jb) y[j0 + tid] = xs[tid];
-  const int r = blockIdx.x * kTB + tid;
-  if (r < j0) {
-    T acc = x[r];
-    for (int se = jb; se > 0; se -= 64) {  // 64-row sub-blocks from the end
-      const int s0 = max(0, se - 64);
-      const T* row = LU + (size_t)(j0 + s0) * n + r;
-#pragma unroll 16
-      for (int i = 0; i < se - s0; ++i) acc = acc - row[(size_t)i * n] * xs[s0 + i];
-    }
-    x[r] = acc;
-  }
-}
-
 template <typename T>
 __global__ void k_copy_t(const T* __restrict__ a, int n, T* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
 }
 
-// tmp: n values of scratch (distinct from b and out).
+// LU column `col` (clamped to n - 1) at row `row` (clamped)
 template <typename T>
-void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* b, T* out,
-                  T* tmp) {
+__device__ __forceinline__ T lu_at(const T* __restrict__ LU, int n, int row, int col) {
+  return LU[(size_t)min(col, n - 1) * n + min(row, n - 1)];
+}
+
+// sum_{c < N} LU(row, col0 + c) wl[c]: loads staged 32 at a time, four
+// partial sums combined in a fixed order.
+template <typename T, int N>
+__device__ __forceinline__ T dot_cols(const T* __restrict__ LU, int n, int row, int col0,
+                                      const T* wl) {
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int h = 0; h < N; h += 32) {
+    T v[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) v[c] = lu_at(LU, n, row, col0 + h + c);
+#pragma unroll
+    for (int c = 0; c < 32; ++c) acc[c & 3] = __builtin_fma(v[c], wl[h + c], acc[c & 3]);
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+// One block of the group: wl[s0 .. s0 + 64) <- Dinv wl[s0 .. s0 + 64) (the
+// block's inverse, column-major, ld 64, zero-padded like wl).
+template <typename T>
+__device__ __forceinline__ void group_block(const T* __restrict__ Dinv, T* wl, int s0,
+                                            T (*part)[64]) {
+  const int t = threadIdx.x, r = t & 63, q = t >> 6;
+  T v[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) v[c] = Dinv[(16 * q + c) * kDB + r];
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc[c & 3] = __builtin_fma(v[c], wl[s0 + 16 * q + c], acc[c & 3]);
+  part[q][r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  __syncthreads();
+  if (t < 64) wl[s0 + t] = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+  __syncthreads();
+}
+
+// w[row0 + r] -= sum over the group's gb columns of LU(row0 + r, j0 + c) wl[c]
+// for r < 64 and row0 + r < lim.
+template <typename T>
+__device__ __forceinline__ void group_rows(const T* __restrict__ LU, int n, int j0, int gb,
+                                           int row0, int lim, const T* wl, T* w,
+                                           T (*part)[64]) {
+  const int t = threadIdx.x, r = t & 63, q = t >> 6;
+  T s = T(0);
+  if (64 * q < gb) s = dot_cols<T, 64>(LU, n, row0 + r, j0 + 64 * q, wl + 64 * q);
+  part[q][r] = s;
+  __syncthreads();
+  if (t < 64 && row0 + t < lim)
+    w[row0 + t] = w[row0 + t] - (((part[0][t] + part[1][t]) + part[2][t]) + part[3][t]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd(const T* __restrict__ LU, int n,
+                                             const T* __restrict__ invL, T* w, T* y, int j0) {
+  __shared__ T wl[kGB];
+  __shared__ T part[4][64];
+  const int t = threadIdx.x;
+  const int gb = min(kGB, n - j0);
+  wl[t] = t < gb ? w[j0 + t] : T(0);
+  __syncthreads();
+  for (int s0 = 0; s0 < gb; s0 += kDB) {
+    group_block(invL + (size_t)(j0 + s0) * kDB, wl, s0, part);
+    // the group's later rows
+    const int row = s0 + kDB + t;  // row within the group
+    T acc = T(0);
+    if (s0 + kDB < gb) acc = dot_cols<T, kDB>(LU, n, j0 + row, j0 + s0, wl + s0);
+    __syncthreads();
+    if (row < gb) wl[row] = wl[row] - acc;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && t < gb) y[j0 + t] = wl[t];
+  group_rows(LU, n, j0, gb, j0 + gb + 64 * blockIdx.x, n, wl, w, part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd(const T* __restrict__ LU, int n,
+                                             const T* __restrict__ invU, T* w, T* y, int j0) {
+  __shared__ T wl[kGB];
+  __shared__ T part[4][64];
+  const int t = threadIdx.x;
+  const int gb = min(kGB, n - j0);
+  wl[t] = t < gb ? w[j0 + t] : T(0);
+  __syncthreads();
+  for (int s0 = (gb - 1) / kDB * kDB; s0 >= 0; s0 -= kDB) {
+    group_block(invU + (size_t)(j0 + s0) * kDB, wl, s0, part);
+    // the group's earlier rows
+    T acc = T(0);
+    if (s0 > 0) acc = dot_cols<T, kDB>(LU, n, j0 + t, j0 + s0, wl + s0);
+    __syncthreads();
+    if (t < s0) wl[t] = wl[t] - acc;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && t < gb) y[j0 + t] = wl[t];
+  group_rows(LU, n, j0, gb, 64 * blockIdx.x, j0, wl, w, part);
+}
+
+// inv: the diagonal blocks' inverses (L^-1 blocks, then U^-1 blocks, each
+// cdiv(n, 64) blocks of 64 x 64).  tmp: n values of scratch (distinct from b
+// and out).
+template <typename T>
+void launch_getrs(hipStream_t s, const T* LU, int n, const int* perm, const T* inv,
+                  const T* b, T* out, T* tmp) {
   if (n <= 0) return;
+  const T* invL = inv;
+  const T* invU = inv + (size_t)cdiv(n, kDB) * kDB * kDB;
   hipLaunchKernelGGL((k_permute<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, b, perm, n, tmp);
-  for (int j0 = 0; j0 < n; j0 += kTB) {
-    const int jb = std::min(kTB, n - j0);
-    const int rest = n - j0 - jb;
-    hipLaunchKernelGGL((k_trsv_lower<T>), dim3(std::max(1, cdiv(rest, kTB))), dim3(kTB), 0, s,
-                       LU, n, tmp, out, j0, jb);
+  for (int j0 = 0; j0 < n; j0 += kGB) {
+    const int rest = n - j0 - std::min(kGB, n - j0);
+    hipLaunchKernelGGL((k_fwd<T>), dim3(std::max(1, cdiv(rest, 64))), dim3(256), 0, s, LU, n,
+                       invL, tmp, out, j0);
   }
-  for (int je = n; je > 0; je -= kTB) {
-    const int j0 = std::max(0, je - kTB), jb = je - j0;
-    hipLaunchKernelGGL((k_trsv_upper<T>), dim3(std::max(1, cdiv(j0, kTB))), dim3(kTB), 0, s,
-                       LU, n, out, tmp, j0, jb);
-  }
+  for (int j0 = (n - 1) / kGB * kGB; j0 >= 0; j0 -= kGB)
+    hipLaunchKernelGGL((k_bwd<T>), dim3(std::max(1, cdiv(j0, 64))), dim3(256), 0, s, LU, n,
+                       invU, out, tmp, j0);
   hipLaunchKernelGGL((k_copy_t<T>), dim3(cdiv(n, 256)), dim3(256), 0, s, (const T*)tmp, n, out);
 }
 
-// Workspace of the factorisation (ints): move list of the current panel,
-// tree nominations and arrival counters of the panel kernel.
+// Workspace of the fallback factorisation (ints): move list of the current
+// panel, tree nominations and arrival counters of the panel kernel.
 size_t lu_work_ints(int n) {
   const int tiles = cdiv(n, kTR);
-  // + the panel's diagonal block (kNB x kNB doubles, 256-byte aligned)
-  return 4 * kNB + 64 + (size_t)2 * tiles * kNB + 2 * (size_t)tiles + 64 +
-         2 * kNB * kNB + 64;
+  return 4 * kNB + 64 + (size_t)2 * tiles * kNB + 2 * (size_t)tiles + 64;
 }
 
-// Blocked right-looking LU.  pivoted = false: checked no-pivot panels
-// (info[1] <- 1 when a check fails: the factors are then not GEPP's and the
-// caller refactors with pivoted = true).  perm = the row permutation.
-template <typename T, int GEMM>
-void lu_factor_t(hipStream_t s, T* A, int n, int* perm, int* info, int* work, bool pivoted) {
+// Elements of the diagonal blocks' inverses: the L^-1 blocks, then the U^-1
+// blocks, 64 x 64 each.
+size_t lu_inv_elems(int n) { return 2 * (size_t)cdiv(n, kDB) * kDB * kDB; }
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// A22 -= L21 U12 after the panel [k0, k0 + kb).
+template <typename T>
+void lu_trailing(hipStream_t s, int prec, T* A, int n, int k0, int kb) {
+  const int rest = n - k0 - kb;
+  const GemmArgs u{rest, rest, kb, A + (size_t)k0 * n + k0 + kb, n,
+                   A + (size_t)(k0 + kb) * n + k0, n, A + (size_t)(k0 + kb) * n + k0 + kb, n,
+                   -1.0, 1.0, nullptr};
+  launch_gemm(s, prec, &u, 1);
+}
+
+// Blocked right-looking LU.  prec: GEMM_F64 (T = double), or GEMM_F32 /
+// GEMM_BF16 (T = float; the trailing updates with FP32 or BF16 operands,
+// the panels in FP32).  pivoted = false: the checked
+// no-pivot path (info[1] <- 1 when a check fails: the factors are then not
+// GEPP's and the caller refactors with pivoted = true).  perm = the row
+// permutation; inv <- the diagonal blocks' inverses (lu_inv_elems).
+template <typename T>
+void lu_factor_t(hipStream_t s, int prec, T* A, int n, int* perm, int* info, int* work, T* inv,
+                 bool pivoted) {
   const int tiles = cdiv(n, kTR);
   int* mv = work;
   int* cand = work + 4 * kNB + 64;
   int* ctr = cand + (size_t)2 * tiles * kNB;
   const int nctr = 2 * tiles;
-  T* dblk = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(ctr + nctr + 64) + 255) &
-                                 ~(uintptr_t)255);
+  T* invL = inv;
+  T* invU = inv + (size_t)cdiv(n, kDB) * kDB * kDB;
   hipLaunchKernelGGL(k_lu_init, dim3(cdiv(std::max(n, nctr), 256)), dim3(256), 0, s, info,
                      ctr, nctr, perm, n);
-  for (int k0 = 0; k0 < n; k0 += kNB) {
-    const int kb = std::min(kNB, n - k0);
-    const int rest = n - k0 - kb;
-    if (pivoted) {
-      hipLaunchKernelGGL((k_tslu<T>), dim3(cdiv(n - k0, kTR)), dim3(kTR), 0, s, A, n, n, k0,
-                         kb, cand, ctr, mv, info);
-      if (rest > 0)
-        hipLaunchKernelGGL((k_tslu_l21<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A, n, n,
-                           k0, kb);
-      hipLaunchKernelGGL((k_apply_moves<T>), dim3(cdiv(n - kb + 1, 4)), dim3(256), 0, s, A,
-                         n, n, k0, kb, (const int*)mv, perm);
-    } else {
-      hipLaunchKernelGGL((k_panel_np<T>), dim3(std::max(1, cdiv(rest, 64))), dim3(64), 0, s,
-                         A, n, n, k0, kb, dblk, info + 1, info);
+  if (!pivoted) {
+    for (int k0 = 0; k0 < n; k0 += kDB) {
+      const int kb = std::min(kDB, n - k0), rest = n - k0 - kb;
+      hipLaunchKernelGGL((k_lu64<T>), dim3(1), dim3(64), 0, s, A, n, n, k0, info + 1, info);
+      if (rest > 0) {
+        const int nb = cdiv(rest, kDB);
+        hipLaunchKernelGGL((k_panel64<T>), dim3(2 * nb), dim3(64), 0, s, A, n, n, k0, nb,
+                           info + 1);
+        lu_trailing(s, prec, A, n, k0, kb);
+      }
     }
-    if (rest > 0) {
-      hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A,
-                         n, n, k0, kb, pivoted ? (const T*)nullptr : (const T*)dblk);
-      dim3 g(cdiv(rest, 64), cdiv(rest, 64));
-      if constexpr (GEMM == 0) {
-        hipLaunchKernelGGL(k_gemm_f64, g, dim3(256), 0, s, (double*)A, n, n, k0, kb);
-      } else if constexpr (GEMM == 1) {
-        hipLaunchKernelGGL((k_gemm_f32<false>), g, dim3(256), 0, s, (float*)A, n,
-                           n, k0, kb);
-      } else {
-        hipLaunchKernelGGL((k_gemm_f32<true>), g, dim3(256), 0, s, (float*)A, n,
-                           n, k0, kb);
+  } else {
+    for (int k0 = 0; k0 < n; k0 += kNB) {
+      const int kb = std::min(kNB, n - k0);
+      const int rest = n - k0 - kb;
+      hipLaunchKernelGGL((k_tslu<T>), dim3(cdiv(n - k0, kTR)), dim3(kTR), 0, s, A, n, n, k0, kb,
+                         cand, ctr, mv, info);
+      if (rest > 0)
+        hipLaunchKernelGGL((k_tslu_l21<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A, n, n, k0,
+                           kb);
+      hipLaunchKernelGGL((k_apply_moves<T>), dim3(cdiv(n - kb + 1, 4)), dim3(256), 0, s, A, n, n,
+                         k0, kb, (const int*)mv, perm);
+      if (rest > 0) {
+        hipLaunchKernelGGL((k_trsm<T>), dim3(cdiv(rest, 64)), dim3(64), 0, s, A, n, n, k0, kb);
+        lu_trailing(s, prec, A, n, k0, kb);
       }
     }
   }
+  hipLaunchKernelGGL((k_diag_inv<T>), dim3(cdiv(n, kDB)), dim3(128), 0, s, (const T*)A, n, n,
+                     invL, invU);
 }
 
 // Factor with the checked no-pivot path; if a check fails, restore A from
 // `saved` and factor with tournament pivoting.  Synchronises the stream once
 // (to read the check).  Returns 1 when pivoting was needed, -1 on a HIP
 // error.
-template <typename T, int GEMM>
-int lu_factor_checked(hipStream_t s, T* A, int n, int* perm, int* info, int* work, T* saved) {
+template <typename T>
+int lu_factor_checked(hipStream_t s, int prec, T* A, int n, int* perm, int* info, int* work,
+                      T* inv, T* saved) {
   const size_t bytes = sizeof(T) * (size_t)n * n;
   if (hipMemcpyAsync(saved, A, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
-  lu_factor_t<T, GEMM>(s, A, n, perm, info, work, false);
+  lu_factor_t<T>(s, prec, A, n, perm, info, work, inv, false);
   int flag = 0;
   if (hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return -1;
   if (!flag) return 0;
   if (hipMemcpyAsync(A, saved, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
-  lu_factor_t<T, GEMM>(s, A, n, perm, info, work, true);
+  lu_factor_t<T>(s, prec, A, n, perm, info, work, inv, true);
   return 1;
 }
 
@@ -1059,35 +1057,42 @@ __global__ __launch_bounds__(1024) void k_fp_solve(const double* __restrict__ m,
 }  // namespace
 
 // LU workspace: ints [info, flag | pad 64][perm n][moves, tree], then an
-// n x n copy of A for the pivoted retry.
+// n x n copy of A for the pivoted retry, then the diagonal blocks' inverses.
 static size_t lu_ints(int n) { return 64 + (size_t)n + lu_work_ints(n); }
 size_t dense_lu_work_bytes(int n) {
-  return (sizeof(int) * lu_ints(n) + 255) / 256 * 256 + sizeof(double) * (size_t)n * n;
+  return align256(sizeof(int) * lu_ints(n)) + align256(sizeof(double) * (size_t)n * n) +
+         sizeof(double) * lu_inv_elems(n);
 }
 int* dense_lu_info(void* work) { return (int*)work; }
 
+static double* lu_saved(void* work, int n) {
+  return (double*)((char*)work + align256(sizeof(int) * lu_ints(n)));
+}
+static double* lu_inv(void* work, int n) {
+  return (double*)((char*)lu_saved(work, n) + align256(sizeof(double) * (size_t)n * n));
+}
+
 int dense_lu_factor(hipStream_t s, double* A, int n, void* work) {
   int* w = (int*)work;
-  double* saved = (double*)((char*)work + (sizeof(int) * lu_ints(n) + 255) / 256 * 256);
-  return lu_factor_checked<double, 0>(s, A, n, w + 64, w, w + 64 + (size_t)n, saved);
+  return lu_factor_checked<double>(s, GEMM_F64, A, n, w + 64, w, w + 64 + (size_t)n,
+                                   lu_inv(work, n), lu_saved(work, n));
 }
 
 void dense_lu_solve(hipStream_t s, const double* LU, int n, void* work, const double* b,
                     double* x) {
   const int* w = (const int*)work;
   // the copy of A kept for a pivoted retry is free once A is factored
-  double* tmp = (double*)((char*)work + (sizeof(int) * lu_ints(n) + 255) / 256 * 256);
-  launch_getrs<double>(s, LU, n, w + 64, b, x, tmp);
+  launch_getrs<double>(s, LU, n, w + 64, lu_inv(work, n), b, x, lu_saved(work, n));
 }
 
 size_t dense_mixed_work_bytes(int n) {
   const size_t nn = (size_t)n * n;
-  return 2 * nn * sizeof(float) + 512 + sizeof(int) * lu_ints(n) + (size_t)n * sizeof(double) +
-         64 * sizeof(double) + 2 * (size_t)n * sizeof(float) + 256 +
-         64 * (size_t)n * sizeof(double) + 256;
+  return 2 * align256(nn * sizeof(float)) + align256(sizeof(int) * lu_ints(n)) +
+         (size_t)n * sizeof(double) + 64 * sizeof(double) + 2 * (size_t)n * sizeof(float) +
+         256 + align256(64 * (size_t)n * sizeof(double)) + sizeof(float) * lu_inv_elems(n);
 }
 
-// Mixed precision: LU of fl32(A) (FP32 or BF16-input MFMA updates), then
+// Mixed precision: LU of fl32(A) (FP32 or BF16-operand MFMA updates), then
 // x_{k+1} = x_k + LU^-1 (b - A x_k) with the residual in FP64.
 int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
                       double* x, int mode, double tol, int max_iter, void* work,
@@ -1096,11 +1101,11 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   char* w = (char*)work;
   const size_t nn = (size_t)n * n;
   float* Af = (float*)w;
-  w += (nn * sizeof(float) + 255) / 256 * 256;
+  w += align256(nn * sizeof(float));
   float* Asaved = (float*)w;
-  w += (nn * sizeof(float) + 255) / 256 * 256;
+  w += align256(nn * sizeof(float));
   int* lw = (int*)w;
-  w += (sizeof(int) * lu_ints(n) + 255) / 256 * 256;
+  w += align256(sizeof(int) * lu_ints(n));
   double* r = (double*)w;
   w += (size_t)n * sizeof(double);
   double* norms = (double*)w;
@@ -1111,13 +1116,15 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
   w += (size_t)n * sizeof(float);
   w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
   double* part = (double*)w;  // residual slices [S][n], 256-byte aligned
+  w += align256(64 * (size_t)n * sizeof(double));
+  float* inv = (float*)w;
   int S = 1, cols = n;
   resid_split(n, &S, &cols);
   int* perm = lw + 64;
   int* tree = lw + 64 + (size_t)n;
   hipLaunchKernelGGL(k_to_f32, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, Af, nn);
-  const int rc = mode == 2 ? lu_factor_checked<float, 2>(s, Af, n, perm, lw, tree, Asaved)
-                           : lu_factor_checked<float, 1>(s, Af, n, perm, lw, tree, Asaved);
+  const int rc = lu_factor_checked<float>(s, mode == 2 ? GEMM_BF16 : GEMM_F32, Af, n, perm, lw,
+                                          tree, inv, Asaved);
   if (rc < 0) return -2;
   (void)hipMemsetAsync(x, 0, sizeof(double) * n, s);
   int it = 0;
@@ -1127,7 +1134,7 @@ int dense_mixed_solve(hipStream_t s, const double* A, int n, const double* b,
                        (const double*)x, cols, part);
     hipLaunchKernelGGL(k_resid_sum, dim3(cdiv(n, 256)), dim3(256), 0, s, (const double*)part,
                        S, n, b, r, rf);
-    launch_getrs<float>(s, Af, n, perm, rf, dxf, Asaved);  // Asaved is free after the LU
+    launch_getrs<float>(s, Af, n, perm, inv, rf, dxf, Asaved);  // Asaved is free after the LU
     hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, s, x, (const float*)dxf,
                        n, norms);
     if (hipMemcpyAsync(h_norms, norms, sizeof(h_norms), hipMemcpyDeviceToHost, s) !=

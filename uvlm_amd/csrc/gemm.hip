@@ -1,0 +1,307 @@
+// gemm.hip -- matrix-core GEMM of the dense solves on gfx950.
+//
+// C = alpha A B + beta C, column-major.  A workgroup of 256 lanes (2 x 2
+// wavefronts) computes a BM x BN tile of C; each wavefront a (BM/2) x (BN/2)
+// block of 16 x 16 MFMA tiles held in accumulator registers.  Operand tiles
+// of BK columns of A and BK rows of B are staged through LDS, double
+// buffered: the global loads of step t + 1 are issued before the matrix
+// instructions of step t, so one barrier per step suffices.
+//
+// The product is formed transposed (D = B^T A^T) so that the 16 lanes of a
+// lane group hold 16 consecutive rows of one column of C: the epilogue's
+// loads and stores of C are contiguous in column-major memory.
+//
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32: a[lane] = X[lane%16]
+// [lane/16], b[lane] = Y[lane/16][lane%16]; the result element r of a lane is
+// D[4r + lane/16][lane%16] (f64) or D[4(lane/16) + r][lane%16] (f32).
+// v_mfma_f32_16x16x32_bf16: lane l holds X[l%16][8(l/16) .. 8(l/16)+7] and
+// Y[8(l/16) .. 8(l/16)+7][l%16]; result as the f32 form.
+#include "gemm.h"
+
+#include <algorithm>
+#include <cstdint>
+
+namespace uvlm {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+inline int gdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  // column of C (within the 16-column block) of result element r
+  static __device__ __forceinline__ int col(int r, int lq) { return 4 * r + lq; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float x, float y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int r, int lq) { return 4 * lq + r; }
+};
+
+struct GemmPair {
+  GemmArgs p0, p1;
+  int tiles0;  // tiles of p0; blocks beyond belong to p1
+};
+
+// Tile of this workgroup: problem fields and the tile's first row / column.
+template <int BM, int BN>
+__device__ __forceinline__ GemmArgs pick(const GemmPair& pp, int* r0, int* c0) {
+  int t = blockIdx.x;
+  const bool second = t >= pp.tiles0;
+  if (second) t -= pp.tiles0;
+  const GemmArgs p = second ? pp.p1 : pp.p0;
+  const int tm = (p.m + BM - 1) / BM;
+  *r0 = (t % tm) * BM;
+  *c0 = (t / tm) * BN;
+  return p;
+}
+
+// Epilogue of one 16-row block of a wavefront: the JN x 4 elements of C
+// this lane updates are read together (clamped in-bounds addresses, so the
+// reads do not wait on one another), then alpha acc + beta C is stored (C
+// not read when beta == 0).  Returns true when a result has |r| > 1.
+template <typename T, int JN, typename Acc, typename ColOf>
+__device__ __forceinline__ bool epilogue_rows(const GemmArgs& p, T* C, int row, int cbase,
+                                              const Acc* acc, ColOf colof) {
+  const T alpha = (T)p.alpha, beta = (T)p.beta;
+  const int rc = min(row, p.m - 1);
+  T cv[JN][4];
+  if (beta != T(0)) {
+#pragma unroll
+    for (int j = 0; j < JN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = min(cbase + 16 * j + colof(r), p.n - 1);
+        cv[j][r] = C[(size_t)col * p.ldc + rc];
+      }
+  }
+  bool big = false;
+#pragma unroll
+  for (int j = 0; j < JN; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = cbase + 16 * j + colof(r);
+      if (row < p.m && col < p.n) {
+        T v = alpha * acc[j][r];
+        if (beta != T(0)) v = v + beta * cv[j][r];
+        C[(size_t)col * p.ldc + row] = v;
+        big |= fabs(v) > T(1);
+      }
+    }
+  return big;
+}
+
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm(GemmPair pp) {
+  constexpr int BK = 16;
+  constexpr int PAD = 16 / (int)sizeof(T);  // 16 bytes per LDS row
+  constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
+  constexpr int LA = BM * BK / 256, LB = BK * BN / 256;
+  __shared__ T sA[2][BK][BM + PAD];
+  __shared__ T sB[2][BK][BN + PAD];
+  int r0, c0;
+  const GemmArgs p = pick<BM, BN>(pp, &r0, &c0);
+  const T* __restrict__ A = (const T*)p.A;
+  const T* __restrict__ B = (const T*)p.B;
+  T* C = (T*)p.C;
+  const int t = threadIdx.x;
+  // global -> register staging: A element e at row ar, column ak + e * (256 / BM);
+  // B element e at row bk, column bc + e * (256 / BK)
+  const int ar = t % BM, ak = t / BM;
+  const int bk = t % BK, bc = t / BK;
+  T ra[LA], rb[LB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {  // rows beyond m only reach unstored results
+      const int k = k0 + ak + e * (256 / BM);
+      const auto v = A[(size_t)min(k, p.k - 1) * p.lda + min(r0 + ar, p.m - 1)];
+      ra[e] = k < p.k ? v : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      const int k = k0 + bk;
+      const auto v = B[(size_t)min(c0 + bc + e * (256 / BK), p.n - 1) * p.ldb + min(k, p.k - 1)];
+      rb[e] = k < p.k ? v : T(0);
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) sA[buf][ak + e * (256 / BM)][ar] = ra[e];
+#pragma unroll
+    for (int e = 0; e < LB; ++e) sB[buf][bk][bc + e * (256 / BK)] = rb[e];
+  };
+  const int w = t >> 6, lane = t & 63, l16 = lane & 15, lq = lane >> 4;
+  const int wr = w >> 1, wc = w & 1;
+  typename Mfma<T>::acc_t acc[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) acc[i][j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+  const int nk = (p.k + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    stage(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nk; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nk) load((s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T fa[IM], fb[JN];
+#pragma unroll
+      for (int i = 0; i < IM; ++i) fa[i] = sA[cur][kk + lq][wr * WM + 16 * i + l16];
+#pragma unroll
+      for (int j = 0; j < JN; ++j) fb[j] = sB[cur][kk + lq][wc * WN + 16 * j + l16];
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j) acc[i][j] = Mfma<T>::mma(fb[j], fa[i], acc[i][j]);
+    }
+    if (s + 1 < nk) stage(cur ^ 1);
+    __syncthreads();
+  }
+  bool big = false;
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+    big |= epilogue_rows<T, JN>(p, C, r0 + wr * WM + 16 * i + l16, c0 + wc * WN, acc[i],
+                                [&](int r) { return Mfma<T>::col(r, lq); });
+  if (big && p.bad) atomicOr(p.bad, 1);
+}
+
+// FP32 data, BF16 operands: BK = 32 (one v_mfma_f32_16x16x32_bf16 deep).
+// LDS rows hold the k index contiguously (8 BF16 = 16 bytes per lane read).
+__device__ __forceinline__ unsigned short to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmPair pp) {
+  constexpr int BK = 32, KP = BK + 8;  // row pitch: 80 bytes
+  constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
+  constexpr int LA = BM * BK / 256, LB = BK * BN / 256;
+  __shared__ __align__(16) unsigned short sA[2][BM][KP];
+  __shared__ __align__(16) unsigned short sB[2][BN][KP];
+  int r0, c0;
+  const GemmArgs p = pick<BM, BN>(pp, &r0, &c0);
+  const float* __restrict__ A = (const float*)p.A;
+  const float* __restrict__ B = (const float*)p.B;
+  float* C = (float*)p.C;
+  const int t = threadIdx.x;
+  const int ar = t % BM, ak = t / BM;
+  const int bk = t % BK, bc = t / BK;
+  float ra[LA], rb[LB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {  // rows beyond m only reach unstored results
+      const int k = k0 + ak + e * (256 / BM);
+      const auto v = A[(size_t)min(k, p.k - 1) * p.lda + min(r0 + ar, p.m - 1)];
+      ra[e] = k < p.k ? v : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      const int k = k0 + bk;
+      const auto v = B[(size_t)min(c0 + bc + e * (256 / BK), p.n - 1) * p.ldb + min(k, p.k - 1)];
+      rb[e] = k < p.k ? v : 0.f;
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) sA[buf][ar][ak + e * (256 / BM)] = to_bf16(ra[e]);
+#pragma unroll
+    for (int e = 0; e < LB; ++e) sB[buf][bc + e * (256 / BK)][bk] = to_bf16(rb[e]);
+  };
+  const int w = t >> 6, lane = t & 63, l16 = lane & 15, lq = lane >> 4;
+  const int wr = w >> 1, wc = w & 1;
+  v4f acc[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int nk = (p.k + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    stage(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nk; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < nk) load((s + 1) * BK);
+    v8bf fa[IM], fb[JN];
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+      fa[i] = *reinterpret_cast<const v8bf*>(&sA[cur][wr * WM + 16 * i + l16][8 * lq]);
+#pragma unroll
+    for (int j = 0; j < JN; ++j)
+      fb[j] = *reinterpret_cast<const v8bf*>(&sB[cur][wc * WN + 16 * j + l16][8 * lq]);
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+    if (s + 1 < nk) stage(cur ^ 1);
+    __syncthreads();
+  }
+  bool big = false;
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+    big |= epilogue_rows<float, JN>(p, C, r0 + wr * WM + 16 * i + l16, c0 + wc * WN, acc[i],
+                                    [&](int r) { return 4 * lq + r; });
+  if (big && p.bad) atomicOr(p.bad, 1);
+}
+
+template <int BM, int BN>
+int tiles_of(const GemmArgs& p) {
+  if (p.m <= 0 || p.n <= 0) return 0;
+  return gdiv(p.m, BM) * gdiv(p.n, BN);
+}
+
+template <int BM, int BN>
+void launch_tile(hipStream_t s, int prec, GemmPair& pp) {
+  pp.tiles0 = tiles_of<BM, BN>(pp.p0);
+  const int total = pp.tiles0 + tiles_of<BM, BN>(pp.p1);
+  if (total == 0) return;
+  if (prec == GEMM_F64)
+    hipLaunchKernelGGL((k_gemm<double, BM, BN>), dim3(total), dim3(256), 0, s, pp);
+  else if (prec == GEMM_F32)
+    hipLaunchKernelGGL((k_gemm<float, BM, BN>), dim3(total), dim3(256), 0, s, pp);
+  else
+    hipLaunchKernelGGL((k_gemm_bf16<BM, BN>), dim3(total), dim3(256), 0, s, pp);
+}
+
+}  // namespace
+
+void launch_gemm(hipStream_t s, int prec, const GemmArgs* p, int count) {
+  GemmPair pp;
+  pp.p0 = p[0];
+  if (count > 1) {
+    pp.p1 = p[1];
+  } else {
+    pp.p1 = p[0];
+    pp.p1.m = 0;
+  }
+  // 128 x 128 tiles when they fill the chip (256 CUs) and the problems
+  // leave in-place use valid; 64 x 64 tiles otherwise.
+  const bool in_place = (pp.p0.C == pp.p0.A || pp.p0.C == pp.p0.B) ||
+                        (pp.p1.m > 0 && (pp.p1.C == pp.p1.A || pp.p1.C == pp.p1.B));
+  if (!in_place && tiles_of<128, 128>(pp.p0) + tiles_of<128, 128>(pp.p1) >= 256)
+    launch_tile<128, 128>(s, prec, pp);
+  else
+    launch_tile<64, 64>(s, prec, pp);
+}
+
+}  // namespace uvlm

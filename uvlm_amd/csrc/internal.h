@@ -193,7 +193,6 @@ class InvSolver {
                      double prev, double tol);
   void release();
 
-  void* handle_ = nullptr;
   double *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr, *r_ = nullptr, *xold_ = nullptr,
          *part_ = nullptr;
   unsigned long long* stat_ = nullptr;

@@ -9,13 +9,13 @@
 // two matrix-vector products (split-column partial sums, reduced in a fixed
 // order, so results are deterministic) instead of a latency-bound LU.  The
 // start is extrapolated from the last two solutions of a run.  X is improved
-// in place by Newton-Schulz steps X <- X (2I - A X) (two library GEMMs) when
+// in place by Newton-Schulz steps X <- X (2I - A X) (two FP64 matrix-core
+// GEMMs, gemm.hip) when
 // the contraction slows, and built initially from X0 = A^T / (||A||_1
 // ||A||_inf), for which the iteration converges for every non-singular A.
 // A batch of sweeps is queued at once; a one-thread check after each sweep
 // raises a device flag on convergence (or divergence) and the later sweeps
 // of the batch return at once, so the host usually reads back once per solve.
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +23,7 @@
 #include <string>
 
 #include "../../include/uvlm_gpu.h"
+#include "gemm.h"
 #include "internal.h"
 
 namespace uvlm {
@@ -256,10 +257,7 @@ double bits_to_double(unsigned long long u) {
 
 }  // namespace
 
-InvSolver::~InvSolver() {
-  release();
-  if (handle_) (void)rocblas_destroy_handle((rocblas_handle)handle_);
-}
+InvSolver::~InvSolver() { release(); }
 
 void InvSolver::release() {
   void* bufs[] = {X_, T_, X2_, r_, xold_, part_, stat_};
@@ -272,19 +270,7 @@ void InvSolver::release() {
   streak_ = 0;
 }
 
-int InvSolver::prepare(hipStream_t s, int n, std::string* err) {
-  if (!handle_) {
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) {
-      *err = "rocblas_create_handle failed";
-      return UVLM_ERR_HIP;
-    }
-    handle_ = h;
-  }
-  if (rocblas_set_stream((rocblas_handle)handle_, s) != rocblas_status_success) {
-    *err = "rocblas_set_stream failed";
-    return UVLM_ERR_HIP;
-  }
+int InvSolver::prepare(hipStream_t, int n, std::string* err) {
   if (n == n_) return UVLM_OK;
   release();
   // split-column shape of the matrix-vector products: ~1024 workgroups
@@ -313,19 +299,15 @@ int InvSolver::prepare(hipStream_t s, int n, std::string* err) {
 // < 0 on error, reason in last_err_).  The step synchronises the stream.
 double InvSolver::newton_schulz(hipStream_t s, const double* A) {
   last_err_.clear();
-  rocblas_handle h = (rocblas_handle)handle_;
   const int n = n_;
-  const double one = 1.0, zero = 0.0;
-  if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, A, n, X_,
-                    n, &zero, T_, n) != rocblas_status_success)
-    return -1.0;
+  const GemmArgs ax{n, n, n, A, n, X_, n, T_, n, 1.0, 0.0, nullptr};  // T = A X
+  launch_gemm(s, GEMM_F64, &ax, 1);
   const long nn = (long)n * n;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_ + kNsSlot, 1);
   hipLaunchKernelGGL(k_two_minus_dev, dim3(std::min(1024, cdiv(nn, 256))), dim3(256), 0, s,
                      T_, n, stat_ + kNsSlot);
-  if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, X_, n, T_,
-                    n, &zero, X2_, n) != rocblas_status_success)
-    return -1.0;
+  const GemmArgs xt{n, n, n, X_, n, T_, n, X2_, n, 1.0, 0.0, nullptr};  // X2 = X (2I - A X)
+  launch_gemm(s, GEMM_F64, &xt, 1);
   std::swap(X_, X2_);
   unsigned long long u = 0;
   if (!read_back(s, &u, stat_ + kNsSlot, sizeof(u), "Newton-Schulz step", &last_err_))
