@@ -36,8 +36,6 @@ namespace uvlm {
 namespace {
 
 constexpr int kNB = 32;
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -68,6 +66,32 @@ constexpr int kFan = 4;   // tree fan-in: kFan * kNB nominations = one tile
 // arithmetic, pivot rows broadcast with readlane.
 constexpr int kDB = 64;
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 16 x 16 x 4 matrix-core step D += X Y: lane l supplies X[l%16][l/16] and
+// Y[l/16][l%16]; result element r of lane l is D[rowof(r, l/16)][l%16].
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+  typedef v4d acc;
+  static __device__ __forceinline__ acc mma(double x, double y, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int rowof(int r, int lq) { return 4 * r + lq; }
+};
+template <>
+struct Mf<float> {
+  typedef v4f acc;
+  static __device__ __forceinline__ acc mma(float x, float y, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int rowof(int r, int lq) { return 4 * lq + r; }
+};
+
+constexpr int kSub = 16;
+
 // v from lane `l` (a constant after unrolling) to every lane
 __device__ __forceinline__ double bcast(double v, int l) {
   const long long u = __double_as_longlong(v);
@@ -79,128 +103,231 @@ __device__ __forceinline__ float bcast(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// The kb x kb diagonal block at k0 (identity beyond kb) factored in place by
-// one wavefront, lane i holding row i: at step j every lane reads the pivot
-// row from lane j and lanes i > j eliminate.  |l| <= 1 checked (flag), the
-// first zero pivot reported (info).
-template <typename T>
-__global__ __launch_bounds__(64) void k_lu64(T* A, int lda, int n, int k0, int* flag,
-                                             int* info) {
-  const int i = threadIdx.x;
-  const int kb = min(kDB, n - k0);
-  const int ic = min(i, kb - 1);
-  T x[kDB];
+// 16 x 16 block product on the matrix cores: acc += op(X) op(Y) with X, Y in
+// LDS (row stride ldx, ldy), X negated when neg.
+template <typename T, int LDX, int LDY>
+__device__ __forceinline__ typename Mf<T>::acc mm16(const T* X, const T* Y, bool neg,
+                                                   typename Mf<T>::acc acc, int l16, int lq) {
 #pragma unroll
-  for (int c = 0; c < kDB; ++c) x[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + ic];
-#pragma unroll
-  for (int c = 0; c < kDB; ++c)
-    if (!(i < kb && c < kb)) x[c] = c == i ? T(1) : T(0);
-  bool bad = false;
-  int zero = -1;
-#pragma unroll
-  for (int j = 0; j < kDB; ++j) {
-    const T piv = bcast(x[j], j);
-    if (piv == T(0) && zero < 0 && j < kb) zero = j;
-    const bool below = i > j;
-    T l = x[j];
-    bad |= below && fabs(l) > fabs(piv);
-    if (piv != T(0)) l = l / piv;
-    if (below) x[j] = l;
-    const T lm = below ? l : T(0);
-#pragma unroll
-    for (int c = j + 1; c < kDB; ++c) x[c] = __builtin_fma(-lm, bcast(x[c], j), x[c]);
+  for (int k = 0; k < kSub; k += 4) {
+    const T x = X[l16 * LDX + k + lq];
+    acc = Mf<T>::mma(neg ? -x : x, Y[(k + lq) * LDY + l16], acc);
   }
-  if (i < kb) {
-#pragma unroll
-    for (int c = 0; c < kDB; ++c)
-      if (c < kb) A[(size_t)(k0 + c) * lda + k0 + i] = x[c];
-  }
-  if (__any(bad) && i == 0) atomicOr(flag, 1);
-  if (i == 0 && zero >= 0) atomicCAS(info, 0, k0 + zero + 1);
+  return acc;
 }
 
-// L21 = A21 U11^-1 (workgroups [0, nrb): 64 rows each, lane = row) and U12 =
-// L11^-1 A12 (workgroups from nrb: 64 columns each, lane = column, the tile
-// staged through LDS so that global loads and stores stay contiguous), by
-// substitution against the factored diagonal block D in LDS; U's diagonal is
-// applied through reciprocals.  |l| <= 1 checked on L21 (flag).
+// LU without interchanges of the 64 x 64 block M (LDS, identity beyond kb)
+// by 256 lanes, in four 16-column stages: the 16 x 16 diagonal block
+// factored in registers by 16 lanes of wavefront 0 (pivot row by readlane)
+// and its inverses Li[b] = L_bb^-1, Ui[b] = U_bb^-1 formed (lane = column);
+// then the blocks below and right of it, M[r][b] U_bb^-1 and L_bb^-1
+// M[b][c], and the trailing blocks M[r][c] -= M[r][b] M[b][c] on the matrix
+// cores.  *bad: an |l| > 1 (thread-local); *zero: first zero pivot (LDS).
 template <typename T>
-__global__ __launch_bounds__(64) void k_panel64(T* A, int lda, int n, int k0, int nrb,
-                                                int* flag) {
-  __shared__ T D[kDB][kDB + 1];
-  __shared__ T W[kDB][kDB + 1];
-  __shared__ T rcp[kDB];
-  const int t = threadIdx.x;
+__device__ void diag_factor(T (*M)[kDB + 1], T (*Li)[kSub][kSub + 1], T (*Ui)[kSub][kSub + 1],
+                            int kb, bool* bad, int* zero) {
+  typedef typename Mf<T>::acc Acc;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, l16 = lane & 15, lq = lane >> 4;
+  for (int b = 0; b < 4; ++b) {
+    const int ob = kSub * b;
+    if (w == 0) {  // 16 x 16 LU, lane i = row ob + i
+      const int i = l16;
+      T x[kSub];
+#pragma unroll
+      for (int c = 0; c < kSub; ++c) x[c] = M[ob + i][ob + c];
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) {
+        const T piv = bcast(x[j], j);
+        if (lane == 0 && piv == T(0) && ob + j < kb) atomicMin(zero, ob + j);
+        const bool below = i > j && lane < kSub;
+        T l = x[j];
+        *bad |= below && fabs(l) > fabs(piv);
+        if (piv != T(0)) l = l / piv;
+        if (below) x[j] = l;
+        const T lm = below ? l : T(0);
+#pragma unroll
+        for (int c = j + 1; c < kSub; ++c) x[c] = __builtin_fma(-lm, bcast(x[c], j), x[c]);
+      }
+      if (lane < kSub) {
+#pragma unroll
+        for (int c = 0; c < kSub; ++c) M[ob + i][ob + c] = x[c];
+      }
+    }
+    __syncthreads();
+    if (w == 0 && lane < 2 * kSub) {  // lanes 0..15: column of L_bb^-1, 16..31: of U_bb^-1
+      const int c = l16;
+      T y[kSub];
+#pragma unroll
+      for (int r = 0; r < kSub; ++r) y[r] = r == c ? T(1) : T(0);
+      if (lane < kSub) {
+#pragma unroll
+        for (int k = 0; k < kSub; ++k) {
+          const T yk = y[k];
+#pragma unroll
+          for (int r = k + 1; r < kSub; ++r) y[r] = __builtin_fma(-M[ob + r][ob + k], yk, y[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kSub; ++r) Li[b][r][c] = y[r];
+      } else {
+#pragma unroll
+        for (int k = kSub - 1; k >= 0; --k) {
+          const T yk = y[k] / M[ob + k][ob + k];
+          y[k] = yk;
+#pragma unroll
+          for (int r = 0; r < k; ++r) y[r] = __builtin_fma(-M[ob + r][ob + k], yk, y[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kSub; ++r) Ui[b][r][c] = y[r];
+      }
+    }
+    __syncthreads();
+    const int rem = 3 - b;  // blocks below (right of) the diagonal block
+    for (int task = w; task < 2 * rem; task += 4) {
+      Acc acc = {T(0), T(0), T(0), T(0)};
+      if (task < rem) {  // M[r][b] <- M[r][b] U_bb^-1
+        const int orow = ob + kSub * (1 + task);
+        acc = mm16<T, kDB + 1, kSub + 1>(&M[orow][ob], &Ui[b][0][0], false, acc, l16, lq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = Mf<T>::rowof(r, lq);
+          M[orow + rr][ob + l16] = acc[r];
+          *bad |= fabs(acc[r]) > T(1);
+        }
+      } else {  // M[b][c] <- L_bb^-1 M[b][c]
+        const int ocol = ob + kSub * (1 + task - rem);
+        acc = mm16<T, kSub + 1, kDB + 1>(&Li[b][0][0], &M[ob][ocol], false, acc, l16, lq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) M[ob + Mf<T>::rowof(r, lq)][ocol + l16] = acc[r];
+      }
+    }
+    __syncthreads();
+    for (int task = w; task < rem * rem; task += 4) {  // M[r][c] -= M[r][b] M[b][c]
+      const int orow = ob + kSub * (1 + task % rem), ocol = ob + kSub * (1 + task / rem);
+      Acc acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = M[orow + Mf<T>::rowof(r, lq)][ocol + l16];
+      acc = mm16<T, kDB + 1, kDB + 1>(&M[orow][ob], &M[ob][ocol], true, acc, l16, lq);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) M[orow + Mf<T>::rowof(r, lq)][ocol + l16] = acc[r];
+    }
+    __syncthreads();
+  }
+}
+
+// One 64-column step of the fast path.  Every workgroup factors the
+// diagonal block at k0 (read from A, identity beyond kb) with diag_factor;
+// workgroup 0 stores the factored block in dblk (not in A: a workgroup of
+// the same launch dispatched after it finished would read the factored
+// block as the original), reports |l| > 1 (flag) and the first zero pivot
+// (info).  Then L21 = A21 U11^-1 (workgroups [0, nrb): 64 rows each) and
+// U12 = L11^-1 A12 (workgroups from nrb: 64 columns each) in four 16-column
+// (16-row) stages on the matrix cores: stage b forms
+//   L21[:, b] = (A21[:, b] - L21[:, <b] U11[<b, b]) U_bb^-1
+//   U12[b, :] = L_bb^-1 (A12[b, :] - L11[b, <b] U12[<b, :])
+// Wavefront w owns the 16 rows (columns) w of the tile, so the stages need
+// no exchange between wavefronts.  The diagonal blocks reach A through
+// k_diag_inv after the factorisation.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel64(T* A, int lda, int n, int k0, int nrb,
+                                                 T* __restrict__ dblk, int* flag, int* info) {
+  typedef typename Mf<T>::acc Acc;
+  __shared__ T D[kDB][kDB + 1];    // diagonal block, factored in place
+  __shared__ T W[kDB][kDB + 1];    // the tile: [row][column]
+  __shared__ T Tm[4][kSub][kSub + 1];
+  __shared__ T Li[4][kSub][kSub + 1], Ui[4][kSub][kSub + 1];
+  __shared__ int s_zero;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, l16 = lane & 15, lq = lane >> 4;
   const int kb = min(kDB, n - k0), s0 = k0 + kb;
+  const bool lower = (int)blockIdx.x < nrb;
+  const bool tile = (int)blockIdx.x < 2 * nrb;
+  // rows (lower) or columns (upper) of the tile, in the matrix
+  const int base = s0 + kDB * (lower ? blockIdx.x : blockIdx.x - nrb);
+  if (t == 0) s_zero = kDB;
   {
-    T v[kDB];
+    const int r = t & 63, cq = t >> 6;
+    T dv[16], wv[16];
 #pragma unroll
-    for (int c = 0; c < kDB; ++c)
-      v[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(t, kb - 1)];
+    for (int e = 0; e < 16; ++e) {
+      const int c = 16 * cq + e;
+      dv[e] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(r, kb - 1)];
+      wv[e] = !tile ? T(0)
+              : lower ? A[(size_t)(k0 + min(c, kb - 1)) * lda + min(base + r, n - 1)]
+                      : A[(size_t)min(base + c, n - 1) * lda + k0 + min(r, kb - 1)];
+    }
 #pragma unroll
-    for (int c = 0; c < kDB; ++c) D[t][c] = (t < kb && c < kb) ? v[c] : (t == c ? T(1) : T(0));
+    for (int e = 0; e < 16; ++e) {
+      const int c = 16 * cq + e;
+      D[r][c] = (r < kb && c < kb) ? dv[e] : (r == c ? T(1) : T(0));
+      W[r][c] = (lower ? c < kb : r < kb) ? wv[e] : T(0);
+    }
   }
   __syncthreads();
-  rcp[t] = T(1) / D[t][t];
-  __syncthreads();
-  if ((int)blockIdx.x < nrb) {
-    const int row = s0 + kDB * blockIdx.x + t, rc = min(row, n - 1);
-    T x[kDB];
+  bool dbad = false;
+  diag_factor<T>(D, Li, Ui, kb, &dbad, &s_zero);
+  if (blockIdx.x == 0) {
+    const int r = t & 63, cq = t >> 6;
 #pragma unroll
-    for (int c = 0; c < kDB; ++c) x[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + rc];
-#pragma unroll
-    for (int c = 0; c < kDB; ++c)
-      if (c >= kb) x[c] = T(0);
-    bool bad = false;
-#pragma unroll
-    for (int k = 0; k < kDB; ++k) {
-      const T xk = x[k] * rcp[k];
-      x[k] = xk;
-      bad |= fabs(xk) > T(1);
-#pragma unroll
-      for (int c = k + 1; c < kDB; ++c) x[c] = __builtin_fma(-xk, D[k][c], x[c]);
-    }
-    if (row < n) {
-#pragma unroll
-      for (int c = 0; c < kDB; ++c)
-        if (c < kb) A[(size_t)(k0 + c) * lda + row] = x[c];
-    }
-    if (__any(bad && row < n) && t == 0) atomicOr(flag, 1);
-  } else {
-    const int cb = s0 + kDB * (blockIdx.x - nrb);  // first column of the tile
-    {
-      T v[kDB];
-#pragma unroll
-      for (int c = 0; c < kDB; ++c)
-        v[c] = A[(size_t)min(cb + c, n - 1) * lda + k0 + min(t, kb - 1)];
-#pragma unroll
-      for (int c = 0; c < kDB; ++c) W[c][t] = t < kb ? v[c] : T(0);  // W[column][row]
-    }
-    __syncthreads();
-    T y[kDB];
-#pragma unroll
-    for (int r = 0; r < kDB; ++r) y[r] = W[t][r];
-#pragma unroll
-    for (int k = 0; k < kDB; ++k) {
-      const T yk = y[k];
-#pragma unroll
-      for (int r = k + 1; r < kDB; ++r) y[r] = __builtin_fma(-D[r][k], yk, y[r]);
-    }
-#pragma unroll
-    for (int r = 0; r < kDB; ++r) W[t][r] = y[r];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < kDB; ++c)
-      if (t < kb && cb + c < n) A[(size_t)(cb + c) * lda + k0 + t] = W[c][t];
+    for (int e = 0; e < 16; ++e) dblk[(16 * cq + e) * kDB + r] = D[r][16 * cq + e];
+    if (__any(dbad) && lane == 0) atomicOr(flag, 1);
+    if (t == 0 && s_zero < kDB) atomicCAS(info, 0, k0 + s_zero + 1);
   }
+  if (!tile) return;
+  bool bad = false;
+  for (int b = 0; b < 4; ++b) {
+    const int ob = kSub * b;
+    Acc acc;
+    if (lower) {  // rows 16w.. of the tile, columns ob..: A21 block minus L21[:, <b] U11[<b, b]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = W[16 * w + Mf<T>::rowof(r, lq)][ob + l16];
+      for (int k = 0; k < ob; k += 4)
+        acc = Mf<T>::mma(-W[16 * w + l16][k + lq], D[k + lq][ob + l16], acc);
+    } else {  // rows ob.. of U12, columns 16w..: A12 block minus L11[b, <b] U12[<b, :]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = W[ob + Mf<T>::rowof(r, lq)][16 * w + l16];
+      for (int k = 0; k < ob; k += 4)
+        acc = Mf<T>::mma(-D[ob + l16][k + lq], W[k + lq][16 * w + l16], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Tm[w][Mf<T>::rowof(r, lq)][l16] = acc[r];
+    __syncthreads();
+    Acc out = {T(0), T(0), T(0), T(0)};
+    if (lower) {  // (..) U_bb^-1
+      out = mm16<T, kSub + 1, kSub + 1>(&Tm[w][0][0], &Ui[b][0][0], false, out, l16, lq);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        W[16 * w + Mf<T>::rowof(r, lq)][ob + l16] = out[r];
+        bad |= fabs(out[r]) > T(1) && base + 16 * w + Mf<T>::rowof(r, lq) < n && ob + l16 < kb;
+      }
+    } else {  // L_bb^-1 (..)
+      out = mm16<T, kSub + 1, kSub + 1>(&Li[b][0][0], &Tm[w][0][0], false, out, l16, lq);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) W[ob + Mf<T>::rowof(r, lq)][16 * w + l16] = out[r];
+    }
+    __syncthreads();
+  }
+  {
+    const int r = t & 63, cq = t >> 6;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int c = 16 * cq + e;
+      if (lower) {
+        if (base + r < n && c < kb) A[(size_t)(k0 + c) * lda + base + r] = W[r][c];
+      } else {
+        if (base + c < n && r < kb) A[(size_t)(base + c) * lda + k0 + r] = W[r][c];
+      }
+    }
+  }
+  if (__any(bad) && lane == 0) atomicOr(flag, 1);
 }
 
 // Inverses of the factored 64 x 64 diagonal blocks (one workgroup per block,
 // identity beyond the matrix), column-major with ld 64 at invL, invU + 4096
 // b: wavefront 0 forms L^-1, wavefront 1 U^-1, one lane per column, by
-// forward and backward elimination on the identity in registers.
+// forward and backward elimination on the identity in registers.  dblk
+// (nullable): the blocks as the fast path left them (64 x 64 column-major
+// each), stored into A here; without it the blocks are read from A.
 template <typename T>
-__global__ __launch_bounds__(128) void k_diag_inv(const T* A, int lda, int n, T* invL,
+__global__ __launch_bounds__(128) void k_diag_inv(T* A, int lda, int n, const T* dblk, T* invL,
                                                   T* invU) {
   __shared__ T M[kDB][kDB + 1];
   __shared__ T rcp[kDB];
@@ -210,11 +337,18 @@ __global__ __launch_bounds__(128) void k_diag_inv(const T* A, int lda, int n, T*
   invU += (size_t)kDB * kDB * blockIdx.x;
   if (t < 64) {
     T v[kDB];
+    const T* db = dblk + (size_t)kDB * kDB * blockIdx.x;
 #pragma unroll
     for (int c = 0; c < kDB; ++c)
-      v[c] = A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(t, kb - 1)];
+      v[c] = dblk ? db[c * kDB + t]
+                  : A[(size_t)(k0 + min(c, kb - 1)) * lda + k0 + min(t, kb - 1)];
 #pragma unroll
     for (int c = 0; c < kDB; ++c) M[t][c] = (t < kb && c < kb) ? v[c] : (t == c ? T(1) : T(0));
+    if (dblk && t < kb) {  // the fast path's factored diagonal block into A
+#pragma unroll
+      for (int c = 0; c < kDB; ++c)
+        if (c < kb) A[(size_t)(k0 + c) * lda + k0 + t] = v[c];
+    }
   }
   __syncthreads();
   if (t < 64) rcp[t] = T(1) / M[t][t];
@@ -697,8 +831,8 @@ size_t lu_work_ints(int n) {
 }
 
 // Elements of the diagonal blocks' inverses: the L^-1 blocks, then the U^-1
-// blocks, 64 x 64 each.
-size_t lu_inv_elems(int n) { return 2 * (size_t)cdiv(n, kDB) * kDB * kDB; }
+// blocks, 64 x 64 each, then the fast path's factored diagonal blocks.
+size_t lu_inv_elems(int n) { return 3 * (size_t)cdiv(n, kDB) * kDB * kDB; }
 
 inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -728,18 +862,16 @@ void lu_factor_t(hipStream_t s, int prec, T* A, int n, int* perm, int* info, int
   const int nctr = 2 * tiles;
   T* invL = inv;
   T* invU = inv + (size_t)cdiv(n, kDB) * kDB * kDB;
+  T* dblk = invU + (size_t)cdiv(n, kDB) * kDB * kDB;  // the fast path's diagonal blocks
   hipLaunchKernelGGL(k_lu_init, dim3(cdiv(std::max(n, nctr), 256)), dim3(256), 0, s, info,
                      ctr, nctr, perm, n);
   if (!pivoted) {
     for (int k0 = 0; k0 < n; k0 += kDB) {
       const int kb = std::min(kDB, n - k0), rest = n - k0 - kb;
-      hipLaunchKernelGGL((k_lu64<T>), dim3(1), dim3(64), 0, s, A, n, n, k0, info + 1, info);
-      if (rest > 0) {
-        const int nb = cdiv(rest, kDB);
-        hipLaunchKernelGGL((k_panel64<T>), dim3(2 * nb), dim3(64), 0, s, A, n, n, k0, nb,
-                           info + 1);
-        lu_trailing(s, prec, A, n, k0, kb);
-      }
+      const int nb = cdiv(rest, kDB);
+      hipLaunchKernelGGL((k_panel64<T>), dim3(std::max(1, 2 * nb)), dim3(256), 0, s, A, n, n, k0,
+                         nb, dblk + (size_t)k0 * kDB, info + 1, info);
+      if (rest > 0) lu_trailing(s, prec, A, n, k0, kb);
     }
   } else {
     for (int k0 = 0; k0 < n; k0 += kNB) {
@@ -758,8 +890,8 @@ void lu_factor_t(hipStream_t s, int prec, T* A, int n, int* perm, int* info, int
       }
     }
   }
-  hipLaunchKernelGGL((k_diag_inv<T>), dim3(cdiv(n, kDB)), dim3(128), 0, s, (const T*)A, n, n,
-                     invL, invU);
+  hipLaunchKernelGGL((k_diag_inv<T>), dim3(cdiv(n, kDB)), dim3(128), 0, s, A, n, n,
+                     pivoted ? (const T*)nullptr : (const T*)dblk, invL, invU);
 }
 
 // Factor with the checked no-pivot path; if a check fails, restore A from
