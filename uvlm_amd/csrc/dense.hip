@@ -116,6 +116,18 @@ __device__ __forceinline__ typename Mf<T>::acc mm16(const T* X, const T* Y, bool
   return acc;
 }
 
+// 1 / d by the hardware reciprocal and one Newton step (a few ulp; far
+// shorter than the division sequence, whose dependent steps cost ~40 cycles
+// each on these chains)
+__device__ __forceinline__ double fast_rcp(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ float fast_rcp(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  return __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);
+}
+
 // LU without interchanges of the 64 x 64 block M (LDS, identity beyond kb)
 // by 256 lanes, in four 16-column stages: the 16 x 16 diagonal block
 // factored in registers by 16 lanes of wavefront 0 (pivot row by readlane)
@@ -142,7 +154,7 @@ __device__ void diag_factor(T (*M)[kDB + 1], T (*Li)[kSub][kSub + 1], T (*Ui)[kS
         const bool below = i > j && lane < kSub;
         T l = x[j];
         *bad |= below && fabs(l) > fabs(piv);
-        if (piv != T(0)) l = l / piv;
+        if (piv != T(0)) l = l * fast_rcp(piv);
         if (below) x[j] = l;
         const T lm = below ? l : T(0);
 #pragma unroll
@@ -169,9 +181,12 @@ __device__ void diag_factor(T (*M)[kDB + 1], T (*Li)[kSub][kSub + 1], T (*Ui)[kS
 #pragma unroll
         for (int r = 0; r < kSub; ++r) Li[b][r][c] = y[r];
       } else {
+        T rd[kSub];
+#pragma unroll
+        for (int k = 0; k < kSub; ++k) rd[k] = fast_rcp(M[ob + k][ob + k]);
 #pragma unroll
         for (int k = kSub - 1; k >= 0; --k) {
-          const T yk = y[k] / M[ob + k][ob + k];
+          const T yk = y[k] * rd[k];
           y[k] = yk;
 #pragma unroll
           for (int r = 0; r < k; ++r) y[r] = __builtin_fma(-M[ob + r][ob + k], yk, y[r]);
