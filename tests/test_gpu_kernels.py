@@ -188,6 +188,37 @@ def test_solve_mixed_precision(ctx, orc, mode):
     assert 1 <= iters <= 60
 
 
+@pytest.mark.parametrize("mode", [N.SOLVE_F32_IR, N.SOLVE_BF16_IR])
+@pytest.mark.parametrize("n", [63, 700, 2100])
+def test_solve_mixed_precision_sizes(ctx, mode, n):
+    """FP32 / BF16-operand LU with FP64 refinement on diagonally dominant
+    systems: ragged 64-column steps and GEMM tiles (n = 63, 700), several
+    256-row solve groups (2100)."""
+    rng = np.random.default_rng(900 + n)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 2.0 * n ** 0.5
+    b = rng.normal(size=n)
+    x, iters = ctx.solve(A, b, mode=mode, tol=1e-14, max_iter=60)
+    xr = np.linalg.solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=1e-10, atol=1e-12 * np.abs(xr).max())
+    assert 1 <= iters <= 60
+
+
+def test_solve_mixed_precision_needs_pivoting(ctx):
+    """An unstructured matrix: the FP32 factorisation falls back to
+    tournament pivoting; refinement still reaches FP64 accuracy.  (BF16
+    operands are left out: at cond(A) ~ 1e3 the BF16 factors do not contract
+    the refinement, kappa * 2^-8 > 1.)"""
+    n = 300
+    rng = np.random.default_rng(4242)
+    A = rng.normal(size=(n, n))
+    b = rng.normal(size=n)
+    x, iters = ctx.solve(A, b, mode=N.SOLVE_F32_IR, tol=1e-14, max_iter=200)
+    res = np.linalg.norm(A @ x - b) / (np.linalg.norm(A) * np.linalg.norm(x))
+    assert res < 1e-14
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-15 * np.linalg.cond(A) * n
+
+
 def test_singular_matrix_reported(ctx):
     from uvlm_amd import UVLMError
     with pytest.raises(UVLMError) as e:
@@ -229,7 +260,7 @@ def test_solve_fullpiv_ties_and_rank(ctx, orc):
     np.testing.assert_array_equal(x, xr)
 
 
-@pytest.mark.parametrize("n", [1, 33, 600])
+@pytest.mark.parametrize("n", [1, 33, 600, 2100])
 def test_solve_inverse_refinement(ctx, orc, n):
     """UVLM_SOLVE_F64_INV from a cold start (inverse built by Newton-Schulz
     from A^T / (||A||_1 ||A||_inf)) agrees with the full-pivoting oracle."""
