@@ -283,6 +283,27 @@ def test_solve_inverse_influence_matrix(ctx, orc):
     np.testing.assert_allclose(x, xr, rtol=1e-11, atol=1e-13 * np.abs(xr).max())
 
 
+def test_solve_inverse_ill_conditioned(ctx, orc):
+    """cond(A) = 1e6: the Newton-Schulz build of an FP32 kept inverse stalls
+    at max|I - A X| ~ 6e-3 (FP32 rounding times cond), above its 1e-3 goal,
+    so the build must fall back to an FP64 inverse; the refined solution
+    then has the FP64 accuracy of the full-pivoting oracle (both ~cond *
+    1e-16 from the true solution).  tol 1e-12: the refinement's own floor
+    here is ~1e-11 relative."""
+    n = 200
+    rng = np.random.default_rng(77)
+    Q1, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    Q2, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = (Q1 * np.logspace(0, -6, n)) @ Q2.T
+    b = rng.normal(size=n)
+    x, sweeps = ctx.solve(A, b, mode=N.SOLVE_F64_INV, tol=1e-12)
+    xr, _ = orc.fullpiv_solve(A, b)
+    np.testing.assert_allclose(x, xr, rtol=0, atol=1e-8 * np.abs(xr).max())
+    r = A @ x - b
+    assert np.abs(r).max() <= 1e-12 * np.abs(A).sum(1).max() * np.abs(x).max()
+    assert sweeps <= 30
+
+
 def test_solve_inverse_singular_reported(ctx):
     from uvlm_amd import UVLMError
     with pytest.raises(UVLMError) as e:

@@ -186,15 +186,18 @@ class InvSolver {
   long long rebuilds() const { return rebuilds_; }
 
  private:
-  int prepare(hipStream_t s, int n, std::string* err);
+  // buffers for n unknowns, X in FP64 (wide) or FP32
+  int prepare(hipStream_t s, int n, bool wide, std::string* err);
   int build(hipStream_t s, const double* A, std::string* err);
   double newton_schulz(hipStream_t s, const double* A);
   void enqueue_sweep(hipStream_t s, const double* A, const double* b, double* x, int k,
                      double prev, double tol);
   void release();
 
-  double *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr, *r_ = nullptr, *xold_ = nullptr,
-         *part_ = nullptr;
+  void *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr;  // FP32, or FP64 when wide_
+  float* Af_ = nullptr;                               // A rounded to FP32
+  double *r_ = nullptr, *xold_ = nullptr, *part_ = nullptr;
+  bool wide_ = false;
   unsigned long long* stat_ = nullptr;
   int n_ = 0, S_ = 1, cols_ = 1;
   bool valid_ = false;

@@ -5,14 +5,19 @@
 // to the next.  The solver keeps X ~ A^-1 of a recent step and refines
 //     x_{k+1} = x_k + X (b - A x_k)
 // with the residual in FP64: the error contracts by rho(I - X A) per sweep
-// and the result solves the current A x = b to FP64 accuracy.  A sweep is
-// two matrix-vector products (split-column partial sums, reduced in a fixed
-// order, so results are deterministic) instead of a latency-bound LU.  The
-// start is extrapolated from the last two solutions of a run.  X is improved
-// in place by Newton-Schulz steps X <- X (2I - A X) (two FP64 matrix-core
-// GEMMs, gemm.hip) when
-// the contraction slows, and built initially from X0 = A^T / (||A||_1
-// ||A||_inf), for which the iteration converges for every non-singular A.
+// and the result solves the current A x = b to FP64 accuracy.  X only sets
+// the contraction, so it is kept in FP32: its rounding (~6e-8 cond(A))
+// bounds rho far below the refresh threshold, halves the bytes of its
+// matrix-vector product and lets its Newton-Schulz steps run on the FP32
+// matrix cores.  A matrix whose FP32 inverse cannot reach that accuracy
+// (the build stalls above 1e-3) is handled with X in FP64.
+// A sweep is two matrix-vector products (split-column partial sums, reduced
+// in a fixed order, so results are deterministic) instead of a
+// latency-bound LU.  The start is extrapolated from the last two solutions
+// of a run.  X is improved in place by Newton-Schulz steps
+// X <- X (2I - A X) (two matrix-core GEMMs, gemm.hip) when the contraction
+// slows, and built initially from X0 = A^T / (||A||_1 ||A||_inf), for which
+// the iteration converges for every non-singular A.
 // A batch of sweeps is queued at once; a one-thread check after each sweep
 // raises a device flag on convergence (or divergence) and the later sweeps
 // of the batch return at once, so the host usually reads back once per solve.
@@ -54,27 +59,35 @@ __device__ inline void block_max_to(double v, unsigned long long* out, double* l
 }
 
 // T <- 2I - T and out <- max |I - T| (grid-stride, one atomic per workgroup)
-__global__ __launch_bounds__(256) void k_two_minus_dev(double* T, int n,
-                                                       unsigned long long* out) {
+template <typename T_>
+__global__ __launch_bounds__(256) void k_two_minus_dev(T_* T, int n, unsigned long long* out) {
   __shared__ double lds[4];
   const long nn = (long)n * n;
   double e = 0.0;
   for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < nn; k += (long)gridDim.x * 256) {
     const long i = k % n, j = k / n;
-    const double id = i == j ? 1.0 : 0.0;
-    const double t = T[k];
-    e = fmax(e, fabs(id - t));
-    T[k] = 2.0 * id - t;
+    const T_ id = i == j ? (T_)1 : (T_)0;
+    const T_ t = T[k];
+    e = fmax(e, (double)fabs(id - t));
+    T[k] = (T_)2 * id - t;
   }
   block_max_to(e, out, lds);
 }
 
 // X <- alpha A^T
-__global__ void k_scaled_transpose(const double* A, int n, double alpha, double* X) {
+template <typename T_>
+__global__ void k_scaled_transpose(const double* A, int n, double alpha, T_* X) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)n * n) return;
   const long i = k % n, j = k / n;
-  X[k] = alpha * A[i * n + j];
+  X[k] = (T_)(alpha * A[i * n + j]);
+}
+
+// Af <- A rounded to FP32 (the operand of the FP32 Newton-Schulz GEMM)
+__global__ void k_round_f32(const double* __restrict__ A, long nn, float* __restrict__ Af) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nn;
+       k += (long)gridDim.x * blockDim.x)
+    Af[k] = (float)A[k];
 }
 
 // out[0] = max_j sum_i |A_ij| (1-norm), out[1] = max_i sum_j |A_ij| (inf-norm),
@@ -101,21 +114,24 @@ __global__ __launch_bounds__(256) void k_colsum_max(const double* __restrict__ A
     atomicMax(out, (unsigned long long)__double_as_longlong((sw[0] + sw[1]) + (sw[2] + sw[3])));
 }
 
-// One thread per row: the row's absolute sum over the columns in order.
-__global__ __launch_bounds__(256) void k_rowsum_max(const double* __restrict__ A, int n,
-                                                    unsigned long long* __restrict__ out) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+// part[s][i] = sum_{j in column slice s} |A[j n + i]|: the row sums of |A|
+// over column slices (one row per thread, coalesced across the workgroup).
+__global__ __launch_bounds__(256) void k_absrow_part(const double* __restrict__ A, int n, int cols,
+                                                     double* __restrict__ part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = blockIdx.y * cols;
+  const int j1 = min(n, j0 + cols);
+  const double* p = A + (long)j0 * n + i;
   double acc = 0.0;
-  if (r < n)
-    for (int c = 0; c < n; ++c) acc += fabs(A[(long)c * n + r]);
-  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
-  if ((threadIdx.x & 63) == 0)
-    atomicMax(out + 1, (unsigned long long)__double_as_longlong(acc));
+  for (int j = j0; j < j1; ++j, p += n) acc += fabs(*p);
+  part[(long)blockIdx.y * n + i] = acc;
 }
-
-// part[s][i] = sum_{j in column slice s} M[j n + i] v[j] (column-major M),
-// s = blockIdx.y; one row per thread, columns in increasing order.
-__global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ M, int n,
+// part[s][i] = sum_{j in column slice s} M[j n + i] v[j] (column-major M,
+// FP64 or FP32 elements, FP64 products and sums), s = blockIdx.y; one row
+// per thread, columns in increasing order.
+template <typename T_>
+__global__ __launch_bounds__(256) void k_gemv_part(const T_* __restrict__ M, int n,
                                                    const double* __restrict__ v, int cols,
                                                    double* __restrict__ part,
                                                    const long long* flag) {
@@ -124,7 +140,7 @@ __global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ M,
   if (i >= n) return;
   const int j0 = blockIdx.y * cols;
   const int j1 = min(n, j0 + cols);
-  const double* p = M + (long)j0 * n + i;
+  const T_* p = M + (long)j0 * n + i;
   double acc = 0.0;
   int j = j0;
   for (; j + 4 <= j1; j += 4) {
@@ -135,7 +151,7 @@ __global__ __launch_bounds__(256) void k_gemv_part(const double* __restrict__ M,
     acc = fma(a3, v[j + 3], acc);
     p += 4L * n;
   }
-  for (; j < j1; ++j, p += n) acc = fma(*p, v[j], acc);
+  for (; j < j1; ++j, p += n) acc = fma((double)*p, v[j], acc);
   part[(long)blockIdx.y * n + i] = acc;
 }
 
@@ -198,21 +214,39 @@ __global__ __launch_bounds__(256) void k_update_sum(const double* __restrict__ p
   }
 }
 
+// out[1] <- max_i sum_s part[s][i] (the inf-norm from k_absrow_part's slices)
+__global__ __launch_bounds__(256) void k_rowmax_sum(const double* __restrict__ part, int S, int n,
+                                                    unsigned long long* out) {
+  __shared__ double lds[kRedGroups][17];
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+  double t = sum_parts(part, S, n, i, lds);
+  if (threadIdx.x < 64) {
+    if (!(threadIdx.x < 16 && i < n)) t = 0.0;
+    for (int off = 32; off > 0; off >>= 1) t = fmax(t, __shfl_xor(t, off));
+    if (threadIdx.x == 0) atomic_max_nonneg(out + 1, t);
+  }
+}
+
 // After sweep k: raise the flag on convergence (max|dx| <= tol max|x|, or a
-// stall at the rounding level: contraction above 1/2 with max|dx| <= 1e3 tol
-// max|x|) or divergence (a growing or non-finite correction).
+// stall at the rounding level: contraction above 1/2, growth included, with
+// max|dx| <= 1e3 tol max|x|) or divergence (a non-finite correction, or one
+// that grows above the rounding level).
 __global__ void k_check(const unsigned long long* norms, int k, double prev, double tol,
                         long long* flag) {
   if (*flag != 0) return;
   const double d = __longlong_as_double((long long)norms[2 * k]);
   const double m = __longlong_as_double((long long)norms[2 * k + 1]);
   const double p = k > 0 ? __longlong_as_double((long long)norms[2 * k - 2]) : prev;
-  if (!(d <= 1e300) || !(m <= 1e300) || (p > 0.0 && d > p)) {
+  if (!(d <= 1e300) || !(m <= 1e300)) {
     *flag = -1 - k;
     return;
   }
   const double rho = p > 0.0 ? d / p : -1.0;
-  if (d <= tol * m || (rho > 0.5 && d <= 1e3 * tol * m)) *flag = 1 + k;
+  if (d <= tol * m || (rho > 0.5 && d <= 1e3 * tol * m)) {
+    *flag = 1 + k;
+    return;
+  }
+  if (p > 0.0 && d > p) *flag = -1 - k;
 }
 
 // x <- 2 x - xold, xold <- x (linear extrapolation of the start)
@@ -260,18 +294,21 @@ double bits_to_double(unsigned long long u) {
 InvSolver::~InvSolver() { release(); }
 
 void InvSolver::release() {
-  void* bufs[] = {X_, T_, X2_, r_, xold_, part_, stat_};
+  void* bufs[] = {X_, T_, X2_, Af_, r_, xold_, part_, stat_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
-  X_ = T_ = X2_ = r_ = xold_ = part_ = nullptr;
+  X_ = T_ = X2_ = nullptr;
+  Af_ = nullptr;
+  r_ = xold_ = part_ = nullptr;
   stat_ = nullptr;
   n_ = 0;
+  wide_ = false;
   valid_ = false;
   streak_ = 0;
 }
 
-int InvSolver::prepare(hipStream_t, int n, std::string* err) {
-  if (n == n_) return UVLM_OK;
+int InvSolver::prepare(hipStream_t, int n, bool wide, std::string* err) {
+  if (n == n_ && wide == wide_) return UVLM_OK;
   release();
   // split-column shape of the matrix-vector products: ~1024 workgroups
   const int rb = cdiv(n, 256);
@@ -279,12 +316,13 @@ int InvSolver::prepare(hipStream_t, int n, std::string* err) {
   cols_ = cdiv(n, want);
   S_ = cdiv(n, cols_);
   const size_t nn = (size_t)n * n;
-  if (hipMalloc(&X_, sizeof(double) * nn) != hipSuccess ||
-      hipMalloc(&T_, sizeof(double) * nn) != hipSuccess ||
-      hipMalloc(&X2_, sizeof(double) * nn) != hipSuccess ||
-      hipMalloc(&r_, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&xold_, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&part_, sizeof(double) * (size_t)S_ * n) != hipSuccess ||
+  const size_t el = wide ? sizeof(double) : sizeof(float);
+  if (hipMalloc(&X_, el * nn) != hipSuccess || hipMalloc(&T_, el * nn) != hipSuccess ||
+      hipMalloc(&X2_, el * nn) != hipSuccess ||
+      (!wide && hipMalloc((void**)&Af_, sizeof(float) * nn) != hipSuccess) ||
+      hipMalloc((void**)&r_, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&xold_, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&part_, sizeof(double) * (size_t)S_ * n) != hipSuccess ||
       hipMalloc((void**)&stat_, sizeof(unsigned long long) * kStatWords) != hipSuccess) {
     (void)hipGetLastError();
     release();
@@ -292,22 +330,36 @@ int InvSolver::prepare(hipStream_t, int n, std::string* err) {
     return UVLM_ERR_OOM;
   }
   n_ = n;
+  wide_ = wide;
   return UVLM_OK;
 }
 
 // One Newton-Schulz step from the current X; returns max |I - A X_old| (or
-// < 0 on error, reason in last_err_).  The step synchronises the stream.
+// < 0 on error, reason in last_err_).  FP32: A is rounded to FP32 and both
+// products run on the FP32 matrix cores.  The step synchronises the stream.
 double InvSolver::newton_schulz(hipStream_t s, const double* A) {
   last_err_.clear();
   const int n = n_;
-  const GemmArgs ax{n, n, n, A, n, X_, n, T_, n, 1.0, 0.0, nullptr};  // T = A X
-  launch_gemm(s, GEMM_F64, &ax, 1);
   const long nn = (long)n * n;
+  const int prec = wide_ ? GEMM_F64 : GEMM_F32;
+  const void* Aop = A;
+  if (!wide_) {
+    hipLaunchKernelGGL(k_round_f32, dim3(std::min(2048, cdiv(nn, 256))), dim3(256), 0, s, A, nn,
+                       Af_);
+    Aop = Af_;
+  }
+  const GemmArgs ax{n, n, n, Aop, n, X_, n, T_, n, 1.0, 0.0, nullptr};  // T = A X
+  launch_gemm(s, prec, &ax, 1);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_ + kNsSlot, 1);
-  hipLaunchKernelGGL(k_two_minus_dev, dim3(std::min(1024, cdiv(nn, 256))), dim3(256), 0, s,
-                     T_, n, stat_ + kNsSlot);
+  const dim3 g2(std::min(1024, cdiv(nn, 256)));
+  if (wide_)
+    hipLaunchKernelGGL(k_two_minus_dev<double>, g2, dim3(256), 0, s, (double*)T_, n,
+                       stat_ + kNsSlot);
+  else
+    hipLaunchKernelGGL(k_two_minus_dev<float>, g2, dim3(256), 0, s, (float*)T_, n,
+                       stat_ + kNsSlot);
   const GemmArgs xt{n, n, n, X_, n, T_, n, X2_, n, 1.0, 0.0, nullptr};  // X2 = X (2I - A X)
-  launch_gemm(s, GEMM_F64, &xt, 1);
+  launch_gemm(s, prec, &xt, 1);
   std::swap(X_, X2_);
   unsigned long long u = 0;
   if (!read_back(s, &u, stat_ + kNsSlot, sizeof(u), "Newton-Schulz step", &last_err_))
@@ -316,13 +368,18 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A) {
   return bits_to_double(u);
 }
 
+// X from X0 = A^T / (||A||_1 ||A||_inf) by Newton-Schulz steps until
+// max |I - A X| < 1e-3.  In FP32 a deviation that stops falling above that
+// (the FP32 rounding of an ill-conditioned A) switches to FP64 and starts over.
 int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
   const int n = n_;
   double* nrm = reinterpret_cast<double*>(stat_);  // words 0 and 1
   unsigned long long* nb = reinterpret_cast<unsigned long long*>(stat_);
   hipLaunchKernelGGL(k_norms_zero, dim3(1), dim3(1), 0, s, nb);
   hipLaunchKernelGGL(k_colsum_max, dim3(n), dim3(256), 0, s, A, n, nb);
-  hipLaunchKernelGGL(k_rowsum_max, dim3(cdiv(n, 256)), dim3(256), 0, s, A, n, nb);
+  hipLaunchKernelGGL(k_absrow_part, dim3(cdiv(n, 256), S_), dim3(256), 0, s, A, n, cols_, part_);
+  hipLaunchKernelGGL(k_rowmax_sum, dim3(cdiv(n, 16)), dim3(256), 0, s, (const double*)part_, S_,
+                     n, nb);
   double h[2] = {0, 0};
   if (!read_back(s, h, nrm, sizeof(h), "matrix norms", err)) return UVLM_ERR_HIP;
   if (!(h[0] > 0.0) || !(h[1] > 0.0)) {
@@ -330,8 +387,14 @@ int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
     return UVLM_ERR_SINGULAR;
   }
   const long nn = (long)n * n;
-  hipLaunchKernelGGL(k_scaled_transpose, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, n,
-                     1.0 / (h[0] * h[1]), X_);
+  if (wide_)
+    hipLaunchKernelGGL(k_scaled_transpose<double>, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, n,
+                       1.0 / (h[0] * h[1]), (double*)X_);
+  else
+    hipLaunchKernelGGL(k_scaled_transpose<float>, dim3(cdiv(nn, 256)), dim3(256), 0, s, A, n,
+                       1.0 / (h[0] * h[1]), (float*)X_);
+  double best = 1e300;
+  int since = 0;  // steps since the deviation last fell below 0.7 of its best
   for (int it = 0; it < 200; ++it) {
     const double e = newton_schulz(s, A);
     if (e < 0.0) {
@@ -343,6 +406,20 @@ int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
       return UVLM_OK;
     }
     if (!(e < 1e300)) break;
+    // below 1/4 the deviation falls at least by half per step (its spectral
+    // part squares); four steps without a 30 % gain there, or 60 steps in
+    // all (1 - e doubles per step from 1 - 2^-60), mean the FP32 floor
+    if (e < 0.7 * best) {
+      best = e;
+      since = 0;
+    } else {
+      ++since;
+    }
+    if (!wide_ && ((best < 0.25 && since >= 4) || it >= 60)) {
+      int rc = prepare(s, n, true, err);
+      if (rc != UVLM_OK) return rc;
+      return build(s, A, err);
+    }
   }
   *err = "inverse solver: Newton-Schulz did not converge (singular matrix?)";
   return UVLM_ERR_NOCONV;
@@ -356,12 +433,16 @@ void InvSolver::enqueue_sweep(hipStream_t s, const double* A, const double* b, d
   const long long* flag = reinterpret_cast<const long long*>(stat_ + kFlagSlot);
   const dim3 g(cdiv(n, 256), S_);
   const int rb = cdiv(n, 16);
-  hipLaunchKernelGGL(k_gemv_part, g, dim3(256), 0, s, A, n, (const double*)x, cols_, part_,
-                     flag);
+  hipLaunchKernelGGL(k_gemv_part<double>, g, dim3(256), 0, s, A, n, (const double*)x, cols_,
+                     part_, flag);
   hipLaunchKernelGGL(k_residual_sum, dim3(rb), dim3(256), 0, s, (const double*)part_, S_, n, b,
                      r_, flag);
-  hipLaunchKernelGGL(k_gemv_part, g, dim3(256), 0, s, (const double*)X_, n, (const double*)r_,
-                     cols_, part_, flag);
+  if (wide_)
+    hipLaunchKernelGGL(k_gemv_part<double>, g, dim3(256), 0, s, (const double*)X_, n,
+                       (const double*)r_, cols_, part_, flag);
+  else
+    hipLaunchKernelGGL(k_gemv_part<float>, g, dim3(256), 0, s, (const float*)X_, n,
+                       (const double*)r_, cols_, part_, flag);
   hipLaunchKernelGGL(k_update_sum, dim3(rb), dim3(256), 0, s, (const double*)part_, S_, n, x,
                      stat_ + 2 * k, flag);
   hipLaunchKernelGGL(k_check, dim3(1), dim3(1), 0, s, (const unsigned long long*)stat_, k, prev,
@@ -370,7 +451,7 @@ void InvSolver::enqueue_sweep(hipStream_t s, const double* A, const double* b, d
 
 int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, double* x,
                      bool warm, double tol, int max_sweeps, int* sweeps, std::string* err) {
-  int rc = prepare(s, n, err);
+  int rc = prepare(s, n, n == n_ && wide_, err);
   if (rc != UVLM_OK) return rc;
   if (sweeps) *sweeps = 0;
   const int nb = cdiv(n, 256);
