@@ -12,9 +12,12 @@ The whole run is done twice in this process:
 
 * the fast form: near/far split of the wake sum (lattice-block moment
   expansions for far clusters, theta 4 near the wings graded beyond 1.5
-  wing radii to 1.3, exact audits every 50 steps), FP32 compensated near
-  sums with FP64 near-core re-evaluation, kept-inverse FP64 solve with
-  iterative refinement;
+  wing radii to 1.2, exact audits every 50 steps), a correction network
+  for the far clusters trained in the run on the run's own wake (steps
+  100, 600, 1100, 1600) and evaluated once per (target tile, cluster) on
+  the BF16 matrix cores, FP32 compensated near sums with FP64 near-core
+  re-evaluation, the wings' moment expansions for wake nodes beyond the
+  first 24 rows, kept-inverse solve (FP32 inverse, FP64 residuals);
 * the plain FP64 method: every sum exact (FP64 segment chains), blocked
   FP64 LU.
 
@@ -37,7 +40,9 @@ state of the plain run, on the host's cores.
 Other modes: --workload c1|c2|c3|c5 runs one configuration alone (same
 window rules, `--precision`, `--solve`, `--farfield` choose the method),
 --workload c4 the 256 batched cases; --no-plain, --no-c2, --no-cpu skip
-legs; --spread adds a second FP64 run of the tandem workload that differs
+legs (--no-net-compare the fast form's run without the network, which is
+otherwise timed after the plain run); --spread adds a second FP64 run of
+the tandem workload that differs
 from the plain one only by rounding (kept-inverse solve instead of the LU)
 and reports, per 100-step window, the fast run's CL/CT differences beside
 the two FP64 runs' spread; --calibrate-copy runs a plain device copy of known size for the PMC
@@ -67,9 +72,13 @@ TRAFFIC = {  # committed PMC summaries (HBM bytes per launch) of the default run
     "k_chain_f64": os.path.join(PROFILES, "k_chain_f64_naca4412_20x80_rk4_traffic.json"),
     "k_ff_near": os.path.join(PROFILES, "k_ff_near_tandem_2x16x64_traffic.json"),
 }
-# The fast form of the tandem workload.
+# The fast form of the tandem workload.  The correction network is trained
+# in the run from its own wake (first at step `first`, then warm-started
+# every `every` steps) and corrects the far clusters from then on.
+NET = {"first": 100, "every": 500, "samples": 30000, "iters_first": 300, "iters": 80,
+       "batch": 1024, "lr": 1e-3}
 FAST = {"precision": "f32c", "solve": "inv", "farfield": 4.0, "ff_reach": 1.5,
-        "ff_theta_min": 1.3, "audit": 50, "ff_tol": 1e-3}
+        "ff_theta_min": 1.2, "audit": 50, "ff_tol": 1e-3, "mlp": NET}
 PLAIN = {"precision": "f64", "solve": "f64", "farfield": 0.0}
 
 
@@ -114,6 +123,11 @@ def run_case(case, method, window, warmup, label):
             sim.set_farfield_focus(method["ff_reach"], method.get("ff_theta_min", 2.0))
     K = max(1, min(window, S))
     first = S - K + 1  # first timed step
+    net = method.get("mlp")
+    w, trainings, t_train = None, 0, 0.0
+    if net:
+        from uvlm_amd.gpu import mlp_init
+        w = mlp_init(5)
     sim.profile(True)
     sim.profile_read()
     sim.profile_read_near()
@@ -126,6 +140,13 @@ def run_case(case, method, window, warmup, label):
             sim.sync()
             tw0 = time.perf_counter()
         sim.step(s)
+        if net and s >= net["first"] and (s - net["first"]) % net["every"] == 0:
+            t0 = time.perf_counter()
+            w, _ = sim.farfield_learn(w, method["farfield"], max_samples=net["samples"],
+                                      iters=net["iters_first"] if trainings == 0 else net["iters"],
+                                      batch=net["batch"], lr=net["lr"], seed=s)
+            t_train += time.perf_counter() - t0
+            trainings += 1
         if s % 200 == 0:
             log(f"{label} step {s}")
     sim.sync()
@@ -140,6 +161,10 @@ def run_case(case, method, window, warmup, label):
            "near": near, "pairs": sim.pairs()}
     if method.get("farfield", 0) > 0:
         out["ff"] = sim.farfield_stats()
+    if net:
+        pairs, trips = sim.farfield_mlp_stats()
+        out["net"] = {"trainings": trainings, "training_s": t_train, "pairs_corrected": pairs,
+                      "pairs_tripped_to_exact": trips}
     if method["solve"] == "inv":
         out["inv"] = sim.inverse_stats()
     return out
@@ -252,6 +277,10 @@ def bench_tandem(args):
     case = W.tandem_c5()
     S = int(case["steps"])
     fast_m = dict(FAST)
+    if args.fast_theta_min:
+        fast_m["ff_theta_min"] = args.fast_theta_min
+    if args.no_net:
+        fast_m.pop("mlp")
     log("fast form")
     fast = run_case(case, fast_m, args.steps, args.warmup, "fast")
     fast_sim = fast.pop("sim")
@@ -267,14 +296,21 @@ def bench_tandem(args):
                    "dt": case["dt"], "run_steps": S, "timed_steps": [first, S],
                    "final_wake_rings": 2 * S * case["cols"],
                    "method": {"sums": "near/far split of the wake: lattice-block moment "
-                                      "expansions (orders 0-3) for far clusters, exact "
-                                      "near blocks in FP32 with FP64 accumulation and "
-                                      "FP64 near-core re-evaluation; bound sheets exact",
+                                      "expansions (orders 0-3) for far clusters with a "
+                                      "learned per-tile correction, exact near blocks in "
+                                      "FP32 with FP64 accumulation and FP64 near-core "
+                                      "re-evaluation; bound sheets exact at the wings and "
+                                      "in the first 24 wake rows, by each wing's moment "
+                                      "expansion beyond",
                               "theta": fast_m["farfield"],
                               "graded": {"reach": fast_m["ff_reach"],
                                          "theta_min": fast_m["ff_theta_min"]},
                               "audit_every": fast_m["audit"],
-                              "solve": "kept-inverse FP64, iterative refinement to 1e-14",
+                              "network": (dict(fast_m["mlp"], arch="32-128-128-128-30 SiLU, "
+                                               "BF16 MFMA inference, trained in the run")
+                                          if fast_m.get("mlp") else None),
+                              "solve": "kept-inverse (FP32 inverse, FP64 residuals), "
+                                       "iterative refinement to 1e-14",
                               "loads": "FP64"}},
         "timesteps_per_s_run": S / fast["t_run"],
         "work_rate_reference_counting": {
@@ -290,6 +326,8 @@ def bench_tandem(args):
     if "ff" in fast:
         err, audits, active = fast["ff"]
         out["farfield"] = {"last_audit_rel_err": err, "audits": audits, "active": bool(active)}
+    if "net" in fast:
+        out["network"] = fast["net"]
     if "inv" in fast:
         sw, rf, rb, _ = fast["inv"]
         out["inverse_solve"] = {"sweeps": sw, "newton_schulz_refreshes": rf, "rebuilds": rb}
@@ -313,6 +351,17 @@ def bench_tandem(args):
             r2 = run_case(case, dict(PLAIN, solve="inv"), args.steps, args.warmup, "fp64-b")
             r2.pop("sim").close()
             out["rounding_spread"] = spread_report(fast, plain, r2)
+    if fast_m.get("mlp") and not args.no_net_compare:
+        log("fast form without the network")
+        m2 = {k: v for k, v in fast_m.items() if k != "mlp"}
+        r2 = run_case(case, m2, args.steps, args.warmup, "fast-no-net")
+        r2.pop("sim").close()
+        cmp = {"ms_per_step_run": 1e3 * r2["t_run"] / S,
+               "with_network_ms_per_step_run": 1e3 * fast["t_run"] / S}
+        if plain_sim is not None:
+            cmp["timesteps_per_s_ratio_run_vs_plain"] = plain["t_run"] / r2["t_run"]
+            cmp.update(load_differences(r2["loads"], plain["loads"]))
+        out["fast_without_network"] = cmp
     if not args.no_c2:
         log("NACA4412 block")
         c2 = W.naca4412_c2()
@@ -430,6 +479,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--fast-theta-min", type=float, default=0.0,
+                    help="tandem fast form: the graded criterion's floor (default %g)"
+                    % FAST["ff_theta_min"])
+    ap.add_argument("--no-net", action="store_true",
+                    help="tandem fast form without the correction network")
+    ap.add_argument("--no-net-compare", action="store_true",
+                    help="skip the fast form's second run without the correction network")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--spread", action="store_true",

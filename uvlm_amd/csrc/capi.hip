@@ -70,14 +70,19 @@ struct uvlm_ctx {
   double batch_pairs = 0.0;
   // far-field split of the wake sum in the advection stages
   uvlm::FarField* ff = nullptr;
+  uvlm::FarField* ffb = nullptr;  // the bound sheets as sources (wake nodes downstream)
   double ff_theta = 0.0, ff_tol = 0.0, ff_last_err = -1.0;
   double ff_reach = 0.0, ff_theta_min = 0.0;  // graded criterion (reach 0: off)
   int ff_audit_every = 0, ff_counter = 0, ff_audits = 0;
   bool ff_off = false;
   // the split's sources are rebuilt only when the wake changed
   long wake_version = 0, ff_version = -1;
+  // the solve's split call also serves the loads' edge midpoints: wake
+  // velocity there (ffm) for this wake and wing state
+  long wing_version = 0, ffm_wake = -1, ffm_wing = -1;
+  Buf wtg, wvel;
   Buf ffm;
-  Buf ffg, ffv, ffx, ffxs, ffstat, ffbox;
+  Buf ffg, ffv, ffx, ffxs, ffstat, ffbox, bnear, ffbnd;
   // kept-inverse solver (UVLM_SOLVE_F64_INV); one for the simulation, one
   // for the stateless uvlm_solve
   uvlm::InvSolver* inv = nullptr;
@@ -316,9 +321,20 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
 // wake's own nodes: their velocities set the circulations and the loads
 // directly, and they cost little.
 constexpr double kWingThetaFactor = 2.0;
-// Audits of wakes of at least kAuditFull nodes sum every kAuditStride-th
-// node exactly (an eighth of the cost of the exact stage).
-constexpr int kAuditFull = 32768, kAuditStride = 8;
+// Audits of wakes of at least kAuditFull nodes sum every stride-th node
+// exactly, stride = max(kAuditStride, nodes / kAuditNodes): at most an
+// eighth of the cost of the exact stage, and ~kAuditNodes sums for the
+// largest wakes.
+constexpr int kAuditFull = 32768, kAuditStride = 8, kAuditNodes = 8192;
+// With the split on, the bound sheets act on the wake's nodes exactly in
+// the first kBoundRows node rows of each wake sheet (the wake at the wings)
+// and through each wing's moment expansion beyond, where a node is farther
+// than kBoundTheta wing radii from the wing's centre (else that wing is
+// summed exactly at the node): relative error ~kBoundTheta^-4 of the bound
+// sheets' velocity at nodes that move wake acting on the wings like a
+// dipole.
+constexpr int kBoundRows = 24;
+constexpr double kBoundTheta = 4.0;
 
 // The near/far split applies: set, not switched off by an audit, a wake.
 bool ff_active(const uvlm_ctx* c) {
@@ -420,6 +436,8 @@ int uvlm_destroy(uvlm_ctx* c) {
   c->batch = nullptr;
   delete c->ff;
   c->ff = nullptr;
+  delete c->ffb;
+  c->ffb = nullptr;
   delete c->inv;
   c->inv = nullptr;
   delete c->t_inv;
@@ -920,6 +938,7 @@ int uvlm_sim_set_wing(uvlm_ctx* c, const double* nodes) {
   TRY(stage_upload(c, c->st_wing, c->wing.p, nodes, sizeof(double) * 3 * nn));
   launch_pack_lattice(c->stream, P<double>(c->wing), c->R + 1, 0, P<double>(c->gamma),
                       c->R, 0, c->num, c->R, c->C, P<RingRec>(c->recs));
+  c->wing_version += 1;
   return check_launch(c);
 }
 
@@ -999,13 +1018,31 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   launch_geometry(c->stream, P<double>(c->wing), c->num, c->R, c->C, P<double>(c->cp),
                   P<double>(c->nrm), P<double>(c->area));
   TRY(stage_upload(c, c->st_vm, c->vm.p, vmorph_cp, sizeof(double) * 3 * N));
-  // wake velocity at the collocation points (through the split when set)
+  // wake velocity at the collocation points (through the split when set;
+  // one split call also takes the loads' edge midpoints: the wake does not
+  // change between the solve and the loads)
   if (ff_active(c)) {
     TRY(ff_current(c));
+    const int E = c->num * c->R * (2 * c->C + 1);
+    const size_t nb = sizeof(double) * 3 * (size_t)(N + E);
+    TRY(ensure(c, c->wtg, nb));
+    TRY(ensure(c, c->wvel, nb));
+    TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
+    HIPCHK(c, hipMemcpyAsync(c->wtg.p, c->cp.p, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice,
+                             c->stream));
+    launch_edge_midpoints(c->stream, P<double>(c->wing), c->num, c->R, c->C,
+                          P<double>(c->wtg) + 3 * (size_t)N);
     std::string e;
-    const int rc = c->ff->velocity(c->stream, P<double>(c->cp), N, kWingThetaFactor * c->ff_theta,
-                                   P<double>(c->uw), &e, nullptr, true);
+    const int rc = c->ff->velocity(c->stream, P<double>(c->wtg), N + E,
+                                   kWingThetaFactor * c->ff_theta, P<double>(c->wvel), &e,
+                                   nullptr, true);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    HIPCHK(c, hipMemcpyAsync(c->uw.p, c->wvel.p, sizeof(double) * 3 * N,
+                             hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ffm.p, P<double>(c->wvel) + 3 * (size_t)N,
+                             sizeof(double) * 3 * E, hipMemcpyDeviceToDevice, c->stream));
+    c->ffm_wake = c->wake_version;
+    c->ffm_wing = c->wing_version;
   } else {
     ChunkPlan plan;
     TRY(sum_sources(c, P<double>(c->cp), N, false, true, &plan));
@@ -1088,14 +1125,16 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
   TRY(stage_upload(c, c->st_area, c->ref_area.p, ref_area, sizeof(double) * c->num));
   ChunkPlan plan;
   if (ff_active(c)) {  // bound sheets exactly, the wake through the split
-    TRY(ff_current(c));
     TRY(sum_sources(c, P<double>(c->mid), E, true, false, &plan));
-    TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
-    std::string e;
-    const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E,
-                                   kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e, nullptr,
-                                   true);
-    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    if (c->ffm_wake != c->wake_version || c->ffm_wing != c->wing_version) {
+      TRY(ff_current(c));
+      TRY(ensure(c, c->ffm, sizeof(double) * 3 * (size_t)E));
+      std::string e;
+      const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E,
+                                     kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e,
+                                     nullptr, true);
+      if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    }
     launch_ff_combine(c->stream, P<double>(c->partial), E, plan.cb, P<double>(c->ffm), E, ff, 0,
                       0, P<double>(c->uflow));
   } else {
@@ -1167,11 +1206,38 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     focus.reach = c->ff_reach;
     focus.theta_min = c->ff_theta_min;
   }
+  // bound sheets as far-field sources for the wake's downstream nodes
+  const bool bound_split = use_ff && rows > kBoundRows;
+  if (bound_split) {
+    if (!c->ffb && !(c->ffb = new (std::nothrow) FarField()))
+      return fail(c, UVLM_ERR_OOM, "far-field state");
+    std::string e;
+    const int rc = c->ffb->build(c->stream, P<double>(c->wing), P<double>(c->gamma), c->num,
+                                 c->R, c->C, &e);
+    if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+    TRY(ensure(c, c->bnear, sizeof(double) * 3 * (size_t)c->num * kBoundRows * C1));
+    TRY(ensure(c, c->ffbnd, vb));
+  }
   auto ff_stage = [&](const double* targets, double* out) -> int {
     ChunkPlan plan;
-    TRY(sum_sources(c, targets, V, true, false, &plan));
     std::string e;
     int rc;
+    const double* bnd;  // taken after sum_sources, which may reallocate c->partial
+    int bcb;
+    if (bound_split) {
+      const int Vn = c->num * kBoundRows * C1;
+      launch_gather_rows(c->stream, targets, rows, 0, c->num, kBoundRows, C1, P<double>(c->bnear));
+      TRY(sum_sources(c, P<double>(c->bnear), Vn, true, false, &plan));
+      rc = c->ffb->root_velocity(c->stream, targets, V, rows * C1, C1, kBoundRows, kBoundTheta,
+                                 P<double>(c->partial), Vn, plan.cb, P<double>(c->ffbnd), &e);
+      if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+      bnd = P<double>(c->ffbnd);
+      bcb = 1;
+    } else {
+      TRY(sum_sources(c, targets, V, true, false, &plan));
+      bnd = P<double>(c->partial);
+      bcb = plan.cb;
+    }
     if (targets == P<double>(c->pos)) {  // the current wake: built for the RHS already
       TRY(ff_current(c));
     } else {  // an intermediate stage's positions
@@ -1182,8 +1248,7 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     }
     rc = c->ff->velocity(c->stream, targets, V, c->ff_theta, P<double>(c->ffv), &e, &focus, true);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
-    launch_ff_combine(c->stream, P<double>(c->partial), V, plan.cb, P<double>(c->ffv), V, ff,
-                      te_period, C1, out);
+    launch_ff_combine(c->stream, bnd, V, bcb, P<double>(c->ffv), V, ff, te_period, C1, out);
     return check_launch(c);
   };
   bool first = true;
@@ -1192,9 +1257,10 @@ int uvlm_sim_advect(uvlm_ctx* c, int scheme, size_t ignore_steps, const double f
     TRY(ff_stage(targets, P<double>(out)));
     if (audit && first) {
       first = false;
-      // Large wakes are audited on every kAuditStride-th node (the subset
-      // moves along with each audit) and keep the split's velocities.
-      const int stride = V >= kAuditFull ? kAuditStride : 1;
+      // Large wakes are audited on every stride-th node (at least
+      // kAuditStride, about kAuditNodes nodes; the subset moves along with
+      // each audit) and keep the split's velocities.
+      const int stride = V >= kAuditFull ? std::max(kAuditStride, V / kAuditNodes) : 1;
       if (stride == 1) {
         TRY(exact_stage(targets, P<double>(c->ffx)));
         launch_ff_maxdiff(c->stream, P<double>(out), P<double>(c->ffx), V,
@@ -1479,22 +1545,26 @@ int mlp_upload(uvlm_ctx* c, const float* w, const float* X, const float* V, size
     return fail(c, UVLM_ERR_OOM, "network state");
   TRY(ensure(c, c->mw, sizeof(float) * UVLM_MLP_PARAMS));
   TRY(ensure(c, c->mx, sizeof(float) * kMlpIn * std::max<size_t>(n, 1)));
-  TRY(ensure(c, c->my, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+  TRY(ensure(c, c->my, sizeof(float) * kMlpOut * std::max<size_t>(n, 1)));
   HIPCHK(c, hipMemcpyAsync(c->mw.p, w, sizeof(float) * UVLM_MLP_PARAMS, hipMemcpyHostToDevice,
                            c->stream));
   if (n)
     HIPCHK(c, hipMemcpyAsync(c->mx.p, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice,
                              c->stream));
   if (V) {
-    TRY(ensure(c, c->mv, sizeof(float) * 3 * std::max<size_t>(n, 1)));
+    TRY(ensure(c, c->mv, sizeof(float) * kMlpOut * std::max<size_t>(n, 1)));
     if (n)
-      HIPCHK(c, hipMemcpyAsync(c->mv.p, V, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+      HIPCHK(c, hipMemcpyAsync(c->mv.p, V, sizeof(float) * kMlpOut * n, hipMemcpyHostToDevice,
                                c->stream));
   }
   return UVLM_OK;
 }
 
 }  // namespace
+
+static_assert(UVLM_MLP_PARAMS == MlpShape::total && UVLM_MLP_IN == kMlpIn &&
+                  UVLM_MLP_OUT == kMlpOut,
+              "network layout of the C-ABI");
 
 int uvlm_farfield_set_mlp(uvlm_ctx* c, const float* weights, double ymax) {
   if (!c) return UVLM_ERR_ARG;
@@ -1531,7 +1601,7 @@ int copy_samples(uvlm_ctx* c, const std::vector<float>& f, const std::vector<flo
   const size_t k = std::min(*count, f.size() / kMlpIn);
   if (k) {
     std::memcpy(feats, f.data(), sizeof(float) * kMlpIn * k);
-    std::memcpy(resid, r.data(), sizeof(float) * 3 * k);
+    std::memcpy(resid, r.data(), sizeof(float) * kMlpOut * k);
   }
   *count = k;
   (void)c;
@@ -1582,11 +1652,25 @@ int sim_samples(uvlm_ctx* c, double theta, size_t want, unsigned seed, std::vect
                      P<double>(c->pos));
   launch_gather_rows_w(c->stream, P<double>(c->wgam), c->capg, c->gstart, c->num, c->wgrows,
                        c->C, P<double>(c->ffg));
+  // the lists of the advection call: graded criterion (when set) and the
+  // per-target far test
+  FFFocus focus;
+  if (c->ff_reach > 0.0) {
+    TRY(ensure(c, c->ffbox, 6 * sizeof(double)));
+    launch_ff_bbox(c->stream, P<double>(c->wing), c->num * (c->R + 1) * (c->C + 1),
+                   P<double>(c->ffbox));
+    focus.box = P<double>(c->ffbox);
+    focus.reach = c->ff_reach;
+    focus.theta_min = c->ff_theta_min;
+  }
   std::string e;
   c->ff_version = -1;
-  int rc = c->ff->wake_velocity(c->stream, P<double>(c->pos), P<double>(c->ffg), c->num,
-                                c->wgrows, c->C, P<double>(c->pos), V, theta,
-                                P<double>(c->ffv), &e);
+  c->ff->set_f32c(prec_of(c->flags) == kF32C);
+  int rc = c->ff->build(c->stream, P<double>(c->pos), P<double>(c->ffg), c->num, c->wgrows, c->C,
+                        &e);
+  if (rc == UVLM_OK)
+    rc = c->ff->velocity(c->stream, P<double>(c->pos), V, theta, P<double>(c->ffv), &e, &focus,
+                         true);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
   rc = c->ff->sample_residuals(c->stream, P<double>(c->pos), V, (int)want, seed, f, r, &e);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
@@ -1698,7 +1782,8 @@ int uvlm_mlp_forward(uvlm_ctx* c, const float* w, const float* feats, size_t n, 
                      P<float>(c->my));
     TRY(check_launch(c));
   }
-  HIPCHK(c, hipMemcpyAsync(y, c->my.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(y, c->my.p, sizeof(float) * kMlpOut * n, hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return UVLM_OK;
 }

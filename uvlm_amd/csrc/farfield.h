@@ -20,11 +20,29 @@ constexpr int kFFQ1 = 3, kFFQ2 = 12, kFFQ3 = 39;  // offsets of Q1, Q2, Q3
 // one sheet; level l merges 2 x 2 blocks of level l - 1.
 constexpr int kFFRows = 4;
 constexpr int kFFCols = 16;
-// MLP correction of the expansion: inputs, hidden width, hidden layers.
+// Network correction of the expansion, one evaluation per (target tile,
+// far cluster): inputs, hidden width, outputs.  The outputs are the
+// expansion's remainder about the tile centre c_T as a quadratic polynomial
+// in delta = (x - c_T) / (kMlpTaylorUnit D), D = |c_T - c|, in units of the
+// leading omitted term: value y[i] (3), gradient y[3 + 3i + j] (9), Hessian
+// y[12 + 6i + m] for (j, k) = 00, 01, 02, 11, 12, 22 (18).  The unit, a
+// typical tile radius over D, weighs the three parts of the training loss
+// by what they add over a tile.
 constexpr int kMlpIn = 32;
 constexpr int kMlpHidden = 128;
+constexpr int kMlpOut = 30;
 // Highest cluster level the network corrects (and is trained on).
 constexpr int kMlpMaxLevel = 2;
+// A (tile, cluster) pair is corrected when the tile's radius is at most
+// this fraction of D (the quadratic then represents the remainder over the
+// tile) and c_T lies outside the cluster's sphere.
+constexpr double kMlpTileRatio = 0.3;
+// Far-list entries of a tile that can carry a correction (the first ones).
+constexpr int kNetCap = 256;
+// Offset of the finite differences that make the training targets, in
+// units of D.
+constexpr double kMlpFdStep = 0.02;
+constexpr double kMlpTaylorUnit = 0.15;
 
 struct alignas(16) FFCluster {
   double c[3];  // expansion centre
@@ -41,12 +59,12 @@ struct FFTree {
 };
 
 // Weights of the correction network: h1 = silu(W1 x + b1), h2, h3 likewise
-// (W2, W3: 128 x 128), y = W4 h3 + b4 (3 outputs).  FP32 master copy; the
-// inference kernel reads BF16-rounded weights.
+// (W2, W3: 128 x 128), y = W4 h3 + b4 (kMlpOut outputs).  FP32 master copy;
+// the inference kernel reads BF16-rounded weights.
 struct MlpShape {
   static constexpr int n_w1 = kMlpHidden * kMlpIn;
   static constexpr int n_w2 = kMlpHidden * kMlpHidden;
-  static constexpr int n_w4 = 3 * kMlpHidden;
+  static constexpr int n_w4 = kMlpOut * kMlpHidden;
   static constexpr int off_w1 = 0;
   static constexpr int off_b1 = off_w1 + n_w1;
   static constexpr int off_w2 = off_b1 + kMlpHidden;
@@ -55,7 +73,7 @@ struct MlpShape {
   static constexpr int off_b3 = off_w3 + n_w2;
   static constexpr int off_w4 = off_b3 + kMlpHidden;
   static constexpr int off_b4 = off_w4 + n_w4;
-  static constexpr int total = off_b4 + 4;
+  static constexpr int total = off_b4 + kMlpOut + 2;  // two pad words
 };
 
 // Graded criterion for targets away from the bodies.  The focus is the
@@ -106,15 +124,24 @@ class FarField {
   // level-0 blocks of the others are summed exactly.  out [nt][3] overwritten.
   // focus (optional): the graded criterion of FFFocus.  per_target: a
   // cluster is also far when each target (not the tile's sphere) clears
-  // theta times its radius; not with the network, whose correction keeps to
-  // the sphere's pairs (on the others it lowered the mean error but raised
-  // the largest, measured at theta 2.5 on a rolled-up tandem wake).
+  // theta times its radius.  With the network, k_ff_tilenet evaluates it
+  // once per (tile, far cluster) and the far kernel adds the corrections.
   int velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
                std::string* err, const FFFocus* focus = nullptr, bool per_target = false);
   // build + velocity.
   int wake_velocity(hipStream_t s, const double* nodes, const double* gamma, int num, int rows,
                     int cols, const double* targets, int nt, double theta, double* out,
                     std::string* err);
+  // Velocity of the built sources (each sheet's root cluster) at nt points
+  // laid out as sheets of per_sheet points in rows of width points, SoA
+  // out [3][nt]: points in the first near_rows rows of each sheet take the
+  // sum of the cb SoA chunks of near_part (a compact copy of those rows,
+  // [num][near_rows][width], chunk stride near_ld); the others the root
+  // clusters' moment expansions where their distance exceeds theta times
+  // the radius, else the sheets' exact sums.
+  int root_velocity(hipStream_t s, const double* x, int nt, int per_sheet, int width,
+                    int near_rows, double theta, const double* near_part, int near_ld, int cb,
+                    double* out, std::string* err);
   // Level-0 blocks summed exactly and clusters expanded, summed over the
   // tiles of the last velocity call (reads the list counts back).
   int list_stats(hipStream_t s, double* near_blocks, double* far_clusters, std::string* err);
@@ -137,9 +164,10 @@ class FarField {
   int mlp_stats(hipStream_t s, unsigned long long* pairs, unsigned long long* trips,
                 std::string* err);
   // Training samples from the interaction lists of the last velocity call
-  // (same targets): up to max_samples far (target, cluster) pairs of
-  // clusters up to kMlpMaxLevel, drawn with a seeded generator; features
-  // [k][kMlpIn] and normalised remainders [k][3] (exact - expansion) to host.
+  // (same targets): up to max_samples (tile, far cluster) pairs that the
+  // network corrects, drawn with a seeded generator; features [k][kMlpIn]
+  // at the tile centre and the remainder's Taylor coefficients [k][kMlpOut]
+  // (exact - expansion, normalised) to host.
   int sample_residuals(hipStream_t s, const double* targets, int nt, int max_samples,
                        unsigned seed, std::vector<float>* feats, std::vector<float>* resid,
                        std::string* err);
@@ -157,11 +185,11 @@ class FarField {
        *d_cl_ = nullptr, *d_mom_ = nullptr, *d_map_ = nullptr, *d_tiles_ = nullptr,
        *d_near_ = nullptr, *d_far_ = nullptr, *d_cnt_ = nullptr, *d_front_ = nullptr,
        *d_part_ = nullptr, *d_w_ = nullptr, *d_wf_ = nullptr, *d_mstat_ = nullptr,
-       *d_samp_ = nullptr;
+       *d_samp_ = nullptr, *d_corr_ = nullptr, *d_tr_ = nullptr, *d_nrow_ = nullptr;
   size_t c_rec_ = 0, c_key_ = 0, c_key2_ = 0, c_idx_ = 0, c_idx2_ = 0,
          c_sort_tmp_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
          c_near_ = 0, c_far_ = 0, c_cnt_ = 0, c_front_ = 0, c_part_ = 0, c_w_ = 0, c_wf_ = 0,
-         c_mstat_ = 0, c_samp_ = 0;
+         c_mstat_ = 0, c_samp_ = 0, c_corr_ = 0, c_tr_ = 0, c_nrow_ = 0;
   // near-sum measurement
   bool prof_ = false, np_live_ = false;
   void* d_np_ = nullptr;

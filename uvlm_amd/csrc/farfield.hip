@@ -726,6 +726,16 @@ __device__ inline void moment_traces(const double* Q, double tr[12]) {
   }
 }
 
+// Traces of every cluster's moments (12 per cluster, one thread each),
+// once per build: the far kernel reads them with the moments.
+__global__ void k_ff_traces(const double* __restrict__ mom, int total, double* __restrict__ tr) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= total) return;
+  double t[12];
+  moment_traces(mom + (long)id * kFFMom, t);
+  for (int k = 0; k < 12; ++k) tr[(long)id * 12 + k] = t[k];
+}
+
 // u (without 1/4pi) of the moments Q at d = x - c, orders 0..3.  With the
 // position indices of Q2, Q3 symmetric, the derivative tensors of 1/r
 // contract to
@@ -819,68 +829,199 @@ __device__ double ff_features(const double* Q, double dx, double dy, double dz, 
   return s * rho * rho * rho * rho * kInv4Pi / (D * D);
 }
 
+// Taylor polynomial of a correction record at delta = (x - c_T) /
+// (kMlpTaylorUnit D) (record: the 30 outputs times the scale, then
+// 1 / (kMlpTaylorUnit D), status), FP32.
+__device__ __forceinline__ void tile_correction(const float* __restrict__ rec, float dx, float dy,
+                                                float dz, float v[3]) {
+  const float q[6] = {0.5f * dx * dx, dx * dy, dx * dz, 0.5f * dy * dy, dy * dz, 0.5f * dz * dz};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float a = rec[i];
+    a = fmaf(rec[3 + 3 * i], dx, a);
+    a = fmaf(rec[4 + 3 * i], dy, a);
+    a = fmaf(rec[5 + 3 * i], dz, a);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) a = fmaf(rec[12 + 6 * i + m], q[m], a);
+    v[i] = a;
+  }
+}
+
+// Whether the network corrects the pair (tile centre, cluster): a cluster
+// up to kMlpMaxLevel, c_T outside its sphere and the tile small against
+// the distance D.  Returns D (0 when not corrected).
+__device__ __forceinline__ double tile_pair_distance(const double4& tc, const FFCluster& c,
+                                                     double* dx, double* dy, double* dz) {
+  *dx = tc.x - c.c[0];
+  *dy = tc.y - c.c[1];
+  *dz = tc.z - c.c[2];
+  const double D = sqrt(*dx * *dx + *dy * *dy + *dz * *dz);
+  const bool ok = c.level <= kMlpMaxLevel && D > c.r && tc.w <= kMlpTileRatio * D;
+  return ok ? D : 0.0;
+}
+
+// Pairs the network corrects, gathered: one thread per (tile, far-list
+// entry e < kNetCap) decides (tile_pair_distance, circulation); an eligible
+// pair takes a row of the compact batch (atomic counter: the row order
+// varies from run to run, the records do not) with its features at the
+// tile centre, its scale s rho^4 / (4pi D^2) and 1 / (kMlpTaylorUnit D);
+// every other entry's correction record is marked 0 (none).
+struct NetRow {
+  int tile, e;
+  float scale, inv;
+};
+
+__global__ __launch_bounds__(64) void k_ff_netrows(FFTree t, const FFCluster* __restrict__ cl,
+                                                   const double* __restrict__ mom,
+                                                   const double4* __restrict__ tiles,
+                                                   const int* __restrict__ far,
+                                                   const int2* __restrict__ cnt,
+                                                   int* __restrict__ nrows,
+                                                   NetRow* __restrict__ rows,
+                                                   float* __restrict__ feats,
+                                                   float* __restrict__ corr) {
+  const int tile = blockIdx.x, e = blockIdx.y * 64 + threadIdx.x;
+  const int count = min(cnt[tile].y, kNetCap);
+  if (e >= count) return;
+  const int id = far[(long)tile * t.total + e];
+  const FFCluster c = cl[id];
+  const double4 tc = tiles[tile];
+  double dx, dy, dz;
+  const double D = tile_pair_distance(tc, c, &dx, &dy, &dz);
+  float f[kMlpIn];
+  const double scale = D > 0.0 ? ff_features(mom + (long)id * kFFMom, dx, dy, dz, c.r, f) : 0.0;
+  if (!(scale > 0.0)) {
+    corr[((long)tile * kNetCap + e) * 32 + 31] = 0.0f;
+    return;
+  }
+  const int r = atomicAdd(nrows, 1);
+  rows[r] = NetRow{tile, e, (float)scale, (float)(1.0 / (kMlpTaylorUnit * D))};
+  for (int k = 0; k < kMlpIn; ++k) feats[(long)r * kMlpIn + k] = f[k];
+}
+
+// The network on the gathered rows, one wavefront per batch of 64 rows
+// (batches b = blockIdx.x, blockIdx.x + gridDim.x, ... below *nrows), on
+// the matrix cores; per row a correction record of 32 floats at its (tile,
+// entry): the outputs times the scale, 1 / (kMlpTaylorUnit D), and the
+// status (1 corrected, -1 an output beyond ymax: the far kernel sums the
+// cluster exactly).
+__global__ __launch_bounds__(64) void k_ff_tilenet(const int* __restrict__ nrows,
+                                                   const NetRow* __restrict__ rows,
+                                                   const float* __restrict__ feats, MlpDev net,
+                                                   float ymax, float* __restrict__ corr) {
+  __shared__ unsigned short lds[2 * kMlpLdsHalfs];
+  __shared__ float ylds[64 * kMlpYStride];
+  const int lane = threadIdx.x;
+  const int n = *nrows;
+  for (int b = blockIdx.x; 64 * b < n; b += gridDim.x) {
+    const int r = 64 * b + lane;
+    float f[kMlpIn];
+    for (int k = 0; k < kMlpIn; ++k) f[k] = r < n ? feats[(long)r * kMlpIn + k] : 0.0f;
+    mlp_wave_bf16(net, f, lds, ylds);
+    if (r < n) {
+      const NetRow q = rows[r];
+      const float* y = ylds + lane * kMlpYStride;
+      float ym = 0.0f;
+      for (int o = 0; o < kMlpOut; ++o) ym = fmaxf(ym, fabsf(y[o]));
+      const bool ok = ym <= ymax;
+      float* rec = corr + ((long)q.tile * kNetCap + q.e) * 32;
+      for (int o = 0; o < kMlpOut; ++o) rec[o] = ok ? y[o] * q.scale : 0.0f;
+      rec[30] = q.inv;
+      rec[31] = ok ? 1.0f : -1.0f;
+    }
+  }
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
 // Far clusters of a tile, group g: moment expansion; with the network
-// (MLP = true) clusters up to kMlpMaxLevel get its correction, and a pair
-// whose output leaves the trained range (|y| > ymax) is summed exactly.
+// (MLP = true) the first kNetCap entries add their correction records
+// (k_ff_tilenet), and an entry whose network output left the trained range
+// is summed exactly.  The group's clusters k = g, g + kGroups, ... are
+// staged kFarBatch at a time (moments, traces, centre) between two
+// barriers and expanded in order.
+constexpr int kFarBatch = 4;
 template <bool MLP>
 __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __restrict__ cl,
                                                const double* __restrict__ mom,
+                                               const double* __restrict__ trs,
                                                const double* __restrict__ rec,
                                                const double* __restrict__ x, int nt,
                                                const int* __restrict__ map,
                                                const int* __restrict__ far,
-                                               const int2* __restrict__ cnt, MlpDev net,
-                                               float ymax, unsigned long long* __restrict__ stats,
+                                               const int2* __restrict__ cnt,
+                                               const double4* __restrict__ tiles,
+                                               const float* __restrict__ corr,
+                                               unsigned long long* __restrict__ stats,
                                                double* __restrict__ part) {
-  __shared__ double sq[kFFMom];
-  __shared__ double sc[5], str_[12];
-  __shared__ unsigned short lds[MLP ? 2 * kMlpLdsHalfs : 1];
-  __shared__ float ylds[MLP ? 64 * 4 : 1];
+  constexpr int kStage = kFFMom + 12 + 4;  // moments, traces, centre and radius
+  __shared__ double sq[kFarBatch][kStage];
+  __shared__ int sid[kFarBatch];
   const int tile = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
   const int tg = tile_target(map, tile, lane);
   const double px = tg >= 0 ? x[3 * (long)tg] : 0.0, py = tg >= 0 ? x[3 * (long)tg + 1] : 0.0,
                pz = tg >= 0 ? x[3 * (long)tg + 2] : 0.0;
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;  // x - c_T
+  if (MLP) {
+    const double4 tc = tiles[tile];
+    ox = (float)(px - tc.x);
+    oy = (float)(py - tc.y);
+    oz = (float)(pz - tc.z);
+  }
   const int count = cnt[tile].y;
   const int* list = far + (long)tile * t.total;
   double ux = 0.0, uy = 0.0, uz = 0.0;
   unsigned npairs = 0, ntrips = 0;
-  for (int k = g; k < count; k += kGroups) {
-    const int id = list[k];
+  for (int k0 = g; k0 < count; k0 += kFarBatch * kGroups) {
+    const int nb = min(kFarBatch, (count - k0 + kGroups - 1) / kGroups);
     __syncthreads();
-    for (int m = lane; m < kFFMom; m += 64) sq[m] = mom[(long)id * kFFMom + m];
-    if (lane < 3) sc[lane] = cl[id].c[lane];
-    if (lane == 3) sc[3] = cl[id].r;
-    if (lane == 4) sc[4] = (double)cl[id].level;
+    for (int q = lane; q < nb * kStage; q += 64) {
+      const int j = q / kStage, m = q % kStage;
+      const int id = list[k0 + j * kGroups];
+      double v;
+      if (m < kFFMom)
+        v = mom[(long)id * kFFMom + m];
+      else if (m < kFFMom + 12)
+        v = trs[(long)id * 12 + m - kFFMom];
+      else if (m < kFFMom + 15)
+        v = cl[id].c[m - kFFMom - 12];
+      else
+        v = cl[id].r;
+      sq[j][m] = v;
+      if (m == 0) sid[j] = id;
+    }
     __syncthreads();
-    if (lane == 0) moment_traces(sq, str_);
-    __syncthreads();
-    const double dx = px - sc[0], dy = py - sc[1], dz = pz - sc[2];
-    double u[3];
-    expand_moments(sq, str_, dx, dy, dz, u);
-    u[0] *= kInv4Pi;
-    u[1] *= kInv4Pi;
-    u[2] *= kInv4Pi;
-    if (MLP && sc[4] <= (double)kMlpMaxLevel) {
-      float f[kMlpIn];
-      const double scale = ff_features(sq, dx, dy, dz, sc[3], f);
-      float y[3];
-      mlp_wave_bf16(net, f, lds, ylds, y);
-      const float ym = fmaxf(fabsf(y[0]), fmaxf(fabsf(y[1]), fabsf(y[2])));
-      if (tg >= 0 && scale > 0.0) {
-        ++npairs;
-        if (ym <= ymax) {
-          u[0] += (double)y[0] * scale;
-          u[1] += (double)y[1] * scale;
-          u[2] += (double)y[2] * scale;
-        } else {  // outside the trained range: the exact sum of the cluster
-          ++ntrips;
-          exact_cluster(t, cl[id], rec, px, py, pz, u);
+    for (int j = 0; j < nb; ++j) {
+      const int k = k0 + j * kGroups;
+      const double* Q = sq[j];
+      const double dx = px - Q[kFFMom + 12], dy = py - Q[kFFMom + 13], dz = pz - Q[kFFMom + 14];
+      double u[3];
+      expand_moments(Q, Q + kFFMom, dx, dy, dz, u);
+      u[0] *= kInv4Pi;
+      u[1] *= kInv4Pi;
+      u[2] *= kInv4Pi;
+      if (MLP && k < kNetCap) {
+        const float* r = corr + ((long)tile * kNetCap + k) * 32;
+        const float status = r[31];
+        if (status != 0.0f && tg >= 0) {
+          ++npairs;
+          if (status > 0.0f) {
+            const float inv = r[30];
+            float v[3];
+            tile_correction(r, ox * inv, oy * inv, oz * inv, v);
+            u[0] += (double)v[0];
+            u[1] += (double)v[1];
+            u[2] += (double)v[2];
+          } else {  // outside the trained range: the exact sum of the cluster
+            ++ntrips;
+            exact_cluster(t, cl[sid[j]], rec, px, py, pz, u);
+          }
         }
       }
+      ux += u[0];
+      uy += u[1];
+      uz += u[2];
     }
-    ux += u[0];
-    uy += u[1];
-    uz += u[2];
   }
   if (tg >= 0) {
     part[((long)(kNearGroups + g) * 3 + 0) * nt + tg] = ux;
@@ -899,37 +1040,129 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
   }
 }
 
-// Training samples: pair q = (tile, far-list entry, lane); row q of feats
-// and resid (normalised remainder of the expansion, exact - expansion, in
-// units of s rho^4 / (4pi D^2)); valid[q] = 0 for an empty lane, a cluster
-// above kMlpMaxLevel or one without circulation.
+// Remainder of a cluster's expansion at x, exact - expansion, over scale.
+__device__ void remainder_at(const FFTree& t, const FFCluster& c, const double* Q,
+                             const double* tr, const double* __restrict__ rec, double x, double y,
+                             double z, double scale, double out[3]) {
+  double ue[3], ux[3];
+  expand_moments(Q, tr, x - c.c[0], y - c.c[1], z - c.c[2], ue);
+  exact_cluster(t, c, rec, x, y, z, ux);
+  for (int d = 0; d < 3; ++d) out[d] = (ux[d] - ue[d] * kInv4Pi) / scale;
+}
+
+// Training samples: sample q = (tile, far-list entry); row q of feats (at
+// the tile centre c_T) and of labels: the remainder's Taylor coefficients
+// about c_T by central differences of step h = kMlpFdStep D over the 19
+// points 0, +-e_j, +-e_j +-e_k (offsets in units of D):
+//   value R(0); gradient (R(e_j) - R(-e_j)) / 2h;
+//   Hessian jj (R(e_j) - 2 R(0) + R(-e_j)) / h^2,
+//   jk (R(e_j + e_k) - R(e_j - e_k) - R(-e_j + e_k) + R(-e_j - e_k)) / 4h^2,
+// gradient and Hessian then in the polynomial's unit kMlpTaylorUnit D
+// (times kMlpTaylorUnit and its square).
+// valid[q] = 0 for an entry the network does not correct (tile_pair_distance)
+// or a cluster without circulation.
 __global__ void k_ff_samples(FFTree t, const FFCluster* __restrict__ cl,
                              const double* __restrict__ mom, const double* __restrict__ rec,
-                             const double* __restrict__ x, const int* __restrict__ map,
-                             const int* __restrict__ far, const int4* __restrict__ pick, int n,
-                             float* __restrict__ feats, float* __restrict__ resid,
-                             int* __restrict__ valid) {
+                             const double4* __restrict__ tiles, const int* __restrict__ far,
+                             const int4* __restrict__ pick, int n, float* __restrict__ feats,
+                             float* __restrict__ labels, int* __restrict__ valid) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int4 pk = pick[q];
-  const int tg = tile_target(map, pk.x, pk.z);
   const int id = far[(long)pk.x * t.total + pk.y];
   const FFCluster c = cl[id];
+  const double4 tc = tiles[pk.x];
   valid[q] = 0;
-  if (tg < 0 || c.level > kMlpMaxLevel) return;
-  const double px = x[3 * (long)tg], py = x[3 * (long)tg + 1], pz = x[3 * (long)tg + 2];
+  double dx, dy, dz;
+  const double D = tile_pair_distance(tc, c, &dx, &dy, &dz);
+  if (!(D > 0.0)) return;
   const double* Q = mom + (long)id * kFFMom;
-  const double dx = px - c.c[0], dy = py - c.c[1], dz = pz - c.c[2];
   float f[kMlpIn];
   const double scale = ff_features(Q, dx, dy, dz, c.r, f);
   if (!(scale > 0.0)) return;
-  double ue[3], ux[3], tr[12];
+  double tr[12];
   moment_traces(Q, tr);
-  expand_moments(Q, tr, dx, dy, dz, ue);
-  exact_cluster(t, c, rec, px, py, pz, ux);
+  const double hD = kMlpFdStep * D, h = kMlpFdStep;
+  const double u1 = kMlpTaylorUnit, u2 = kMlpTaylorUnit * kMlpTaylorUnit;
+  double R0[3];
+  remainder_at(t, c, Q, tr, rec, tc.x, tc.y, tc.z, scale, R0);
+  float* L = labels + (long)q * kMlpOut;
+  for (int i = 0; i < 3; ++i) L[i] = (float)R0[i];
+  double Rp[3][3], Rm[3][3];  // R(+h e_j), R(-h e_j)
+  for (int j = 0; j < 3; ++j) {
+    const double ej[3] = {j == 0 ? hD : 0.0, j == 1 ? hD : 0.0, j == 2 ? hD : 0.0};
+    remainder_at(t, c, Q, tr, rec, tc.x + ej[0], tc.y + ej[1], tc.z + ej[2], scale, Rp[j]);
+    remainder_at(t, c, Q, tr, rec, tc.x - ej[0], tc.y - ej[1], tc.z - ej[2], scale, Rm[j]);
+    for (int i = 0; i < 3; ++i) L[3 + 3 * i + j] = (float)(u1 * (Rp[j][i] - Rm[j][i]) / (2.0 * h));
+  }
+  const int pj[6] = {0, 0, 0, 1, 1, 2}, pk2[6] = {0, 1, 2, 1, 2, 2};
+  for (int m = 0; m < 6; ++m) {
+    const int j = pj[m], k = pk2[m];
+    if (j == k) {
+      for (int i = 0; i < 3; ++i)
+        L[12 + 6 * i + m] = (float)(u2 * (Rp[j][i] - 2.0 * R0[i] + Rm[j][i]) / (h * h));
+    } else {
+      double o[3] = {0.0, 0.0, 0.0}, a[3], b[3], cc[3], d[3];
+      o[j] = hD;
+      o[k] = hD;
+      remainder_at(t, c, Q, tr, rec, tc.x + o[0], tc.y + o[1], tc.z + o[2], scale, a);
+      remainder_at(t, c, Q, tr, rec, tc.x - o[0], tc.y - o[1], tc.z - o[2], scale, d);
+      o[k] = -hD;
+      remainder_at(t, c, Q, tr, rec, tc.x + o[0], tc.y + o[1], tc.z + o[2], scale, b);
+      remainder_at(t, c, Q, tr, rec, tc.x - o[0], tc.y - o[1], tc.z - o[2], scale, cc);
+      // a = R(e_j + e_k), b = R(e_j - e_k), cc = R(-e_j + e_k), d = R(-e_j - e_k)
+      for (int i = 0; i < 3; ++i)
+        L[12 + 6 * i + m] = (float)(u2 * (a[i] - b[i] - cc[i] + d[i]) / (4.0 * h * h));
+    }
+  }
   for (int k = 0; k < kMlpIn; ++k) feats[(long)q * kMlpIn + k] = f[k];
-  for (int d = 0; d < 3; ++d) resid[(long)q * 3 + d] = (float)((ux[d] - ue[d] * kInv4Pi) / scale);
   valid[q] = 1;
+}
+
+// Bound-sheet velocity at the wake's nodes (sheets of per_sheet points,
+// rows of width points): the first near_rows rows of each sheet take the
+// exact chain sums of a compact copy ([num][near_rows][width], cb SoA
+// chunks of near_ld, combined in chunk order); the others the moment
+// expansion of each source sheet's root cluster where their distance from
+// its centre exceeds theta times its radius, else that sheet's exact sum.
+// out: SoA [3][nt].
+__global__ void k_bound_far(FFTree t, const FFCluster* __restrict__ cl,
+                            const double* __restrict__ mom, const double* __restrict__ trs,
+                            const double* __restrict__ rec, const double* __restrict__ x, int nt,
+                            int per_sheet, int width, int near_rows, double theta,
+                            const double* __restrict__ near_part, int near_ld, int cb,
+                            double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nt) return;
+  const int n = p / per_sheet, q = p % per_sheet;
+  double u[3] = {0.0, 0.0, 0.0};
+  if (q / width < near_rows) {
+    const long k = (long)n * near_rows * width + q;
+    for (int d = 0; d < 3; ++d) {
+      double b = near_part[(long)d * near_ld + k];
+      for (int c = 1; c < cb; ++c) b = b + near_part[((long)c * 3 + d) * near_ld + k];
+      u[d] = b;
+    }
+  } else {
+    const double px = x[3 * (long)p], py = x[3 * (long)p + 1], pz = x[3 * (long)p + 2];
+    const int L = t.levels - 1;
+    for (int w = 0; w < t.num; ++w) {
+      const int id = ff_id(t, L, w, 0, 0);
+      const FFCluster c = cl[id];
+      const double dx = px - c.c[0], dy = py - c.c[1], dz = pz - c.c[2];
+      double v[3];
+      if (dx * dx + dy * dy + dz * dz > theta * theta * c.r * c.r) {
+        expand_moments(mom + (long)id * kFFMom, trs + (long)id * 12, dx, dy, dz, v);
+        for (int d = 0; d < 3; ++d) v[d] *= kInv4Pi;
+      } else {
+        exact_cluster(t, c, rec, px, py, pz, v);
+      }
+      u[0] += v[0];
+      u[1] += v[1];
+      u[2] += v[2];
+    }
+  }
+  for (int d = 0; d < 3; ++d) out[(long)d * nt + p] = u[d];
 }
 
 // out[t] = near groups (in order) + far groups (in order).
@@ -1068,7 +1301,8 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 FarField::~FarField() {
   void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_sort_tmp_,
                   d_box_,  d_cl_,    d_mom_,  d_map_,  d_tiles_, d_near_, d_far_,
-                  d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_, d_np_};
+                  d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_, d_np_, d_corr_,
+                  d_tr_,   d_nrow_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& e : ev_used_) {
@@ -1187,6 +1421,7 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
   if ((rc = ensure(&d_box_, &c_box_, sizeof(double) * 8, err)) ||
       (rc = ensure(&d_cl_, &c_cl_, sizeof(FFCluster) * (size_t)t.total, err)) ||
       (rc = ensure(&d_mom_, &c_mom_, sizeof(double) * kFFMom * (size_t)t.total, err)) ||
+      (rc = ensure(&d_tr_, &c_tr_, sizeof(double) * 12 * (size_t)t.total, err)) ||
       (rc = ensure(&d_rec_, &c_rec_, sizeof(double) * kRec * (size_t)n0, err)))
     return rc;
   hipLaunchKernelGGL(k_ff_level0, dim3(n0), dim3(kMomThreads), 0, s, t, nodes, gamma,
@@ -1194,6 +1429,8 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
   for (int lv = 1; lv < t.levels; ++lv)
     hipLaunchKernelGGL(k_ff_parent, dim3(num * t.nbr[lv] * t.nbc[lv]), dim3(kMomThreads), 0, s,
                        t, lv, (FFCluster*)d_cl_, (double*)d_mom_);
+  hipLaunchKernelGGL(k_ff_traces, dim3(cdiv(t.total, 256)), dim3(256), 0, s,
+                     (const double*)d_mom_, t.total, (double*)d_tr_);
   if (hipGetLastError() != hipSuccess) {
     *err = "far-field build launch failed";
     return UVLM_ERR_HIP;
@@ -1240,8 +1477,7 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      (double4*)d_tiles_);
   const bool graded = focus && focus->box && focus->reach > 0.0;
   hipLaunchKernelGGL(k_ff_lists, dim3(ntiles), dim3(64), 0, s, t, (const FFCluster*)d_cl_,
-                     (const double4*)d_tiles_, targets, (const int*)map, per_target && !mlp_on_,
-                     theta,
+                     (const double4*)d_tiles_, targets, (const int*)map, per_target, theta,
                      graded ? focus->box : nullptr,
                      graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
                      (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
@@ -1284,16 +1520,35 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
     ev_used_.push_back(ev);
   }
   if (mlp_on_) {
+    const size_t cap = (size_t)ntiles * kNetCap;
+    if ((rc = ensure(&d_corr_, &c_corr_, sizeof(float) * 32 * cap, err)) ||
+        (rc = ensure(&d_nrow_, &c_nrow_, 256 + (sizeof(NetRow) + sizeof(float) * kMlpIn) * cap,
+                     err)))
+      return rc;
+    int* nrows = (int*)d_nrow_;
+    NetRow* rows = (NetRow*)((char*)d_nrow_ + 256);
+    float* feats = (float*)(rows + cap);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, nrows);
+    hipLaunchKernelGGL(k_ff_netrows, dim3(ntiles, kNetCap / 64), dim3(64), 0, s, t,
+                       (const FFCluster*)d_cl_, (const double*)d_mom_,
+                       (const double4*)d_tiles_, (const int*)d_far_, (const int2*)d_cnt_, nrows,
+                       rows, feats, (float*)d_corr_);
+    hipLaunchKernelGGL(k_ff_tilenet, dim3(std::min<size_t>(cdiv((long)cap, 64), 2048)), dim3(64),
+                       0, s, (const int*)nrows, (const NetRow*)rows, (const float*)feats,
+                       mlp_dev_view(d_w_), ymax_, (float*)d_corr_);
     hipLaunchKernelGGL(k_ff_far<true>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
-                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
+                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_tr_,
+                       (const double*)d_rec_,
                        targets, nt, (const int*)map, (const int*)d_far_, (const int2*)d_cnt_,
-                       mlp_dev_view(d_w_), ymax_, (unsigned long long*)d_mstat_,
-                       (double*)d_part_);
+                       (const double4*)d_tiles_, (const float*)d_corr_,
+                       (unsigned long long*)d_mstat_, (double*)d_part_);
   } else {
     hipLaunchKernelGGL(k_ff_far<false>, dim3(ntiles, kGroups), dim3(64), 0, s, t,
-                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
+                       (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_tr_,
+                       (const double*)d_rec_,
                        targets, nt, (const int*)map, (const int*)d_far_, (const int2*)d_cnt_,
-                       MlpDev{}, 0.0f, (unsigned long long*)nullptr, (double*)d_part_);
+                       (const double4*)nullptr, (const float*)nullptr,
+                       (unsigned long long*)nullptr, (double*)d_part_);
   }
   hipLaunchKernelGGL(k_ff_sum, dim3(cdiv(nt, 256)), dim3(256), 0, s, (const double*)d_part_, nt,
                      out);
@@ -1320,6 +1575,25 @@ int FarField::wake_velocity(hipStream_t s, const double* nodes, const double* ga
   int rc = build(s, nodes, gamma, num, rows, cols, err);
   if (rc != UVLM_OK) return rc;
   return velocity(s, targets, nt, theta, out, err);
+}
+
+int FarField::root_velocity(hipStream_t s, const double* x, int nt, int per_sheet, int width,
+                            int near_rows, double theta, const double* near_part, int near_ld,
+                            int cb, double* out, std::string* err) {
+  if (nt <= 0) return UVLM_OK;
+  if (!built_) {
+    *err = "far field: no sources built";
+    return UVLM_ERR_STATE;
+  }
+  hipLaunchKernelGGL(k_bound_far, dim3(cdiv(nt, 256)), dim3(256), 0, s, tree_,
+                     (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_tr_,
+                     (const double*)d_rec_, x, nt, per_sheet, width, near_rows, theta, near_part,
+                     near_ld, cb, out);
+  if (hipGetLastError() != hipSuccess) {
+    *err = "bound far-field launch failed";
+    return UVLM_ERR_HIP;
+  }
+  return UVLM_OK;
 }
 
 int FarField::list_stats(hipStream_t s, double* near_leaves, double* far_clusters,
@@ -1415,17 +1689,17 @@ int FarField::sample_residuals(hipStream_t s, const double* targets, int nt, int
   for (int q = 0; q < max_samples; ++q) {
     const long e = (long)(next() % (unsigned long long)pre[ntiles]);
     const int tile = (int)(std::upper_bound(pre.begin(), pre.end(), e) - pre.begin()) - 1;
-    pick[q] = int4{tile, (int)(e - pre[tile]), (int)(next() % 64), 0};
+    pick[q] = int4{tile, (int)(e - pre[tile]), 0, 0};
   }
   int rc;
   const size_t sb = (size_t)max_samples;
-  if ((rc = ensure(&d_samp_, &c_samp_, sb * (sizeof(int4) + sizeof(float) * (kMlpIn + 3) +
+  if ((rc = ensure(&d_samp_, &c_samp_, sb * (sizeof(int4) + sizeof(float) * (kMlpIn + kMlpOut) +
                                              sizeof(int)) + 256, err)))
     return rc;
   int4* dpick = (int4*)d_samp_;
   float* df = (float*)(dpick + sb);
   float* dr = df + sb * kMlpIn;
-  int* dv = (int*)(dr + sb * 3);
+  int* dv = (int*)(dr + sb * kMlpOut);
   if (hipMemcpyAsync(dpick, pick.data(), sizeof(int4) * sb, hipMemcpyHostToDevice, s) !=
       hipSuccess) {
     *err = "sample upload failed";
@@ -1433,9 +1707,9 @@ int FarField::sample_residuals(hipStream_t s, const double* targets, int nt, int
   }
   hipLaunchKernelGGL(k_ff_samples, dim3(cdiv(max_samples, 64)), dim3(64), 0, s, tree_,
                      (const FFCluster*)d_cl_, (const double*)d_mom_, (const double*)d_rec_,
-                     targets, (const int*)d_map_, (const int*)d_far_, (const int4*)dpick,
+                     (const double4*)d_tiles_, (const int*)d_far_, (const int4*)dpick,
                      max_samples, df, dr, dv);
-  std::vector<float> hf(sb * kMlpIn), hr(sb * 3);
+  std::vector<float> hf(sb * kMlpIn), hr(sb * kMlpOut);
   std::vector<int> hv(sb);
   if (hipMemcpyAsync(hf.data(), df, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, s) !=
           hipSuccess ||
@@ -1450,7 +1724,7 @@ int FarField::sample_residuals(hipStream_t s, const double* targets, int nt, int
   for (size_t q = 0; q < sb; ++q)
     if (hv[q]) {
       feats->insert(feats->end(), hf.begin() + q * kMlpIn, hf.begin() + (q + 1) * kMlpIn);
-      resid->insert(resid->end(), hr.begin() + q * 3, hr.begin() + (q + 1) * 3);
+      resid->insert(resid->end(), hr.begin() + q * kMlpOut, hr.begin() + (q + 1) * kMlpOut);
     }
   return UVLM_OK;
 }

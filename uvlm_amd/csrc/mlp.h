@@ -17,13 +17,13 @@ size_t mlp_dev_bytes();
 MlpDev mlp_dev_view(const void* base);
 // FP32 parameters (MlpShape layout, device) -> inference copy (device).
 void launch_mlp_to_dev(hipStream_t s, const float* w, void* dev);
-// BF16 inference of n samples (features [n][32] -> y [n][3]), device arrays.
+// BF16 inference of n samples (features [n][32] -> y [n][kMlpOut]), device arrays.
 void launch_mlp_infer(hipStream_t s, MlpDev net, const float* X, int n, float* Y);
 
 class MlpTrainer {
  public:
   ~MlpTrainer();
-  // FP32 forward of n samples: Y [n][3] (device arrays).
+  // FP32 forward of n samples: Y [n][kMlpOut] (device arrays).
   int forward(hipStream_t s, const float* w, const float* X, int n, float* Y, std::string* err);
   // Loss 1/2 mean |y - V|^2 and its gradient (MlpShape layout) over n
   // samples; g and loss are device pointers.

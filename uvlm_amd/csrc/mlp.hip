@@ -46,13 +46,13 @@ __global__ void k_mlp_to_dev(const float* __restrict__ w, unsigned short* __rest
     hw[kOffW2 + t] = f32_to_bf16(w[S::off_w2 + t]);
     hw[kOffW3 + t] = f32_to_bf16(w[S::off_w3 + t]);
   }
-  if (t < kMlpW4Rows * H) hw[kOffW4 + t] = t < 3 * H ? f32_to_bf16(w[S::off_w4 + t]) : 0;
+  if (t < kMlpW4Rows * H) hw[kOffW4 + t] = t < kMlpOut * H ? f32_to_bf16(w[S::off_w4 + t]) : 0;
   if (t < H) {
     fb[t] = w[S::off_b1 + t];
     fb[H + t] = w[S::off_b2 + t];
     fb[2 * H + t] = w[S::off_b3 + t];
   }
-  if (t < kMlpW4Rows) fb[3 * H + t] = t < 3 ? w[S::off_b4 + t] : 0.0f;
+  if (t < kMlpW4Rows) fb[3 * H + t] = t < kMlpOut ? w[S::off_b4 + t] : 0.0f;
 }
 
 // One wavefront per 64 samples; rows beyond n run on zero features and are
@@ -60,14 +60,14 @@ __global__ void k_mlp_to_dev(const float* __restrict__ w, unsigned short* __rest
 __global__ __launch_bounds__(64) void k_mlp_infer(MlpDev net, const float* __restrict__ X, int n,
                                                   float* __restrict__ Y) {
   __shared__ unsigned short lds[2 * kMlpLdsHalfs];
-  __shared__ float ylds[64 * 4];
+  __shared__ float ylds[64 * kMlpYStride];
   const int row = blockIdx.x * 64 + threadIdx.x;
   float f[kMlpIn];
   for (int k = 0; k < kMlpIn; ++k) f[k] = row < n ? X[(long)row * kMlpIn + k] : 0.0f;
-  float y[3];
-  mlp_wave_bf16(net, f, lds, ylds, y);
+  mlp_wave_bf16(net, f, lds, ylds);
   if (row < n)
-    for (int o = 0; o < 3; ++o) Y[(long)row * 3 + o] = y[o];
+    for (int o = 0; o < kMlpOut; ++o)
+      Y[(long)row * kMlpOut + o] = ylds[threadIdx.x * kMlpYStride + o];
 }
 
 // WT[k][n] = W[n][k] for the four layers (k-major copies for the forward).
@@ -83,9 +83,9 @@ __global__ void k_transpose_all(const float* __restrict__ w, float* __restrict__
     wt[H * IN + k * H + n] = w[S::off_w2 + t];
     wt[H * IN + H * H + k * H + n] = w[S::off_w3 + t];
   }
-  if (t < 3 * H) {  // W4 [3][H] -> WT4 [H][3]
+  if (t < kMlpOut * H) {  // W4 [OUT][H] -> WT4 [H][OUT]
     const int n = t / H, k = t % H;
-    wt[H * IN + 2 * H * H + k * 3 + n] = w[S::off_w4 + t];
+    wt[H * IN + 2 * H * H + k * kMlpOut + n] = w[S::off_w4 + t];
   }
 }
 
@@ -110,9 +110,10 @@ __global__ void k_out_grad(const float* __restrict__ Y, const float* __restrict_
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   float l = 0.0f;
-  for (int o = 0; o < 3; ++o) {
-    const float e = Y[3 * b + o] - V[3 * b + o];
-    dY[3 * b + o] = e / (float)B;
+  for (int o = 0; o < kMlpOut; ++o) {
+    const long q = (long)kMlpOut * b + o;
+    const float e = Y[q] - V[q];
+    dY[q] = e / (float)B;
     l = fmaf(e, e, l);
   }
   lrow[b] = 0.5f * l;
@@ -178,13 +179,13 @@ __global__ void k_gather_rows(const float* __restrict__ X, const float* __restri
                               const int* __restrict__ idx, int B, float* __restrict__ Xb,
                               float* __restrict__ Vb) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B * (IN + 3)) return;
-  const int b = t / (IN + 3), k = t % (IN + 3);
+  if (t >= B * (IN + kMlpOut)) return;
+  const int b = t / (IN + kMlpOut), k = t % (IN + kMlpOut);
   const long r = idx[b];
   if (k < IN)
     Xb[(long)b * IN + k] = X[r * IN + k];
   else
-    Vb[(long)b * 3 + k - IN] = V[r * 3 + k - IN];
+    Vb[(long)b * kMlpOut + k - IN] = V[r * kMlpOut + k - IN];
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
@@ -242,8 +243,9 @@ int MlpTrainer::ensure(int rows, std::string* err) {
     *p = nullptr;
   }
   const size_t R = (size_t)rows, C = (size_t)cdiv(rows, kChunkRows);
-  const size_t sizes[] = {R * H, R * H, R * H, R * H, R * H, R * H, R * 3, R * 3, R * H, R * H,
-                          R * H, (size_t)MlpShape::total, C * H * (H + 1), R, R * IN, R * 3,
+  const size_t O = kMlpOut;
+  const size_t sizes[] = {R * H, R * H, R * H, R * H, R * H, R * H, R * O, R * O, R * H, R * H,
+                          R * H, (size_t)MlpShape::total, C * H * (H + 1), R, R * IN, R * O,
                           (size_t)MlpShape::total, (size_t)MlpShape::total,
                           (size_t)MlpShape::total, 4};
   for (int i = 0; i < 20; ++i)
@@ -270,8 +272,8 @@ void MlpTrainer::fwd(hipStream_t s, const float* w, const float* X, int B) {
                      B, Z2_, H2_);
   hipLaunchKernelGGL(k_fwd, dim3(g), dim3(256), 0, s, (const float*)H2_, H, wt3, w + S::off_b3, H,
                      B, Z3_, H3_);
-  hipLaunchKernelGGL(k_fwd, dim3(cdiv((long)B * 3, 256)), dim3(256), 0, s, (const float*)H3_, H,
-                     wt4, w + S::off_b4, 3, B, Y_, (float*)nullptr);
+  hipLaunchKernelGGL(k_fwd, dim3(cdiv((long)B * kMlpOut, 256)), dim3(256), 0, s,
+                     (const float*)H3_, H, wt4, w + S::off_b4, kMlpOut, B, Y_, (float*)nullptr);
 }
 
 void MlpTrainer::bwd(hipStream_t s, const float* w, const float* X, const float* V, int B,
@@ -283,12 +285,12 @@ void MlpTrainer::bwd(hipStream_t s, const float* w, const float* X, const float*
   hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, (const float*)lrow_, B, loss);
   const int gh = cdiv((long)B * H, 256);
   // layer 4
-  hipLaunchKernelGGL(k_wgrad, dim3(cdiv(3 * (H + 1), 256), C), dim3(256), 0, s,
-                     (const float*)dY_, 3, (const float*)H3_, H, B, P_);
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3(cdiv(3 * (H + 1), 256)), dim3(256), 0, s,
-                     (const float*)P_, C, 3, H, g, S::off_w4, S::off_b4);
-  hipLaunchKernelGGL(k_bwd_data, dim3(gh), dim3(256), 0, s, (const float*)dY_, 3, w + S::off_w4,
-                     H, (const float*)Z3_, B, D3_);
+  hipLaunchKernelGGL(k_wgrad, dim3(cdiv(kMlpOut * (H + 1), 256), C), dim3(256), 0, s,
+                     (const float*)dY_, kMlpOut, (const float*)H3_, H, B, P_);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(cdiv(kMlpOut * (H + 1), 256)), dim3(256), 0, s,
+                     (const float*)P_, C, kMlpOut, H, g, S::off_w4, S::off_b4);
+  hipLaunchKernelGGL(k_bwd_data, dim3(gh), dim3(256), 0, s, (const float*)dY_, kMlpOut,
+                     w + S::off_w4, H, (const float*)Z3_, B, D3_);
   // layer 3
   hipLaunchKernelGGL(k_wgrad, dim3(cdiv(H * (H + 1), 256), C), dim3(256), 0, s,
                      (const float*)D3_, H, (const float*)H2_, H, B, P_);
@@ -308,7 +310,7 @@ void MlpTrainer::bwd(hipStream_t s, const float* w, const float* X, const float*
                      (const float*)D1_, H, X, IN, B, P_);
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(cdiv(H * (IN + 1), 256)), dim3(256), 0, s,
                      (const float*)P_, C, H, IN, g, S::off_w1, S::off_b1);
-  hipLaunchKernelGGL(k_fill_f, dim3(1), dim3(64), 0, s, g + S::off_b4 + 3, 1, 0.0f);  // pad
+  hipLaunchKernelGGL(k_fill_f, dim3(1), dim3(64), 0, s, g + S::off_b4 + kMlpOut, 2, 0.0f);  // pad
 }
 
 int MlpTrainer::forward(hipStream_t s, const float* w, const float* X, int n, float* Y,
@@ -317,7 +319,7 @@ int MlpTrainer::forward(hipStream_t s, const float* w, const float* X, int n, fl
   int rc = ensure(n, err);
   if (rc != UVLM_OK) return rc;
   fwd(s, w, X, n);
-  if (hipMemcpyAsync(Y, Y_, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, s) !=
+  if (hipMemcpyAsync(Y, Y_, sizeof(float) * kMlpOut * (size_t)n, hipMemcpyDeviceToDevice, s) !=
       hipSuccess) {
     *err = "network output copy failed";
     return UVLM_ERR_HIP;
@@ -394,7 +396,7 @@ int MlpTrainer::train(hipStream_t s, float* w, const float* X, const float* V, i
     return UVLM_ERR_OOM;
   }
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_gather_rows, dim3(cdiv((long)batch * (IN + 3), 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_gather_rows, dim3(cdiv((long)batch * (IN + kMlpOut), 256)), dim3(256), 0, s,
                        X, V, (const int*)(idx_ + (size_t)it * batch), batch, Xb_, Vb_);
     fwd(s, w, Xb_, batch);
     bwd(s, w, Xb_, Vb_, batch, g_, dloss ? dloss + it : loss_);

@@ -1,15 +1,17 @@
 // mlp_dev.h -- BF16 MFMA inference of the far-field correction network for
-// one wavefront of 64 samples (one per lane): used by the far-field kernel
-// and by the stand-alone inference entry point.
+// one wavefront of 64 samples (one per lane): used by the tile-correction
+// kernel of the far field and by the stand-alone inference entry point.
 //
 // Network: h1 = silu(W1 f + b1), h2 = silu(W2 h1 + b2), h3 = silu(W3 h2 + b3),
-// y = W4 h3 + b4 (32 -> 128 -> 128 -> 128 -> 3).  Weights and activations are
-// rounded to BF16 (round to nearest even), products accumulate in FP32
-// (v_mfma_f32_16x16x16bf16_1k), biases and activations are FP32.
+// y = W4 h3 + b4 (32 -> 128 -> 128 -> 128 -> 30).  Weights and activations
+// are rounded to BF16 (round to nearest even), products accumulate in FP32
+// on gfx950's v_mfma_f32_16x16x32_bf16, biases and activations are FP32.
 //
-// MFMA operand layout (16 x 16 x 16, one wavefront): A[i][k] at lane
-// i + 16 (k / 4), element k % 4; B[k][j] at lane j + 16 (k / 4), element
-// k % 4; D[i][j] at lane j + 16 (i / 4), element i % 4.
+// MFMA operand layout (16 x 16 x 32, one wavefront): lane l holds
+// A[l % 16][8 (l / 16) .. 8 (l / 16) + 7] and B[8 (l / 16) .. + 7][l % 16]
+// (8 BF16 each); the result D[4 (l / 16) + r][l % 16] is element r of the
+// lane's accumulator.  Here A = activations (row = sample), B[k][j] =
+// W[j][k], so a lane holds 4 samples' values of one output.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,27 +21,31 @@
 namespace uvlm {
 
 // Inference copy of the network: weights as BF16 bits, row-major [out][in]
-// (W4 padded with zero rows to 16 outputs), biases FP32 (b4 padded to 16).
+// (W4 padded with zero rows to 32 outputs), biases FP32 (b4 padded to 32).
 struct MlpDev {
   const unsigned short* w1;  // [128][32]
   const unsigned short* w2;  // [128][128]
   const unsigned short* w3;  // [128][128]
-  const unsigned short* w4;  // [16][128]
+  const unsigned short* w4;  // [32][128]
   const float* b1;
   const float* b2;
   const float* b3;
-  const float* b4;  // [16]
+  const float* b4;  // [32]
 };
-constexpr int kMlpW4Rows = 16;
+constexpr int kMlpW4Rows = 32;
 constexpr size_t kMlpDevHalfs =
     (size_t)kMlpHidden * kMlpIn + 2 * (size_t)kMlpHidden * kMlpHidden + kMlpW4Rows * kMlpHidden;
 constexpr size_t kMlpDevFloats = 3 * kMlpHidden + kMlpW4Rows;
 // LDS row stride (BF16 elements) of the activation buffers: 128 + 8 keeps
-// the 8-byte fragment reads of the 16 rows of a tile on distinct banks.
+// rows 16-byte aligned and the 16-byte fragment reads of a tile's 16 rows
+// on distinct banks.
 constexpr int kMlpLdsStride = kMlpHidden + 8;
 constexpr int kMlpLdsHalfs = 64 * kMlpLdsStride;
+// Row stride (floats) of the output buffer: 32 outputs + 1 against bank
+// conflicts of the per-lane reads.
+constexpr int kMlpYStride = kMlpW4Rows + 1;
 
-typedef short mlp_v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_v8bf __attribute__((ext_vector_type(8)));
 typedef float mlp_v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline unsigned short f32_to_bf16(float f) {
@@ -57,7 +63,8 @@ __host__ __device__ inline unsigned short f32_to_bf16(float f) {
 __device__ inline float mlp_silu(float z) { return z / (1.0f + __expf(-z)); }
 
 // out[64 x N] = act(in[64 x K] W^T + b) for one wavefront.  in/out: LDS,
-// BF16, row stride kMlpLdsStride.  NT = N / 16 output tiles.
+// BF16, row stride kMlpLdsStride; without SILU the FP32 results go to yout
+// (row stride kMlpYStride).  NT = N / 16 output tiles, K a multiple of 32.
 template <int K, int NT, bool SILU>
 __device__ inline void mlp_layer(const unsigned short* __restrict__ W, const float* __restrict__ b,
                                  const unsigned short* in, unsigned short* out, float* yout) {
@@ -69,19 +76,19 @@ __device__ inline void mlp_layer(const unsigned short* __restrict__ W, const flo
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mlp_v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const int k = k0 + 4 * lq;
-    mlp_v4s bf[NT];
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + 8 * lq;
+    mlp_v8bf bf[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      bf[nt] = *reinterpret_cast<const mlp_v4s*>(W + (size_t)(16 * nt + l16) * K + k);
+      bf[nt] = *reinterpret_cast<const mlp_v8bf*>(W + (size_t)(16 * nt + l16) * K + k);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      const mlp_v4s af =
-          *reinterpret_cast<const mlp_v4s*>(in + (16 * mt + l16) * kMlpLdsStride + k);
+      const mlp_v8bf af =
+          *reinterpret_cast<const mlp_v8bf*>(in + (16 * mt + l16) * kMlpLdsStride + k);
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[nt], acc[mt][nt], 0, 0, 0);
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[nt], acc[mt][nt], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -96,17 +103,18 @@ __device__ inline void mlp_layer(const unsigned short* __restrict__ W, const flo
         const float z = acc[mt][nt][r] + bn;
         if (SILU)
           out[row * kMlpLdsStride + n] = f32_to_bf16(mlp_silu(z));
-        else if (n < 3)
-          yout[row * 4 + n] = z;
+        else
+          yout[row * kMlpYStride + n] = z;
       }
   }
 }
 
-// y (3 values per lane) of the lane's features f[32].  lds: 2 x kMlpLdsHalfs
-// BF16 buffers; ylds: 64 x 4 floats.  For one-wavefront workgroups; every
-// lane calls it together (it contains workgroup barriers).
+// Forward pass of the 64 lanes' features f[32]: outputs of lane l in
+// ylds[l * kMlpYStride + o], o < kMlpOut.  lds: 2 x kMlpLdsHalfs BF16
+// buffers; ylds: 64 x kMlpYStride floats.  For one-wavefront workgroups;
+// every lane calls it together (it contains workgroup barriers).
 __device__ inline void mlp_wave_bf16(const MlpDev& net, const float* f, unsigned short* lds,
-                                     float* ylds, float y[3]) {
+                                     float* ylds) {
   const int lane = threadIdx.x & 63;
   unsigned short* A = lds;
   unsigned short* B = lds + kMlpLdsHalfs;
@@ -119,11 +127,8 @@ __device__ inline void mlp_wave_bf16(const MlpDev& net, const float* f, unsigned
   __syncthreads();
   mlp_layer<kMlpHidden, kMlpHidden / 16, true>(net.w3, net.b3, A, B, nullptr);
   __syncthreads();
-  mlp_layer<kMlpHidden, 1, false>(net.w4, net.b4, B, nullptr, ylds);
+  mlp_layer<kMlpHidden, kMlpW4Rows / 16, false>(net.w4, net.b4, B, nullptr, ylds);
   __syncthreads();
-  y[0] = ylds[lane * 4 + 0];
-  y[1] = ylds[lane * 4 + 1];
-  y[2] = ylds[lane * 4 + 2];
 }
 
 }  // namespace uvlm

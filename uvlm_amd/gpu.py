@@ -36,9 +36,9 @@ def _pf(a):
 
 
 def _samples(L, check, call, max_samples):
-    """Run a sample-drawing entry point; (features [k][32], targets [k][3])."""
+    """Run a sample-drawing entry point; (features [k][32], targets [k][30])."""
     f = np.zeros((max_samples, N.MLP_IN), dtype=np.float32)
-    r = np.zeros((max_samples, 3), dtype=np.float32)
+    r = np.zeros((max_samples, N.MLP_OUT), dtype=np.float32)
     cnt = C.c_size_t(max_samples)
     check(call(_pf(f), _pf(r), C.byref(cnt)))
     return f[:cnt.value].copy(), r[:cnt.value].copy()
@@ -49,7 +49,8 @@ def mlp_init(seed: int = 0) -> np.ndarray:
     UVLM_MLP_PARAMS): uniform(-a, a) weights with a = sqrt(3 / fan_in) (x 0.1
     for the output layer), zero biases."""
     rng = np.random.default_rng(seed)
-    shapes = [(128, 32), (128,), (128, 128), (128,), (128, 128), (128,), (3, 128), (3,)]
+    shapes = [(128, 32), (128,), (128, 128), (128,), (128, 128), (128,), (N.MLP_OUT, 128),
+              (N.MLP_OUT,)]
     parts = []
     for k, shp in enumerate(shapes):
         if len(shp) == 2:
@@ -57,7 +58,7 @@ def mlp_init(seed: int = 0) -> np.ndarray:
             parts.append(rng.uniform(-a, a, size=shp).reshape(-1))
         else:
             parts.append(np.zeros(shp))
-    return np.concatenate(parts + [np.zeros(1)]).astype(np.float32)
+    return np.concatenate(parts + [np.zeros(2)]).astype(np.float32)
 
 
 class Context:
@@ -153,13 +154,13 @@ class Context:
     # --- far-field correction network ---------------------------------------
     def mlp_forward(self, w, X, mode=N.MLP_FP32):
         w, X = _f32(w), _f32(X).reshape(-1, N.MLP_IN)
-        y = np.zeros((X.shape[0], 3), dtype=np.float32)
+        y = np.zeros((X.shape[0], N.MLP_OUT), dtype=np.float32)
         self._check(self._L.uvlm_mlp_forward(self._h, _pf(w), _pf(X), X.shape[0], _pf(y), mode),
                     "uvlm_mlp_forward")
         return y
 
     def mlp_grad(self, w, X, V):
-        w, X, V = _f32(w), _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, 3)
+        w, X, V = _f32(w), _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, N.MLP_OUT)
         g = np.zeros(N.MLP_PARAMS, dtype=np.float32)
         loss = C.c_float()
         self._check(self._L.uvlm_mlp_grad(self._h, _pf(w), _pf(X), _pf(V), X.shape[0], _pf(g),
@@ -168,7 +169,7 @@ class Context:
 
     def mlp_train(self, w, X, V, iters, batch, lr, seed=1):
         w = _f32(w).copy()
-        X, V = _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, 3)
+        X, V = _f32(X).reshape(-1, N.MLP_IN), _f32(V).reshape(-1, N.MLP_OUT)
         losses = np.zeros(iters, dtype=np.float32)
         self._check(self._L.uvlm_mlp_train(self._h, _pf(w), _pf(X), _pf(V), X.shape[0], iters,
                                            batch, float(lr), seed, _pf(losses)),
