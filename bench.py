@@ -14,7 +14,7 @@ The whole run is done twice in this process:
   expansions for far clusters, theta 4 near the wings graded beyond 1.5
   wing radii to 1.2, exact audits every 50 steps), a correction network
   for the far clusters trained in the run on the run's own wake (steps
-  100, 600, 1100, 1600) and evaluated once per (target tile, cluster) on
+  100, 700, 1300, 1900) and evaluated once per (target tile, cluster) on
   the BF16 matrix cores, FP32 compensated near sums with FP64 near-core
   re-evaluation, the wings' moment expansions for wake nodes beyond the
   first 24 rows, kept-inverse solve (FP32 inverse, FP64 residuals);
@@ -40,8 +40,8 @@ state of the plain run, on the host's cores.
 Other modes: --workload c1|c2|c3|c5 runs one configuration alone (same
 window rules, `--precision`, `--solve`, `--farfield` choose the method),
 --workload c4 the 256 batched cases; --no-plain, --no-c2, --no-cpu skip
-legs (--no-net-compare the fast form's run without the network, which is
-otherwise timed after the plain run); --spread adds a second FP64 run of
+legs; --net-compare also times the fast form without the correction
+network (after the plain run); --spread adds a second FP64 run of
 the tandem workload that differs
 from the plain one only by rounding (kept-inverse solve instead of the LU)
 and reports, per 100-step window, the fast run's CL/CT differences beside
@@ -75,7 +75,7 @@ TRAFFIC = {  # committed PMC summaries (HBM bytes per launch) of the default run
 # The fast form of the tandem workload.  The correction network is trained
 # in the run from its own wake (first at step `first`, then warm-started
 # every `every` steps) and corrects the far clusters from then on.
-NET = {"first": 100, "every": 500, "samples": 30000, "iters_first": 300, "iters": 80,
+NET = {"first": 100, "every": 600, "samples": 16000, "iters_first": 200, "iters": 50,
        "batch": 1024, "lr": 1e-3}
 FAST = {"precision": "f32c", "solve": "inv", "farfield": 4.0, "ff_reach": 1.5,
         "ff_theta_min": 1.2, "audit": 50, "ff_tol": 1e-3, "mlp": NET}
@@ -351,7 +351,7 @@ def bench_tandem(args):
             r2 = run_case(case, dict(PLAIN, solve="inv"), args.steps, args.warmup, "fp64-b")
             r2.pop("sim").close()
             out["rounding_spread"] = spread_report(fast, plain, r2)
-    if fast_m.get("mlp") and not args.no_net_compare:
+    if fast_m.get("mlp") and args.net_compare:
         log("fast form without the network")
         m2 = {k: v for k, v in fast_m.items() if k != "mlp"}
         r2 = run_case(case, m2, args.steps, args.warmup, "fast-no-net")
@@ -484,8 +484,9 @@ def main():
                     % FAST["ff_theta_min"])
     ap.add_argument("--no-net", action="store_true",
                     help="tandem fast form without the correction network")
-    ap.add_argument("--no-net-compare", action="store_true",
-                    help="skip the fast form's second run without the correction network")
+    ap.add_argument("--net-compare", action="store_true",
+                    help="also time the fast form without the correction network "
+                         "(fast_without_network)")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--spread", action="store_true",

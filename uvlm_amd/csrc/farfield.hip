@@ -38,6 +38,9 @@ __host__ __device__ inline int cdiv(long a, long b) { return (int)((a + b - 1) /
 constexpr int kTileT = 64;        // targets per tile
 constexpr int kGroups = 8;        // partial-sum groups per tile, far clusters
 constexpr int kNearGroups = 64;   // partial-sum groups per tile, near blocks
+// The near walk loads the next block's record into registers during the
+// current walk (else it loads each record when it needs it, fewer registers).
+constexpr bool kNearPrefetch = true;
 constexpr int kMomThreads = 128;  // >= kFFMom: one thread per moment component
 
 // --- tree layout ------------------------------------------------------------
@@ -87,6 +90,50 @@ __global__ __launch_bounds__(1024) void k_bbox(const double* __restrict__ x, int
     for (int w = 1; w < 16; ++w) v = d < 3 ? fmin(v, s[d][w]) : fmax(v, s[d][w]);
     box[d] = v;
   }
+}
+
+// Bounding box of many points in two launches: kBoxParts workgroups each
+// reduce a contiguous range to part[g][6] (min x, y, z, max x, y, z), then
+// one wavefront reduces the parts in order into box[6].
+constexpr int kBoxParts = 64;
+__global__ __launch_bounds__(256) void k_bbox_part(const double* __restrict__ x, int n,
+                                                   double* __restrict__ part) {
+  __shared__ double s[6][4];
+  const int per = (n + kBoxParts - 1) / kBoxParts;
+  const int q0 = blockIdx.x * per, q1 = min(n, q0 + per);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int q = q0 + threadIdx.x; q < q1; q += 256)
+    for (int d = 0; d < 3; ++d) {
+      const double v = x[3 * (long)q + d];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d) {
+      s[d][threadIdx.x >> 6] = lo[d];
+      s[3 + d][threadIdx.x >> 6] = hi[d];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int d = threadIdx.x;
+    double v = s[d][0];
+    for (int w = 1; w < 4; ++w) v = d < 3 ? fmin(v, s[d][w]) : fmax(v, s[d][w]);
+    part[blockIdx.x * 6 + d] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_bbox_fin(const double* __restrict__ part,
+                                                 double* __restrict__ box) {
+  const int d = threadIdx.x;
+  if (d >= 6) return;
+  double v = part[d];
+  for (int g = 1; g < kBoxParts; ++g) v = d < 3 ? fmin(v, part[g * 6 + d]) : fmax(v, part[g * 6 + d]);
+  box[d] = v;
 }
 
 __device__ inline unsigned long long spread21(unsigned long long v) {
@@ -670,28 +717,33 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
   const int count = cnt[tile].x;
   const int* list = near + (long)tile * n0;
   double ux = 0.0, uy = 0.0, uz = 0.0;
-  double pre[kPer];
+  double pre[kNearPrefetch ? kPer : 1];
   int id = g < count ? list[g] : 0;
   auto load = [&](int b) {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int k = lane + 64 * u;
-      pre[u] = k < kLen ? rec[(long)b * kRec + k] : 0.0;
+      pre[kNearPrefetch ? u : 0] = k < kLen ? rec[(long)b * kRec + k] : 0.0;
+      if (!kNearPrefetch) sr[lane + 64 * u] = pre[0];
     }
   };
-  if (g < count) load(id);
+  if (kNearPrefetch && g < count) load(id);
   unsigned segs = 0;  // segments walked by this group (for the pair count)
   for (int k = g; k < count; k += kNearGroups) {
     const FFCluster c = cl[id];
     const int nr = c.r1 - c.r0, nc = c.c1 - c.c0;
     segs += (unsigned)((nr + 1) * nc + (nc + 1) * nr);
     __syncthreads();
+    if (kNearPrefetch) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) sr[lane + 64 * u] = pre[u];
+      for (int u = 0; u < kPer; ++u) sr[lane + 64 * u] = pre[u];
+    } else {
+      load(id);
+    }
     __syncthreads();
     if (k + kNearGroups < count) {  // next block's record, in flight during the walk
       id = list[k + kNearGroups];
-      load(id);
+      if (kNearPrefetch) load(id);
     }
     if (F32)
       walk_record32(sr, nr, nc, px, py, pz, ux, uy, uz);
@@ -910,7 +962,7 @@ __global__ __launch_bounds__(64) void k_ff_tilenet(const int* __restrict__ nrows
                                                    const float* __restrict__ feats, MlpDev net,
                                                    float ymax, float* __restrict__ corr) {
   __shared__ unsigned short lds[2 * kMlpLdsHalfs];
-  __shared__ float ylds[64 * kMlpYStride];
+  float* ylds = reinterpret_cast<float*>(lds);  // outputs over the first buffer
   const int lane = threadIdx.x;
   const int n = *nrows;
   for (int b = blockIdx.x; 64 * b < n; b += gridDim.x) {
@@ -929,6 +981,7 @@ __global__ __launch_bounds__(64) void k_ff_tilenet(const int* __restrict__ nrows
       rec[30] = q.inv;
       rec[31] = ok ? 1.0f : -1.0f;
     }
+    __syncthreads();  // the outputs' buffer takes the next batch's features
   }
 }
 
@@ -957,6 +1010,7 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
   constexpr int kStage = kFFMom + 12 + 4;  // moments, traces, centre and radius
   __shared__ double sq[kFarBatch][kStage];
   __shared__ int sid[kFarBatch];
+  __shared__ float srec[MLP ? kFarBatch : 1][32];  // correction records
   const int tile = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
   const int tg = tile_target(map, tile, lane);
   const double px = tg >= 0 ? x[3 * (long)tg] : 0.0, py = tg >= 0 ? x[3 * (long)tg + 1] : 0.0,
@@ -990,6 +1044,11 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
       sq[j][m] = v;
       if (m == 0) sid[j] = id;
     }
+    if (MLP)
+      for (int q = lane; q < nb * 32; q += 64) {
+        const int j = q / 32, m = q % 32, k = k0 + j * kGroups;
+        srec[j][m] = k < kNetCap ? corr[((long)tile * kNetCap + k) * 32 + m] : 0.0f;
+      }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
       const int k = k0 + j * kGroups;
@@ -1000,8 +1059,8 @@ __global__ __launch_bounds__(64) void k_ff_far(FFTree t, const FFCluster* __rest
       u[0] *= kInv4Pi;
       u[1] *= kInv4Pi;
       u[2] *= kInv4Pi;
-      if (MLP && k < kNetCap) {
-        const float* r = corr + ((long)tile * kNetCap + k) * 32;
+      if (MLP) {
+        const float* r = srec[MLP ? j : 0];
         const float status = r[31];
         if (status != 0.0f && tg >= 0) {
           ++npairs;
@@ -1418,7 +1477,7 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
   t.total = off;
   const int n0 = num * t.nbr[0] * t.nbc[0];
   int rc;
-  if ((rc = ensure(&d_box_, &c_box_, sizeof(double) * 8, err)) ||
+  if ((rc = ensure(&d_box_, &c_box_, sizeof(double) * (8 + 6 * kBoxParts), err)) ||
       (rc = ensure(&d_cl_, &c_cl_, sizeof(FFCluster) * (size_t)t.total, err)) ||
       (rc = ensure(&d_mom_, &c_mom_, sizeof(double) * kFFMom * (size_t)t.total, err)) ||
       (rc = ensure(&d_tr_, &c_tr_, sizeof(double) * 12 * (size_t)t.total, err)) ||
@@ -1466,7 +1525,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
     return rc;
   // target tiles: the targets in Morton order of their own box
   double* tbox = (double*)d_box_;
-  hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, s, targets, nt, tbox);
+  hipLaunchKernelGGL(k_bbox_part, dim3(kBoxParts), dim3(256), 0, s, targets, nt, tbox + 8);
+  hipLaunchKernelGGL(k_bbox_fin, dim3(1), dim3(64), 0, s, (const double*)(tbox + 8), tbox);
   hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
                      (const double*)tbox, (unsigned long long*)d_key_, (int*)d_idx_);
   if ((rc = sort_pairs(s, nt, err)) != UVLM_OK) return rc;

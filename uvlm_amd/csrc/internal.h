@@ -189,7 +189,8 @@ class InvSolver {
   // buffers for n unknowns, X in FP64 (wide) or FP32
   int prepare(hipStream_t s, int n, bool wide, std::string* err);
   int build(hipStream_t s, const double* A, std::string* err);
-  double newton_schulz(hipStream_t s, const double* A);
+  // one Newton-Schulz step (low: BF16 operands); wait: read the deviation back
+  double newton_schulz(hipStream_t s, const double* A, bool low, bool wait);
   void enqueue_sweep(hipStream_t s, const double* A, const double* b, double* x, int k,
                      double prev, double tol);
   void release();
@@ -202,6 +203,7 @@ class InvSolver {
   int n_ = 0, S_ = 1, cols_ = 1;
   bool valid_ = false;
   int streak_ = 0, last_sweeps_ = 0;
+  double end_rho_ = 1.0;  // contraction observed in the last solve
   long long sweeps_total_ = 0, refreshes_ = 0, rebuilds_ = 0;
   std::string last_err_;
 };

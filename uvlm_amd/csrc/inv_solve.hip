@@ -38,7 +38,11 @@ constexpr int kMaxBatch = 16;           // sweeps queued per host read
 constexpr int kNsSlot = 2 * kMaxBatch;  // Newton-Schulz deviation word
 constexpr int kFlagSlot = kNsSlot + 1;  // 0 running, 1 + k converged at k, -1 - k diverged
 constexpr int kStatWords = kFlagSlot + 1;
-constexpr double kRefreshRho = 0.15;    // contraction above which X is refreshed
+constexpr double kRefreshRho = 0.05;    // contraction above which X is refreshed
+// A warm solve starts with a BF16 refresh when the previous solve contracted
+// more slowly than this (the matrix of a time-stepping run drifts by about
+// as much per step).
+constexpr double kPreRefreshRho = 0.02;
 constexpr int kRedGroups = 16;          // partial-sum groups per row in the reductions
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -336,12 +340,16 @@ int InvSolver::prepare(hipStream_t, int n, bool wide, std::string* err) {
 
 // One Newton-Schulz step from the current X; returns max |I - A X_old| (or
 // < 0 on error, reason in last_err_).  FP32: A is rounded to FP32 and both
-// products run on the FP32 matrix cores.  The step synchronises the stream.
-double InvSolver::newton_schulz(hipStream_t s, const double* A) {
+// products run on the FP32 matrix cores, or with low on the BF16 matrix
+// cores (FP32 data rounded to BF16 operands, FP32 accumulation: the step
+// then squares the deviation down to a floor of ~1e-2, measured 0.007 in
+// spectral radius on the tandem wings' matrix, enough to contract each
+// sweep ~100x).  The step synchronises the stream.
+double InvSolver::newton_schulz(hipStream_t s, const double* A, bool low, bool wait) {
   last_err_.clear();
   const int n = n_;
   const long nn = (long)n * n;
-  const int prec = wide_ ? GEMM_F64 : GEMM_F32;
+  const int prec = wide_ ? GEMM_F64 : low ? GEMM_BF16 : GEMM_F32;
   const void* Aop = A;
   if (!wide_) {
     hipLaunchKernelGGL(k_round_f32, dim3(std::min(2048, cdiv(nn, 256))), dim3(256), 0, s, A, nn,
@@ -361,10 +369,11 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A) {
   const GemmArgs xt{n, n, n, X_, n, T_, n, X2_, n, 1.0, 0.0, nullptr};  // X2 = X (2I - A X)
   launch_gemm(s, prec, &xt, 1);
   std::swap(X_, X2_);
+  ++refreshes_;
+  if (!wait) return 0.0;
   unsigned long long u = 0;
   if (!read_back(s, &u, stat_ + kNsSlot, sizeof(u), "Newton-Schulz step", &last_err_))
     return -1.0;
-  ++refreshes_;
   return bits_to_double(u);
 }
 
@@ -396,7 +405,7 @@ int InvSolver::build(hipStream_t s, const double* A, std::string* err) {
   double best = 1e300;
   int since = 0;  // steps since the deviation last fell below 0.7 of its best
   for (int it = 0; it < 200; ++it) {
-    const double e = newton_schulz(s, A);
+    const double e = newton_schulz(s, A, false, true);
     if (e < 0.0) {
       *err = last_err_.empty() ? "inverse solver: Newton-Schulz GEMM failed" : last_err_;
       return UVLM_ERR_HIP;
@@ -464,6 +473,8 @@ int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, dou
     hipLaunchKernelGGL(k_copy_vec, dim3(nb), dim3(256), 0, s, (const double*)x, xold_, n);
   }
   if (!valid_ && (rc = build(s, A, err)) != UVLM_OK) return rc;
+  else if (warm && end_rho_ > kPreRefreshRho)
+    (void)newton_schulz(s, A, true, false);  // queued ahead of the sweeps, no read-back
   double prev = -1.0;
   int refreshed = 0, done = 0;
   bool rebuilt = false;
@@ -486,6 +497,17 @@ int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, dou
     if (flag > 0) {
       last_sweeps_ = done;
       if (warm) ++streak_;
+      // contraction of this solve: geometric mean over the sweeps before
+      // the converged one (that one sits at the rounding level)
+      const int kc = (int)flag - 1;
+      const double d0 = bits_to_double(u[0]);
+      if (kc >= 2 && d0 > 0.0) {
+        const double dl = bits_to_double(u[2 * (kc - 1)]);
+        end_rho_ = std::pow(std::max(dl, 1e-300) / d0, 1.0 / (kc - 1));
+      } else {
+        end_rho_ = kc >= 2 ? 1.0 : 0.0;
+      }
+      if (refreshed > 0 || prev > 0.0) end_rho_ = std::max(end_rho_, kPreRefreshRho * 2);
       return UVLM_OK;
     }
     if (flag < 0) {
@@ -506,9 +528,11 @@ int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, dou
     const double dp = batch > 1 ? bits_to_double(u[2 * (batch - 2)]) : prev;
     const double rho = dp > 0.0 ? d / dp : -1.0;
     prev = d;
-    // slow contraction: improve X (each step squares I - A X)
+    // slow contraction: improve X (each step squares I - A X); the first
+    // refresh of a solve on the BF16 matrix cores, later ones in FP32 (a
+    // contraction the BF16 floor does not bring under the threshold)
     if (rho > kRefreshRho && refreshed < 3) {
-      if (newton_schulz(s, A) < 0.0) {
+      if (newton_schulz(s, A, refreshed == 0, true) < 0.0) {
         *err = last_err_.empty() ? "inverse solver: Newton-Schulz GEMM failed" : last_err_;
         return UVLM_ERR_HIP;
       }

@@ -65,54 +65,64 @@ __device__ inline float mlp_silu(float z) { return z / (1.0f + __expf(-z)); }
 // out[64 x N] = act(in[64 x K] W^T + b) for one wavefront.  in/out: LDS,
 // BF16, row stride kMlpLdsStride; without SILU the FP32 results go to yout
 // (row stride kMlpYStride).  NT = N / 16 output tiles, K a multiple of 32.
+// The output tiles are formed kMlpChunk at a time (4 x kMlpChunk
+// accumulators of 4 floats), so the registers stay within one wavefront's
+// budget without spilling.
+constexpr int kMlpChunk = 2;
 template <int K, int NT, bool SILU>
 __device__ inline void mlp_layer(const unsigned short* __restrict__ W, const float* __restrict__ b,
                                  const unsigned short* in, unsigned short* out, float* yout) {
   const int lane = threadIdx.x & 63;
   const int l16 = lane & 15, lq = lane >> 4;
-  mlp_v4f acc[4][NT];
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mlp_v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    const int k = k0 + 8 * lq;
-    mlp_v8bf bf[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-      bf[nt] = *reinterpret_cast<const mlp_v8bf*>(W + (size_t)(16 * nt + l16) * K + k);
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      const mlp_v8bf af =
-          *reinterpret_cast<const mlp_v8bf*>(in + (16 * mt + l16) * kMlpLdsStride + k);
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[nt], acc[mt][nt], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int n = 16 * nt + l16;
-    const float bn = b[n];
+  constexpr int NC = NT < kMlpChunk ? NT : kMlpChunk;
+  static_assert(NT % NC == 0, "output tiles in whole chunks");
+  for (int n0 = 0; n0 < NT; n0 += NC) {
+    mlp_v4f acc[4][NC];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * mt + 4 * lq + r;
-        const float z = acc[mt][nt][r] + bn;
-        if (SILU)
-          out[row * kMlpLdsStride + n] = f32_to_bf16(mlp_silu(z));
-        else
-          yout[row * kMlpYStride + n] = z;
+      for (int c = 0; c < NC; ++c) acc[mt][c] = mlp_v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      const int k = k0 + 8 * lq;
+      mlp_v8bf bf[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        bf[c] = *reinterpret_cast<const mlp_v8bf*>(W + (size_t)(16 * (n0 + c) + l16) * K + k);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const mlp_v8bf af =
+            *reinterpret_cast<const mlp_v8bf*>(in + (16 * mt + l16) * kMlpLdsStride + k);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          acc[mt][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[c], acc[mt][c], 0, 0, 0);
       }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int n = 16 * (n0 + c) + l16;
+      const float bn = b[n];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * mt + 4 * lq + r;
+          const float z = acc[mt][c][r] + bn;
+          if (SILU)
+            out[row * kMlpLdsStride + n] = f32_to_bf16(mlp_silu(z));
+          else
+            yout[row * kMlpYStride + n] = z;
+        }
+    }
   }
 }
 
 // Forward pass of the 64 lanes' features f[32]: outputs of lane l in
 // ylds[l * kMlpYStride + o], o < kMlpOut.  lds: 2 x kMlpLdsHalfs BF16
-// buffers; ylds: 64 x kMlpYStride floats.  For one-wavefront workgroups;
-// every lane calls it together (it contains workgroup barriers).
+// buffers; ylds: 64 x kMlpYStride floats, which may be the first of the
+// two buffers (the last layer reads only the second).  For one-wavefront
+// workgroups; every lane calls it together (it contains workgroup barriers).
+static_assert(64 * kMlpYStride * 4 <= kMlpLdsHalfs * 2, "outputs fit one activation buffer");
 __device__ inline void mlp_wave_bf16(const MlpDev& net, const float* f, unsigned short* lds,
                                      float* ylds) {
   const int lane = threadIdx.x & 63;
