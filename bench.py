@@ -75,8 +75,8 @@ TRAFFIC = {  # committed PMC summaries (HBM bytes per launch) of the default run
 # The fast form of the tandem workload.  The correction network is trained
 # in the run from its own wake (first at step `first`, then warm-started
 # every `every` steps) and corrects the far clusters from then on.
-NET = {"first": 100, "every": 600, "samples": 16000, "iters_first": 200, "iters": 50,
-       "batch": 1024, "lr": 1e-3}
+NET = {"first": 100, "every": 600, "samples": 12000, "iters_first": 80, "iters": 30,
+       "batch": 4096, "lr": 2e-3}
 FAST = {"precision": "f32c", "solve": "inv", "farfield": 4.0, "ff_reach": 1.5,
         "ff_theta_min": 1.2, "audit": 50, "ff_tol": 1e-3, "mlp": NET}
 PLAIN = {"precision": "f64", "solve": "f64", "farfield": 0.0}

@@ -185,7 +185,10 @@ __device__ inline MomIndex mom_index(int m) {
   return MomIndex{3, r / 27, (r / 9) % 3, (r / 3) % 3, r % 3};
 }
 
-// Component of the moments of one segment (A = a - c, E = e).
+// Component of the moments of one segment (A = a - c, E = e); the thirds
+// are products with the rounded 1/3 (an FP64 division in this per-segment
+// loop costs ~100 cycles of latency).
+constexpr double kThird = 1.0 / 3.0;
 __device__ __forceinline__ double segment_moment(const MomIndex& x, double g, const double* A,
                                                  const double* E) {
   const double ge = g * E[x.i];
@@ -193,12 +196,12 @@ __device__ __forceinline__ double segment_moment(const MomIndex& x, double g, co
   if (x.type == 0) return ge;
   if (x.type == 1) return ge * (A[j] + 0.5 * E[j]);
   if (x.type == 2) {
-    const double p2 = A[j] * A[k] + 0.5 * (A[j] * E[k] + E[j] * A[k]) + E[j] * E[k] / 3.0;
+    const double p2 = A[j] * A[k] + 0.5 * (A[j] * E[k] + E[j] * A[k]) + E[j] * E[k] * kThird;
     return ge * p2;
   }
   const double p3 = A[j] * A[k] * A[l] +
                     0.5 * (A[j] * A[k] * E[l] + A[j] * E[k] * A[l] + E[j] * A[k] * A[l]) +
-                    (A[j] * E[k] * E[l] + E[j] * A[k] * E[l] + E[j] * E[k] * A[l]) / 3.0 +
+                    (A[j] * E[k] * E[l] + E[j] * A[k] * E[l] + E[j] * E[k] * A[l]) * kThird +
                     0.25 * E[j] * E[k] * E[l];
   return ge * p3;
 }
@@ -474,12 +477,23 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
   const int tile = blockIdx.x, lane = threadIdx.x;
   const double4 tc = tiles[tile];
   __shared__ double tx[kTileT], ty[kTileT], tz[kTileT];
+  double blo[3], bhi[3];  // the targets' box (a tighter test than the sphere)
   {  // the tile's targets (lane 0's stands in for the missing ones of the last tile)
     int tg = tile_target(map, tile, lane);
     if (tg < 0) tg = tile_target(map, tile, 0);
-    tx[lane] = x[3 * (long)tg];
-    ty[lane] = x[3 * (long)tg + 1];
-    tz[lane] = x[3 * (long)tg + 2];
+    const double p[3] = {x[3 * (long)tg], x[3 * (long)tg + 1], x[3 * (long)tg + 2]};
+    tx[lane] = p[0];
+    ty[lane] = p[1];
+    tz[lane] = p[2];
+    for (int d = 0; d < 3; ++d) {
+      double lo = p[d], hi = p[d];
+      for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off));
+        hi = fmax(hi, __shfl_xor(hi, off));
+      }
+      blo[d] = lo;
+      bhi[d] = hi;
+    }
   }
   if (fbox) {  // graded criterion away from the focus
     const double4 f = focus_sphere(fbox);
@@ -512,10 +526,18 @@ __global__ __launch_bounds__(64) void k_ff_lists(FFTree t, const FFCluster* __re
         bool is_far = dist - tc.w > reach_c;
         if (per_target && !is_far && dist + tc.w > reach_c) {
           const double lim2 = reach_c * reach_c;
-          is_far = true;
-          for (int q = 0; q < kTileT && is_far; ++q) {
-            const double ex = c.c[0] - tx[q], ey = c.c[1] - ty[q], ez = c.c[2] - tz[q];
-            is_far = ex * ex + ey * ey + ez * ez > lim2;
+          double b2 = 0.0;  // squared distance from the centre to the targets' box
+          for (int d = 0; d < 3; ++d) {
+            const double e = fmax(fmax(blo[d] - c.c[d], c.c[d] - bhi[d]), 0.0);
+            b2 += e * e;
+          }
+          is_far = b2 > lim2;
+          if (!is_far) {
+            is_far = true;
+            for (int q = 0; q < kTileT && is_far; ++q) {
+              const double ex = c.c[0] - tx[q], ey = c.c[1] - ty[q], ez = c.c[2] - tz[q];
+              is_far = ex * ex + ey * ey + ez * ez > lim2;
+            }
           }
         }
         if (is_far) {
