@@ -708,12 +708,36 @@ __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double*
 // Near-count keys of the tiles: the near kernel takes tiles in decreasing
 // order of their near lists (heaviest first), so that the longest
 // workgroups do not start last and run alone at the end of the launch.
-__global__ void k_near_keys(const int2* __restrict__ cnt, int ntiles,
-                            unsigned long long* __restrict__ key, int* __restrict__ idx) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  key[t] = (unsigned long long)(0x7fffffff - cnt[t].x);
-  idx[t] = t;
+// One workgroup: a counting sort of the tiles by near count / 8 (1024
+// buckets, the heaviest first; within a bucket in the order the atomics
+// hand out, which changes no result: every (tile, group) workgroup writes
+// its own partial sums).
+constexpr int kOrderBuckets = 1024;
+__global__ __launch_bounds__(1024) void k_order_heavy(const int2* __restrict__ cnt, int ntiles,
+                                                      int* __restrict__ order) {
+  __shared__ int hist[kOrderBuckets];
+  const int t = threadIdx.x;
+  hist[t] = 0;
+  __syncthreads();
+  auto bucket = [&](int q) {  // 0 = heaviest
+    return kOrderBuckets - 1 - min(cnt[q].x >> 3, kOrderBuckets - 1);
+  };
+  for (int q = t; q < ntiles; q += 1024) atomicAdd(&hist[bucket(q)], 1);
+  __syncthreads();
+  // exclusive prefix sum over the buckets (Hillis-Steele on the inclusive sum)
+  const int mine = hist[t];
+  int v = mine;
+  for (int off = 1; off < kOrderBuckets; off <<= 1) {
+    __syncthreads();
+    const int add = t >= off ? hist[t - off] : 0;
+    __syncthreads();
+    v += add;
+    hist[t] = v;
+  }
+  __syncthreads();
+  hist[t] = v - mine;
+  __syncthreads();
+  for (int q = t; q < ntiles; q += 1024) order[atomicAdd(&hist[bucket(q)], 1)] = q;
 }
 
 // Group g of a tile sums the tile's near blocks g, g + kNearGroups, ...: each
@@ -1564,10 +1588,9 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                      graded ? focus->reach : 0.0, graded ? focus->theta_min : theta,
                      (int*)d_near_, (int*)d_far_, (int2*)d_cnt_, (int*)d_front_);
   // heaviest tiles first (the Morton sort's buffers are free again)
-  hipLaunchKernelGGL(k_near_keys, dim3(cdiv(ntiles, 256)), dim3(256), 0, s,
-                     (const int2*)d_cnt_, ntiles, (unsigned long long*)d_key_, (int*)d_idx_);
-  if ((rc = sort_pairs(s, ntiles, err)) != UVLM_OK) return rc;
-  const int* order = (const int*)d_idx2_;
+  hipLaunchKernelGGL(k_order_heavy, dim3(1), dim3(1024), 0, s, (const int2*)d_cnt_, ntiles,
+                     (int*)d_idx_);
+  const int* order = (const int*)d_idx_;
   unsigned long long* npairs = nullptr;
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if (prof_) {
