@@ -117,7 +117,11 @@ int uvlm_ring_induced_velocity(uvlm_ctx* ctx, const double* targets,
                                unsigned flags, double* out);
 
 /* Replaces SimpleSimulator::CalcMatrix (simple_simulator.cpp:76-101):
- * A[K][L] = VORING_L(cpos_K; Gamma=1) . normals_K, written row-major [N][N]. */
+ * A[K][L] = VORING_L(cpos_K; Gamma=1) . normals_K, written row-major [N][N].
+ * flags: UVLM_PREC_F64_EXACT evaluates the reference's operation order;
+ * UVLM_PREC_F64 and UVLM_PREC_F32C the FP64 rsqrt form of the simulation's
+ * fast modes (one rsqrt per corner, one reciprocal per segment, ~1e-14 of
+ * max |A| from the reference order). */
 int uvlm_influence_matrix(uvlm_ctx* ctx, const double* nodes, size_t num,
                           size_t rows, size_t cols, const double* cpos,
                           const double* normals, unsigned flags, double* A);
@@ -310,6 +314,14 @@ int uvlm_mlp_train(uvlm_ctx* ctx, float* weights, const float* feats, const floa
  * their expansion at every target of the tile; a pair with an output
  * |y| > ymax (outside the trained range) is summed exactly instead. */
 int uvlm_farfield_set_mlp(uvlm_ctx* ctx, const float* weights, double ymax);
+/* Order in which the split tiles n_targets targets (no reference
+ * counterpart; exposed for tests): the targets' 63-bit Morton keys in
+ * their bounding box, 21 bits per axis interleaved x, y, z from bit 0
+ * (keys[n_targets], unsorted), and the target indices sorted by key, equal
+ * keys in index order, padded with -1 to a multiple of 64
+ * (order[64 * ceil(n_targets / 64)]). */
+int uvlm_farfield_target_order(uvlm_ctx* ctx, const double* targets, size_t n_targets,
+                               unsigned long long* keys, int* order);
 /* (target, cluster) pairs corrected and pairs tripped to the exact sum
  * since the network was set. */
 int uvlm_farfield_mlp_stats(uvlm_ctx* ctx, unsigned long long* pairs,

@@ -31,6 +31,38 @@ def wavy_lattice(rows, cols, seed=0, num=1):
     return np.stack(sheets), gamma
 
 
+def morton_keys(x):
+    """63-bit Morton keys of points in their bounding box, 21 bits per axis
+    (x in bit 0, y in bit 1, z in bit 2 of each triple), IEEE FP64 like the
+    device's key kernel."""
+    lo, hi = x.min(0), x.max(0)
+    w = hi - lo
+    keys = np.zeros(x.shape[0], dtype=np.uint64)
+    for d in range(3):
+        t = (x[:, d] - lo[d]) / w[d] if w[d] > 0 else np.zeros(x.shape[0])
+        c = np.floor(np.clip(t, 0.0, 1.0) * 2097151.0).astype(np.uint64)
+        for b in range(21):
+            keys |= ((c >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + d)
+    return keys
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 4097, 9000, 300000, 1200000])
+def test_target_order_is_a_stable_sort_of_the_morton_keys(ctx, n):
+    """The split's tile order comes from its own LSD radix sort: the keys are
+    the Morton keys of a direct restatement, the order is numpy's stable
+    argsort of them (equal keys in index order), padded with -1; sizes
+    cover one partial chunk, several rows and rows of several chunks."""
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=(n, 3)) * np.array([20.0, 3.0, 0.5])
+    if n > 100:
+        x[::5] = x[7]                              # repeated points: equal keys
+        x[1::11, 2] = np.round(x[1::11, 2], 1)     # shared high digits
+    keys, order = ctx.farfield_target_order(x)
+    np.testing.assert_array_equal(keys, morton_keys(x))
+    np.testing.assert_array_equal(order[:n], np.argsort(keys, kind="stable"))
+    assert (order[n:] == -1).all() and order.size == (n + 63) // 64 * 64
+
+
 def test_all_near_is_the_exact_sum(ctx, orc):
     nodes, gamma = wavy_lattice(37, 40, seed=1, num=2)
     rng = np.random.default_rng(2)

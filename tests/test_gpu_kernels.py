@@ -153,6 +153,30 @@ def test_influence_matrix_bitwise(ctx, orc):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("flags", [N.PREC_F64, N.PREC_F32C])
+def test_influence_matrix_fast_form(ctx, orc, flags):
+    """The fast modes' rsqrt form of the matrix: the oracle's entries to
+    1e-13 of max |A|; extra targets on a node, on a segment's line and near
+    the cut-off get the reference's zero or finite entries."""
+    rng = np.random.default_rng(6)
+    nodes, _ = rand_lattice(rng, 2, 6, 10)
+    cp, nrm, _ = orc.lattice_geometry(nodes)
+    ref = orc.influence_matrix(nodes, cp, nrm)
+    got = ctx.influence_matrix(nodes, cp, nrm, flags=flags)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    # collocation points moved onto a node, the middle of an edge and 1e-7
+    # off an edge line (|d|^2 ~ 1e-16 |r0|^2 < 1e-10: cut), rest unchanged
+    cp2 = cp.copy()
+    flat = nodes.reshape(-1, 3)
+    cp2[0] = nodes[0, 0, 0]
+    cp2[1] = 0.5 * (nodes[0, 2, 3] + nodes[0, 2, 4])
+    cp2[2] = nodes[0, 4, 5] + np.array([0.0, 0.0, 1e-7])
+    ref2 = orc.influence_matrix(nodes, cp2, nrm)
+    got2 = ctx.influence_matrix(nodes, cp2, nrm, flags=flags)
+    assert np.isfinite(got2).all() and flat.shape[0] > 3
+    np.testing.assert_allclose(got2, ref2, rtol=0, atol=1e-13 * np.abs(ref2).max())
+
+
 @pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 257, 600])
 def test_solve_f64(ctx, orc, n):
     rng = np.random.default_rng(n)

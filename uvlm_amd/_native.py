@@ -43,6 +43,7 @@ GPU_SYMBOLS = [
     "uvlm_sim_inverse_stats", "uvlm_sim_refine_stats",
     "uvlm_mlp_forward", "uvlm_mlp_grad", "uvlm_mlp_train", "uvlm_farfield_set_mlp",
     "uvlm_farfield_mlp_stats", "uvlm_farfield_samples", "uvlm_sim_farfield_samples",
+    "uvlm_farfield_target_order",
     "uvlm_sim_farfield_learn",
     "uvlm_batch_init", "uvlm_batch_run", "uvlm_batch_autotune", "uvlm_batch_steps",
     "uvlm_batch_get_gamma", "uvlm_batch_get_wing", "uvlm_batch_get_wake",
@@ -170,6 +171,7 @@ def gpu_lib():
         L.uvlm_farfield_set_mlp.argtypes = [_vp, _fp, C.c_double]
         _ull = C.POINTER(C.c_ulonglong)
         L.uvlm_farfield_mlp_stats.argtypes = [_vp, _ull, _ull]
+        L.uvlm_farfield_target_order.argtypes = [_vp, _dp, _sz, _ull, C.POINTER(C.c_int)]
         L.uvlm_farfield_samples.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
                                             C.c_uint, _fp, _fp, C.POINTER(_sz)]
         L.uvlm_sim_farfield_samples.argtypes = [_vp, C.c_double, C.c_uint, _fp, _fp,

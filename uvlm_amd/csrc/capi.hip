@@ -709,7 +709,7 @@ int uvlm_influence_matrix(uvlm_ctx* c, const double* nodes, size_t num,
                           const double* normals, unsigned flags, double* A) {
   if (!c) return UVLM_ERR_ARG;
   TRY(set_dev(c));
-  (void)flags;
+  TRY(check_prec(c, flags));
   const size_t N = num * rows * cols;
   if (N == 0) return UVLM_OK;
   if (!nodes || !cpos || !normals || !A) return fail(c, UVLM_ERR_ARG, "null argument");
@@ -727,8 +727,12 @@ int uvlm_influence_matrix(uvlm_ctx* c, const double* nodes, size_t num,
                            hipMemcpyHostToDevice, c->stream));
   launch_pack_lattice(c->stream, P<double>(c->t_a), (int)rows + 1, 0, nullptr, 0,
                       0, (int)num, (int)rows, (int)cols, P<RingRec>(c->t_recs));
-  launch_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b),
-                P<double>(c->t_c), (int)N, c->cut, P<double>(c->t_A));
+  if (prec_of(flags) == kF64Exact)
+    launch_matrix(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b), P<double>(c->t_c),
+                  (int)N, c->cut, P<double>(c->t_A));
+  else
+    launch_matrix_fast(c->stream, P<RingRec>(c->t_recs), P<double>(c->t_b), P<double>(c->t_c),
+                       (int)N, c->cut, P<double>(c->t_A));
   TRY(check_launch(c));
   // device matrix is column-major: A^T row-major == A column-major
   double* tmp = new (std::nothrow) double[N * N];
@@ -1051,8 +1055,12 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
   }
   launch_rhs(c->stream, P<double>(c->uw), P<double>(c->vm), P<double>(c->nrm), N, ff,
              P<double>(c->rhs));
-  launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
-                c->cut, P<double>(c->A));
+  if (prec_of(c->flags) == kF64Exact)
+    launch_matrix(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
+                  c->cut, P<double>(c->A));
+  else
+    launch_matrix_fast(c->stream, P<RingRec>(c->recs), P<double>(c->cp), P<double>(c->nrm), N,
+                       c->cut, P<double>(c->A));
   TRY(check_launch(c));
   c->pairs_eval += 4.0 * N * (double)N;
   c->pairs_ref += 4.0 * N * (double)N + 4.0 * N * (double)Nw;
@@ -1608,6 +1616,35 @@ int copy_samples(uvlm_ctx* c, const std::vector<float>& f, const std::vector<flo
   return UVLM_OK;
 }
 }  // namespace
+
+int uvlm_farfield_target_order(uvlm_ctx* c, const double* targets, size_t nt,
+                               unsigned long long* keys, int* order) {
+  if (!c || (nt && (!targets || !keys || !order))) return UVLM_ERR_ARG;
+  if (nt > (1u << 26)) return fail(c, UVLM_ERR_ARG, "problem too large");
+  TRY(set_dev(c));
+  if (nt == 0) return UVLM_OK;
+  if (!c->ff) {
+    c->ff = new (std::nothrow) FarField();
+    if (c->ff) c->ff->set_profile(c->prof);
+  }
+  if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
+  const size_t padded = (nt + 63) / 64 * 64;
+  TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
+  TRY(ensure(c, c->t_out, std::max(sizeof(unsigned long long) * nt, sizeof(int) * padded)));
+  TRY(ensure(c, c->t_b, sizeof(int) * padded));
+  HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice,
+                           c->stream));
+  std::string e;
+  const int rc = c->ff->morton_order(c->stream, P<double>(c->t_a), (int)nt, (int)(padded / 64),
+                                     P<int>(c->t_b), P<unsigned long long>(c->t_out), &e);
+  if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
+  HIPCHK(c, hipMemcpyAsync(keys, c->t_out.p, sizeof(unsigned long long) * nt,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(order, c->t_b.p, sizeof(int) * padded, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return UVLM_OK;
+}
 
 int uvlm_farfield_samples(uvlm_ctx* c, const double* targets, size_t nt, const double* nodes,
                           const double* gamma, size_t num, size_t rows, size_t cols,

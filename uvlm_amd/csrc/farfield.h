@@ -128,6 +128,11 @@ class FarField {
   // once per (tile, far cluster) and the far kernel adds the corrections.
   int velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
                std::string* err, const FFFocus* focus = nullptr, bool per_target = false);
+  // Targets tiled in Morton order of their bounding box (own stable radix
+  // sort of 63-bit keys): map[ntiles * 64] gets the target indices, -1
+  // beyond nt; keys_out (device, optional) receives the unsorted keys.
+  int morton_order(hipStream_t s, const double* targets, int nt, int ntiles, int* map,
+                   unsigned long long* keys_out, std::string* err);
   // build + velocity.
   int wake_velocity(hipStream_t s, const double* nodes, const double* gamma, int num, int rows,
                     int cols, const double* targets, int nt, double theta, double* out,
@@ -174,20 +179,20 @@ class FarField {
 
  private:
   int ensure(void** p, size_t* cap, size_t bytes, std::string* err);
-  int sort_pairs(hipStream_t s, int n, std::string* err);
+
 
   FFTree tree_;
   bool built_ = false, mlp_on_ = false, f32c_ = false;
   float ymax_ = 0.0f;
   // device buffers
   void *d_rec_ = nullptr, *d_key_ = nullptr, *d_key2_ = nullptr,
-       *d_idx_ = nullptr, *d_idx2_ = nullptr, *d_sort_tmp_ = nullptr, *d_box_ = nullptr,
+       *d_idx_ = nullptr, *d_idx2_ = nullptr, *d_hist_ = nullptr, *d_box_ = nullptr,
        *d_cl_ = nullptr, *d_mom_ = nullptr, *d_map_ = nullptr, *d_tiles_ = nullptr,
        *d_near_ = nullptr, *d_far_ = nullptr, *d_cnt_ = nullptr, *d_front_ = nullptr,
        *d_part_ = nullptr, *d_w_ = nullptr, *d_wf_ = nullptr, *d_mstat_ = nullptr,
        *d_samp_ = nullptr, *d_corr_ = nullptr, *d_tr_ = nullptr, *d_nrow_ = nullptr;
   size_t c_rec_ = 0, c_key_ = 0, c_key2_ = 0, c_idx_ = 0, c_idx2_ = 0,
-         c_sort_tmp_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
+         c_hist_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
          c_near_ = 0, c_far_ = 0, c_cnt_ = 0, c_front_ = 0, c_part_ = 0, c_w_ = 0, c_wf_ = 0,
          c_mstat_ = 0, c_samp_ = 0, c_corr_ = 0, c_tr_ = 0, c_nrow_ = 0;
   // near-sum measurement

@@ -18,8 +18,6 @@
 // T_m = the m-th derivative tensor of 1/r.  Level-0 moments are integrated
 // segment by segment (exact polynomials in the segment parameter); parents
 // translate their children's.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -96,9 +94,12 @@ __global__ __launch_bounds__(1024) void k_bbox(const double* __restrict__ x, int
 // reduce a contiguous range to part[g][6] (min x, y, z, max x, y, z), then
 // one wavefront reduces the parts in order into box[6].
 constexpr int kBoxParts = 64;
+// It also clears the digit counts of the Morton sort (hz[nz]).
 __global__ __launch_bounds__(256) void k_bbox_part(const double* __restrict__ x, int n,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part, int* __restrict__ hz,
+                                                   int nz) {
   __shared__ double s[6][4];
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < nz; q += kBoxParts * 256) hz[q] = 0;
   const int per = (n + kBoxParts - 1) / kBoxParts;
   const int q0 = blockIdx.x * per, q1 = min(n, q0 + per);
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -146,10 +147,116 @@ __device__ inline unsigned long long spread21(unsigned long long v) {
   return v;
 }
 
-// 63-bit Morton key of each point in its box (21 bits per axis).
+// --- stable radix sort of the targets' Morton keys ---------------------------
+// LSD, kRsPasses passes of kRsBits-bit digits over the 63-bit keys (with
+// their target indices).  The keys are cut into nb <= kRsMaxRows contiguous
+// rows of `per` keys (a multiple of kRsChunk); pass p reads h_p[nb][256],
+// the digit counts of every row of its input, and the workgroup of row b
+// places each of its keys of digit d at
+//   (keys of digits < d) + (keys of digit d in rows < b)
+//   + (keys of digit d before it in row b),
+// the last term from wavefront ballots over chunks of kRsChunk keys.  While
+// placing them it counts the next pass's digits per destination row into
+// h_{p+1} (integer atomics: the counts do not depend on their order), so a
+// pass is one launch.  Stable, so the order equals any stable sort's.
+constexpr int kRsBits = 8, kRsBins = 1 << kRsBits, kRsPasses = 8;  // 64 >= 63 key bits
+constexpr int kRsThreads = 256, kRsItems = 16, kRsChunk = kRsThreads * kRsItems;
+constexpr int kRsMaxRows = 128;
+struct RsPlan {
+  int nb, per;
+};
+inline RsPlan rs_plan(int n) {
+  const int per = cdiv(cdiv(n, kRsMaxRows), kRsChunk) * kRsChunk;
+  return RsPlan{cdiv(n, per), per};
+}
+
+// One pass over the keys (kin, iin) -> (kout, iout): kout may be null (the
+// last pass needs no keys); iout gets -1 at positions n..pad_to-1 (the tile
+// map's padding).  hnext may be null (no next pass).
+__global__ __launch_bounds__(kRsThreads) void k_rs_pass(
+    const unsigned long long* __restrict__ kin, const int* __restrict__ iin,
+    unsigned long long* __restrict__ kout, int* __restrict__ iout, int n, int per, int nb,
+    int shift, const int* __restrict__ hist, int* __restrict__ hnext, int pad_to) {
+  __shared__ int cnt[kRsItems][4][kRsBins];  // per (item, wavefront, digit)
+  __shared__ int base[kRsBins];
+  __shared__ int wsum[4];
+  const int t = threadIdx.x, b = blockIdx.x, lane = t & 63, w = t >> 6;
+  // thread t owns digit t: keys of digit t in all rows and in rows before b
+  int total = 0, pre = 0;
+  for (int r = 0; r < nb; ++r) {
+    const int v = hist[r * kRsBins + t];
+    total += v;
+    pre += r < b ? v : 0;
+  }
+  // exclusive prefix of the totals over the digits
+  int inc = total;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(inc, off);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int q = 0; q < w; ++q) before += wsum[q];
+  int run = before + inc - total + pre;  // next free position of digit t for this row
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int lo = b * per, hi = min(n, lo + per);
+  for (int c0 = lo; c0 < hi; c0 += kRsChunk) {
+    for (int i = 0; i < kRsItems; ++i)
+      for (int q = 0; q < 4; ++q) cnt[i][q][t] = 0;
+    base[t] = run;
+    __syncthreads();
+    unsigned long long key[kRsItems];
+    int id[kRsItems], rk[kRsItems];
+    for (int i = 0; i < kRsItems; ++i) {
+      const int e = c0 + i * kRsThreads + t;
+      key[i] = e < hi ? kin[e] : 0ull;
+      id[i] = e < hi ? iin[e] : -1;
+    }
+    for (int i = 0; i < kRsItems; ++i) {
+      const bool ok = c0 + i * kRsThreads + t < hi;
+      const int d = (int)((key[i] >> shift) & (kRsBins - 1));
+      unsigned long long peers = __ballot(ok);  // lanes of this wavefront with digit d
+      for (int bit = 0; bit < kRsBits; ++bit) {
+        const bool on = (d >> bit) & 1;
+        const unsigned long long m = __ballot(on);
+        peers &= on ? m : ~m;
+      }
+      rk[i] = __popcll(peers & lt);
+      if (ok && rk[i] == 0) cnt[i][w][d] = __popcll(peers);
+    }
+    __syncthreads();
+    // digit t: exclusive prefix over (item, wavefront), the keys' order
+    int s = 0;
+    for (int i = 0; i < kRsItems; ++i)
+      for (int q = 0; q < 4; ++q) {
+        const int c = cnt[i][q][t];
+        cnt[i][q][t] = s;
+        s += c;
+      }
+    __syncthreads();
+    for (int i = 0; i < kRsItems; ++i) {
+      if (c0 + i * kRsThreads + t >= hi) continue;
+      const int d = (int)((key[i] >> shift) & (kRsBins - 1));
+      const int dst = base[d] + cnt[i][w][d] + rk[i];
+      if (kout) kout[dst] = key[i];
+      iout[dst] = id[i];
+      if (hnext)
+        atomicAdd(&hnext[(dst / per) * kRsBins + (int)((key[i] >> (shift + kRsBits)) & (kRsBins - 1))],
+                  1);
+    }
+    run += s;
+    __syncthreads();  // cnt and base are rewritten by the next chunk
+  }
+  if (b == 0)
+    for (int q = n + t; q < pad_to; q += kRsThreads) iout[q] = -1;
+}
+
+// 63-bit Morton key of each point in its box (21 bits per axis); counts the
+// keys' lowest digits per row of the sort plan into h0 (cleared beforehand).
 __global__ void k_morton(const double* __restrict__ x, int n,
                          const double* __restrict__ box, unsigned long long* __restrict__ key,
-                         int* __restrict__ idx) {
+                         int* __restrict__ idx, int per, int* __restrict__ h0) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   unsigned long long k = 0;
@@ -163,6 +270,7 @@ __global__ void k_morton(const double* __restrict__ x, int n,
   }
   key[q] = k;
   idx[q] = q;
+  atomicAdd(&h0[(q / per) * kRsBins + (int)(k & (kRsBins - 1))], 1);
 }
 
 // Component m of the moments: type 0 (m < 3), 1 (< 12), 2 (< 39), 3 (< 120)
@@ -417,11 +525,6 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_parent(FFTree t, int l,
 }
 
 // --- targets -------------------------------------------------------------------
-__global__ void k_tile_map(const int* __restrict__ perm, int nt, int ntiles, int* __restrict__ map) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < ntiles * kTileT) map[k] = k < nt ? perm[k] : -1;
-}
-
 __device__ __forceinline__ int tile_target(const int* map, int tile, int lane) {
   return map[tile * kTileT + lane];
 }
@@ -1404,7 +1507,7 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 
 // ===========================================================================
 FarField::~FarField() {
-  void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_sort_tmp_,
+  void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_hist_,
                   d_box_,  d_cl_,    d_mom_,  d_map_,  d_tiles_, d_near_, d_far_,
                   d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_, d_np_, d_corr_,
                   d_tr_,   d_nrow_};
@@ -1472,22 +1575,47 @@ int FarField::ensure(void** p, size_t* cap, size_t bytes, std::string* err) {
   return UVLM_OK;
 }
 
-// (d_key_, d_idx_) -> sorted (d_key2_, d_idx2_), n pairs, stable.
-int FarField::sort_pairs(hipStream_t s, int n, std::string* err) {
-  size_t bytes = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned long long*)d_key_,
-                                         (unsigned long long*)d_key2_, (const int*)d_idx_,
-                                         (int*)d_idx2_, n, 0, 63, s) != hipSuccess) {
-    *err = "radix sort sizing failed";
+// Targets in Morton order of their own bounding box: map[ntiles * kTileT]
+// gets the target indices (stable in the index among equal keys), -1 beyond
+// nt; keys_out (device, optional) the unsorted keys.
+int FarField::morton_order(hipStream_t s, const double* targets, int nt, int ntiles, int* map,
+                           unsigned long long* keys_out, std::string* err) {
+  const RsPlan rp = rs_plan(nt);
+  const size_t nk = (size_t)nt, nh = (size_t)kRsPasses * rp.nb * kRsBins;
+  int rc;
+  if ((rc = ensure(&d_key_, &c_key_, sizeof(unsigned long long) * nk, err)) ||
+      (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned long long) * nk, err)) ||
+      (rc = ensure(&d_idx_, &c_idx_, sizeof(int) * nk, err)) ||
+      (rc = ensure(&d_idx2_, &c_idx2_, sizeof(int) * nk, err)) ||
+      (rc = ensure(&d_hist_, &c_hist_, sizeof(int) * nh, err)) ||
+      (rc = ensure(&d_box_, &c_box_, sizeof(double) * (8 + 6 * kBoxParts), err)))
+    return rc;
+  double* tbox = (double*)d_box_;
+  int* hist = (int*)d_hist_;
+  hipLaunchKernelGGL(k_bbox_part, dim3(kBoxParts), dim3(256), 0, s, targets, nt, tbox + 8, hist,
+                     (int)nh);
+  hipLaunchKernelGGL(k_bbox_fin, dim3(1), dim3(64), 0, s, (const double*)(tbox + 8), tbox);
+  hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
+                     (const double*)tbox, (unsigned long long*)d_key_, (int*)d_idx_, rp.per, hist);
+  if (keys_out &&
+      hipMemcpyAsync(keys_out, d_key_, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice,
+                     s) != hipSuccess) {
+    *err = "key copy failed";
     return UVLM_ERR_HIP;
   }
-  int rc = ensure(&d_sort_tmp_, &c_sort_tmp_, bytes, err);
-  if (rc != UVLM_OK) return rc;
-  bytes = c_sort_tmp_;
-  if (hipcub::DeviceRadixSort::SortPairs(d_sort_tmp_, bytes, (const unsigned long long*)d_key_,
-                                         (unsigned long long*)d_key2_, (const int*)d_idx_,
-                                         (int*)d_idx2_, n, 0, 63, s) != hipSuccess) {
-    *err = "radix sort failed";
+  unsigned long long* K[2] = {(unsigned long long*)d_key_, (unsigned long long*)d_key2_};
+  int* I[2] = {(int*)d_idx_, (int*)d_idx2_};
+  for (int p = 0; p < kRsPasses; ++p) {
+    const bool last = p + 1 == kRsPasses;
+    const int* h = hist + (size_t)p * rp.nb * kRsBins;
+    int* hn = last ? nullptr : hist + (size_t)(p + 1) * rp.nb * kRsBins;
+    hipLaunchKernelGGL(k_rs_pass, dim3(rp.nb), dim3(kRsThreads), 0, s,
+                       (const unsigned long long*)K[p & 1], (const int*)I[p & 1],
+                       last ? nullptr : K[(p + 1) & 1], last ? map : I[(p + 1) & 1], nt, rp.per,
+                       rp.nb, p * kRsBits, h, hn, last ? ntiles * kTileT : 0);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    *err = "Morton order launch failed";
     return UVLM_ERR_HIP;
   }
   return UVLM_OK;
@@ -1554,13 +1682,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
   const FFTree& t = tree_;
   const int ntiles = cdiv(nt, kTileT);
   int rc;
-  const size_t nk = (size_t)nt;
   const int n0 = t.num * t.nbr[0] * t.nbc[0];
-  if ((rc = ensure(&d_key_, &c_key_, sizeof(unsigned long long) * nk, err)) ||
-      (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned long long) * nk, err)) ||
-      (rc = ensure(&d_idx_, &c_idx_, sizeof(int) * nk, err)) ||
-      (rc = ensure(&d_idx2_, &c_idx2_, sizeof(int) * nk, err)) ||
-      (rc = ensure(&d_map_, &c_map_, sizeof(int) * (size_t)ntiles * kTileT, err)) ||
+  if ((rc = ensure(&d_map_, &c_map_, sizeof(int) * (size_t)ntiles * kTileT, err)) ||
       (rc = ensure(&d_tiles_, &c_tiles_, sizeof(double4) * (size_t)ntiles, err)) ||
       (rc = ensure(&d_near_, &c_near_, sizeof(int) * (size_t)ntiles * n0, err)) ||
       (rc = ensure(&d_far_, &c_far_, sizeof(int) * (size_t)ntiles * t.total, err)) ||
@@ -1570,15 +1693,8 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
                    sizeof(double) * 3 * (kNearGroups + kGroups) * (size_t)nt, err)))
     return rc;
   // target tiles: the targets in Morton order of their own box
-  double* tbox = (double*)d_box_;
-  hipLaunchKernelGGL(k_bbox_part, dim3(kBoxParts), dim3(256), 0, s, targets, nt, tbox + 8);
-  hipLaunchKernelGGL(k_bbox_fin, dim3(1), dim3(64), 0, s, (const double*)(tbox + 8), tbox);
-  hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
-                     (const double*)tbox, (unsigned long long*)d_key_, (int*)d_idx_);
-  if ((rc = sort_pairs(s, nt, err)) != UVLM_OK) return rc;
   int* map = (int*)d_map_;
-  hipLaunchKernelGGL(k_tile_map, dim3(cdiv((long)ntiles * kTileT, 256)), dim3(256), 0, s,
-                     (const int*)d_idx2_, nt, ntiles, map);
+  if ((rc = morton_order(s, targets, nt, ntiles, map, nullptr, err)) != UVLM_OK) return rc;
   hipLaunchKernelGGL(k_ff_tiles, dim3(ntiles), dim3(64), 0, s, targets, (const int*)map,
                      (double4*)d_tiles_);
   const bool graded = focus && focus->box && focus->reach > 0.0;

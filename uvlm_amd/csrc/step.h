@@ -9,6 +9,9 @@ void launch_geometry(hipStream_t s, const double* nodes, int num, int R, int C,
                      double* cp, double* nrm, double* area);
 void launch_matrix(hipStream_t s, const RingRec* recs, const double* cp,
                    const double* nrm, int N, const CutOff& cut, double* A);
+// Same matrix in the rsqrt form of the FP64-chain and F32C modes.
+void launch_matrix_fast(hipStream_t s, const RingRec* recs, const double* cp,
+                        const double* nrm, int N, const CutOff& cut, double* A);
 // Legacy ring order (A[L*N + K] = ring_L.BiotSavartLaw(cp_K, 1) . n_K).
 void launch_ring_matrix(hipStream_t s, const RingRec* recs, const double* cp,
                         const double* nrm, int N, const CutOff& cut, double* A);

@@ -7,6 +7,7 @@
 // tree order instead of the reference's OpenMP reduction.
 #include <cmath>
 
+#include "devmath.h"
 #include "internal.h"
 #include "step.h"
 
@@ -108,6 +109,62 @@ __global__ __launch_bounds__(256) void k_matrix(const RingRec* __restrict__ recs
         u = add(u, seg_exact(p[k], p[(k + 1) & 3], x, cut));
     }
     A[(size_t)(l0 + l) * N + K] = dot(u, nl);
+  }
+}
+
+// Rsqrt form of the same matrix for the FP64-chain and F32C modes: per
+// (collocation point, ring) one refined hardware rsqrt per corner (shared by
+// the two segments that meet there) and one refined reciprocal per segment,
+// FMA-contracted, each segment's term projected on the normal before the
+// sum; A = (sum of d.n r0.(r1/|r1| - r2/|r2|) / |d|^2) / 4pi.  The cut-off
+// tests are the segment chains' (|d|^2 < eps, |r1|, |r2| < eps on the
+// squares).  Agrees with the reference order to ~1e-14 of max |A| (the
+// rsqrt and reciprocal carry one Newton step each, as in the chains).
+__global__ __launch_bounds__(256) void k_matrix_fast(const RingRec* __restrict__ recs,
+                                                     const double* __restrict__ cp,
+                                                     const double* __restrict__ nrm, int N,
+                                                     CutOff cut, double* __restrict__ A) {
+  __shared__ double2 tile[kMatRings * 8];
+  const int l0 = blockIdx.y * kMatRings;
+  const int cnt = min(kMatRings, N - l0);
+  const double2* src = reinterpret_cast<const double2*>(recs + l0);
+  for (int q = threadIdx.x; q < cnt * 8; q += 256) tile[q] = src[q];
+  __syncthreads();
+  const int K = blockIdx.x * 256 + threadIdx.x;
+  if (K >= N) return;
+  const d3 x = ld3(cp + 3 * (size_t)K);
+  const d3 nl = ld3(nrm + 3 * (size_t)K);
+  const double* tl = reinterpret_cast<const double*>(tile);
+  for (int l = 0; l < cnt; ++l) {
+    const double* r = tl + 16 * l;
+    double rx[4], ry[4], rz[4], rs[4], inv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      rx[k] = x.x - r[3 * k];
+      ry[k] = x.y - r[3 * k + 1];
+      rz[k] = x.z - r[3 * k + 2];
+      rs[k] = __builtin_fma(rz[k], rz[k], __builtin_fma(ry[k], ry[k], rx[k] * rx[k]));
+      inv[k] = dev::rsq64_1(rs[k]);
+    }
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int b = (k + 1) & 3;
+      const double r0x = r[3 * b] - r[3 * k], r0y = r[3 * b + 1] - r[3 * k + 1],
+                   r0z = r[3 * b + 2] - r[3 * k + 2];
+      const double dx = __builtin_fma(r0y, rz[k], -r0z * ry[k]);
+      const double dy = __builtin_fma(r0z, rx[k], -r0x * rz[k]);
+      const double dz = __builtin_fma(r0x, ry[k], -r0y * rx[k]);
+      const double dsq = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      const double o1 = __builtin_fma(r0z, rz[k], __builtin_fma(r0y, ry[k], r0x * rx[k]));
+      const double o2 = __builtin_fma(r0z, rz[b], __builtin_fma(r0y, ry[b], r0x * rx[b]));
+      const double tt = __builtin_fma(o1, inv[k], -o2 * inv[b]);
+      const double dn = __builtin_fma(dz, nl.z, __builtin_fma(dy, nl.y, dx * nl.x));
+      const bool cutk = (dsq < cut.eps) | (rs[k] < cut.eps2) | (rs[b] < cut.eps2);
+      const double t = dn * tt * dev::rcp64_1(dsq);
+      a += cutk ? 0.0 : t;
+    }
+    A[(size_t)(l0 + l) * N + K] = a * kInv4Pi;
   }
 }
 
@@ -312,6 +369,12 @@ void launch_matrix(hipStream_t s, const RingRec* recs, const double* cp,
                    const double* nrm, int N, const CutOff& cut, double* A) {
   dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
   hipLaunchKernelGGL(k_matrix<false>, g, dim3(256), 0, s, recs, cp, nrm, N, cut, A);
+}
+
+void launch_matrix_fast(hipStream_t s, const RingRec* recs, const double* cp,
+                        const double* nrm, int N, const CutOff& cut, double* A) {
+  dim3 g(cdiv(N, 256), cdiv(N, kMatRings));
+  hipLaunchKernelGGL(k_matrix_fast, g, dim3(256), 0, s, recs, cp, nrm, N, cut, A);
 }
 
 void launch_ring_matrix(hipStream_t s, const RingRec* recs, const double* cp,

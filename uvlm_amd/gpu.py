@@ -151,6 +151,19 @@ class Context:
                     "uvlm_farfield_velocity")
         return out
 
+    def farfield_target_order(self, targets):
+        """Morton keys of the targets and the split's tile order
+        (uvlm_farfield_target_order): keys [n] uint64, order [64 ceil(n/64)]
+        int32 padded with -1."""
+        x = _f64(targets).reshape(-1, 3)
+        n = x.shape[0]
+        keys = np.zeros(n, dtype=np.uint64)
+        order = np.zeros((n + 63) // 64 * 64, dtype=np.int32)
+        self._check(self._L.uvlm_farfield_target_order(
+            self._h, _p(x), n, keys.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+            order.ctypes.data_as(C.POINTER(C.c_int))), "uvlm_farfield_target_order")
+        return keys, order
+
     # --- far-field correction network ---------------------------------------
     def mlp_forward(self, w, X, mode=N.MLP_FP32):
         w, X = _f32(w), _f32(X).reshape(-1, N.MLP_IN)
