@@ -165,10 +165,31 @@ def test_strided_audit_of_a_large_wake(orc):
     # audits at steps 1, 31, ..., 271; the last one's wake has 2 * 272 * 65 nodes
     assert audits == 10 and active and 0 < err < 1e-3
     b.set_farfield(theta=2.0, audit_every=10, tol=1e-15)
+    prev, _ = b.wake()  # node rows 1.. of the state the next step audits
     b.step(272)  # audited: 2 * 273 * 65 wake nodes, every 8th summed exactly
     b.sync()
     err, audits, active = b.farfield_stats()
     assert audits == 2 and not active and err > 1e-15
+    # The reported difference is the strided maximum over that state:
+    # nodes off + 8k (off = audits before this one, mod 8), trailing-edge
+    # rows skipped, against the oracle's exact stage velocity there; the
+    # split's velocities are the Euler update's (post - pre) / dt.
+    R, C1, dt = case["rows"], case["cols"] + 1, case["dt"]
+    ff = np.array(case["forward_flight"])
+    wing, gam = b.wing_nodes(), b.gamma()
+    post, wgam = b.wake()
+    te = wing[:, R]
+    np.testing.assert_array_equal(post[:, 0], te + dt * -ff)
+    pre = np.concatenate([te[:, None], prev], axis=1)
+    vel = ((post - pre) / dt).reshape(-1, 3)
+    V, per = pre.shape[0] * pre.shape[1] * C1, pre.shape[1] * C1
+    idx = np.arange(1 % 8, V, 8)
+    idx = idx[idx % per >= C1]
+    x = pre.reshape(-1, 3)[idx]
+    exact = -ff + orc.induced_velocity(x, wing, gam) + orc.induced_velocity(x, pre, wgam)
+    want = np.abs(vel[idx] - exact).max() / np.abs(exact).max()
+    assert 1e-9 < want < 1e-3
+    np.testing.assert_allclose(err, want, rtol=1e-6)
 
 
 def test_simulation_learns_the_correction(orc):

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
     z = targets[3 * t + 2];
   }
   double pax = 0.0, pay = 0.0, paz = 0.0;  // previous node (FP64, near-core path)
-  dev::End32 A = {0.f, 0.f, 0.f, 0.f, 0.f, true}, B;  // element 0 starts a chain (g = 0)
+  dev::End32 A = {0.f, 0.f, 0.f, 0.f, 0.f}, B;  // element 0 starts a chain (g = 0)
   if (rb > 0) {
     const double* p = recs[rb - 1].p;
     pax = p[0];
@@ -132,7 +132,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
       const int kend = min(cnt, k0 + 16);
       // run start: previous node for the FP64 re-evaluation pass below
       const double qx = pax, qy = pay, qz = paz;
-      unsigned redo = 0;  // elements of the run to re-evaluate (bit k - k0)
+      // lanes with an element of the run to re-evaluate in FP64 (wave mask)
+      unsigned long long redo = 0ull;
+      const dev::End32 A0 = A;  // the run's first end, for that pass
       // Two elements per trip, the end point alternating between A and B;
       // chain starts (g = 0, r0 = 0) fall in the band and are not re-evaluated.
       int k = k0;
@@ -142,16 +144,15 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         const float* f0 = reinterpret_cast<const float*>(e0 + 4);
         const float* f1 = reinterpret_cast<const float*>(e1 + 4);
         dev::end32(e0, x, y, z, B);
-        redo |= (unsigned)(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0) << (k - k0);
+        redo |= __ballot(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0);
         dev::end32(e1, x, y, z, A);
-        redo |= (unsigned)(dev::elem32(B, A, f1, f1[4], fx, fy, fz) && e1[3] != 0.0)
-                << (k + 1 - k0);
+        redo |= __ballot(dev::elem32(B, A, f1, f1[4], fx, fy, fz) && e1[3] != 0.0);
       }
       if (k < kend) {
         const double* e0 = tl + 8 * k;
         const float* f0 = reinterpret_cast<const float*>(e0 + 4);
         dev::end32(e0, x, y, z, B);
-        redo |= (unsigned)(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0) << (k - k0);
+        redo |= __ballot(dev::elem32(A, B, f0, f0[4], fx, fy, fz) && e0[3] != 0.0);
         A = B;
       }
       {
@@ -160,12 +161,16 @@ __global__ __launch_bounds__(kBlock) void k_chain_f32c(
         pay = el[1];
         paz = el[2];
       }
-      if (__any(redo != 0)) {  // rare: pairs near the cut-off again, in FP64
+      if (redo) {  // rare: pairs near the cut-off again, in FP64
+        const bool mine = (redo >> __lane_id()) & 1ull;
         double ax = qx, ay = qy, az = qz;
+        dev::End32 P = A0, Q;
         for (int k = k0; k < kend; ++k) {
           const double* e = tl + 8 * k;
-          if ((redo >> (k - k0)) & 1u)
+          dev::end32(e, x, y, z, Q);
+          if (mine && e[3] != 0.0 && dev::in32(P, Q, reinterpret_cast<const float*>(e + 4)))
             dev::seg_ref(ax, ay, az, e[0], e[1], e[2], x, y, z, e[3], ux, uy, uz);
+          P = Q;
           ax = e[0];
           ay = e[1];
           az = e[2];

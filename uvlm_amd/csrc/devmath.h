@@ -124,10 +124,9 @@ constexpr float kBand32 = 0x1p-42f;
 
 // End point of a segment seen from the target, in FP32: offsets (x - p
 // formed in FP64, so exact to FP32 rounding relative to |x - p|), squared
-// distance, its reciprocal root, and whether it lies in the cut-off band.
+// distance and its reciprocal root.
 struct End32 {
   float rx, ry, rz, rs, inv;
-  bool tiny;
 };
 
 __device__ __forceinline__ void end32(const double* p, double x, double y, double z, End32& e) {
@@ -136,30 +135,57 @@ __device__ __forceinline__ void end32(const double* p, double x, double y, doubl
   e.rz = (float)(z - p[2]);
   e.rs = __builtin_fmaf(e.rz, e.rz, __builtin_fmaf(e.ry, e.ry, e.rx * e.rx));
   e.inv = __builtin_amdgcn_rsqf(e.rs);
-  e.tiny = e.rs < 1e-18f;
+}
+
+// d = r0 x (x - a) of a segment a -> b (f = (float)(b - a), ...) and |d|^2.
+__device__ __forceinline__ void cross32(const End32& a, const float* f, float& dx, float& dy,
+                                        float& dz, float& dsq) {
+  const float r0x = f[0], r0y = f[1], r0z = f[2];
+  dx = __builtin_fmaf(r0y, a.rz, -r0z * a.ry);
+  dy = __builtin_fmaf(r0z, a.rx, -r0x * a.rz);
+  dz = __builtin_fmaf(r0x, a.ry, -r0y * a.rx);
+  dsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// The cut-off band of the FP32 law: |d|^2 below 2e-9 plus the FP32 rounding
+// bound of d, or |r| < 1e-9 at either end.  Ends = false drops the end
+// tests, which the first implies for segments shorter than sqrt(kShortSeg2):
+// an end within 1e-9 of the target gives |d| <= |r0| 1e-9 plus the rounding
+// of x - a, which the band's bound covers.
+template <bool Ends>
+__device__ __forceinline__ bool band32(const End32& a, const End32& b, const float* f,
+                                       float dsq) {
+  const bool core = dsq < __builtin_fmaf(f[3], a.rs, 2e-9f);
+  return Ends ? core | (fminf(a.rs, b.rs) < 1e-18f) : core;
 }
 
 // One segment a -> b of strength g (with 1/4pi): f = (float)(b - a) and
-// kBand32 |b - a|^2.  Pairs within a band around the reference's cut-off
-// (|r| < 1e-9, |d|^2 < 2e-9 plus the FP32 rounding bound of d) add nothing
-// here and are reported, for the caller's FP64 re-evaluation.
+// kBand32 |b - a|^2 (so |b - a|^2 = f[3] 2^42 exactly).  Pairs within the
+// band add nothing here and are reported, for the caller's FP64
+// re-evaluation; in32 tells again, for the same ends, whether elem32
+// reported the pair (the same operations, so the same decision).
+template <bool Ends = true>
 __device__ __forceinline__ bool elem32(const End32& a, const End32& b, const float* f, float g,
                                        float& fx, float& fy, float& fz) {
+  float dx, dy, dz, dsq;
+  cross32(a, f, dx, dy, dz, dsq);
   const float r0x = f[0], r0y = f[1], r0z = f[2];
-  const float dx = __builtin_fmaf(r0y, a.rz, -r0z * a.ry);
-  const float dy = __builtin_fmaf(r0z, a.rx, -r0x * a.rz);
-  const float dz = __builtin_fmaf(r0x, a.ry, -r0y * a.rx);
-  const float dsq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
   const float oa = __builtin_fmaf(r0z, a.rz, __builtin_fmaf(r0y, a.ry, r0x * a.rx));
-  const float ob = __builtin_fmaf(r0z, b.rz, __builtin_fmaf(r0y, b.ry, r0x * b.rx));
+  const float ob = __builtin_fmaf(-f[3], 0x1p42f, oa);  // r0.(x - b) = r0.(x - a) - |r0|^2
   const float tt = __builtin_fmaf(oa, a.inv, -ob * b.inv);
-  const bool nearcore = (dsq < __builtin_fmaf(f[3], a.rs, 2e-9f)) | a.tiny | b.tiny;
+  const bool nearcore = band32<Ends>(a, b, f, dsq);
   float K = tt * g * __builtin_amdgcn_rcpf(dsq);
   K = nearcore ? 0.f : K;
   fx = __builtin_fmaf(dx, K, fx);
   fy = __builtin_fmaf(dy, K, fy);
   fz = __builtin_fmaf(dz, K, fz);
   return nearcore;
+}
+template <bool Ends = true>
+__device__ __forceinline__ bool in32(const End32& a, const End32& b, const float* f) {
+  float dx, dy, dz, dsq;
+  cross32(a, f, dx, dy, dz, dsq);
+  return band32<Ends>(a, b, f, dsq);
 }
 
 }  // namespace dev

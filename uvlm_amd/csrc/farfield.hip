@@ -739,6 +739,7 @@ constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 // (|r| < 1e-9, |d|^2 < 2e-9 plus the FP32 rounding bound of d) are
 // re-evaluated in FP64 exactly as the reference does.  Two elements per
 // trip, the end point alternating between two register sets.
+template <bool Ends>
 __device__ __forceinline__ void chain32(const double* R, int n0, int stride_nodes, int count,
                                         const double* S, const float* F, int s0, double x,
                                         double y, double z, double& ux, double& uy,
@@ -748,27 +749,31 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
   dev::End32 A, B;
   dev::end32(p, x, y, z, A);
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  unsigned redo = 0;  // elements to re-evaluate in FP64 (bit k)
-  static_assert(kFFCols <= 32 && kFFRows <= 32, "chain masks hold 32 elements");
+  // lanes with an element of this chain to re-evaluate in FP64 (a wave mask:
+  // scalar bookkeeping)
+  unsigned long long redo = 0ull;
   int k = 0;
   for (; k + 2 <= count; k += 2) {
     dev::end32(p + step, x, y, z, B);
-    redo |= (unsigned)dev::elem32(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz) << k;
+    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz));
     p += 2 * step;
     dev::end32(p, x, y, z, A);
-    redo |= (unsigned)dev::elem32(B, A, F + 4 * (s0 + k + 1), (float)S[s0 + k + 1], fx, fy, fz)
-            << (k + 1);
+    redo |= __ballot(dev::elem32<Ends>(B, A, F + 4 * (s0 + k + 1), (float)S[s0 + k + 1], fx, fy, fz));
   }
   if (k < count) {
     dev::end32(p + step, x, y, z, B);
-    redo |= (unsigned)dev::elem32(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz) << k;
+    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz));
   }
-  if (__any(redo != 0)) {  // rare: a second, FP64 pass over the flagged elements
+  if (redo) {  // rare: a second pass finds the flagged elements, in FP64
+    const bool mine = (redo >> __lane_id()) & 1ull;
     p = R + 3 * n0;
+    dev::end32(p, x, y, z, A);
     for (int e = 0; e < count; ++e) {
       const double* b = p + step;
-      if ((redo >> e) & 1u)
+      dev::end32(b, x, y, z, B);
+      if (mine && dev::in32<Ends>(A, B, F + 4 * (s0 + e)))
         dev::seg_ref(p[0], p[1], p[2], b[0], b[1], b[2], x, y, z, S[s0 + e], ux, uy, uz);
+      A = B;
       p = b;
     }
   }
@@ -777,16 +782,17 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
   uz += (double)fz;
 }
 
+template <bool Ends>
 __device__ __forceinline__ void walk_record32(const double* R, int nr, int nc, double px,
                                               double py, double pz, double& ux, double& uy,
                                               double& uz) {
   const double* S = R + kRecStr;
   const float* F = reinterpret_cast<const float*>(R + kRecSeg);
   for (int i = 0; i <= nr; ++i)  // node rows: spanwise segments
-    chain32(R, i * (nc + 1), 1, nc, S, F, i * nc, px, py, pz, ux, uy, uz);
+    chain32<Ends>(R, i * (nc + 1), 1, nc, S, F, i * nc, px, py, pz, ux, uy, uz);
   const int s1 = (nr + 1) * nc;
   for (int j = 0; j <= nc; ++j)  // node columns: chordwise segments
-    chain32(R, j, nc + 1, nr, S, F, s1 + j * nr, px, py, pz, ux, uy, uz);
+    chain32<Ends>(R, j, nc + 1, nr, S, F, s1 + j * nr, px, py, pz, ux, uy, uz);
 }
 
 // Exact velocity (with 1/4pi) at one target of the level-0 blocks under
@@ -894,8 +900,12 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
       id = list[k + kNearGroups];
       if (kNearPrefetch) load(id);
     }
-    if (F32)
-      walk_record32(sr, nr, nc, px, py, pz, ux, uy, uz);
+    // FP32 for blocks of short segments, whose end-point cut-offs the band
+    // implies; the FP64 walk with the end-point tests otherwise (a block
+    // 3e4 across: not met in practice, and one FP32 variant keeps the
+    // kernel at 5 waves per SIMD)
+    if (F32 && c.r < kShortRadius)
+      walk_record32<false>(sr, nr, nc, px, py, pz, ux, uy, uz);
     else if (c.r < kShortRadius)
       walk_record<false>(sr, nr, nc, px, py, pz, ux, uy, uz);
     else
