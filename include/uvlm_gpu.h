@@ -315,13 +315,13 @@ int uvlm_mlp_train(uvlm_ctx* ctx, float* weights, const float* feats, const floa
  * |y| > ymax (outside the trained range) is summed exactly instead. */
 int uvlm_farfield_set_mlp(uvlm_ctx* ctx, const float* weights, double ymax);
 /* Order in which the split tiles n_targets targets (no reference
- * counterpart; exposed for tests): the targets' 63-bit Morton keys in
- * their bounding box, 21 bits per axis interleaved x, y, z from bit 0
+ * counterpart; exposed for tests): the targets' 30-bit Morton keys in
+ * their bounding box, 10 bits per axis interleaved x, y, z from bit 0
  * (keys[n_targets], unsorted), and the target indices sorted by key, equal
  * keys in index order, padded with -1 to a multiple of 64
  * (order[64 * ceil(n_targets / 64)]). */
 int uvlm_farfield_target_order(uvlm_ctx* ctx, const double* targets, size_t n_targets,
-                               unsigned long long* keys, int* order);
+                               unsigned* keys, int* order);
 /* (target, cluster) pairs corrected and pairs tripped to the exact sum
  * since the network was set. */
 int uvlm_farfield_mlp_stats(uvlm_ctx* ctx, unsigned long long* pairs,

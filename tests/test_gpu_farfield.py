@@ -32,26 +32,28 @@ def wavy_lattice(rows, cols, seed=0, num=1):
 
 
 def morton_keys(x):
-    """63-bit Morton keys of points in their bounding box, 21 bits per axis
+    """30-bit Morton keys of points in their bounding box, 10 bits per axis
     (x in bit 0, y in bit 1, z in bit 2 of each triple), IEEE FP64 like the
     device's key kernel."""
     lo, hi = x.min(0), x.max(0)
     w = hi - lo
-    keys = np.zeros(x.shape[0], dtype=np.uint64)
+    keys = np.zeros(x.shape[0], dtype=np.uint32)
     for d in range(3):
         t = (x[:, d] - lo[d]) / w[d] if w[d] > 0 else np.zeros(x.shape[0])
-        c = np.floor(np.clip(t, 0.0, 1.0) * 2097151.0).astype(np.uint64)
-        for b in range(21):
-            keys |= ((c >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + d)
+        c = np.floor(np.clip(t, 0.0, 1.0) * 1023.0).astype(np.uint32)
+        for b in range(10):
+            keys |= ((c >> np.uint32(b)) & np.uint32(1)) << np.uint32(3 * b + d)
     return keys
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 4097, 9000, 300000, 1200000])
+@pytest.mark.parametrize("n", [1, 63, 64, 1025, 9000, 16384, 16385, 300000, 1200000])
 def test_target_order_is_a_stable_sort_of_the_morton_keys(ctx, n):
     """The split's tile order comes from its own LSD radix sort: the keys are
     the Morton keys of a direct restatement, the order is numpy's stable
     argsort of them (equal keys in index order), padded with -1; sizes
-    cover one partial chunk, several rows and rows of several chunks."""
+    cover the one-workgroup sort (up to 16384 keys, partial and full
+    wavefronts) and the multi-workgroup one (one partial chunk, several
+    rows, rows of several chunks)."""
     rng = np.random.default_rng(n)
     x = rng.normal(size=(n, 3)) * np.array([20.0, 3.0, 0.5])
     if n > 100:
@@ -218,6 +220,45 @@ def test_simulation_learns_the_correction(orc):
     la, lb = a.loads(), b.loads()
     scale = np.abs(la).max(axis=0)
     assert np.all(np.abs(la - lb).max(axis=0) / scale < 1e-3)
+
+
+def test_full_size_fast_form_with_the_network():
+    """The benchmark's fast form at the tandem workload's full size (2 x
+    16 x 64 rings): split with the graded criterion, F32C near sums,
+    kept-inverse solve, the correction network trained in the run at step
+    100 (same settings as the benchmark): CL/CT within 1e-4 of the plain
+    FP64 run over steps 0..30 (later the wake's roll-up amplifies any
+    rounding difference, so the 160-step run means are held to 1e-3), the
+    audits pass and the network corrects far pairs without tripping."""
+    from uvlm_amd import _native as N
+    from uvlm_amd.gpu import mlp_init
+    steps = 160
+    base = W.tandem_c5(steps=steps)
+    plain = Simulation(dict(base, precision=N.PREC_F64, solve_mode=N.SOLVE_F64), steps)
+    fast = Simulation(dict(base, precision=N.PREC_F32C, solve_mode=N.SOLVE_F64_INV), steps)
+    fast.set_farfield(theta=4.0, audit_every=50, tol=1e-3)
+    fast.set_farfield_focus(reach=1.5, theta_min=1.2)
+    w = mlp_init(5)
+    for s in range(steps + 1):
+        if s % 40 == 0:
+            print(f"[full-size network] step {s}", flush=True)
+        plain.step(s)
+        fast.step(s)
+        if s == 100:
+            w, loss = fast.farfield_learn(w, 4.0, max_samples=12000, iters=80, batch=4096,
+                                          lr=2e-3, seed=s)
+            assert np.isfinite(loss)
+    plain.sync()
+    fast.sync()
+    pairs, trips = fast.farfield_mlp_stats()
+    err, audits, active = fast.farfield_stats()
+    assert pairs > 0 and trips == 0
+    assert active and audits == 4 and err < 1e-3
+    lp, lf = plain.loads(), fast.loads()
+    assert np.isfinite(lf).all() and lf.shape == lp.shape == (steps + 1, 2, 2)
+    scale = np.abs(lp).max(axis=0)
+    assert (np.abs(lf[:31] - lp[:31]).max(axis=0) / scale).max() <= 1e-4
+    assert (np.abs(lf.mean(axis=0) - lp.mean(axis=0)) / scale).max() <= 1e-3
 
 
 def test_graded_criterion_away_from_the_wings(orc):

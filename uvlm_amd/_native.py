@@ -171,7 +171,8 @@ def gpu_lib():
         L.uvlm_farfield_set_mlp.argtypes = [_vp, _fp, C.c_double]
         _ull = C.POINTER(C.c_ulonglong)
         L.uvlm_farfield_mlp_stats.argtypes = [_vp, _ull, _ull]
-        L.uvlm_farfield_target_order.argtypes = [_vp, _dp, _sz, _ull, C.POINTER(C.c_int)]
+        L.uvlm_farfield_target_order.argtypes = [_vp, _dp, _sz, C.POINTER(C.c_uint),
+                                                 C.POINTER(C.c_int)]
         L.uvlm_farfield_samples.argtypes = [_vp, _dp, _sz, _dp, _dp, _sz, _sz, _sz, C.c_double,
                                             C.c_uint, _fp, _fp, C.POINTER(_sz)]
         L.uvlm_sim_farfield_samples.argtypes = [_vp, C.c_double, C.c_uint, _fp, _fp,

@@ -321,6 +321,9 @@ int sum_sources(uvlm_ctx* c, const double* targets, int nt, bool bound, bool wak
 // wake's own nodes: their velocities set the circulations and the loads
 // directly, and they cost little.
 constexpr double kWingThetaFactor = 2.0;
+// Tags of the wings' target sets in the split (their Morton order is kept
+// between steps; the wings' points keep their arrangement as they move).
+constexpr int kOrderWingTargets = 1, kOrderEdgeMidpoints = 2;
 // Audits of wakes of at least kAuditFull nodes sum every stride-th node
 // exactly, stride = max(kAuditStride, nodes / kAuditNodes): at most an
 // eighth of the cost of the exact stage, and ~kAuditNodes sums for the
@@ -1039,7 +1042,7 @@ int uvlm_sim_solve(uvlm_ctx* c, const double* vmorph_cp, const double ff[3],
     std::string e;
     const int rc = c->ff->velocity(c->stream, P<double>(c->wtg), N + E,
                                    kWingThetaFactor * c->ff_theta, P<double>(c->wvel), &e,
-                                   nullptr, true);
+                                   nullptr, true, kOrderWingTargets);
     if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     HIPCHK(c, hipMemcpyAsync(c->uw.p, c->wvel.p, sizeof(double) * 3 * N,
                              hipMemcpyDeviceToDevice, c->stream));
@@ -1140,7 +1143,7 @@ int uvlm_sim_loads(uvlm_ctx* c, const double* vmorph_edges, const double ff[3],
       std::string e;
       const int rc = c->ff->velocity(c->stream, P<double>(c->mid), E,
                                      kWingThetaFactor * c->ff_theta, P<double>(c->ffm), &e,
-                                     nullptr, true);
+                                     nullptr, true, kOrderEdgeMidpoints);
       if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
     }
     launch_ff_combine(c->stream, P<double>(c->partial), E, plan.cb, P<double>(c->ffm), E, ff, 0,
@@ -1618,7 +1621,7 @@ int copy_samples(uvlm_ctx* c, const std::vector<float>& f, const std::vector<flo
 }  // namespace
 
 int uvlm_farfield_target_order(uvlm_ctx* c, const double* targets, size_t nt,
-                               unsigned long long* keys, int* order) {
+                               unsigned* keys, int* order) {
   if (!c || (nt && (!targets || !keys || !order))) return UVLM_ERR_ARG;
   if (nt > (1u << 26)) return fail(c, UVLM_ERR_ARG, "problem too large");
   TRY(set_dev(c));
@@ -1630,15 +1633,15 @@ int uvlm_farfield_target_order(uvlm_ctx* c, const double* targets, size_t nt,
   if (!c->ff) return fail(c, UVLM_ERR_OOM, "far-field state");
   const size_t padded = (nt + 63) / 64 * 64;
   TRY(ensure(c, c->t_a, sizeof(double) * 3 * nt));
-  TRY(ensure(c, c->t_out, std::max(sizeof(unsigned long long) * nt, sizeof(int) * padded)));
+  TRY(ensure(c, c->t_out, std::max(sizeof(unsigned) * nt, sizeof(int) * padded)));
   TRY(ensure(c, c->t_b, sizeof(int) * padded));
   HIPCHK(c, hipMemcpyAsync(c->t_a.p, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice,
                            c->stream));
   std::string e;
   const int rc = c->ff->morton_order(c->stream, P<double>(c->t_a), (int)nt, (int)(padded / 64),
-                                     P<int>(c->t_b), P<unsigned long long>(c->t_out), &e);
+                                     P<int>(c->t_b), P<unsigned>(c->t_out), &e);
   if (rc != UVLM_OK) return fail(c, rc, "%s", e.c_str());
-  HIPCHK(c, hipMemcpyAsync(keys, c->t_out.p, sizeof(unsigned long long) * nt,
+  HIPCHK(c, hipMemcpyAsync(keys, c->t_out.p, sizeof(unsigned) * nt,
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(order, c->t_b.p, sizeof(int) * padded, hipMemcpyDeviceToHost,
                            c->stream));

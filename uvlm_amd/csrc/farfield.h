@@ -126,13 +126,20 @@ class FarField {
   // cluster is also far when each target (not the tile's sphere) clears
   // theta times its radius.  With the network, k_ff_tilenet evaluates it
   // once per (tile, far cluster) and the far kernel adds the corrections.
+  // order_tag != 0: the targets are a set the caller identifies by this tag
+  // (e.g. the wings' collocation points and edge midpoints), whose
+  // arrangement changes little between calls: their Morton order is kept
+  // from a previous call with the same tag and count and recomputed every
+  // kOrderReuse calls.  The order only groups targets into tiles; every
+  // tile's centre, radius and lists are formed from the current positions.
   int velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
-               std::string* err, const FFFocus* focus = nullptr, bool per_target = false);
+               std::string* err, const FFFocus* focus = nullptr, bool per_target = false,
+               int order_tag = 0);
   // Targets tiled in Morton order of their bounding box (own stable radix
-  // sort of 63-bit keys): map[ntiles * 64] gets the target indices, -1
+  // sort of 30-bit keys): map[ntiles * 64] gets the target indices, -1
   // beyond nt; keys_out (device, optional) receives the unsorted keys.
   int morton_order(hipStream_t s, const double* targets, int nt, int ntiles, int* map,
-                   unsigned long long* keys_out, std::string* err);
+                   unsigned* keys_out, std::string* err);
   // build + velocity.
   int wake_velocity(hipStream_t s, const double* nodes, const double* gamma, int num, int rows,
                     int cols, const double* targets, int nt, double theta, double* out,
@@ -195,6 +202,10 @@ class FarField {
          c_hist_ = 0, c_box_ = 0, c_cl_ = 0, c_mom_ = 0, c_map_ = 0, c_tiles_ = 0,
          c_near_ = 0, c_far_ = 0, c_cnt_ = 0, c_front_ = 0, c_part_ = 0, c_w_ = 0, c_wf_ = 0,
          c_mstat_ = 0, c_samp_ = 0, c_corr_ = 0, c_tr_ = 0, c_nrow_ = 0;
+  // kept Morton order of a tagged target set
+  void* d_tmap_ = nullptr;
+  size_t c_tmap_ = 0;
+  int tmap_tag_ = 0, tmap_nt_ = -1, tmap_uses_ = 0;
   // near-sum measurement
   bool prof_ = false, np_live_ = false;
   void* d_np_ = nullptr;

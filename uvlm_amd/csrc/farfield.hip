@@ -94,12 +94,9 @@ __global__ __launch_bounds__(1024) void k_bbox(const double* __restrict__ x, int
 // reduce a contiguous range to part[g][6] (min x, y, z, max x, y, z), then
 // one wavefront reduces the parts in order into box[6].
 constexpr int kBoxParts = 64;
-// It also clears the digit counts of the Morton sort (hz[nz]).
 __global__ __launch_bounds__(256) void k_bbox_part(const double* __restrict__ x, int n,
-                                                   double* __restrict__ part, int* __restrict__ hz,
-                                                   int nz) {
+                                                   double* __restrict__ part) {
   __shared__ double s[6][4];
-  for (int q = blockIdx.x * 256 + threadIdx.x; q < nz; q += kBoxParts * 256) hz[q] = 0;
   const int per = (n + kBoxParts - 1) / kBoxParts;
   const int q0 = blockIdx.x * per, q1 = min(n, q0 + per);
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -137,31 +134,25 @@ __global__ __launch_bounds__(64) void k_bbox_fin(const double* __restrict__ part
   box[d] = v;
 }
 
-__device__ inline unsigned long long spread21(unsigned long long v) {
-  v &= 0x1fffffull;
-  v = (v | (v << 32)) & 0x1f00000000ffffull;
-  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
-  v = (v | (v << 8)) & 0x100f00f00f00f00full;
-  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
-
 // --- stable radix sort of the targets' Morton keys ---------------------------
-// LSD, kRsPasses passes of kRsBits-bit digits over the 63-bit keys (with
-// their target indices).  The keys are cut into nb <= kRsMaxRows contiguous
-// rows of `per` keys (a multiple of kRsChunk); pass p reads h_p[nb][256],
-// the digit counts of every row of its input, and the workgroup of row b
-// places each of its keys of digit d at
+// LSD over the 30-bit keys (with their target indices): kRsPasses passes of
+// 8-bit digits.  Large sets: the keys are cut into nb <= kRsMaxRows rows of
+// `per` keys (a multiple of kRsChunk); per pass, k_rs_hist counts each
+// row's digits (LDS atomics) and k_rs_pass places the keys of row b with
+// digit d at
 //   (keys of digits < d) + (keys of digit d in rows < b)
 //   + (keys of digit d before it in row b),
-// the last term from wavefront ballots over chunks of kRsChunk keys.  While
-// placing them it counts the next pass's digits per destination row into
-// h_{p+1} (integer atomics: the counts do not depend on their order), so a
-// pass is one launch.  Stable, so the order equals any stable sort's.
-constexpr int kRsBits = 8, kRsBins = 1 << kRsBits, kRsPasses = 8;  // 64 >= 63 key bits
+// the last term from wavefront ballots over chunks of kRsChunk keys.  Small
+// sets (<= kRsSmall keys, the wings' targets): one workgroup computes the
+// keys and runs every pass in one launch (k_rs_small).  Stable, so the
+// order equals any stable sort's of the same keys.
+constexpr int kMortonBits = 10;  // per axis: cells of 1/1024 of the box
+constexpr int kRsBits = 8, kRsBins = 1 << kRsBits, kRsPasses = 4;  // 32 >= 30 key bits
 constexpr int kRsThreads = 256, kRsItems = 16, kRsChunk = kRsThreads * kRsItems;
 constexpr int kRsMaxRows = 128;
+constexpr int kOrderReuse = 50;  // calls a tagged target set keeps its order
+constexpr int kRsSmallThreads = 1024, kRsSmallItems = 16;
+constexpr int kRsSmall = kRsSmallThreads * kRsSmallItems;
 struct RsPlan {
   int nb, per;
 };
@@ -170,23 +161,98 @@ inline RsPlan rs_plan(int n) {
   return RsPlan{cdiv(n, per), per};
 }
 
+// Bits 0..9 of v spread to bits 0, 3, 6, ..., 27.
+__device__ inline unsigned spread10(unsigned v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton key of point q in the box (10 bits per axis, x in bit 0 of
+// each triple): cells of 1/1024 of the box, far finer than a tile of 64
+// targets; points of one cell keep their index order.
+__device__ inline unsigned morton_key(const double* __restrict__ x, long q,
+                                      const double* __restrict__ box) {
+  unsigned k = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double p = x[3 * q + d];
+    const double w = box[3 + d] - box[d];
+    double t = w > 0.0 ? (p - box[d]) / w : 0.0;
+    t = fmin(fmax(t, 0.0), 1.0);
+    const unsigned c = (unsigned)(t * (double)((1 << kMortonBits) - 1));
+    k |= spread10(c) << d;
+  }
+  return k;
+}
+
+__global__ void k_morton(const double* __restrict__ x, int n, const double* __restrict__ box,
+                         unsigned* __restrict__ key, int* __restrict__ idx) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  key[q] = morton_key(x, q, box);
+  if (idx) idx[q] = q;
+}
+
+// Lanes of this wavefront with the caller's digit (among lanes with ok).
+__device__ __forceinline__ unsigned long long rs_peers(int d, bool ok) {
+  unsigned long long m = __ballot(ok);
+  for (int b = 0; b < kRsBits; ++b) {
+    const bool on = (d >> b) & 1;
+    const unsigned long long x = __ballot(on);
+    m &= on ? x : ~x;
+  }
+  return m;
+}
+
+// hist[b][256]: digit counts of row b of the keys.
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const unsigned* __restrict__ kin, int n,
+                                                         int per, int shift,
+                                                         int* __restrict__ hist) {
+  __shared__ int h[kRsBins];
+  const int t = threadIdx.x, b = blockIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const int lo = b * per, hi = min(n, lo + per);
+  for (int e = lo + t; e < hi; e += kRsThreads) atomicAdd(&h[(kin[e] >> shift) & (kRsBins - 1)], 1);
+  __syncthreads();
+  hist[b * kRsBins + t] = h[t];
+}
+
 // One pass over the keys (kin, iin) -> (kout, iout): kout may be null (the
 // last pass needs no keys); iout gets -1 at positions n..pad_to-1 (the tile
-// map's padding).  hnext may be null (no next pass).
+// map's padding).
 __global__ __launch_bounds__(kRsThreads) void k_rs_pass(
-    const unsigned long long* __restrict__ kin, const int* __restrict__ iin,
-    unsigned long long* __restrict__ kout, int* __restrict__ iout, int n, int per, int nb,
-    int shift, const int* __restrict__ hist, int* __restrict__ hnext, int pad_to) {
+    const unsigned* __restrict__ kin, const int* __restrict__ iin,
+    unsigned* __restrict__ kout, int* __restrict__ iout, int n, int per, int nb,
+    int shift, const int* __restrict__ hist, int pad_to) {
   __shared__ int cnt[kRsItems][4][kRsBins];  // per (item, wavefront, digit)
   __shared__ int base[kRsBins];
   __shared__ int wsum[4];
   const int t = threadIdx.x, b = blockIdx.x, lane = t & 63, w = t >> 6;
+  const int lo = b * per, hi = min(n, lo + per);
+  // the first chunk's keys are in flight while the offsets are formed
+  unsigned key[kRsItems];
+  int id[kRsItems], rk[kRsItems];
+  for (int i = 0; i < kRsItems; ++i) {
+    const int e = lo + i * kRsThreads + t;
+    key[i] = e < hi ? kin[e] : 0u;
+    id[i] = e < hi ? iin[e] : -1;
+  }
   // thread t owns digit t: keys of digit t in all rows and in rows before b
+  // (16 rows' counts loaded before they are summed)
   int total = 0, pre = 0;
-  for (int r = 0; r < nb; ++r) {
-    const int v = hist[r * kRsBins + t];
-    total += v;
-    pre += r < b ? v : 0;
+  for (int r0 = 0; r0 < nb; r0 += 16) {
+    int v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = r0 + q < nb ? hist[(r0 + q) * kRsBins + t] : 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      total += v[q];
+      pre += r0 + q < b ? v[q] : 0;
+    }
   }
   // exclusive prefix of the totals over the digits
   int inc = total;
@@ -200,40 +266,42 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_pass(
   for (int q = 0; q < w; ++q) before += wsum[q];
   int run = before + inc - total + pre;  // next free position of digit t for this row
   const unsigned long long lt = (1ull << lane) - 1ull;
-  const int lo = b * per, hi = min(n, lo + per);
   for (int c0 = lo; c0 < hi; c0 += kRsChunk) {
+    if (c0 > lo)
+      for (int i = 0; i < kRsItems; ++i) {
+        const int e = c0 + i * kRsThreads + t;
+        key[i] = e < hi ? kin[e] : 0u;
+        id[i] = e < hi ? iin[e] : -1;
+      }
     for (int i = 0; i < kRsItems; ++i)
       for (int q = 0; q < 4; ++q) cnt[i][q][t] = 0;
     base[t] = run;
     __syncthreads();
-    unsigned long long key[kRsItems];
-    int id[kRsItems], rk[kRsItems];
-    for (int i = 0; i < kRsItems; ++i) {
-      const int e = c0 + i * kRsThreads + t;
-      key[i] = e < hi ? kin[e] : 0ull;
-      id[i] = e < hi ? iin[e] : -1;
-    }
     for (int i = 0; i < kRsItems; ++i) {
       const bool ok = c0 + i * kRsThreads + t < hi;
       const int d = (int)((key[i] >> shift) & (kRsBins - 1));
-      unsigned long long peers = __ballot(ok);  // lanes of this wavefront with digit d
-      for (int bit = 0; bit < kRsBits; ++bit) {
-        const bool on = (d >> bit) & 1;
-        const unsigned long long m = __ballot(on);
-        peers &= on ? m : ~m;
-      }
+      const unsigned long long peers = rs_peers(d, ok);
       rk[i] = __popcll(peers & lt);
       if (ok && rk[i] == 0) cnt[i][w][d] = __popcll(peers);
     }
     __syncthreads();
     // digit t: exclusive prefix over (item, wavefront), the keys' order
+    // (all counts read before the first is rewritten)
     int s = 0;
-    for (int i = 0; i < kRsItems; ++i)
-      for (int q = 0; q < 4; ++q) {
-        const int c = cnt[i][q][t];
-        cnt[i][q][t] = s;
-        s += c;
-      }
+    {
+      int c[kRsItems * 4];
+#pragma unroll
+      for (int i = 0; i < kRsItems; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) c[4 * i + q] = cnt[i][q][t];
+#pragma unroll
+      for (int i = 0; i < kRsItems; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          cnt[i][q][t] = s;
+          s += c[4 * i + q];
+        }
+    }
     __syncthreads();
     for (int i = 0; i < kRsItems; ++i) {
       if (c0 + i * kRsThreads + t >= hi) continue;
@@ -241,9 +309,6 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_pass(
       const int dst = base[d] + cnt[i][w][d] + rk[i];
       if (kout) kout[dst] = key[i];
       iout[dst] = id[i];
-      if (hnext)
-        atomicAdd(&hnext[(dst / per) * kRsBins + (int)((key[i] >> (shift + kRsBits)) & (kRsBins - 1))],
-                  1);
     }
     run += s;
     __syncthreads();  // cnt and base are rewritten by the next chunk
@@ -252,25 +317,97 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_pass(
     for (int q = n + t; q < pad_to; q += kRsThreads) iout[q] = -1;
 }
 
-// 63-bit Morton key of each point in its box (21 bits per axis); counts the
-// keys' lowest digits per row of the sort plan into h0 (cleared beforehand).
-__global__ void k_morton(const double* __restrict__ x, int n,
-                         const double* __restrict__ box, unsigned long long* __restrict__ key,
-                         int* __restrict__ idx, int per, int* __restrict__ h0) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  unsigned long long k = 0;
-  for (int d = 0; d < 3; ++d) {
-    const double p = x[3 * (long)q + d];
-    const double w = box[3 + d] - box[d];
-    double t = w > 0.0 ? (p - box[d]) / w : 0.0;
-    t = fmin(fmax(t, 0.0), 1.0);
-    const unsigned long long c = (unsigned long long)(t * 2097151.0);
-    k |= spread21(c) << d;
+// The whole sort of n <= kRsSmall keys in one workgroup: keys of the points
+// x in the box to (kA, iA), kRsPasses passes between (kA, iA) and (kB, iB)
+// in global memory (the workgroup's barriers order its own global accesses),
+// digit counts per (item, wavefront) in LDS; the last pass writes the tile
+// map (-1 at n..pad_to-1).
+__global__ __launch_bounds__(kRsSmallThreads) void k_rs_small(
+    const double* __restrict__ x, int n, const double* __restrict__ box, unsigned* kA, int* iA,
+    unsigned* kB, int* iB, int* __restrict__ map, int pad_to) {
+  constexpr int kW = kRsSmallThreads / 64;  // wavefronts
+  __shared__ unsigned short cnt[kRsSmallItems][kW][kRsBins];
+  __shared__ int part[4][kRsBins];  // per quarter of the (item, wavefront) sequence
+  __shared__ int base[kRsBins];
+  __shared__ int wsum[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int dd = t & (kRsBins - 1), qt = t >> 8;  // digit and quarter this thread serves
+  for (int e = t; e < n; e += kRsSmallThreads) {
+    kA[e] = morton_key(x, e, box);
+    iA[e] = e;
   }
-  key[q] = k;
-  idx[q] = q;
-  atomicAdd(&h0[(q / per) * kRsBins + (int)(k & (kRsBins - 1))], 1);
+  __syncthreads();
+  const int items = (n + kRsSmallThreads - 1) / kRsSmallThreads;
+  const int L = items * kW, f0 = qt * L / 4, f1 = (qt + 1) * L / 4;  // this quarter's entries
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int p = 0; p < kRsPasses; ++p) {
+    const bool last = p + 1 == kRsPasses;
+    const unsigned* kin = (p & 1) ? kB : kA;
+    const int* iin = (p & 1) ? iB : iA;
+    unsigned* kout = (p & 1) ? kA : kB;
+    int* iout = last ? map : (p & 1) ? iA : iB;
+    const int shift = p * kRsBits;
+    for (int f = f0; f < f1; ++f) cnt[f / kW][f % kW][dd] = 0;
+    unsigned key[kRsSmallItems];
+    int id[kRsSmallItems], rk[kRsSmallItems];
+#pragma unroll
+    for (int i = 0; i < kRsSmallItems; ++i) {
+      const int e = i * kRsSmallThreads + t;
+      key[i] = i < items && e < n ? kin[e] : 0u;
+      id[i] = i < items && e < n ? iin[e] : -1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRsSmallItems; ++i) {
+      if (i >= items) break;  // uniform
+      const bool ok = i * kRsSmallThreads + t < n;
+      const int d = (int)((key[i] >> shift) & (kRsBins - 1));
+      const unsigned long long peers = rs_peers(d, ok);
+      rk[i] = __popcll(peers & lt);
+      if (ok && rk[i] == 0) cnt[i][w][d] = (unsigned short)__popcll(peers);
+    }
+    __syncthreads();
+    // digit dd: sums of the quarters, totals, exclusive prefix over digits
+    int s = 0;
+    for (int f = f0; f < f1; ++f) s += cnt[f / kW][f % kW][dd];
+    part[qt][dd] = s;
+    __syncthreads();
+    int qoff = 0, total = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int v = part[q][dd];
+      qoff += q < qt ? v : 0;
+      total += v;
+    }
+    if (qt == 0) {  // the first four wavefronts scan the 256 totals
+      int inc = total;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off);
+        if (lane >= off) inc += u;
+      }
+      if (lane == 63) wsum[w] = inc;
+      base[dd] = inc - total;  // within this wavefront's 64 digits
+    }
+    __syncthreads();
+    int run = base[dd] + qoff;
+    for (int q = 0; q < (dd >> 6); ++q) run += wsum[q];
+    for (int f = f0; f < f1; ++f) {
+      const int c = cnt[f / kW][f % kW][dd];
+      cnt[f / kW][f % kW][dd] = (unsigned short)run;
+      run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRsSmallItems; ++i) {
+      if (i >= items) break;
+      if (i * kRsSmallThreads + t >= n) continue;
+      const int d = (int)((key[i] >> shift) & (kRsBins - 1));
+      const int dst = cnt[i][w][d] + rk[i];
+      if (!last) kout[dst] = key[i];
+      iout[dst] = id[i];
+    }
+    __syncthreads();  // this pass's global writes before the next pass reads them
+  }
+  for (int q = n + t; q < pad_to; q += kRsSmallThreads) map[q] = -1;
 }
 
 // Component m of the moments: type 0 (m < 3), 1 (< 12), 2 (< 39), 3 (< 120)
@@ -1518,7 +1655,7 @@ void launch_gather_rows_w(hipStream_t s, const double* rb, int cap, int row0, in
 // ===========================================================================
 FarField::~FarField() {
   void* bufs[] = {d_rec_,  d_key_,   d_key2_, d_idx_,   d_idx2_, d_hist_,
-                  d_box_,  d_cl_,    d_mom_,  d_map_,  d_tiles_, d_near_, d_far_,
+                  d_box_,  d_cl_,    d_mom_,  d_map_,  d_tmap_, d_tiles_, d_near_, d_far_,
                   d_cnt_,  d_front_, d_part_, d_w_,    d_wf_,    d_mstat_, d_samp_, d_np_, d_corr_,
                   d_tr_,   d_nrow_};
   for (void* p : bufs)
@@ -1589,40 +1726,41 @@ int FarField::ensure(void** p, size_t* cap, size_t bytes, std::string* err) {
 // gets the target indices (stable in the index among equal keys), -1 beyond
 // nt; keys_out (device, optional) the unsorted keys.
 int FarField::morton_order(hipStream_t s, const double* targets, int nt, int ntiles, int* map,
-                           unsigned long long* keys_out, std::string* err) {
+                           unsigned* keys_out, std::string* err) {
   const RsPlan rp = rs_plan(nt);
-  const size_t nk = (size_t)nt, nh = (size_t)kRsPasses * rp.nb * kRsBins;
+  const size_t nk = (size_t)nt, nh = (size_t)rp.nb * kRsBins;
   int rc;
-  if ((rc = ensure(&d_key_, &c_key_, sizeof(unsigned long long) * nk, err)) ||
-      (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned long long) * nk, err)) ||
+  if ((rc = ensure(&d_key_, &c_key_, sizeof(unsigned) * nk, err)) ||
+      (rc = ensure(&d_key2_, &c_key2_, sizeof(unsigned) * nk, err)) ||
       (rc = ensure(&d_idx_, &c_idx_, sizeof(int) * nk, err)) ||
       (rc = ensure(&d_idx2_, &c_idx2_, sizeof(int) * nk, err)) ||
       (rc = ensure(&d_hist_, &c_hist_, sizeof(int) * nh, err)) ||
       (rc = ensure(&d_box_, &c_box_, sizeof(double) * (8 + 6 * kBoxParts), err)))
     return rc;
   double* tbox = (double*)d_box_;
-  int* hist = (int*)d_hist_;
-  hipLaunchKernelGGL(k_bbox_part, dim3(kBoxParts), dim3(256), 0, s, targets, nt, tbox + 8, hist,
-                     (int)nh);
+  hipLaunchKernelGGL(k_bbox_part, dim3(kBoxParts), dim3(256), 0, s, targets, nt, tbox + 8);
   hipLaunchKernelGGL(k_bbox_fin, dim3(1), dim3(64), 0, s, (const double*)(tbox + 8), tbox);
-  hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
-                     (const double*)tbox, (unsigned long long*)d_key_, (int*)d_idx_, rp.per, hist);
-  if (keys_out &&
-      hipMemcpyAsync(keys_out, d_key_, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice,
-                     s) != hipSuccess) {
-    *err = "key copy failed";
-    return UVLM_ERR_HIP;
-  }
-  unsigned long long* K[2] = {(unsigned long long*)d_key_, (unsigned long long*)d_key2_};
+  if (keys_out)
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
+                       (const double*)tbox, keys_out, (int*)nullptr);
+  unsigned* K[2] = {(unsigned*)d_key_, (unsigned*)d_key2_};
   int* I[2] = {(int*)d_idx_, (int*)d_idx2_};
-  for (int p = 0; p < kRsPasses; ++p) {
-    const bool last = p + 1 == kRsPasses;
-    const int* h = hist + (size_t)p * rp.nb * kRsBins;
-    int* hn = last ? nullptr : hist + (size_t)(p + 1) * rp.nb * kRsBins;
-    hipLaunchKernelGGL(k_rs_pass, dim3(rp.nb), dim3(kRsThreads), 0, s,
-                       (const unsigned long long*)K[p & 1], (const int*)I[p & 1],
-                       last ? nullptr : K[(p + 1) & 1], last ? map : I[(p + 1) & 1], nt, rp.per,
-                       rp.nb, p * kRsBits, h, hn, last ? ntiles * kTileT : 0);
+  if (nt <= kRsSmall) {
+    hipLaunchKernelGGL(k_rs_small, dim3(1), dim3(kRsSmallThreads), 0, s, targets, nt,
+                       (const double*)tbox, K[0], I[0], K[1], I[1], map, ntiles * kTileT);
+  } else {
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(nt, 256)), dim3(256), 0, s, targets, nt,
+                       (const double*)tbox, K[0], I[0]);
+    int* hist = (int*)d_hist_;
+    for (int p = 0; p < kRsPasses; ++p) {
+      const bool last = p + 1 == kRsPasses;
+      hipLaunchKernelGGL(k_rs_hist, dim3(rp.nb), dim3(kRsThreads), 0, s,
+                         (const unsigned*)K[p & 1], nt, rp.per, p * kRsBits, hist);
+      hipLaunchKernelGGL(k_rs_pass, dim3(rp.nb), dim3(kRsThreads), 0, s,
+                         (const unsigned*)K[p & 1], (const int*)I[p & 1],
+                         last ? nullptr : K[(p + 1) & 1], last ? map : I[(p + 1) & 1], nt, rp.per,
+                         rp.nb, p * kRsBits, (const int*)hist, last ? ntiles * kTileT : 0);
+    }
   }
   if (hipGetLastError() != hipSuccess) {
     *err = "Morton order launch failed";
@@ -1683,7 +1821,7 @@ int FarField::build(hipStream_t s, const double* nodes, const double* gamma, int
 }
 
 int FarField::velocity(hipStream_t s, const double* targets, int nt, double theta, double* out,
-                       std::string* err, const FFFocus* focus, bool per_target) {
+                       std::string* err, const FFFocus* focus, bool per_target, int order_tag) {
   if (nt <= 0) return UVLM_OK;
   if (!built_) {
     *err = "far field: no sources built";
@@ -1702,9 +1840,25 @@ int FarField::velocity(hipStream_t s, const double* targets, int nt, double thet
       (rc = ensure(&d_part_, &c_part_,
                    sizeof(double) * 3 * (kNearGroups + kGroups) * (size_t)nt, err)))
     return rc;
-  // target tiles: the targets in Morton order of their own box
+  // target tiles: the targets in Morton order of their own box (kept for a
+  // tagged set)
   int* map = (int*)d_map_;
-  if ((rc = morton_order(s, targets, nt, ntiles, map, nullptr, err)) != UVLM_OK) return rc;
+  if (order_tag != 0) {
+    const void* before = d_tmap_;
+    if ((rc = ensure(&d_tmap_, &c_tmap_, sizeof(int) * (size_t)ntiles * kTileT, err)) != UVLM_OK)
+      return rc;
+    map = (int*)d_tmap_;
+    if (order_tag != tmap_tag_ || nt != tmap_nt_ || d_tmap_ != before ||
+        tmap_uses_ >= kOrderReuse) {
+      if ((rc = morton_order(s, targets, nt, ntiles, map, nullptr, err)) != UVLM_OK) return rc;
+      tmap_tag_ = order_tag;
+      tmap_nt_ = nt;
+      tmap_uses_ = 0;
+    }
+    ++tmap_uses_;
+  } else if ((rc = morton_order(s, targets, nt, ntiles, map, nullptr, err)) != UVLM_OK) {
+    return rc;
+  }
   hipLaunchKernelGGL(k_ff_tiles, dim3(ntiles), dim3(64), 0, s, targets, (const int*)map,
                      (double4*)d_tiles_);
   const bool graded = focus && focus->box && focus->reach > 0.0;

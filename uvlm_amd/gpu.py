@@ -153,14 +153,14 @@ class Context:
 
     def farfield_target_order(self, targets):
         """Morton keys of the targets and the split's tile order
-        (uvlm_farfield_target_order): keys [n] uint64, order [64 ceil(n/64)]
+        (uvlm_farfield_target_order): keys [n] uint32, order [64 ceil(n/64)]
         int32 padded with -1."""
         x = _f64(targets).reshape(-1, 3)
         n = x.shape[0]
-        keys = np.zeros(n, dtype=np.uint64)
+        keys = np.zeros(n, dtype=np.uint32)
         order = np.zeros((n + 63) // 64 * 64, dtype=np.int32)
         self._check(self._L.uvlm_farfield_target_order(
-            self._h, _p(x), n, keys.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+            self._h, _p(x), n, keys.ctypes.data_as(C.POINTER(C.c_uint)),
             order.ctypes.data_as(C.POINTER(C.c_int))), "uvlm_farfield_target_order")
         return keys, order
 
