@@ -1094,9 +1094,13 @@ __global__ void k_ff_traces(const double* __restrict__ mom, int total, double* _
 // (Qd = Q contracted with d over its last position index).
 __device__ void expand_moments(const double* Q, const double* tr, double dx, double dy, double dz,
                                double u[3]) {
+  // FMA-contracted by hand (this file is built without contraction, which
+  // the reference-order kernels need): half the VALU instructions of the
+  // separate products and sums; the expansion's truncation, ~(r/|d|)^4,
+  // dwarfs the difference in rounding.
   const double d[3] = {dx, dy, dz};
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  const double ir = rsqrt(r2);
+  const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  const double ir = dev::rsq64_1(r2);
   const double i2 = ir * ir, i3 = i2 * ir, i5 = i3 * i2, i7 = i5 * i2, i9 = i7 * i2;
   const double* Q1 = Q + kFFQ1;
   const double* Q2 = Q + kFFQ2;
@@ -1108,23 +1112,28 @@ __device__ void expand_moments(const double* Q, const double* tr, double dx, dou
       double q2d = 0.0, q3dd = 0.0;
       for (int k = 0; k < 3; ++k) {
         const double* q3 = Q3 + ((c * 3 + b) * 3 + k) * 3;
-        const double q3d = q3[0] * d[0] + q3[1] * d[1] + q3[2] * d[2];
-        q2d += Q2[(c * 3 + b) * 3 + k] * d[k];
-        q3dd += q3d * d[k];
+        const double q3d = __builtin_fma(q3[2], d[2], __builtin_fma(q3[1], d[1], q3[0] * d[0]));
+        q2d = __builtin_fma(Q2[(c * 3 + b) * 3 + k], d[k], q2d);
+        q3dd = __builtin_fma(q3d, d[k], q3dd);
       }
-      q1d += Q1[c * 3 + b] * d[b];
-      q2dd += q2d * d[b];
-      q3ddd += q3dd * d[b];
-      trd += tr[3 + 3 * c + b] * d[b];
-      be[c][b] = Q1[c * 3 + b] * i3 + 3.0 * q2d * i5 + 7.5 * q3dd * i7 -
-                 1.5 * tr[3 + 3 * c + b] * i5;
+      const double t = tr[3 + 3 * c + b];
+      q1d = __builtin_fma(Q1[c * 3 + b], d[b], q1d);
+      q2dd = __builtin_fma(q2d, d[b], q2dd);
+      q3ddd = __builtin_fma(q3dd, d[b], q3ddd);
+      trd = __builtin_fma(t, d[b], trd);
+      // Q1 i3 + 3 q2d i5 + 7.5 q3dd i7 - 1.5 t i5
+      be[c][b] = __builtin_fma(Q1[c * 3 + b], i3,
+                               __builtin_fma(__builtin_fma(3.0, q2d, -1.5 * t), i5, 7.5 * q3dd * i7));
     }
-    al[c] = -Q[c] * i3 - 3.0 * q1d * i5 - 7.5 * q2dd * i7 + 1.5 * tr[c] * i5 -
-            17.5 * q3ddd * i9 + 7.5 * trd * i7;
+    // -Q0 i3 - 3 q1d i5 - 7.5 q2dd i7 + 1.5 tr i5 - 17.5 q3ddd i9 + 7.5 trd i7
+    al[c] = __builtin_fma(-Q[c], i3,
+                          __builtin_fma(__builtin_fma(-3.0, q1d, 1.5 * tr[c]), i5,
+                                        __builtin_fma(__builtin_fma(-7.5, q2dd, 7.5 * trd), i7,
+                                                      -17.5 * q3ddd * i9)));
   }
-  u[0] = (d[1] * al[2] - d[2] * al[1]) + (be[2][1] - be[1][2]);
-  u[1] = (d[2] * al[0] - d[0] * al[2]) + (be[0][2] - be[2][0]);
-  u[2] = (d[0] * al[1] - d[1] * al[0]) + (be[1][0] - be[0][1]);
+  u[0] = __builtin_fma(d[1], al[2], -d[2] * al[1]) + (be[2][1] - be[1][2]);
+  u[1] = __builtin_fma(d[2], al[0], -d[0] * al[2]) + (be[0][2] - be[2][0]);
+  u[2] = __builtin_fma(d[0], al[1], -d[1] * al[0]) + (be[1][0] - be[0][1]);
 }
 
 // Network features of (target, cluster) from its moments Q, offset d = x - c
