@@ -43,6 +43,11 @@ constexpr double kRefreshRho = 0.05;    // contraction above which X is refreshe
 // more slowly than this (the matrix of a time-stepping run drifts by about
 // as much per step).
 constexpr double kPreRefreshRho = 0.02;
+// ... for matrices of at most this order: the refresh's two GEMMs grow as
+// N^3 against the sweeps' N^2 (measured: at N = 2048 the queued refreshes
+// take a run from 0.83 to 0.5 ms of solve per step, at N = 4096 they cost
+// 1.4 ms per step more than the sweeps they save).
+constexpr int kPreRefreshMaxN = 3072;
 constexpr int kRedGroups = 16;          // partial-sum groups per row in the reductions
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -473,7 +478,7 @@ int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, dou
     hipLaunchKernelGGL(k_copy_vec, dim3(nb), dim3(256), 0, s, (const double*)x, xold_, n);
   }
   if (!valid_ && (rc = build(s, A, err)) != UVLM_OK) return rc;
-  else if (warm && end_rho_ > kPreRefreshRho)
+  else if (warm && end_rho_ > kPreRefreshRho && n <= kPreRefreshMaxN)
     (void)newton_schulz(s, A, true, false);  // queued ahead of the sweeps, no read-back
   double prev = -1.0;
   int refreshed = 0, done = 0;
