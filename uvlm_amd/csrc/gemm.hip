@@ -202,29 +202,66 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmPair pp) {
   const float* __restrict__ B = (const float*)p.B;
   float* C = (float*)p.C;
   const int t = threadIdx.x;
-  const int ar = t % BM, ak = t / BM;
-  const int bk = t % BK, bc = t / BK;
+  // Each thread stages LA (LB) consecutive k of one row of A (column of B),
+  // so its BF16 values go to LDS as 16-byte writes (LA / 8 of them) instead
+  // of one 2-byte write each.  A (column-major, rows contiguous): lanes take
+  // consecutive rows, so every load instruction is coalesced.  B (k
+  // contiguous in a column): a thread's LB values are contiguous, loaded 16
+  // bytes at a time when the run lies inside the column and is aligned.
+  constexpr int GA = 256 / BM, GB = 256 / BN;  // k groups per row / column
+  const int ar = t % BM, ak = t / BM;          // row of A, k group
+  const int bc = t / GB, bk = t % GB;          // column of B, k group
   float ra[LA], rb[LB];
+  const bool bvec = (p.ldb & 3) == 0 && ((uintptr_t)B & 15) == 0;
   auto load = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < LA; ++e) {  // rows beyond m only reach unstored results
-      const int k = k0 + ak + e * (256 / BM);
+      const int k = k0 + ak * LA + e;
       const auto v = A[(size_t)min(k, p.k - 1) * p.lda + min(r0 + ar, p.m - 1)];
       ra[e] = k < p.k ? v : 0.f;
     }
+    const int kb = k0 + bk * LB;
+    const float* bcol = B + (size_t)min(c0 + bc, p.n - 1) * p.ldb;
+    if (bvec && kb + LB <= p.k) {
 #pragma unroll
-    for (int e = 0; e < LB; ++e) {
-      const int k = k0 + bk;
-      const auto v = B[(size_t)min(c0 + bc + e * (256 / BK), p.n - 1) * p.ldb + min(k, p.k - 1)];
-      rb[e] = k < p.k ? v : 0.f;
+      for (int e = 0; e < LB; e += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(bcol + kb + e);
+        rb[e] = v.x;
+        rb[e + 1] = v.y;
+        rb[e + 2] = v.z;
+        rb[e + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < LB; ++e) {
+        const int k = kb + e;
+        const float v = bcol[min(k, p.k - 1)];
+        rb[e] = k < p.k ? v : 0.f;
+      }
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < LA; ++e) sA[buf][ar][ak + e * (256 / BM)] = to_bf16(ra[e]);
+    for (int e = 0; e < LA; e += 8) {
+      uint4 q;
+      q.x = to_bf16(ra[e]) | ((unsigned)to_bf16(ra[e + 1]) << 16);
+      q.y = to_bf16(ra[e + 2]) | ((unsigned)to_bf16(ra[e + 3]) << 16);
+      q.z = to_bf16(ra[e + 4]) | ((unsigned)to_bf16(ra[e + 5]) << 16);
+      q.w = to_bf16(ra[e + 6]) | ((unsigned)to_bf16(ra[e + 7]) << 16);
+      *reinterpret_cast<uint4*>(&sA[buf][ar][ak * LA + e]) = q;
+    }
 #pragma unroll
-    for (int e = 0; e < LB; ++e) sB[buf][bc + e * (256 / BK)][bk] = to_bf16(rb[e]);
+    for (int e = 0; e < LB; e += 8) {
+      uint4 q;
+      q.x = to_bf16(rb[e]) | ((unsigned)to_bf16(rb[e + 1]) << 16);
+      q.y = to_bf16(rb[e + 2]) | ((unsigned)to_bf16(rb[e + 3]) << 16);
+      q.z = to_bf16(rb[e + 4]) | ((unsigned)to_bf16(rb[e + 5]) << 16);
+      q.w = to_bf16(rb[e + 6]) | ((unsigned)to_bf16(rb[e + 7]) << 16);
+      *reinterpret_cast<uint4*>(&sB[buf][bc][bk * LB + e]) = q;
+    }
   };
+  static_assert(LA % 8 == 0 && LB % 8 == 0 && GA * LA == BK && GB * LB == BK,
+                "each thread stages whole 16-byte runs of k");
   const int w = t >> 6, lane = t & 63, l16 = lane & 15, lq = lane >> 4;
   const int wr = w >> 1, wc = w & 1;
   v4f acc[IM][JN];
