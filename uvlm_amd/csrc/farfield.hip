@@ -51,11 +51,17 @@ constexpr int kL0Segs = (kFFRows + 1) * kFFCols + (kFFCols + 1) * kFFRows;
 // Near-sum record of a level-0 block: its nodes (row-major, nc + 1 per row),
 // its segment strengths / 4pi (spanwise segments by node row, then
 // chordwise by node column) and, for the FP32 walk, each segment's vector
-// and squared length rounded to FP32 (4 floats per segment, same order);
-// contiguous so one wavefront loads it coalesced.
+// and squared length rounded to FP32 (4 floats per segment, same order) and
+// its strength rounded to FP32; contiguous so one wavefront loads it
+// coalesced.  The FP64 walk stages nodes and strengths (the first kRecSeg
+// doubles), the FP32 walk nodes, vectors and FP32 strengths (kRec32 doubles
+// in LDS, the FP64 strengths skipped).
 constexpr int kRecStr = 3 * kL0Nodes;
 constexpr int kRecSeg = kRecStr + kL0Segs;  // FP32 segment vectors (as floats)
-constexpr int kRec = kRecSeg + 2 * kL0Segs;
+constexpr int kRecG = kRecSeg + 2 * kL0Segs;  // FP32 strengths (as floats)
+constexpr int kRec = kRecG + kL0Segs / 2;
+constexpr int kRec32 = kRec - kL0Segs;  // staged by the FP32 walk
+static_assert(kL0Segs % 2 == 0, "FP32 strengths fill whole doubles");
 
 // Bounding box of n points x[n][3]: box[0..2] min, box[3..5] max (one
 // workgroup).
@@ -359,7 +365,7 @@ __global__ __launch_bounds__(kRsSmallThreads) void k_rs_small(
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kRsSmallItems; ++i) {
-      if (i >= items) break;  // uniform
+      if (i >= items) continue;  // uniform
       const bool ok = i * kRsSmallThreads + t < n;
       const int d = (int)((key[i] >> shift) & (kRsBins - 1));
       const unsigned long long peers = rs_peers(d, ok);
@@ -398,7 +404,7 @@ __global__ __launch_bounds__(kRsSmallThreads) void k_rs_small(
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kRsSmallItems; ++i) {
-      if (i >= items) break;
+      if (i >= items) continue;
       if (i * kRsSmallThreads + t >= n) continue;
       const int d = (int)((key[i] >> shift) & (kRsBins - 1));
       const int dst = cnt[i][w][d] + rk[i];
@@ -525,6 +531,7 @@ __global__ __launch_bounds__(kMomThreads) void k_ff_level0(FFTree t,
       gb = gl(ia, ja);
     }
     R[kRecStr + s] = ga * kInv4Pi - gb * kInv4Pi;
+    reinterpret_cast<float*>(R + kRecG)[s] = (float)(ga * kInv4Pi - gb * kInv4Pi);
     const double* pa = sp + 3 * (ia * (nc + 1) + ja);
     const double* pb = sp + 3 * (ib * (nc + 1) + jb);
     sA[3 * s] = pa[0] - cx;
@@ -855,9 +862,9 @@ __device__ __forceinline__ void chain64(const double* R, int n0, int stride, int
 // All chains of a record (nr x nc rings): node rows (spanwise segments),
 // then node columns (chordwise).
 template <bool Ends>
-__device__ __forceinline__ void walk_record(const double* R, int nr, int nc, double px, double py,
-                                            double pz, double& ux, double& uy, double& uz) {
-  const double* S = R + kRecStr;
+__device__ __forceinline__ void walk_record(const double* R, const double* S, int nr, int nc,
+                                            double px, double py, double pz, double& ux,
+                                            double& uy, double& uz) {
   for (int i = 0; i <= nr; ++i)
     chain64<Ends>(R, i * (nc + 1), 1, nc, S, i * nc, px, py, pz, ux, uy, uz);
   const int s1 = (nr + 1) * nc;
@@ -878,9 +885,9 @@ constexpr double kShortRadius = 1.5e4;  // (2 r)^2 < dev::kShortSeg2
 // trip, the end point alternating between two register sets.
 template <bool Ends>
 __device__ __forceinline__ void chain32(const double* R, int n0, int stride_nodes, int count,
-                                        const double* S, const float* F, int s0, double x,
-                                        double y, double z, double& ux, double& uy,
-                                        double& uz) {
+                                        const double* __restrict__ S, const float* F,
+                                        const float* G, int s0, double x, double y, double z,
+                                        double& ux, double& uy, double& uz) {
   const double* p = R + 3 * n0;
   const int step = 3 * stride_nodes;
   dev::End32 A, B;
@@ -892,14 +899,14 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
   int k = 0;
   for (; k + 2 <= count; k += 2) {
     dev::end32(p + step, x, y, z, B);
-    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz));
+    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), G[s0 + k], fx, fy, fz));
     p += 2 * step;
     dev::end32(p, x, y, z, A);
-    redo |= __ballot(dev::elem32<Ends>(B, A, F + 4 * (s0 + k + 1), (float)S[s0 + k + 1], fx, fy, fz));
+    redo |= __ballot(dev::elem32<Ends>(B, A, F + 4 * (s0 + k + 1), G[s0 + k + 1], fx, fy, fz));
   }
   if (k < count) {
     dev::end32(p + step, x, y, z, B);
-    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), (float)S[s0 + k], fx, fy, fz));
+    redo |= __ballot(dev::elem32<Ends>(A, B, F + 4 * (s0 + k), G[s0 + k], fx, fy, fz));
   }
   if (redo) {  // rare: a second pass finds the flagged elements, in FP64
     const bool mine = (redo >> __lane_id()) & 1ull;
@@ -919,17 +926,20 @@ __device__ __forceinline__ void chain32(const double* R, int n0, int stride_node
   uz += (double)fz;
 }
 
+// R: the record as the FP32 walk stages it (nodes, segment vectors, FP32
+// strengths); S: the FP64 strengths (in the global record, read only by the
+// rare FP64 re-evaluation).
 template <bool Ends>
-__device__ __forceinline__ void walk_record32(const double* R, int nr, int nc, double px,
-                                              double py, double pz, double& ux, double& uy,
-                                              double& uz) {
-  const double* S = R + kRecStr;
-  const float* F = reinterpret_cast<const float*>(R + kRecSeg);
+__device__ __forceinline__ void walk_record32(const double* R, const double* S, int nr, int nc,
+                                              double px, double py, double pz, double& ux,
+                                              double& uy, double& uz) {
+  const float* F = reinterpret_cast<const float*>(R + kRecStr);
+  const float* G = reinterpret_cast<const float*>(R + kRecStr + 2 * kL0Segs);
   for (int i = 0; i <= nr; ++i)  // node rows: spanwise segments
-    chain32<Ends>(R, i * (nc + 1), 1, nc, S, F, i * nc, px, py, pz, ux, uy, uz);
+    chain32<Ends>(R, i * (nc + 1), 1, nc, S, F, G, i * nc, px, py, pz, ux, uy, uz);
   const int s1 = (nr + 1) * nc;
   for (int j = 0; j <= nc; ++j)  // node columns: chordwise segments
-    chain32<Ends>(R, j, nc + 1, nr, S, F, s1 + j * nr, px, py, pz, ux, uy, uz);
+    chain32<Ends>(R, j, nc + 1, nr, S, F, G, s1 + j * nr, px, py, pz, ux, uy, uz);
 }
 
 // Exact velocity (with 1/4pi) at one target of the level-0 blocks under
@@ -943,8 +953,8 @@ __device__ void exact_cluster(const FFTree& t, const FFCluster& c, const double*
     for (int bj = bj0; bj < bj1; ++bj) {
       const int nr = min(t.rows, (bi + 1) * kFFRows) - bi * kFFRows;
       const int nc = min(t.cols, (bj + 1) * kFFCols) - bj * kFFCols;
-      walk_record<true>(rec + (long)ff_id(t, 0, c.n, bi, bj) * kRec, nr, nc, px, py, pz, ux, uy,
-                        uz);
+      const double* R = rec + (long)ff_id(t, 0, c.n, bi, bj) * kRec;
+      walk_record<true>(R, R + kRecStr, nr, nc, px, py, pz, ux, uy, uz);
     }
   u[0] = ux;
   u[1] = uy;
@@ -990,7 +1000,7 @@ __global__ __launch_bounds__(1024) void k_order_heavy(const int2* __restrict__ c
 // block's record is staged in LDS (the next one is loaded into registers
 // meanwhile) and walked for every lane's target (FP64, or FP32 compensated).
 template <bool F32>
-__global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl,
+__global__ __launch_bounds__(64, 5) void k_ff_near(const FFCluster* __restrict__ cl,
                                                 const double* __restrict__ rec,
                                                 const double* __restrict__ x, int nt,
                                                 const int* __restrict__ map, int n0,
@@ -999,7 +1009,7 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
                                                 const int* __restrict__ order,
                                                 double* __restrict__ part,
                                                 unsigned long long* __restrict__ npairs) {
-  constexpr int kLen = F32 ? kRec : kRecSeg;  // the FP64 walk needs no FP32 vectors
+  constexpr int kLen = F32 ? kRec32 : kRecSeg;  // doubles staged (see kRec32)
   constexpr int kPer = (kLen + 63) / 64;
   __shared__ double sr[kPer * 64];
   const int tile = order[blockIdx.x], g = blockIdx.y, lane = threadIdx.x;
@@ -1015,7 +1025,8 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int k = lane + 64 * u;
-      pre[kNearPrefetch ? u : 0] = k < kLen ? rec[(long)b * kRec + k] : 0.0;
+      const int src = F32 && k >= kRecStr ? k + kL0Segs : k;  // the FP32 walk skips S
+      pre[kNearPrefetch ? u : 0] = k < kLen ? rec[(long)b * kRec + src] : 0.0;
       if (!kNearPrefetch) sr[lane + 64 * u] = pre[0];
     }
   };
@@ -1023,6 +1034,7 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
   unsigned segs = 0;  // segments walked by this group (for the pair count)
   for (int k = g; k < count; k += kNearGroups) {
     const FFCluster c = cl[id];
+    const double* S = rec + (long)id * kRec + kRecStr;  // FP64 strengths, global
     const int nr = c.r1 - c.r0, nc = c.c1 - c.c0;
     segs += (unsigned)((nr + 1) * nc + (nc + 1) * nr);
     __syncthreads();
@@ -1042,11 +1054,13 @@ __global__ __launch_bounds__(64) void k_ff_near(const FFCluster* __restrict__ cl
     // 3e4 across: not met in practice, and one FP32 variant keeps the
     // kernel at 5 waves per SIMD)
     if (F32 && c.r < kShortRadius)
-      walk_record32<false>(sr, nr, nc, px, py, pz, ux, uy, uz);
+      walk_record32<false>(sr, S, nr, nc, px, py, pz, ux, uy, uz);
+    else if (F32)
+      walk_record<true>(sr, S, nr, nc, px, py, pz, ux, uy, uz);
     else if (c.r < kShortRadius)
-      walk_record<false>(sr, nr, nc, px, py, pz, ux, uy, uz);
+      walk_record<false>(sr, sr + kRecStr, nr, nc, px, py, pz, ux, uy, uz);
     else
-      walk_record<true>(sr, nr, nc, px, py, pz, ux, uy, uz);
+      walk_record<true>(sr, sr + kRecStr, nr, nc, px, py, pz, ux, uy, uz);
   }
   if (tg >= 0) {
     part[((long)g * 3 + 0) * nt + tg] = ux;
