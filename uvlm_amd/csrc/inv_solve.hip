@@ -37,7 +37,8 @@ namespace {
 constexpr int kMaxBatch = 16;           // sweeps queued per host read
 constexpr int kNsSlot = 2 * kMaxBatch;  // Newton-Schulz deviation word
 constexpr int kFlagSlot = kNsSlot + 1;  // 0 running, 1 + k converged at k, -1 - k diverged
-constexpr int kStatWords = kFlagSlot + 1;
+constexpr int kCtrSlot = kFlagSlot + 1;  // workgroups of k_update_sum done (reset by the last)
+constexpr int kStatWords = kCtrSlot + 1;
 constexpr double kRefreshRho = 0.05;    // contraction above which X is refreshed
 // A warm solve starts with a BF16 refresh when the previous solve contracted
 // more slowly than this (the matrix of a time-stepping run drifts by about
@@ -195,13 +196,20 @@ __global__ __launch_bounds__(256) void k_residual_sum(const double* __restrict__
   if (threadIdx.x < 16 && i < n) r[i] = b[i] - t;
 }
 
-// x += sum_s part[s]; out[0], out[1] <- max |dx|, max |x| (after the update)
+__device__ void check_sweep(unsigned long long* norms, int k, double prev, double tol,
+                            long long* flag);
+
+// x += sum_s part[s]; norms[2k], norms[2k + 1] <- max |dx|, max |x| (after
+// the update); the last workgroup to finish then checks the sweep
+// (check_sweep), so a sweep is four launches.
 __global__ __launch_bounds__(256) void k_update_sum(const double* __restrict__ part, int S,
                                                     int n, double* __restrict__ x,
-                                                    unsigned long long* out,
-                                                    const long long* flag) {
+                                                    unsigned long long* norms, int k,
+                                                    double prev, double tol, long long* flag,
+                                                    unsigned* ctr) {
   __shared__ double lds[kRedGroups][17];
   if (*flag != 0) return;
+  unsigned long long* out = norms + 2 * k;
   const int i = blockIdx.x * 16 + (threadIdx.x & 15);
   const double dx = sum_parts(part, S, n, i, lds);
   if (threadIdx.x < 64) {  // wave 0; its lanes 16..63 contribute zeros
@@ -219,6 +227,12 @@ __global__ __launch_bounds__(256) void k_update_sum(const double* __restrict__ p
     if (threadIdx.x == 0) {
       atomic_max_nonneg(out, a);
       atomic_max_nonneg(out + 1, m);
+      __threadfence();
+      if (atomicAdd(ctr, 1u) == gridDim.x - 1) {  // every workgroup's maxima are in
+        *ctr = 0u;
+        __threadfence();
+        check_sweep(norms, k, prev, tol, flag);
+      }
     }
   }
 }
@@ -239,13 +253,18 @@ __global__ __launch_bounds__(256) void k_rowmax_sum(const double* __restrict__ p
 // After sweep k: raise the flag on convergence (max|dx| <= tol max|x|, or a
 // stall at the rounding level: contraction above 1/2, growth included, with
 // max|dx| <= 1e3 tol max|x|) or divergence (a non-finite correction, or one
-// that grows above the rounding level).
-__global__ void k_check(const unsigned long long* norms, int k, double prev, double tol,
-                        long long* flag) {
-  if (*flag != 0) return;
-  const double d = __longlong_as_double((long long)norms[2 * k]);
-  const double m = __longlong_as_double((long long)norms[2 * k + 1]);
-  const double p = k > 0 ? __longlong_as_double((long long)norms[2 * k - 2]) : prev;
+// that grows above the rounding level).  Run by the last workgroup of the
+// sweep's update (the norms are read as atomics: the other workgroups'
+// maxima reach it through the device-scope atomics and fences).
+__device__ void check_sweep(unsigned long long* norms, int k, double prev, double tol,
+                            long long* flag) {
+  auto ld = [](unsigned long long* q) {
+    return __longlong_as_double(
+        (long long)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  };
+  const double d = ld(norms + 2 * k);
+  const double m = ld(norms + 2 * k + 1);
+  const double p = k > 0 ? ld(norms + 2 * k - 2) : prev;
   if (!(d <= 1e300) || !(m <= 1e300)) {
     *flag = -1 - k;
     return;
@@ -316,7 +335,7 @@ void InvSolver::release() {
   streak_ = 0;
 }
 
-int InvSolver::prepare(hipStream_t, int n, bool wide, std::string* err) {
+int InvSolver::prepare(hipStream_t s, int n, bool wide, std::string* err) {
   if (n == n_ && wide == wide_) return UVLM_OK;
   release();
   // split-column shape of the matrix-vector products: ~1024 workgroups
@@ -340,6 +359,8 @@ int InvSolver::prepare(hipStream_t, int n, bool wide, std::string* err) {
   }
   n_ = n;
   wide_ = wide;
+  // the sweep counter starts at zero (its last user resets it)
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_, kStatWords);
   return UVLM_OK;
 }
 
@@ -458,9 +479,8 @@ void InvSolver::enqueue_sweep(hipStream_t s, const double* A, const double* b, d
     hipLaunchKernelGGL(k_gemv_part<float>, g, dim3(256), 0, s, (const float*)X_, n,
                        (const double*)r_, cols_, part_, flag);
   hipLaunchKernelGGL(k_update_sum, dim3(rb), dim3(256), 0, s, (const double*)part_, S_, n, x,
-                     stat_ + 2 * k, flag);
-  hipLaunchKernelGGL(k_check, dim3(1), dim3(1), 0, s, (const unsigned long long*)stat_, k, prev,
-                     tol, reinterpret_cast<long long*>(stat_ + kFlagSlot));
+                     stat_, k, prev, tol, reinterpret_cast<long long*>(stat_ + kFlagSlot),
+                     reinterpret_cast<unsigned*>(stat_ + kCtrSlot));
 }
 
 int InvSolver::solve(hipStream_t s, const double* A, int n, const double* b, double* x,
