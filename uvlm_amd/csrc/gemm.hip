@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 namespace uvlm {
 namespace {
@@ -189,7 +190,9 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
   return (unsigned short)(u >> 16);
 }
 
-template <int BM, int BN>
+// IN16: the operands are already BF16 in memory (their rounding is to_bf16's,
+// so the products equal the FP32-data form's bit for bit).
+template <int BM, int BN, bool IN16>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmPair pp) {
   constexpr int BK = 32, KP = BK + 8;  // row pitch: 80 bytes
   constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
@@ -198,8 +201,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmPair pp) {
   __shared__ __align__(16) unsigned short sB[2][BN][KP];
   int r0, c0;
   const GemmArgs p = pick<BM, BN>(pp, &r0, &c0);
-  const float* __restrict__ A = (const float*)p.A;
-  const float* __restrict__ B = (const float*)p.B;
+  typedef typename std::conditional<IN16, unsigned short, float>::type E;
+  const E* __restrict__ A = (const E*)p.A;
+  const E* __restrict__ B = (const E*)p.B;
   float* C = (float*)p.C;
   const int t = threadIdx.x;
   // Each thread stages LA (LB) consecutive k of one row of A (column of B),
@@ -211,54 +215,67 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmPair pp) {
   constexpr int GA = 256 / BM, GB = 256 / BN;  // k groups per row / column
   const int ar = t % BM, ak = t / BM;          // row of A, k group
   const int bc = t / GB, bk = t % GB;          // column of B, k group
-  float ra[LA], rb[LB];
-  const bool bvec = (p.ldb & 3) == 0 && ((uintptr_t)B & 15) == 0;
+  E ra[LA], rb[LB];
+  constexpr int VEC = 16 / (int)sizeof(E);  // elements per 16-byte load of B
+  const bool bvec = (p.ldb % VEC) == 0 && ((uintptr_t)B & 15) == 0;
   auto load = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < LA; ++e) {  // rows beyond m only reach unstored results
       const int k = k0 + ak * LA + e;
-      const auto v = A[(size_t)min(k, p.k - 1) * p.lda + min(r0 + ar, p.m - 1)];
-      ra[e] = k < p.k ? v : 0.f;
+      const E v = A[(size_t)min(k, p.k - 1) * p.lda + min(r0 + ar, p.m - 1)];
+      ra[e] = k < p.k ? v : E(0);
     }
     const int kb = k0 + bk * LB;
-    const float* bcol = B + (size_t)min(c0 + bc, p.n - 1) * p.ldb;
+    const E* bcol = B + (size_t)min(c0 + bc, p.n - 1) * p.ldb;
     if (bvec && kb + LB <= p.k) {
 #pragma unroll
-      for (int e = 0; e < LB; e += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(bcol + kb + e);
-        rb[e] = v.x;
-        rb[e + 1] = v.y;
-        rb[e + 2] = v.z;
-        rb[e + 3] = v.w;
+      for (int e = 0; e < LB; e += VEC) {
+        const uint4 v = *reinterpret_cast<const uint4*>(bcol + kb + e);
+        if constexpr (IN16) {
+          rb[e] = (E)(v.x & 0xffffu);
+          rb[e + 1] = (E)(v.x >> 16);
+          rb[e + 2] = (E)(v.y & 0xffffu);
+          rb[e + 3] = (E)(v.y >> 16);
+          rb[e + 4] = (E)(v.z & 0xffffu);
+          rb[e + 5] = (E)(v.z >> 16);
+          rb[e + 6] = (E)(v.w & 0xffffu);
+          rb[e + 7] = (E)(v.w >> 16);
+        } else {
+          rb[e] = __uint_as_float(v.x);
+          rb[e + 1] = __uint_as_float(v.y);
+          rb[e + 2] = __uint_as_float(v.z);
+          rb[e + 3] = __uint_as_float(v.w);
+        }
       }
     } else {
 #pragma unroll
       for (int e = 0; e < LB; ++e) {
         const int k = kb + e;
-        const float v = bcol[min(k, p.k - 1)];
-        rb[e] = k < p.k ? v : 0.f;
+        const E v = bcol[min(k, p.k - 1)];
+        rb[e] = k < p.k ? v : E(0);
       }
     }
   };
+  auto pack = [](const E* r) {  // 8 operands as BF16, 16 bytes
+    uint4 q;
+    if constexpr (IN16) {
+      q.x = r[0] | ((unsigned)r[1] << 16);
+      q.y = r[2] | ((unsigned)r[3] << 16);
+      q.z = r[4] | ((unsigned)r[5] << 16);
+      q.w = r[6] | ((unsigned)r[7] << 16);
+    } else {
+      q.x = to_bf16(r[0]) | ((unsigned)to_bf16(r[1]) << 16);
+      q.y = to_bf16(r[2]) | ((unsigned)to_bf16(r[3]) << 16);
+      q.z = to_bf16(r[4]) | ((unsigned)to_bf16(r[5]) << 16);
+      q.w = to_bf16(r[6]) | ((unsigned)to_bf16(r[7]) << 16);
+    }
+    return q;
+  };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < LA; e += 8) {
-      uint4 q;
-      q.x = to_bf16(ra[e]) | ((unsigned)to_bf16(ra[e + 1]) << 16);
-      q.y = to_bf16(ra[e + 2]) | ((unsigned)to_bf16(ra[e + 3]) << 16);
-      q.z = to_bf16(ra[e + 4]) | ((unsigned)to_bf16(ra[e + 5]) << 16);
-      q.w = to_bf16(ra[e + 6]) | ((unsigned)to_bf16(ra[e + 7]) << 16);
-      *reinterpret_cast<uint4*>(&sA[buf][ar][ak * LA + e]) = q;
-    }
+    for (int e = 0; e < LA; e += 8) *reinterpret_cast<uint4*>(&sA[buf][ar][ak * LA + e]) = pack(ra + e);
 #pragma unroll
-    for (int e = 0; e < LB; e += 8) {
-      uint4 q;
-      q.x = to_bf16(rb[e]) | ((unsigned)to_bf16(rb[e + 1]) << 16);
-      q.y = to_bf16(rb[e + 2]) | ((unsigned)to_bf16(rb[e + 3]) << 16);
-      q.z = to_bf16(rb[e + 4]) | ((unsigned)to_bf16(rb[e + 5]) << 16);
-      q.w = to_bf16(rb[e + 6]) | ((unsigned)to_bf16(rb[e + 7]) << 16);
-      *reinterpret_cast<uint4*>(&sB[buf][bc][bk * LB + e]) = q;
-    }
+    for (int e = 0; e < LB; e += 8) *reinterpret_cast<uint4*>(&sB[buf][bc][bk * LB + e]) = pack(rb + e);
   };
   static_assert(LA % 8 == 0 && LB % 8 == 0 && GA * LA == BK && GB * LB == BK,
                 "each thread stages whole 16-byte runs of k");
@@ -316,11 +333,24 @@ void launch_tile(hipStream_t s, int prec, GemmPair& pp) {
     hipLaunchKernelGGL((k_gemm<double, BM, BN>), dim3(total), dim3(256), 0, s, pp);
   else if (prec == GEMM_F32)
     hipLaunchKernelGGL((k_gemm<float, BM, BN>), dim3(total), dim3(256), 0, s, pp);
+  else if (prec == GEMM_BF16)
+    hipLaunchKernelGGL((k_gemm_bf16<BM, BN, false>), dim3(total), dim3(256), 0, s, pp);
   else
-    hipLaunchKernelGGL((k_gemm_bf16<BM, BN>), dim3(total), dim3(256), 0, s, pp);
+    hipLaunchKernelGGL((k_gemm_bf16<BM, BN, true>), dim3(total), dim3(256), 0, s, pp);
+}
+
+__global__ void k_round_bf16(const float* __restrict__ x, long n, unsigned short* __restrict__ y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = to_bf16(x[i]);
 }
 
 }  // namespace
+
+void launch_round_bf16(hipStream_t s, const float* x, long n, unsigned short* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_round_bf16, dim3((unsigned)std::min<long>(2048, gdiv(n, 256))), dim3(256), 0,
+                     s, x, n, y);
+}
 
 void launch_gemm(hipStream_t s, int prec, const GemmArgs* p, int count) {
   GemmPair pp;

@@ -197,6 +197,7 @@ class InvSolver {
 
   void *X_ = nullptr, *T_ = nullptr, *X2_ = nullptr;  // FP32, or FP64 when wide_
   float* Af_ = nullptr;                               // A rounded to FP32
+  unsigned short *Ab_ = nullptr, *Xb_ = nullptr, *Tb_ = nullptr;  // BF16 operands
   double *r_ = nullptr, *xold_ = nullptr, *part_ = nullptr;
   bool wide_ = false;
   unsigned long long* stat_ = nullptr;

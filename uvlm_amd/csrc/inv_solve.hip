@@ -322,11 +322,12 @@ double bits_to_double(unsigned long long u) {
 InvSolver::~InvSolver() { release(); }
 
 void InvSolver::release() {
-  void* bufs[] = {X_, T_, X2_, Af_, r_, xold_, part_, stat_};
+  void* bufs[] = {X_, T_, X2_, Af_, Ab_, Xb_, Tb_, r_, xold_, part_, stat_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   X_ = T_ = X2_ = nullptr;
   Af_ = nullptr;
+  Ab_ = Xb_ = Tb_ = nullptr;
   r_ = xold_ = part_ = nullptr;
   stat_ = nullptr;
   n_ = 0;
@@ -348,6 +349,9 @@ int InvSolver::prepare(hipStream_t s, int n, bool wide, std::string* err) {
   if (hipMalloc(&X_, el * nn) != hipSuccess || hipMalloc(&T_, el * nn) != hipSuccess ||
       hipMalloc(&X2_, el * nn) != hipSuccess ||
       (!wide && hipMalloc((void**)&Af_, sizeof(float) * nn) != hipSuccess) ||
+      (!wide && hipMalloc((void**)&Ab_, sizeof(unsigned short) * nn) != hipSuccess) ||
+      (!wide && hipMalloc((void**)&Xb_, sizeof(unsigned short) * nn) != hipSuccess) ||
+      (!wide && hipMalloc((void**)&Tb_, sizeof(unsigned short) * nn) != hipSuccess) ||
       hipMalloc((void**)&r_, sizeof(double) * n) != hipSuccess ||
       hipMalloc((void**)&xold_, sizeof(double) * n) != hipSuccess ||
       hipMalloc((void**)&part_, sizeof(double) * (size_t)S_ * n) != hipSuccess ||
@@ -382,8 +386,16 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A, bool low, bool w
                        Af_);
     Aop = Af_;
   }
-  const GemmArgs ax{n, n, n, Aop, n, X_, n, T_, n, 1.0, 0.0, nullptr};  // T = A X
-  launch_gemm(s, prec, &ax, 1);
+  // BF16 products read operands rounded once to BF16 (half the bytes of
+  // the FP32 data per tile; the same roundings as rounding while staging)
+  const bool b16 = prec == GEMM_BF16;
+  if (b16) {
+    launch_round_bf16(s, Af_, nn, Ab_);
+    launch_round_bf16(s, (const float*)X_, nn, Xb_);
+  }
+  const GemmArgs ax{n, n, n, b16 ? (const void*)Ab_ : Aop, n, b16 ? (const void*)Xb_ : X_, n,
+                    T_, n, 1.0, 0.0, nullptr};  // T = A X
+  launch_gemm(s, b16 ? GEMM_BF16_IN : prec, &ax, 1);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, stat_ + kNsSlot, 1);
   const dim3 g2(std::min(1024, cdiv(nn, 256)));
   if (wide_)
@@ -392,8 +404,10 @@ double InvSolver::newton_schulz(hipStream_t s, const double* A, bool low, bool w
   else
     hipLaunchKernelGGL(k_two_minus_dev<float>, g2, dim3(256), 0, s, (float*)T_, n,
                        stat_ + kNsSlot);
-  const GemmArgs xt{n, n, n, X_, n, T_, n, X2_, n, 1.0, 0.0, nullptr};  // X2 = X (2I - A X)
-  launch_gemm(s, prec, &xt, 1);
+  if (b16) launch_round_bf16(s, (const float*)T_, nn, Tb_);
+  const GemmArgs xt{n, n, n, b16 ? (const void*)Xb_ : X_, n, b16 ? (const void*)Tb_ : T_, n,
+                    X2_, n, 1.0, 0.0, nullptr};  // X2 = X (2I - A X)
+  launch_gemm(s, b16 ? GEMM_BF16_IN : prec, &xt, 1);
   std::swap(X_, X2_);
   ++refreshes_;
   if (!wait) return 0.0;
